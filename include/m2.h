@@ -1,0 +1,101 @@
+/*
+ * m2.h -- C ABI of the MI355X RHF Fock-build engine (libm2.so).
+ *
+ * This is the thin extern "C" layer between the host C++ Fock /
+ * IntegralEngine / SCF classes (molecular2_amd/csrc/host) and the HIP kernels.
+ * Plain pointers and sizes only.
+ *
+ * Conventions (the reference's, src/fockbuild.cpp and inc/eigen_wrapper.hpp:8):
+ *   - matrices are n x n FP64; all of them are symmetric, so row- and
+ *     column-major storage coincide (the unsymmetrised intermediates never
+ *     leave the library);
+ *   - D is the full closed-shell density 2 C_occ C_occ^T, or a difference
+ *     Delta-D; J = sum (uv|ls) D_ls and K = sum (ul|vs) D_ls, so the Fock
+ *     contribution is J - K/2 (src/fockbuild.cpp:161-163);
+ *   - the DF path takes unscaled C_occ (n x nocc, column-major) and returns J
+ *     and K over C C^T (src/fockbuild.cpp:171-174: jkints = 2J - K);
+ *   - basis functions: shells in the caller's order, Cartesian components in
+ *     the standard order (xx, xy, xz, yy, yz, zz, ...), pure shells m=-l..l
+ *     for l >= 2, p shells always x, y, z; contraction coefficients are passed
+ *     raw and normalised inside as libint2::Shell does (src/basis.cpp:33-66).
+ *
+ * Host buffers belong to the caller, device buffers to the context.  Calls
+ * are synchronous on return unless named *_device (those are ordered on the
+ * context's stream and also synchronous on return).  One host thread per
+ * context.  A non-zero status means failure; m2_last_error() says why (the
+ * C++ wrapper turns it into Error(code, msg), inc/error.hpp:18-35).
+ */
+#ifndef M2_H
+#define M2_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct m2_ctx m2_ctx;
+typedef int m2_status;
+
+enum { M2_OK = 0, M2_ERR_ARG = 1, M2_ERR_HIP = 2, M2_ERR_STATE = 3, M2_ERR_NUMERIC = 4 };
+
+/* J/K precision flags */
+enum {
+    M2_FP64 = 0,       /* every quartet in FP64 */
+    M2_MIXED = 1,      /* quartets with bound < fp32 threshold in FP32 */
+};
+
+/* which basis */
+enum { M2_ORBITAL = 0, M2_AUXILIARY = 1 };
+
+/* Context: device selection, stream, device buffers.  No reference
+ * counterpart (the reference is single-process, single-threaded CPU code);
+ * it owns what IntegralEngine and Fock own in src/integrals.cpp:42-124 and
+ * src/fock.cpp:32-73. */
+m2_status m2_ctx_create(m2_ctx** ctx, int device);
+void m2_ctx_destroy(m2_ctx* ctx);
+const char* m2_last_error(const m2_ctx* ctx);
+/* the HIP stream (hipStream_t) every call of this context runs on */
+void* m2_ctx_stream(m2_ctx* ctx);
+
+/* Basis.  Replaces Basis::addShell + libint2::Shell construction
+ * (src/basis.cpp:33-66) and IntegralEngine::map_shell_to_basis_function /
+ * nbasis (src/integrals.cpp:329-371).  exps/coefs hold sum(nprim) values;
+ * centers 3*nsh (bohr); atoms nsh.  For M2_ORBITAL this also builds the
+ * shell-pair data and the Schwarz bounds on the device. */
+m2_status m2_basis_set(m2_ctx* ctx, int which, int nsh, const int* l, const int* nprim,
+                       const double* exps, const double* coefs, const double* centers,
+                       const int* atoms, int pure);
+m2_status m2_basis_info(m2_ctx* ctx, int which, int* nbf, int* nsh);
+/* normalised coefficients of one shell (libint2 convention), for tests */
+m2_status m2_shell_coefs(m2_ctx* ctx, int which, int shell, double* out);
+
+/* IntegralEngine::compute_schwarz_ints (src/integrals.cpp:471-513):
+ * Q[nsh*nsh], Q(s1,s2) = sqrt(max |(ab|ab)|). */
+m2_status m2_schwarz(m2_ctx* ctx, double* Q);
+
+/* IntegralEngine::compute_shellblock_norm (src/integrals.cpp:451-469). */
+m2_status m2_shellblock_norm(m2_ctx* ctx, const double* A, double* out);
+
+/* IntegralEngine::compute_1body_ints for overlap, kinetic and nuclear
+ * attraction (src/integrals.cpp:394-449); point charges Z at xyz (bohr).
+ * Any of S, T, V may be NULL. */
+m2_status m2_one_body(m2_ctx* ctx, int natom, const double* Z, const double* xyz,
+                      double* S, double* T, double* V);
+
+/* Fock::formJKdirect (src/fockbuild.cpp:64-164): J and K of D, both
+ * multiplied by `multiplier`, quartets screened by
+ * max_6blocks(|D|) Q12 Q34 < precision.  flags: M2_FP64 or M2_MIXED. */
+m2_status m2_jk_direct(m2_ctx* ctx, const double* D, double multiplier, double precision,
+                       unsigned flags, double* J, double* K);
+/* same on device buffers (n*n doubles each) */
+m2_status m2_jk_direct_device(m2_ctx* ctx, const double* dD, double multiplier, double precision,
+                              unsigned flags, double* dJ, double* dK);
+/* quartets with Schwarz bound x block norm below this run in FP32 under
+ * M2_MIXED (default 1e-4) */
+m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below);
+/* statistics of the last J/K build: computed quartets (FP64, FP32), tasks */
+m2_status m2_last_stats(m2_ctx* ctx, long long* q64, long long* q32, long long* ntasks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,716 @@
+// C ABI of the engine (include/m2.h): context, basis and shell-pair set-up,
+// Schwarz bounds, one-electron integrals and the direct J/K driver.
+#include "m2.h"
+#include "m2_common.hpp"
+#include "boys.hpp"
+#include "generated/sph_tables.inc"
+#include "generated/jk_dispatch.inc"
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace m2 {
+void launch_schwarz0(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
+void launch_schwarz1(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
+void launch_schwarz2(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
+void launch_schwarz3(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
+void launch_schwarz4(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
+void launch_schwarz5(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
+void launch_one_body(const struct OneBodyArgs&, hipStream_t);
+void launch_blocknorm(const double* A, int n, int nsh, const int* sh_bf, const int* sh_nbf, double* out, hipStream_t);
+void launch_symmetrize(double* J, double* K, double* Jo, double* Ko, int n, hipStream_t);
+void launch_zero(double* p, size_t n, hipStream_t);
+}  // namespace m2
+
+using namespace m2;
+
+namespace {
+
+constexpr int NPAIRCLASS = 6;
+const int PAIR_L[NPAIRCLASS][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+typedef void (*sw_launcher)(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
+const sw_launcher SW_LAUNCH[NPAIRCLASS] = {launch_schwarz0, launch_schwarz1, launch_schwarz2,
+                                           launch_schwarz3, launch_schwarz4, launch_schwarz5};
+
+int pair_class(int la, int lb) {
+    for (int i = 0; i < NPAIRCLASS; ++i)
+        if (PAIR_L[i][0] == la && PAIR_L[i][1] == lb) return i;
+    return -1;
+}
+
+struct HostShell {
+    int l = 0, atom = -1, bf = 0, nbf = 0;
+    double O[3] = {0, 0, 0};
+    std::vector<double> a, c;
+};
+
+double dfact(int n) {
+    double r = 1.0;
+    for (int k = n; k > 1; k -= 2) r *= k;
+    return r;
+}
+
+// libint2::Shell normalisation: primitive factor of x^l exp(-a r^2), then the
+// contracted x^l function scaled to unit norm.
+void normalise(HostShell& s) {
+    const double pi32 = std::pow(M_PI, 1.5);
+    const int l = s.l;
+    for (size_t k = 0; k < s.a.size(); ++k) {
+        const double ta = 2.0 * s.a[k];
+        s.c[k] *= std::sqrt(std::pow(2.0, l) * std::pow(ta, l + 1.5) / (pi32 * dfact(2 * l - 1)));
+    }
+    double nrm = 0.0;
+    for (size_t p = 0; p < s.a.size(); ++p)
+        for (size_t q = 0; q < s.a.size(); ++q)
+            nrm += s.c[p] * s.c[q] * dfact(2 * l - 1) * pi32 / (std::pow(2.0, l) * std::pow(s.a[p] + s.a[q], l + 1.5));
+    nrm = 1.0 / std::sqrt(nrm);
+    for (auto& x : s.c) x *= nrm;
+}
+
+// Boys F_m(T) on the host, for the device tables: series
+// F_M(T) = e^-T sum_k (2T)^k / ((2M+1)(2M+3)...(2M+2k+1)) then downward recursion.
+void boys_host(int mmax, double T, double* F) {
+    const double et = std::exp(-T);
+    double term = 1.0 / (2 * mmax + 1), sum = term;
+    for (int k = 1; k < 2000; ++k) {
+        term *= 2.0 * T / (2 * mmax + 2 * k + 1);
+        sum += term;
+        if (term < 1e-20 * sum) break;
+    }
+    F[mmax] = et * sum;
+    for (int m = mmax - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + et) / (2 * m + 1);
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(cnt, 1) * sizeof(T));
+        if (e == hipSuccess) n = cnt;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct PairGroup {
+    int begin, end;   // global pair index range, Q descending inside
+};
+
+struct PairClassSeg {
+    int begin = 0, end = 0;
+    std::vector<PairGroup> groups;
+};
+
+struct BasisData {
+    int nsh = 0, nbf = 0, pure = 1;
+    std::vector<HostShell> sh;
+    std::vector<int> h_bf, h_nbf, h_l;
+    std::vector<double> h_O;
+    DevBuf<double> d_O;
+    DevBuf<int> d_bf, d_nbf;
+};
+
+}  // namespace
+
+struct m2_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    BasisData obs, aux;
+    // shell pairs of the orbital basis
+    std::vector<int2> pair_sh;
+    std::vector<int> pair_off, pair_np;
+    std::vector<double> pair_Q;
+    std::vector<double> Qsh;   // nsh x nsh
+    PairClassSeg cls[NPAIRCLASS];
+    DevBuf<PrimPair> d_prims;
+    DevBuf<int> d_pair_off, d_pair_np;
+    DevBuf<int2> d_pair_sh;
+    DevBuf<double> d_pair_Q;
+    // Boys tables
+    DevBuf<double> d_boys;
+    DevBuf<float> d_boys_f;
+    // J/K work buffers
+    DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
+    DevBuf<Task> d_tasks;
+    std::vector<Task> h_tasks;
+    std::vector<double> h_Dn;
+    double fp32_below = 1e-4;
+    long long last_tasks = 0;
+};
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);              \
+            return M2_ERR_HIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+#define ARGCHK(cond, msg)             \
+    do {                              \
+        if (!(cond)) {                \
+            ctx->err = (msg);         \
+            return M2_ERR_ARG;        \
+        }                             \
+    } while (0)
+
+namespace {
+
+BoysTables tables(m2_ctx* ctx) { return BoysTables{ctx->d_boys.p, ctx->d_boys_f.p}; }
+
+m2_status upload_basis(m2_ctx* ctx, BasisData& b) {
+    b.h_bf.resize(b.nsh);
+    b.h_nbf.resize(b.nsh);
+    b.h_l.resize(b.nsh);
+    b.h_O.resize(3 * b.nsh);
+    for (int s = 0; s < b.nsh; ++s) {
+        b.h_bf[s] = b.sh[s].bf;
+        b.h_nbf[s] = b.sh[s].nbf;
+        b.h_l[s] = b.sh[s].l;
+        for (int k = 0; k < 3; ++k) b.h_O[3 * s + k] = b.sh[s].O[k];
+    }
+    HIPCHK(b.d_O.reserve(3 * b.nsh));
+    HIPCHK(b.d_bf.reserve(b.nsh));
+    HIPCHK(b.d_nbf.reserve(b.nsh));
+    HIPCHK(hipMemcpy(b.d_O.p, b.h_O.data(), sizeof(double) * 3 * b.nsh, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.d_bf.p, b.h_bf.data(), sizeof(int) * b.nsh, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.d_nbf.p, b.h_nbf.data(), sizeof(int) * b.nsh, hipMemcpyHostToDevice));
+    return M2_OK;
+}
+
+int np_bucket(int np) {
+    // primitive-pair count buckets: lanes of one wavefront differ by < 2x
+    static const int edges[] = {1, 2, 4, 8, 16, 32, 64, 128, 1 << 30};
+    for (int i = 0; i < 9; ++i)
+        if (np <= edges[i]) return i;
+    return 9;
+}
+
+JKArgs jk_args(m2_ctx* ctx) {
+    JKArgs a{};
+    a.prims = ctx->d_prims.p;
+    a.pair_off = ctx->d_pair_off.p;
+    a.pair_np = ctx->d_pair_np.p;
+    a.pair_sh = ctx->d_pair_sh.p;
+    a.pair_Q = ctx->d_pair_Q.p;
+    a.sh_O = ctx->obs.d_O.p;
+    a.sh_bf = ctx->obs.d_bf.p;
+    a.Dn = ctx->d_Dn.p;
+    a.n = ctx->obs.nbf;
+    a.nsh = ctx->obs.nsh;
+    return a;
+}
+
+// Build canonical shell pairs (la >= lb; equal l: a >= b), their primitive
+// pair records, Schwarz bounds (device), then order each class by
+// (primitive-count bucket, Q descending).
+m2_status build_pairs(m2_ctx* ctx) {
+    BasisData& b = ctx->obs;
+    struct P {
+        int a, b, cls;
+        std::vector<PrimPair> pp;
+    };
+    std::vector<P> pairs;
+    const double sqrt2pi52 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+    for (int s1 = 0; s1 < b.nsh; ++s1)
+        for (int s2 = 0; s2 < b.nsh; ++s2) {
+            const HostShell &A = b.sh[s1], &B = b.sh[s2];
+            if (A.l < B.l || (A.l == B.l && s1 < s2)) continue;
+            P pr;
+            pr.a = s1;
+            pr.b = s2;
+            pr.cls = pair_class(A.l, B.l);
+            if (pr.cls < 0) {
+                ctx->err = "m2_basis_set: orbital shells must have l <= 2";
+                return M2_ERR_ARG;
+            }
+            double AB2 = 0;
+            for (int k = 0; k < 3; ++k) AB2 += (A.O[k] - B.O[k]) * (A.O[k] - B.O[k]);
+            for (size_t i = 0; i < A.a.size(); ++i)
+                for (size_t j = 0; j < B.a.size(); ++j) {
+                    const double a = A.a[i], bb = B.a[j], p = a + bb;
+                    const double ex = std::exp(-a * bb / p * AB2);
+                    // primitive-pair screen far below any FP64 J/K tolerance
+                    if (ex < 1e-20) continue;
+                    PrimPair q{};
+                    q.p = p;
+                    q.inv2p = 0.5 / p;
+                    q.K = sqrt2pi52 * ex * A.c[i] * B.c[j] / p;
+                    for (int k = 0; k < 3; ++k) {
+                        q.P[k] = (a * A.O[k] + bb * B.O[k]) / p;
+                        q.PA[k] = q.P[k] - A.O[k];
+                    }
+                    pr.pp.push_back(q);
+                }
+            pairs.push_back(std::move(pr));
+        }
+    // first layout: by class, for the Schwarz pass
+    std::stable_sort(pairs.begin(), pairs.end(), [](const P& x, const P& y) { return x.cls < y.cls; });
+    auto upload = [&](void) -> m2_status {
+        const int npair = (int)pairs.size();
+        ctx->pair_sh.resize(npair);
+        ctx->pair_off.resize(npair);
+        ctx->pair_np.resize(npair);
+        std::vector<PrimPair> prims;
+        for (int i = 0; i < npair; ++i) {
+            ctx->pair_sh[i] = make_int2(pairs[i].a, pairs[i].b);
+            ctx->pair_off[i] = (int)prims.size();
+            ctx->pair_np[i] = (int)pairs[i].pp.size();
+            prims.insert(prims.end(), pairs[i].pp.begin(), pairs[i].pp.end());
+        }
+        HIPCHK(ctx->d_prims.reserve(prims.size()));
+        HIPCHK(ctx->d_pair_off.reserve(npair));
+        HIPCHK(ctx->d_pair_np.reserve(npair));
+        HIPCHK(ctx->d_pair_sh.reserve(npair));
+        HIPCHK(ctx->d_pair_Q.reserve(npair));
+        if (!prims.empty())
+            HIPCHK(hipMemcpy(ctx->d_prims.p, prims.data(), sizeof(PrimPair) * prims.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_pair_off.p, ctx->pair_off.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_pair_np.p, ctx->pair_np.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_pair_sh.p, ctx->pair_sh.data(), sizeof(int2) * npair, hipMemcpyHostToDevice));
+        return M2_OK;
+    };
+    m2_status st = upload();
+    if (st) return st;
+    // Schwarz bounds on the device, all pairs, FP64
+    const int npair = (int)pairs.size();
+    std::vector<double> Q(npair, 0.0);
+    {
+        JKArgs a = jk_args(ctx);
+        int i0 = 0;
+        while (i0 < npair) {
+            int c = pairs[i0].cls, i1 = i0;
+            while (i1 < npair && pairs[i1].cls == c) ++i1;
+            SW_LAUNCH[c](b.pure, a, tables(ctx), i0, i1 - i0, ctx->d_pair_Q.p, ctx->stream);
+            HIPCHK(hipGetLastError());
+            i0 = i1;
+        }
+        HIPCHK(hipMemcpyAsync(Q.data(), ctx->d_pair_Q.p, sizeof(double) * npair, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->Qsh.assign((size_t)b.nsh * b.nsh, 0.0);
+    for (int i = 0; i < npair; ++i) {
+        ctx->Qsh[(size_t)pairs[i].a * b.nsh + pairs[i].b] = Q[i];
+        ctx->Qsh[(size_t)pairs[i].b * b.nsh + pairs[i].a] = Q[i];
+    }
+    // final layout: class, np bucket, Q descending
+    std::vector<int> order(npair);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (pairs[x].cls != pairs[y].cls) return pairs[x].cls < pairs[y].cls;
+        int bx = np_bucket((int)pairs[x].pp.size()), by = np_bucket((int)pairs[y].pp.size());
+        if (bx != by) return bx > by;
+        return Q[x] > Q[y];
+    });
+    std::vector<P> sorted;
+    std::vector<double> Qs;
+    sorted.reserve(npair);
+    for (int i : order) {
+        sorted.push_back(std::move(pairs[i]));
+        Qs.push_back(Q[i]);
+    }
+    pairs.swap(sorted);
+    st = upload();
+    if (st) return st;
+    ctx->pair_Q = Qs;
+    HIPCHK(hipMemcpy(ctx->d_pair_Q.p, Qs.data(), sizeof(double) * npair, hipMemcpyHostToDevice));
+    for (int c = 0; c < NPAIRCLASS; ++c) ctx->cls[c] = PairClassSeg{};
+    int i0 = 0;
+    while (i0 < npair) {
+        int c = pairs[i0].cls;
+        PairClassSeg& seg = ctx->cls[c];
+        seg.begin = i0;
+        int i1 = i0;
+        while (i1 < npair && pairs[i1].cls == c) {
+            int bk = np_bucket((int)pairs[i1].pp.size()), g0 = i1;
+            while (i1 < npair && pairs[i1].cls == c && np_bucket((int)pairs[i1].pp.size()) == bk) ++i1;
+            seg.groups.push_back(PairGroup{g0, i1});
+        }
+        seg.end = i1;
+        i0 = i1;
+    }
+    return M2_OK;
+}
+
+// number of leading pairs of [g.begin, g.end) (Q descending) with Q >= thr
+int count_ge(const std::vector<double>& Q, int b, int e, double thr) {
+    int lo = b, hi = e;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (Q[mid] >= thr) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - b;
+}
+
+struct ComboRange {
+    int X, Y, t0, t1;
+};
+
+// Work list: for every bra class X >= ket class Y, every bra pair i and every
+// primitive-count group of Y, the ket pairs j whose Q_j clears
+// precision / (Q_i Dmax) -- a superset of the exactly screened quartets,
+// with j <= i inside one class (unique pairs of pairs, src/fockbuild.cpp:88-109).
+std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax) {
+    std::vector<ComboRange> ranges;
+    auto& T = ctx->h_tasks;
+    T.clear();
+    const auto& Q = ctx->pair_Q;
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        const PairClassSeg& sx = ctx->cls[X];
+        for (int Y = 0; Y <= X; ++Y) {
+            const PairClassSeg& sy = ctx->cls[Y];
+            ComboRange r{X, Y, (int)T.size(), 0};
+            for (int i = sx.begin; i < sx.end; ++i) {
+                const double qi = Q[i];
+                if (qi <= 0.0 || Dmax <= 0.0) continue;
+                const double thr = precision / (qi * Dmax);
+                for (const PairGroup& g : sy.groups) {
+                    int cnt = count_ge(Q, g.begin, g.end, thr);
+                    int end = g.begin + cnt;
+                    if (X == Y) end = std::min(end, i + 1);
+                    for (int j = g.begin; j < end; j += 64) T.push_back(Task{i, j, std::min(64, end - j), 0});
+                }
+            }
+            r.t1 = (int)T.size();
+            if (r.t1 > r.t0) ranges.push_back(r);
+        }
+    }
+    return ranges;
+}
+
+m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, double precision, unsigned flags,
+                         double* dJ, double* dK) {
+    BasisData& b = ctx->obs;
+    const int n = b.nbf, nsh = b.nsh;
+    ARGCHK(n > 0, "m2_jk_direct: no orbital basis set");
+    ARGCHK(flags <= M2_MIXED, "m2_jk_direct: unknown flags");
+    HIPCHK(ctx->d_Dn.reserve((size_t)nsh * nsh));
+    HIPCHK(ctx->d_J.reserve((size_t)n * n));
+    HIPCHK(ctx->d_K.reserve((size_t)n * n));
+    launch_blocknorm(dD, n, nsh, b.d_bf.p, b.d_nbf.p, ctx->d_Dn.p, ctx->stream);
+    HIPCHK(hipGetLastError());
+    ctx->h_Dn.resize((size_t)nsh * nsh);
+    HIPCHK(hipMemcpyAsync(ctx->h_Dn.data(), ctx->d_Dn.p, sizeof(double) * nsh * nsh, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double Dmax = 0.0;
+    for (double x : ctx->h_Dn) Dmax = std::max(Dmax, x);
+    std::vector<ComboRange> ranges = build_tasks(ctx, precision, Dmax);
+    ctx->last_tasks = (long long)ctx->h_tasks.size();
+    if (!ctx->h_tasks.empty()) {
+        HIPCHK(ctx->d_tasks.reserve(ctx->h_tasks.size()));
+        HIPCHK(hipMemcpyAsync(ctx->d_tasks.p, ctx->h_tasks.data(), sizeof(Task) * ctx->h_tasks.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+    }
+    launch_zero(ctx->d_J.p, (size_t)n * n, ctx->stream);
+    launch_zero(ctx->d_K.p, (size_t)n * n, ctx->stream);
+    JKArgs a = jk_args(ctx);
+    a.D = dD;
+    a.J = ctx->d_J.p;
+    a.K = ctx->d_K.p;
+    a.precision = precision;
+    a.fp32_below = ctx->fp32_below;
+    a.scale = multiplier;
+    for (const ComboRange& r : ranges) {
+        if (flags == M2_FP64) {
+            JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p + r.t0, r.t1 - r.t0, 0, ctx->stream);
+        } else {
+            JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p + r.t0, r.t1 - r.t0, 1, ctx->stream);
+            JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p + r.t0, r.t1 - r.t0, 2, ctx->stream);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    launch_symmetrize(ctx->d_J.p, ctx->d_K.p, dJ, dK, n, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return M2_OK;
+}
+
+}  // namespace
+
+namespace m2 {
+struct OneBodyArgs {
+    const double* sh_O;
+    const int* sh_l;
+    const int* sh_bf;
+    const int* sh_np;
+    const int* sh_poff;
+    const double* exps;
+    const double* coefs;
+    int nsh, n, pure, natom;
+    const double* Z;
+    const double* xyz;
+    const double* sph;
+    double *S, *T, *V;
+    BoysTables tab;
+};
+}  // namespace m2
+
+extern "C" {
+
+m2_status m2_ctx_create(m2_ctx** out, int device) {
+    if (!out) return M2_ERR_ARG;
+    *out = nullptr;
+    m2_ctx* ctx = new m2_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "m2_ctx_create: %s\n", hipGetErrorString(e));
+        delete ctx;
+        return M2_ERR_HIP;
+    }
+    // Boys tables, F_m(k * 0.1) for m < BOYS_MTAB
+    std::vector<double> tab((size_t)BOYS_NGRID * BOYS_MTAB);
+    std::vector<float> tabf(tab.size());
+    for (int k = 0; k < BOYS_NGRID; ++k) boys_host(BOYS_MTAB - 1, k * BOYS_STEP, &tab[(size_t)k * BOYS_MTAB]);
+    for (size_t i = 0; i < tab.size(); ++i) tabf[i] = (float)tab[i];
+    if (ctx->d_boys.reserve(tab.size()) != hipSuccess || ctx->d_boys_f.reserve(tab.size()) != hipSuccess ||
+        hipMemcpy(ctx->d_boys.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ctx->d_boys_f.p, tabf.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        std::fprintf(stderr, "m2_ctx_create: device allocation failed\n");
+        m2_ctx_destroy(ctx);
+        return M2_ERR_HIP;
+    }
+    *out = ctx;
+    return M2_OK;
+}
+
+void m2_ctx_destroy(m2_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (BasisData* b : {&ctx->obs, &ctx->aux}) {
+        b->d_O.release();
+        b->d_bf.release();
+        b->d_nbf.release();
+    }
+    ctx->d_prims.release();
+    ctx->d_pair_off.release();
+    ctx->d_pair_np.release();
+    ctx->d_pair_sh.release();
+    ctx->d_pair_Q.release();
+    ctx->d_boys.release();
+    ctx->d_boys_f.release();
+    ctx->d_D.release();
+    ctx->d_J.release();
+    ctx->d_K.release();
+    ctx->d_Jo.release();
+    ctx->d_Ko.release();
+    ctx->d_Dn.release();
+    ctx->d_tasks.release();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* m2_last_error(const m2_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* m2_ctx_stream(m2_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+m2_status m2_basis_set(m2_ctx* ctx, int which, int nsh, const int* l, const int* nprim, const double* exps,
+                       const double* coefs, const double* centers, const int* atoms, int pure) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(which == M2_ORBITAL || which == M2_AUXILIARY, "m2_basis_set: which must be M2_ORBITAL or M2_AUXILIARY");
+    ARGCHK(nsh > 0 && l && nprim && exps && coefs && centers, "m2_basis_set: null or empty input");
+    HIPCHK(hipSetDevice(ctx->device));
+    BasisData& b = (which == M2_ORBITAL) ? ctx->obs : ctx->aux;
+    b.nsh = nsh;
+    b.pure = pure ? 1 : 0;
+    b.sh.assign(nsh, HostShell{});
+    int off = 0, bf = 0;
+    for (int s = 0; s < nsh; ++s) {
+        HostShell& S = b.sh[s];
+        ARGCHK(l[s] >= 0 && l[s] <= 4 && nprim[s] > 0, "m2_basis_set: shell l must be 0..4 and nprim > 0");
+        S.l = l[s];
+        S.atom = atoms ? atoms[s] : -1;
+        for (int k = 0; k < 3; ++k) S.O[k] = centers[3 * s + k];
+        S.a.assign(exps + off, exps + off + nprim[s]);
+        S.c.assign(coefs + off, coefs + off + nprim[s]);
+        for (double x : S.a) ARGCHK(x > 0.0, "m2_basis_set: exponents must be positive");
+        off += nprim[s];
+        normalise(S);
+        S.nbf = b.pure ? 2 * S.l + 1 : (S.l + 1) * (S.l + 2) / 2;
+        S.bf = bf;
+        bf += S.nbf;
+    }
+    b.nbf = bf;
+    m2_status st = upload_basis(ctx, b);
+    if (st) return st;
+    if (which == M2_ORBITAL) return build_pairs(ctx);
+    return M2_OK;
+}
+
+m2_status m2_basis_info(m2_ctx* ctx, int which, int* nbf, int* nsh) {
+    if (!ctx) return M2_ERR_ARG;
+    BasisData& b = (which == M2_ORBITAL) ? ctx->obs : ctx->aux;
+    if (nbf) *nbf = b.nbf;
+    if (nsh) *nsh = b.nsh;
+    return M2_OK;
+}
+
+m2_status m2_shell_coefs(m2_ctx* ctx, int which, int shell, double* out) {
+    if (!ctx) return M2_ERR_ARG;
+    BasisData& b = (which == M2_ORBITAL) ? ctx->obs : ctx->aux;
+    ARGCHK(shell >= 0 && shell < b.nsh && out, "m2_shell_coefs: bad shell index");
+    std::copy(b.sh[shell].c.begin(), b.sh[shell].c.end(), out);
+    return M2_OK;
+}
+
+m2_status m2_schwarz(m2_ctx* ctx, double* Q) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(Q && ctx->obs.nsh > 0, "m2_schwarz: no orbital basis or null output");
+    std::copy(ctx->Qsh.begin(), ctx->Qsh.end(), Q);
+    return M2_OK;
+}
+
+m2_status m2_shellblock_norm(m2_ctx* ctx, const double* A, double* out) {
+    if (!ctx) return M2_ERR_ARG;
+    BasisData& b = ctx->obs;
+    ARGCHK(A && out && b.nbf > 0, "m2_shellblock_norm: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)b.nbf * b.nbf;
+    HIPCHK(ctx->d_D.reserve(n2));
+    HIPCHK(ctx->d_Dn.reserve((size_t)b.nsh * b.nsh));
+    HIPCHK(hipMemcpyAsync(ctx->d_D.p, A, sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
+    launch_blocknorm(ctx->d_D.p, b.nbf, b.nsh, b.d_bf.p, b.d_nbf.p, ctx->d_Dn.p, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ctx->d_Dn.p, sizeof(double) * b.nsh * b.nsh, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return M2_OK;
+}
+
+m2_status m2_one_body(m2_ctx* ctx, int natom, const double* Z, const double* xyz, double* S, double* T, double* V) {
+    if (!ctx) return M2_ERR_ARG;
+    BasisData& b = ctx->obs;
+    ARGCHK(b.nbf > 0, "m2_one_body: no orbital basis set");
+    ARGCHK(natom >= 0 && (natom == 0 || (Z && xyz)), "m2_one_body: bad charges");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = b.nbf;
+    std::vector<int> np(b.nsh), poff(b.nsh);
+    std::vector<double> ex, cf;
+    for (int s = 0; s < b.nsh; ++s) {
+        np[s] = (int)b.sh[s].a.size();
+        poff[s] = (int)ex.size();
+        ex.insert(ex.end(), b.sh[s].a.begin(), b.sh[s].a.end());
+        cf.insert(cf.end(), b.sh[s].c.begin(), b.sh[s].c.end());
+    }
+    DevBuf<int> d_l, d_np, d_poff;
+    DevBuf<double> d_ex, d_cf, d_Z, d_xyz, d_S, d_T, d_V, d_sph;
+    m2_status st = M2_OK;
+    auto fail = [&](hipError_t e, const char* what) {
+        ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+        st = M2_ERR_HIP;
+    };
+    hipError_t e = hipSuccess;
+    if ((e = d_l.reserve(b.nsh)) || (e = d_np.reserve(b.nsh)) || (e = d_poff.reserve(b.nsh)) ||
+        (e = d_ex.reserve(ex.size())) || (e = d_cf.reserve(cf.size())) || (e = d_Z.reserve(natom + 1)) ||
+        (e = d_xyz.reserve(3 * natom + 3)) || (e = d_S.reserve((size_t)n * n)) || (e = d_T.reserve((size_t)n * n)) ||
+        (e = d_V.reserve((size_t)n * n)) || (e = d_sph.reserve(sizeof(SPH_TAB) / sizeof(double)))) {
+        fail(e, "m2_one_body: hipMalloc");
+    }
+    if (!st) {
+        (void)hipMemcpy(d_l.p, b.h_l.data(), sizeof(int) * b.nsh, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_np.p, np.data(), sizeof(int) * b.nsh, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_poff.p, poff.data(), sizeof(int) * b.nsh, hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_ex.p, ex.data(), sizeof(double) * ex.size(), hipMemcpyHostToDevice);
+        (void)hipMemcpy(d_cf.p, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice);
+        if (natom > 0) {
+            (void)hipMemcpy(d_Z.p, Z, sizeof(double) * natom, hipMemcpyHostToDevice);
+            (void)hipMemcpy(d_xyz.p, xyz, sizeof(double) * 3 * natom, hipMemcpyHostToDevice);
+        }
+        e = hipMemcpy(d_sph.p, SPH_TAB, sizeof(SPH_TAB), hipMemcpyHostToDevice);
+        if (e) fail(e, "m2_one_body: upload");
+    }
+    if (!st) {
+        OneBodyArgs a{};
+        a.sh_O = b.d_O.p;
+        a.sh_l = d_l.p;
+        a.sh_bf = b.d_bf.p;
+        a.sh_np = d_np.p;
+        a.sh_poff = d_poff.p;
+        a.exps = d_ex.p;
+        a.coefs = d_cf.p;
+        a.nsh = b.nsh;
+        a.n = n;
+        a.pure = b.pure;
+        a.natom = natom;
+        a.Z = d_Z.p;
+        a.xyz = d_xyz.p;
+        a.sph = d_sph.p;
+        a.tab = tables(ctx);
+        a.S = d_S.p;
+        a.T = d_T.p;
+        a.V = d_V.p;
+        launch_one_body(a, ctx->stream);
+        e = hipGetLastError();
+        if (!e) e = hipStreamSynchronize(ctx->stream);
+        if (!e && S) e = hipMemcpy(S, d_S.p, sizeof(double) * n * n, hipMemcpyDeviceToHost);
+        if (!e && T) e = hipMemcpy(T, d_T.p, sizeof(double) * n * n, hipMemcpyDeviceToHost);
+        if (!e && V) e = hipMemcpy(V, d_V.p, sizeof(double) * n * n, hipMemcpyDeviceToHost);
+        if (e) fail(e, "m2_one_body: kernel");
+    }
+    for (auto* x : {&d_l, &d_np, &d_poff}) x->release();
+    for (auto* x : {&d_ex, &d_cf, &d_Z, &d_xyz, &d_S, &d_T, &d_V, &d_sph}) x->release();
+    return st;
+}
+
+m2_status m2_jk_direct_device(m2_ctx* ctx, const double* dD, double multiplier, double precision, unsigned flags,
+                              double* dJ, double* dK) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(dD && dJ && dK, "m2_jk_direct_device: null buffer");
+    HIPCHK(hipSetDevice(ctx->device));
+    return jk_device_core(ctx, dD, multiplier, precision, flags, dJ, dK);
+}
+
+m2_status m2_jk_direct(m2_ctx* ctx, const double* D, double multiplier, double precision, unsigned flags, double* J,
+                       double* K) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(D && J && K, "m2_jk_direct: null buffer");
+    ARGCHK(ctx->obs.nbf > 0, "m2_jk_direct: no orbital basis set");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)ctx->obs.nbf * ctx->obs.nbf;
+    HIPCHK(ctx->d_D.reserve(n2));
+    HIPCHK(ctx->d_Jo.reserve(n2));
+    HIPCHK(ctx->d_Ko.reserve(n2));
+    HIPCHK(hipMemcpyAsync(ctx->d_D.p, D, sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
+    m2_status st = jk_device_core(ctx, ctx->d_D.p, multiplier, precision, flags, ctx->d_Jo.p, ctx->d_Ko.p);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(J, ctx->d_Jo.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(K, ctx->d_Ko.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return M2_OK;
+}
+
+m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(fp32_below >= 0.0, "m2_set_mixed_threshold: threshold must be >= 0");
+    ctx->fp32_below = fp32_below;
+    return M2_OK;
+}
+
+m2_status m2_last_stats(m2_ctx* ctx, long long* q64, long long* q32, long long* ntasks) {
+    if (!ctx) return M2_ERR_ARG;
+    if (q64) *q64 = -1;
+    if (q32) *q32 = -1;
+    if (ntasks) *ntasks = ctx->last_tasks;
+    return M2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Device Boys function F_m(T), m = 0..M, for the ERI kernels.
+//
+// FP64: 8-term Taylor expansion of F_M about the nearest grid point
+// (|dT| <= 0.05, truncation < 1e-15 relative), then downward recursion
+// F_m = (2T F_{m+1} + e^-T) / (2m+1), which is stable.  For T >= 40 the
+// asymptotic F_0 = sqrt(pi/T)/2 (erfc(sqrt(40)) < 1e-18) and upward
+// recursion F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable for T > m.
+// FP32: 5-term Taylor from a float table (truncation < 3e-9 relative) with
+// the same downward recursion; the large-T branch is evaluated in FP64.
+#pragma once
+#include "m2_common.hpp"
+
+namespace m2 {
+
+struct BoysTables {
+    const double* d;   // [BOYS_NGRID][BOYS_MTAB]
+    const float* f;
+};
+
+template <typename R>
+__device__ __forceinline__ R sqrt_inv(R x);
+template <>
+__device__ __forceinline__ double sqrt_inv<double>(double x) { return rsqrt(x); }
+template <>
+__device__ __forceinline__ float sqrt_inv<float>(float x) { return rsqrtf(x); }
+
+template <typename R, int M>
+struct BoysEval;
+
+template <int M>
+struct BoysEval<double, M> {
+    static __device__ __forceinline__ void run(double T, double* F, const BoysTables& tab) {
+        if (T < BOYS_TMAX) {
+            const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
+            const double d = k * BOYS_STEP - T;
+            const double* r = tab.d + k * BOYS_MTAB + M;
+            double s = r[7];
+            s = r[6] + d * (1.0 / 7.0) * s;
+            s = r[5] + d * (1.0 / 6.0) * s;
+            s = r[4] + d * (1.0 / 5.0) * s;
+            s = r[3] + d * (1.0 / 4.0) * s;
+            s = r[2] + d * (1.0 / 3.0) * s;
+            s = r[1] + d * (1.0 / 2.0) * s;
+            s = r[0] + d * s;
+            F[M] = s;
+            if constexpr (M > 0) {
+                const double e = exp(-T);
+                const double t2 = 2.0 * T;
+#pragma unroll
+                for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
+            }
+        } else {
+            const double it = 1.0 / T;
+            F[0] = 0.88622692545275801365 * sqrt(it);   // sqrt(pi)/2 / sqrt(T)
+            if constexpr (M > 0) {
+                const double e = exp(-T);
+                const double h = 0.5 * it;
+#pragma unroll
+                for (int m = 0; m < M; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+            }
+        }
+    }
+};
+
+template <int M>
+struct BoysEval<float, M> {
+    static __device__ __forceinline__ void run(float T, float* F, const BoysTables& tab) {
+        if (T < float(BOYS_TMAX)) {
+            const int k = __float2int_rn(T * float(1.0 / BOYS_STEP));
+            const float d = k * float(BOYS_STEP) - T;
+            const float* r = tab.f + k * BOYS_MTAB + M;
+            float s = r[4];
+            s = r[3] + d * (1.0f / 4.0f) * s;
+            s = r[2] + d * (1.0f / 3.0f) * s;
+            s = r[1] + d * (1.0f / 2.0f) * s;
+            s = r[0] + d * s;
+            F[M] = s;
+            if constexpr (M > 0) {
+                const float e = __expf(-T);
+                const float t2 = 2.0f * T;
+#pragma unroll
+                for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0f / (2 * m + 1));
+            }
+        } else {
+            double Fd[M + 1];
+            BoysEval<double, M>::run(double(T), Fd, tab);
+#pragma unroll
+            for (int m = 0; m <= M; ++m) F[m] = float(Fd[m]);
+        }
+    }
+};
+
+}  // namespace m2

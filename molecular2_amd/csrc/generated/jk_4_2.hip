@@ -1,0 +1,9 @@
+// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.
+#include "../jk_kernel.hpp"
+namespace m2 {
+#include "eri_2111.inc"
+void launch_jk_4_2(int pure, const JKArgs& a, const BoysTables& tab, const Task* t, int nt, int mode, hipStream_t st) {
+  if (pure) launch_jk<2, 1, 1, 1, true>(a, tab, t, nt, mode, st);
+  else launch_jk<2, 1, 1, 1, false>(a, tab, t, nt, mode, st);
+}
+}  // namespace m2

@@ -1,0 +1,55 @@
+// Shared device/host definitions of the MI355X Fock-build engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace m2 {
+
+// Primitive shell-pair record, one per surviving primitive pair of a shell
+// pair (a,b) with la >= lb.  K folds the contraction coefficients, the
+// Gaussian-product prefactor exp(-ab/p |AB|^2), sqrt(2 pi^{5/2}) and 1/p, so
+// that [00|00]^(m) = K_ab K_cd / sqrt(p+q) F_m(T).
+struct alignas(16) PrimPair {
+    double p;       // a + b
+    double inv2p;   // 1 / (2p)
+    double K;
+    double P[3];    // product centre
+    double PA[3];   // P - A (A = centre of the pair's first shell)
+    double pad;
+};
+static_assert(sizeof(PrimPair) == 80, "PrimPair layout");
+
+// Boys table: F_m(T_k) at T_k = k * BOYS_STEP, k = 0..BOYS_NGRID-1,
+// m = 0..BOYS_MTAB-1.  Beyond BOYS_TMAX the asymptotic branch is used.
+constexpr int BOYS_MTAB = 26;
+constexpr int BOYS_NGRID = 401;
+constexpr double BOYS_STEP = 0.1;
+constexpr double BOYS_TMAX = 40.0;
+
+struct JKArgs {
+    const PrimPair* prims;
+    const int* pair_off;     // first PrimPair of each shell pair
+    const int* pair_np;      // number of PrimPairs of each shell pair
+    const int2* pair_sh;     // (a, b) shell indices, la >= lb
+    const double* pair_Q;    // Schwarz bound of each pair
+    const double* sh_O;      // shell centres, 3 per shell
+    const int* sh_bf;        // first basis function of each shell
+    const double* Dn;        // shell-block inf-norms of D, nsh x nsh
+    const double* D;         // density (or Delta-D), n x n
+    double* J;               // unsymmetrised accumulators, n x n
+    double* K;
+    int n;
+    int nsh;
+    double precision;        // quartet screening threshold
+    double fp32_below;       // quartets with bound < this run in FP32 (mixed mode)
+    double scale;            // multiplier
+};
+
+struct Task {                // one wavefront of quartets: bra pair x 1..64 ket pairs
+    int bra;
+    int ket0;
+    int nket;
+    int pad;
+};
+
+}  // namespace m2

@@ -1,0 +1,77 @@
+"""Parity of the HIP direct J/K path (m2_jk_direct) with the oracle's
+restatement of Fock::formJKdirect (src/fockbuild.cpp:64-164)."""
+import numpy as np
+import pytest
+
+from molecular2_amd import molecules as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_density(n, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    C = rng.normal(size=(n, max(2, n // 4)))
+    D = 2.0 * C @ C.T / n
+    return scale * (D + D.T) / 2
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_water_one_body_and_schwarz(oracle, engine, pure):
+    mol = M.water()
+    sh = mol.shells("cc-pvdz")
+    print("case: water one-body/schwarz pure=%s" % pure, flush=True)
+    ob = oracle.Basis(sh, pure=pure)
+    nbf = engine.set_basis(sh, pure=pure)
+    assert nbf == ob.nbf == (24 if pure else 25)
+    for s in range(len(sh)):
+        np.testing.assert_allclose(engine.shell_coefs(s, len(sh[s].exps)), ob.norm_coefs(s), rtol=1e-13, atol=0)
+    S, T, V = engine.one_body(mol.Z, mol.xyz)
+    So, To, Vo = ob.one_body(mol.Z, mol.xyz)
+    for a, b in ((S, So), (T, To), (V, Vo)):
+        assert np.max(np.abs(a - b)) < 1e-12
+    Q = engine.schwarz()
+    assert np.max(np.abs(Q - ob.schwarz())) < 1e-12
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_water_jk_fp64(oracle, engine, pure):
+    mol = M.water()
+    sh = mol.shells("cc-pvdz")
+    print("case: water jk pure=%s" % pure, flush=True)
+    ob = oracle.Basis(sh, pure=pure)
+    engine.set_basis(sh, pure=pure)
+    D = _rand_density(ob.nbf, 1)
+    Q = ob.schwarz()
+    for mult in (1.0, 0.5):
+        Jo, Ko = ob.jk_direct(D, Q, 1e-12, mult)
+        J, K = engine.jk(D, mult, 1e-12)
+        assert np.max(np.abs(J - Jo)) < 1e-10, np.max(np.abs(J - Jo))
+        assert np.max(np.abs(K - Ko)) < 1e-10, np.max(np.abs(K - Ko))
+
+
+def test_benzene_jk_fp64(oracle, engine):
+    mol = M.benzene()
+    sh = mol.shells("cc-pvdz")
+    print("case: benzene jk fp64", flush=True)
+    ob = oracle.Basis(sh, pure=True)
+    engine.set_basis(sh, pure=True)
+    D = _rand_density(ob.nbf, 2)
+    Jo, Ko = ob.jk_direct(D, ob.schwarz(), 1e-12, 1.0)
+    J, K = engine.jk(D, 1.0, 1e-12)
+    assert np.max(np.abs(J - Jo)) < 1e-10, np.max(np.abs(J - Jo))
+    assert np.max(np.abs(K - Ko)) < 1e-10, np.max(np.abs(K - Ko))
+
+
+def test_water_scf_energy_gpu_jk(oracle, engine):
+    """Oracle SCF driver with the HIP J/K: the 6d water energy pin
+    (test/h2otest.out_1:124, -75.990178761701)."""
+    mol = M.water_h2otest()
+    sh = mol.shells("cc-pvdz")
+    print("case: water 6d scf", flush=True)
+    engine.set_basis(sh, pure=False)
+    S, T, V = engine.one_body(mol.Z, mol.xyz)
+    jk = lambda P, m: engine.jk(P, m, 1e-12)
+    scf = oracle.RHF(S, T + V, mol.enuc(), mol.nel, jk, enconverge=1e-11, densconverge=1e-9)
+    E = scf.run()
+    assert scf.converged
+    assert abs(E - (-75.990178761701)) < 1e-8
