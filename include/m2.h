@@ -95,6 +95,44 @@ m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below);
 /* statistics of the last J/K build: computed quartets (FP64, FP32), tasks */
 m2_status m2_last_stats(m2_ctx* ctx, long long* q64, long long* q32, long long* ntasks);
 
+/* Density-fitted J/K: Fock::formJKdf / compute_2body_fock_df
+ * (src/fockbuild.cpp:171-174,585-660) with the 3- and 2-centre integrals of
+ * IntegralEngine::compute_eris_3index / compute_eris_2index
+ * (src/eris.cpp:89-156,304-356).  Needs the M2_AUXILIARY basis (l <= 3).
+ * m2_df_setup builds the fitted tensor (3c/2c integrals, FP64 Cholesky of the
+ * metric, B = (xy|K) L^-T) on the device; m2_jk_df calls it on first use.
+ * Cocc is n x nocc column-major (unscaled occupied orbitals); J and K are
+ * over Cocc Cocc^T.  fp32 != 0: FP32 GEMMs with FP32 accumulation (MFMA);
+ * fp32 == 0: FP64 GEMMs. */
+m2_status m2_df_setup(m2_ctx* ctx);
+m2_status m2_jk_df(m2_ctx* ctx, const double* Cocc, int nocc, int fp32, double* J, double* K);
+m2_status m2_jk_df_device(m2_ctx* ctx, const double* dCocc, int nocc, int fp32, double* dJ, double* dK);
+/* timing of the last DF call when profiling is on: ms of X = C^T B, of
+ * K = X^T X, of the J part, and their sum */
+m2_status m2_df_timing(m2_ctx* ctx, double* ms4);
+
+/* Dense FP64 linear algebra on the device for the SCF driver (host buffers,
+ * column-major).  m2_eigh replaces Eigen::SelfAdjointEigenSolver in
+ * Fock::formOrthog / Fock::diagonalise (src/fock.cpp:124-142,228-233):
+ * ascending eigenvalues w[n], eigenvectors V (n x n, columns).  m2_gemm is
+ * C = alpha op(A) op(B) + beta C with op = 'N' or 'T' (the matrix products of
+ * SCF::calcErr and Fock::transform, src/scf.cpp:52-69, src/fock.cpp:156-172). */
+m2_status m2_eigh(m2_ctx* ctx, int n, const double* A, double* w, double* V);
+m2_status m2_gemm(m2_ctx* ctx, char ta, char tb, int m, int n, int k, double alpha, const double* A,
+                  int lda, const double* B, int ldb, double beta, double* C, int ldc);
+
+/* Kernel profiling (no reference counterpart; the reference only times whole
+ * SCF iterations, src/logger.cpp:451-461).  When on, every J/K launch of a
+ * class combination (bra pair class X, ket pair class Y, index 6X+Y with
+ * classes ss, ps, pp, ds, dp, dd) is bracketed by HIP events on the context
+ * stream and its computed quartets / primitive quartets are counted on the
+ * device.  m2_get_profile fills 36-entry arrays accumulated since the last
+ * reset. */
+m2_status m2_set_profiling(m2_ctx* ctx, int on);
+m2_status m2_get_profile(m2_ctx* ctx, double* ms, long long* launches, long long* quartets,
+                         long long* prim_quartets);
+m2_status m2_reset_profile(m2_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

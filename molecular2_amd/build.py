@@ -29,18 +29,20 @@ ARCH = "gfx950"
 # -munsafe-fp-atomics: FP64 atomicAdd lowers to global_atomic_add_f64.
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-I" + CSRC, "-I" + INCLUDE, "-Wno-unused-result"]
+            "-I" + CSRC, "-I" + os.path.join(CSRC, "host"), "-I" + INCLUDE, "-Wno-unused-result"]
 
 
 def sources():
     srcs = sorted(glob.glob(os.path.join(GEN, "*.hip")))
     srcs += sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     srcs += sorted(glob.glob(os.path.join(CSRC, "*.cpp")))
+    srcs += sorted(glob.glob(os.path.join(CSRC, "host", "*.cpp")))
     return srcs
 
 
 def _headers():
     hs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(GEN, "*.inc")) + \
+        glob.glob(os.path.join(CSRC, "host", "*.hpp")) + \
         glob.glob(os.path.join(INCLUDE, "*.h"))
     return max((os.path.getmtime(h) for h in hs), default=0.0)
 
@@ -86,7 +88,8 @@ def build(force: bool = False, jobs: int = 0, verbose: bool = True) -> str:
     objs.sort()
     newest = max(os.path.getmtime(o) for o in objs)
     if force or not os.path.exists(LIBM2) or os.path.getmtime(LIBM2) < newest:
-        cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIBM2 + ".tmp"] + objs
+        cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIBM2 + ".tmp"] + objs + \
+            ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stderr[-6000:])

@@ -1,8 +1,6 @@
 // C ABI of the engine (include/m2.h): context, basis and shell-pair set-up,
 // Schwarz bounds, one-electron integrals and the direct J/K driver.
-#include "m2.h"
-#include "m2_common.hpp"
-#include "boys.hpp"
+#include "ctx.hpp"
 #include "generated/sph_tables.inc"
 #include "generated/jk_dispatch.inc"
 
@@ -16,39 +14,28 @@
 #include <vector>
 
 namespace m2 {
-void launch_schwarz0(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
-void launch_schwarz1(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
-void launch_schwarz2(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
-void launch_schwarz3(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
-void launch_schwarz4(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
-void launch_schwarz5(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
 void launch_one_body(const struct OneBodyArgs&, hipStream_t);
 void launch_blocknorm(const double* A, int n, int nsh, const int* sh_bf, const int* sh_nbf, double* out, hipStream_t);
 void launch_symmetrize(double* J, double* K, double* Jo, double* Ko, int n, hipStream_t);
 void launch_zero(double* p, size_t n, hipStream_t);
+hipError_t heavy_upload(int device);
+void la_release(m2_ctx* ctx);
+bool heavy_has(int la, int lb, int lc, int ld);
+hipError_t launch_jk_heavy(int device, int la, int lb, int lc, int ld, int pure, const JKArgs& a,
+                           const BoysTables& tab, const Task* tasks, int ntasks, int mode, hipStream_t st);
+hipError_t launch_schwarz_heavy(int device, int la, int lb, int pure, const JKArgs& a, const BoysTables& tab,
+                                int p0, int np, double* Q, hipStream_t st);
 }  // namespace m2
 
 using namespace m2;
 
 namespace {
 
-constexpr int NPAIRCLASS = 6;
-const int PAIR_L[NPAIRCLASS][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
-typedef void (*sw_launcher)(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);
-const sw_launcher SW_LAUNCH[NPAIRCLASS] = {launch_schwarz0, launch_schwarz1, launch_schwarz2,
-                                           launch_schwarz3, launch_schwarz4, launch_schwarz5};
-
 int pair_class(int la, int lb) {
     for (int i = 0; i < NPAIRCLASS; ++i)
         if (PAIR_L[i][0] == la && PAIR_L[i][1] == lb) return i;
     return -1;
 }
-
-struct HostShell {
-    int l = 0, atom = -1, bf = 0, nbf = 0;
-    double O[3] = {0, 0, 0};
-    std::vector<double> a, c;
-};
 
 double dfact(int n) {
     double r = 1.0;
@@ -87,89 +74,8 @@ void boys_host(int mmax, double T, double* F) {
     for (int m = mmax - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + et) / (2 * m + 1);
 }
 
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t reserve(size_t cnt) {
-        if (cnt <= n && p) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(cnt, 1) * sizeof(T));
-        if (e == hipSuccess) n = cnt;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-struct PairGroup {
-    int begin, end;   // global pair index range, Q descending inside
-};
-
-struct PairClassSeg {
-    int begin = 0, end = 0;
-    std::vector<PairGroup> groups;
-};
-
-struct BasisData {
-    int nsh = 0, nbf = 0, pure = 1;
-    std::vector<HostShell> sh;
-    std::vector<int> h_bf, h_nbf, h_l;
-    std::vector<double> h_O;
-    DevBuf<double> d_O;
-    DevBuf<int> d_bf, d_nbf;
-};
-
 }  // namespace
 
-struct m2_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    BasisData obs, aux;
-    // shell pairs of the orbital basis
-    std::vector<int2> pair_sh;
-    std::vector<int> pair_off, pair_np;
-    std::vector<double> pair_Q;
-    std::vector<double> Qsh;   // nsh x nsh
-    PairClassSeg cls[NPAIRCLASS];
-    DevBuf<PrimPair> d_prims;
-    DevBuf<int> d_pair_off, d_pair_np;
-    DevBuf<int2> d_pair_sh;
-    DevBuf<double> d_pair_Q;
-    // Boys tables
-    DevBuf<double> d_boys;
-    DevBuf<float> d_boys_f;
-    // J/K work buffers
-    DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
-    DevBuf<Task> d_tasks;
-    std::vector<Task> h_tasks;
-    std::vector<double> h_Dn;
-    double fp32_below = 1e-4;
-    long long last_tasks = 0;
-};
-
-#define HIPCHK(expr)                                                                   \
-    do {                                                                               \
-        hipError_t _e = (expr);                                                        \
-        if (_e != hipSuccess) {                                                        \
-            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);              \
-            return M2_ERR_HIP;                                                         \
-        }                                                                              \
-    } while (0)
-
-#define ARGCHK(cond, msg)             \
-    do {                              \
-        if (!(cond)) {                \
-            ctx->err = (msg);         \
-            return M2_ERR_ARG;        \
-        }                             \
-    } while (0)
 
 namespace {
 
@@ -298,8 +204,13 @@ m2_status build_pairs(m2_ctx* ctx) {
         while (i0 < npair) {
             int c = pairs[i0].cls, i1 = i0;
             while (i1 < npair && pairs[i1].cls == c) ++i1;
-            SW_LAUNCH[c](b.pure, a, tables(ctx), i0, i1 - i0, ctx->d_pair_Q.p, ctx->stream);
-            HIPCHK(hipGetLastError());
+            if (SW_LAUNCH[c]) {
+                SW_LAUNCH[c](b.pure, a, tables(ctx), i0, i1 - i0, ctx->d_pair_Q.p, ctx->stream);
+                HIPCHK(hipGetLastError());
+            } else {
+                HIPCHK(launch_schwarz_heavy(ctx->device, PAIR_L[c][0], PAIR_L[c][1], b.pure, a, tables(ctx), i0,
+                                            i1 - i0, ctx->d_pair_Q.p, ctx->stream));
+            }
             i0 = i1;
         }
         HIPCHK(hipMemcpyAsync(Q.data(), ctx->d_pair_Q.p, sizeof(double) * npair, hipMemcpyDeviceToHost, ctx->stream));
@@ -361,34 +272,51 @@ int count_ge(const std::vector<double>& Q, int b, int e, double thr) {
 }
 
 struct ComboRange {
-    int X, Y, t0, t1;
+    int X, Y, t0, tmid, t1;   // FP64 tasks [t0, tmid), FP32 tasks [tmid, t1)
 };
 
 // Work list: for every bra class X >= ket class Y, every bra pair i and every
 // primitive-count group of Y, the ket pairs j whose Q_j clears
 // precision / (Q_i Dmax) -- a superset of the exactly screened quartets,
 // with j <= i inside one class (unique pairs of pairs, src/fockbuild.cpp:88-109).
-std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax) {
+// Mixed precision: the ket prefix with Q_i Q_j Dmax >= fp32_below stays FP64,
+// the rest (every quartet bound < fp32_below) goes to the FP32 list.
+// Register classes take up to 64 kets per task (one per lane); table-driven
+// heavy classes take HEAVY_CHUNK (one wavefront walks them in turn).
+constexpr int HEAVY_CHUNK = 4;
+
+std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed) {
     std::vector<ComboRange> ranges;
     auto& T = ctx->h_tasks;
     T.clear();
     const auto& Q = ctx->pair_Q;
+    std::vector<Task> t32;
     for (int X = 0; X < NPAIRCLASS; ++X) {
         const PairClassSeg& sx = ctx->cls[X];
         for (int Y = 0; Y <= X; ++Y) {
             const PairClassSeg& sy = ctx->cls[Y];
-            ComboRange r{X, Y, (int)T.size(), 0};
+            const int chunk = JK_COMBO[X][Y] ? 64 : HEAVY_CHUNK;
+            ComboRange r{X, Y, (int)T.size(), 0, 0};
+            t32.clear();
             for (int i = sx.begin; i < sx.end; ++i) {
                 const double qi = Q[i];
                 if (qi <= 0.0 || Dmax <= 0.0) continue;
                 const double thr = precision / (qi * Dmax);
+                const double thr64 = mixed ? std::max(precision, ctx->fp32_below) / (qi * Dmax) : thr;
                 for (const PairGroup& g : sy.groups) {
-                    int cnt = count_ge(Q, g.begin, g.end, thr);
-                    int end = g.begin + cnt;
-                    if (X == Y) end = std::min(end, i + 1);
-                    for (int j = g.begin; j < end; j += 64) T.push_back(Task{i, j, std::min(64, end - j), 0});
+                    int end = g.begin + count_ge(Q, g.begin, g.end, thr);
+                    int mid = g.begin + count_ge(Q, g.begin, g.end, thr64);
+                    if (X == Y) {
+                        end = std::min(end, i + 1);
+                        mid = std::min(mid, i + 1);
+                    }
+                    mid = std::min(mid, end);
+                    for (int j = g.begin; j < mid; j += chunk) T.push_back(Task{i, j, std::min(chunk, mid - j), 0});
+                    for (int j = mid; j < end; j += chunk) t32.push_back(Task{i, j, std::min(chunk, end - j), 0});
                 }
             }
+            r.tmid = (int)T.size();
+            T.insert(T.end(), t32.begin(), t32.end());
             r.t1 = (int)T.size();
             if (r.t1 > r.t0) ranges.push_back(r);
         }
@@ -412,7 +340,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     HIPCHK(hipStreamSynchronize(ctx->stream));
     double Dmax = 0.0;
     for (double x : ctx->h_Dn) Dmax = std::max(Dmax, x);
-    std::vector<ComboRange> ranges = build_tasks(ctx, precision, Dmax);
+    std::vector<ComboRange> ranges = build_tasks(ctx, precision, Dmax, flags == M2_MIXED);
     ctx->last_tasks = (long long)ctx->h_tasks.size();
     if (!ctx->h_tasks.empty()) {
         HIPCHK(ctx->d_tasks.reserve(ctx->h_tasks.size()));
@@ -422,24 +350,60 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     launch_zero(ctx->d_J.p, (size_t)n * n, ctx->stream);
     launch_zero(ctx->d_K.p, (size_t)n * n, ctx->stream);
     JKArgs a = jk_args(ctx);
+    a.counters = nullptr;
+    if (ctx->prof) {
+        HIPCHK(ctx->d_counters.reserve(72));
+        launch_zero((double*)ctx->d_counters.p, 72, ctx->stream);
+        a.counters = ctx->d_counters.p;
+        while (ctx->ev.size() < 2 * ranges.size() + 2) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ctx->ev.push_back(e);
+        }
+    }
     a.D = dD;
     a.J = ctx->d_J.p;
     a.K = ctx->d_K.p;
     a.precision = precision;
     a.fp32_below = ctx->fp32_below;
     a.scale = multiplier;
-    for (const ComboRange& r : ranges) {
-        if (flags == M2_FP64) {
-            JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p + r.t0, r.t1 - r.t0, 0, ctx->stream);
-        } else {
-            JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p + r.t0, r.t1 - r.t0, 1, ctx->stream);
-            JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p + r.t0, r.t1 - r.t0, 2, ctx->stream);
+    for (size_t ri = 0; ri < ranges.size(); ++ri) {
+        const ComboRange& r = ranges[ri];
+        a.combo = 6 * r.X + r.Y;
+        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * ri], ctx->stream));
+        // mode 0: FP64 kernel on [t0, tmid); mode 2: FP32 kernel on [tmid, t1)
+        for (int mode : {0, 2}) {
+            const int b0 = mode == 0 ? r.t0 : r.tmid, b1 = mode == 0 ? r.tmid : r.t1;
+            if (b1 <= b0) continue;
+            const Task* tp = ctx->d_tasks.p + b0;
+            if (JK_COMBO[r.X][r.Y]) {
+                JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), tp, b1 - b0, mode, ctx->stream);
+                HIPCHK(hipGetLastError());
+            } else {
+                HIPCHK(launch_jk_heavy(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1],
+                                       b.pure, a, tables(ctx), tp, b1 - b0, mode, ctx->stream));
+            }
         }
-        HIPCHK(hipGetLastError());
+        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * ri + 1], ctx->stream));
     }
     launch_symmetrize(ctx->d_J.p, ctx->d_K.p, dJ, dK, n, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) {
+        unsigned long long cnt[72];
+        HIPCHK(hipMemcpy(cnt, ctx->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        for (size_t ri = 0; ri < ranges.size(); ++ri) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * ri], ctx->ev[2 * ri + 1]));
+            const int c = 6 * ranges[ri].X + ranges[ri].Y;
+            ctx->prof_ms[c] += ms;
+            ctx->prof_launch[c] += 1;
+        }
+        for (int c = 0; c < 36; ++c) {
+            ctx->prof_q[c] += (long long)cnt[2 * c];
+            ctx->prof_pq[c] += (long long)cnt[2 * c + 1];
+        }
+    }
     return M2_OK;
 }
 
@@ -475,6 +439,11 @@ m2_status m2_ctx_create(m2_ctx** out, int device) {
     if (e != hipSuccess) {
         std::fprintf(stderr, "m2_ctx_create: %s\n", hipGetErrorString(e));
         delete ctx;
+        return M2_ERR_HIP;
+    }
+    if (heavy_upload(device) != hipSuccess) {
+        std::fprintf(stderr, "m2_ctx_create: program table upload failed\n");
+        m2_ctx_destroy(ctx);
         return M2_ERR_HIP;
     }
     // Boys tables, F_m(k * 0.1) for m < BOYS_MTAB
@@ -516,6 +485,13 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_Ko.release();
     ctx->d_Dn.release();
     ctx->d_tasks.release();
+    ctx->d_counters.release();
+    for (auto* x : {&ctx->d_B64, &ctx->d_dfV, &ctx->d_dfC, &ctx->d_dfX, &ctx->d_dfG, &ctx->d_dfJ, &ctx->d_dfK}) x->release();
+    for (auto* x : {&ctx->d_B32, &ctx->d_dfC32, &ctx->d_dfX32, &ctx->d_dfG32, &ctx->d_dfJ32, &ctx->d_dfK32}) x->release();
+    for (auto* x : {&ctx->d_aux_off, &ctx->d_aux_np, &ctx->d_aux_lists}) x->release();
+    ctx->d_aux_prims.release();
+    la_release(ctx);
+    for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -531,6 +507,7 @@ m2_status m2_basis_set(m2_ctx* ctx, int which, int nsh, const int* l, const int*
     ARGCHK(nsh > 0 && l && nprim && exps && coefs && centers, "m2_basis_set: null or empty input");
     HIPCHK(hipSetDevice(ctx->device));
     BasisData& b = (which == M2_ORBITAL) ? ctx->obs : ctx->aux;
+    ctx->df_ready = false;   // the fitted tensor depends on both bases
     b.nsh = nsh;
     b.pure = pure ? 1 : 0;
     b.sh.assign(nsh, HostShell{});
@@ -702,6 +679,32 @@ m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below) {
     if (!ctx) return M2_ERR_ARG;
     ARGCHK(fp32_below >= 0.0, "m2_set_mixed_threshold: threshold must be >= 0");
     ctx->fp32_below = fp32_below;
+    return M2_OK;
+}
+
+m2_status m2_set_profiling(m2_ctx* ctx, int on) {
+    if (!ctx) return M2_ERR_ARG;
+    ctx->prof = on != 0;
+    return M2_OK;
+}
+
+m2_status m2_get_profile(m2_ctx* ctx, double* ms, long long* launches, long long* quartets, long long* primq) {
+    if (!ctx) return M2_ERR_ARG;
+    for (int c = 0; c < 36; ++c) {
+        if (ms) ms[c] = ctx->prof_ms[c];
+        if (launches) launches[c] = ctx->prof_launch[c];
+        if (quartets) quartets[c] = ctx->prof_q[c];
+        if (primq) primq[c] = ctx->prof_pq[c];
+    }
+    return M2_OK;
+}
+
+m2_status m2_reset_profile(m2_ctx* ctx) {
+    if (!ctx) return M2_ERR_ARG;
+    for (int c = 0; c < 36; ++c) {
+        ctx->prof_ms[c] = 0.0;
+        ctx->prof_launch[c] = ctx->prof_q[c] = ctx->prof_pq[c] = 0;
+    }
     return M2_OK;
 }
 

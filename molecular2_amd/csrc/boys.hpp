@@ -50,13 +50,12 @@ struct BoysEval<double, M> {
                 for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
             }
         } else {
-            const double it = 1.0 / T;
-            F[0] = 0.88622692545275801365 * sqrt(it);   // sqrt(pi)/2 / sqrt(T)
+            const double rs = rsqrt(T);
+            F[0] = 0.88622692545275801365 * rs;   // sqrt(pi)/2 / sqrt(T)
             if constexpr (M > 0) {
-                const double e = exp(-T);
-                const double h = 0.5 * it;
+                const double h = 0.5 * rs * rs;
 #pragma unroll
-                for (int m = 0; m < M; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+                for (int m = 0; m < M; ++m) F[m + 1] = (2 * m + 1) * F[m] * h;
             }
         }
     }

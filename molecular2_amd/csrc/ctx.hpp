@@ -1,0 +1,130 @@
+// Private: the context object behind the C ABI (shared by the engine's
+// translation units).
+#pragma once
+#include "m2.h"
+#include "m2_common.hpp"
+#include "boys.hpp"
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+struct rocblas_handle_opaque;   // rocBLAS handle (typedef'd pointer in rocblas headers)
+
+namespace m2 {
+
+constexpr int NPAIRCLASS = 6;
+const int PAIR_L[NPAIRCLASS][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+struct HostShell {
+    int l = 0, atom = -1, bf = 0, nbf = 0;
+    double O[3] = {0, 0, 0};
+    std::vector<double> a, c;
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(cnt, 1) * sizeof(T));
+        if (e == hipSuccess) n = cnt;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct PairGroup {
+    int begin, end;   // global pair index range, Q descending inside
+};
+
+struct PairClassSeg {
+    int begin = 0, end = 0;
+    std::vector<PairGroup> groups;
+};
+
+struct BasisData {
+    int nsh = 0, nbf = 0, pure = 1;
+    std::vector<HostShell> sh;
+    std::vector<int> h_bf, h_nbf, h_l;
+    std::vector<double> h_O;
+    m2::DevBuf<double> d_O;
+    m2::DevBuf<int> d_bf, d_nbf;
+};
+
+
+}  // namespace m2
+
+struct m2_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    m2::BasisData obs, aux;
+    // shell pairs of the orbital basis
+    std::vector<int2> pair_sh;
+    std::vector<int> pair_off, pair_np;
+    std::vector<double> pair_Q;
+    std::vector<double> Qsh;   // nsh x nsh
+    m2::PairClassSeg cls[m2::NPAIRCLASS];
+    m2::DevBuf<m2::PrimPair> d_prims;
+    m2::DevBuf<int> d_pair_off, d_pair_np;
+    m2::DevBuf<int2> d_pair_sh;
+    m2::DevBuf<double> d_pair_Q;
+    // Boys tables
+    m2::DevBuf<double> d_boys;
+    m2::DevBuf<float> d_boys_f;
+    // J/K work buffers
+    m2::DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
+    m2::DevBuf<m2::Task> d_tasks;
+    std::vector<m2::Task> h_tasks;
+    std::vector<double> h_Dn;
+    double fp32_below = 1e-4;
+    long long last_tasks = 0;
+    // profiling (m2_set_profiling): per class combo kernel time and counters
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    double prof_ms[36] = {0};
+    long long prof_launch[36] = {0};
+    long long prof_q[36] = {0}, prof_pq[36] = {0};
+    m2::DevBuf<unsigned long long> d_counters;
+    // density fitting
+    std::vector<m2::PrimPair> h_aux_prims;
+    m2::DevBuf<m2::PrimPair> d_aux_prims;
+    m2::DevBuf<int> d_aux_off, d_aux_np, d_aux_lists;
+    m2::DevBuf<double> d_B64, d_dfV, d_dfC, d_dfX, d_dfG, d_dfJ, d_dfK;
+    m2::DevBuf<float> d_B32, d_dfC32, d_dfX32, d_dfG32, d_dfJ32, d_dfK32;
+    bool df_ready = false;
+    double df_last_ms[4] = {0, 0, 0, 0};   // X = C^T B, K = X^T X, J, total (profiling)
+    // dense linear algebra (rocBLAS / rocSOLVER)
+    void* blas = nullptr;
+    m2::DevBuf<double> la_a, la_b, la_c, la_w;
+    m2::DevBuf<int> la_info;
+};
+
+namespace m2 {
+m2_status la_handle(m2_ctx* ctx, void** h);
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);              \
+            return M2_ERR_HIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+#define ARGCHK(cond, msg)             \
+    do {                              \
+        if (!(cond)) {                \
+            ctx->err = (msg);         \
+            return M2_ERR_ARG;        \
+        }                             \
+    } while (0)

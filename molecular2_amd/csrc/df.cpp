@@ -1,0 +1,380 @@
+// Density-fitted J/K (the reference's Fock::formJKdf / compute_2body_fock_df,
+// src/fockbuild.cpp:171-174,585-660).
+//
+// Set-up (once per geometry, FP64 on the device):
+//   M[K][x][y] = (K|xy)       3-centre integrals, generated class kernels
+//   V = (K|L), V = L L^T      2-centre metric, rocSOLVER dpotrf
+//   B = M L^-T                rocBLAS dtrsm (row (x,y), column K)
+//   stored as Bx[x][K][y] in FP64 and FP32 (the fitted tensor; n*n*naux).
+// Per call, with C = C_occ (n x nocc, column-major) and Bx read as the
+// column-major n x (n*naux) matrix B(y, (x,K)):
+//   X  = C^T B                  GEMM  nocc x (n*naux), inner n         (xiK)
+//   K  = Z^T Z, Z = X viewed as (naux*nocc) x n                        (k)
+//   g_K = sum_{x,i} X(i,(x,K)) C(x,i)                                  (Jtmp)
+//   J(:,x) = B(:, (x,.)) g      strided-batched GEMV over x            (j)
+// The GEMMs are FP32 with FP32 accumulation on the MFMA units (rocBLAS
+// sgemm) in the default mode, or FP64 (dgemm) when asked for FP64.
+#include "ctx.hpp"
+#include "df_kernel.hpp"
+
+#include <rocblas/rocblas.h>
+
+#include <chrono>
+#include <cmath>
+
+namespace m2 {
+void launch_df3_0(int, int, const DF3Args&, const BoysTables&, const Task*, int, hipStream_t);
+void launch_df3_1(int, int, const DF3Args&, const BoysTables&, const Task*, int, hipStream_t);
+void launch_df3_2(int, int, const DF3Args&, const BoysTables&, const Task*, int, hipStream_t);
+void launch_df3_3(int, int, const DF3Args&, const BoysTables&, const Task*, int, hipStream_t);
+void launch_df2(int lp, int lq, int pure, const DF2Args& a, const BoysTables& tab, hipStream_t st);
+m2_status potrf_device(m2_ctx* ctx, int n, double* dA, int* dinfo);
+
+typedef void (*df3_launcher)(int, int, const DF3Args&, const BoysTables&, const Task*, int, hipStream_t);
+static const df3_launcher DF3[4] = {launch_df3_0, launch_df3_1, launch_df3_2, launch_df3_3};
+
+// [K][x][y] -> [x][K][y] in FP64 and FP32
+__global__ void k_df_reorder(const double* __restrict__ M, double* __restrict__ B64, float* __restrict__ B32, int n,
+                             int naux) {
+    const size_t tot = (size_t)n * n * naux;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t y = t % n;
+        const size_t K = (t / n) % naux;
+        const size_t x = t / ((size_t)n * naux);
+        const double v = M[(K * n + x) * n + y];
+        B64[t] = v;
+        B32[t] = (float)v;
+    }
+}
+
+// g[K] = sum_{x,i} X[x][K][i] C[x + i*n]   (X as written by the X = C^T B GEMM)
+template <typename R>
+__global__ __launch_bounds__(256) void k_df_gamma(const R* __restrict__ X, const R* __restrict__ C, R* __restrict__ g,
+                                                  int n, int naux, int nocc) {
+    const int K = blockIdx.x;
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int t = threadIdx.x; t < n * nocc; t += blockDim.x) {
+        const int x = t / nocc, i = t % nocc;
+        s += double(X[((size_t)x * naux + K) * nocc + i]) * double(C[(size_t)i * n + x]);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) g[K] = R(red[0]);
+}
+
+template <typename R>
+__global__ void k_to_double(const R* __restrict__ a, double* __restrict__ b, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        b[t] = double(a[t]);
+}
+
+__global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        b[t] = float(a[t]);
+}
+
+}  // namespace m2
+
+using namespace m2;
+
+namespace {
+
+BoysTables df_tables(m2_ctx* ctx) { return BoysTables{ctx->d_boys.p, ctx->d_boys_f.p}; }
+
+m2_status df_setup(m2_ctx* ctx) {
+    BasisData& ob = ctx->obs;
+    BasisData& ab = ctx->aux;
+    ARGCHK(ob.nbf > 0, "m2_df_setup: no orbital basis set");
+    ARGCHK(ab.nbf > 0, "m2_df_setup: no auxiliary basis set");
+    for (const auto& s : ab.sh) ARGCHK(s.l <= 3, "m2_df_setup: auxiliary shells must have l <= 3");
+    const int n = ob.nbf, naux = ab.nbf;
+    // aux "pairs": each aux primitive with the unit s function
+    const double sqrt2pi52 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+    std::vector<PrimPair> pp;
+    std::vector<int> off(ab.nsh), np(ab.nsh);
+    for (int s = 0; s < ab.nsh; ++s) {
+        off[s] = (int)pp.size();
+        np[s] = (int)ab.sh[s].a.size();
+        for (size_t k = 0; k < ab.sh[s].a.size(); ++k) {
+            PrimPair q{};
+            const double a = ab.sh[s].a[k];
+            q.p = a;
+            q.inv2p = 0.5 / a;
+            q.K = sqrt2pi52 * ab.sh[s].c[k] / a;
+            for (int d = 0; d < 3; ++d) {
+                q.P[d] = ab.sh[s].O[d];
+                q.PA[d] = 0.0;
+            }
+            pp.push_back(q);
+        }
+    }
+    std::vector<int> lists;
+    int lbeg[5] = {0};
+    for (int l = 0; l <= 3; ++l) {
+        lbeg[l] = (int)lists.size();
+        for (int s = 0; s < ab.nsh; ++s)
+            if (ab.sh[s].l == l) lists.push_back(s);
+    }
+    lbeg[4] = (int)lists.size();
+    HIPCHK(ctx->d_aux_prims.reserve(pp.size()));
+    HIPCHK(ctx->d_aux_off.reserve(ab.nsh));
+    HIPCHK(ctx->d_aux_np.reserve(ab.nsh));
+    HIPCHK(ctx->d_aux_lists.reserve(lists.size()));
+    HIPCHK(hipMemcpy(ctx->d_aux_prims.p, pp.data(), sizeof(PrimPair) * pp.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_aux_off.p, off.data(), sizeof(int) * ab.nsh, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_aux_np.p, np.data(), sizeof(int) * ab.nsh, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_aux_lists.p, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice));
+
+    // 3-centre integrals into M[K][x][y]
+    const size_t nB = (size_t)n * n * naux;
+    DevBuf<double> M;
+    HIPCHK(M.reserve(nB));
+    std::vector<Task> tasks;
+    struct R3 {
+        int lp, Y, t0, t1;
+    };
+    std::vector<R3> rr;
+    for (int l = 0; l <= 3; ++l)
+        for (int Y = 0; Y < NPAIRCLASS; ++Y) {
+            const PairClassSeg& seg = ctx->cls[Y];
+            if (seg.end <= seg.begin) continue;
+            R3 r{l, Y, (int)tasks.size(), 0};
+            for (int k = lbeg[l]; k < lbeg[l + 1]; ++k)
+                for (int j = seg.begin; j < seg.end; j += 64)
+                    tasks.push_back(Task{lists[k], j, std::min(64, seg.end - j), 0});
+            r.t1 = (int)tasks.size();
+            if (r.t1 > r.t0) rr.push_back(r);
+        }
+    DevBuf<Task> dt;
+    HIPCHK(dt.reserve(tasks.size()));
+    HIPCHK(hipMemcpy(dt.p, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice));
+    DF3Args a3{};
+    a3.aux_prims = ctx->d_aux_prims.p;
+    a3.aux_off = ctx->d_aux_off.p;
+    a3.aux_np = ctx->d_aux_np.p;
+    a3.aux_bf = ab.d_bf.p;
+    a3.prims = ctx->d_prims.p;
+    a3.pair_off = ctx->d_pair_off.p;
+    a3.pair_np = ctx->d_pair_np.p;
+    a3.pair_sh = ctx->d_pair_sh.p;
+    a3.sh_O = ob.d_O.p;
+    a3.sh_bf = ob.d_bf.p;
+    a3.M = M.p;
+    a3.n = n;
+    a3.naux = naux;
+    for (const R3& r : rr) {
+        DF3[r.lp](r.Y, ab.pure, a3, df_tables(ctx), dt.p + r.t0, r.t1 - r.t0, ctx->stream);
+        HIPCHK(hipGetLastError());
+    }
+    // 2-centre metric
+    HIPCHK(ctx->d_dfV.reserve((size_t)naux * naux));
+    for (int lp = 0; lp <= 3; ++lp)
+        for (int lq = 0; lq <= lp; ++lq) {
+            DF2Args a2{};
+            a2.aux_prims = ctx->d_aux_prims.p;
+            a2.aux_off = ctx->d_aux_off.p;
+            a2.aux_np = ctx->d_aux_np.p;
+            a2.aux_bf = ab.d_bf.p;
+            a2.P_list = ctx->d_aux_lists.p + lbeg[lp];
+            a2.nP = lbeg[lp + 1] - lbeg[lp];
+            a2.Q_list = ctx->d_aux_lists.p + lbeg[lq];
+            a2.nQ = lbeg[lq + 1] - lbeg[lq];
+            a2.V = ctx->d_dfV.p;
+            a2.naux = naux;
+            launch_df2(lp, lq, ab.pure, a2, df_tables(ctx), ctx->stream);
+            HIPCHK(hipGetLastError());
+        }
+    // Cholesky and fitting
+    DevBuf<int> info;
+    HIPCHK(info.reserve(1));
+    m2_status st = potrf_device(ctx, naux, ctx->d_dfV.p, info.p);
+    if (st) return st;
+    int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (hinfo != 0) {
+        ctx->err = "m2_df_setup: the Coulomb metric is not positive definite (dpotrf info " + std::to_string(hinfo) + ")";
+        return M2_ERR_NUMERIC;
+    }
+    void* hv = nullptr;
+    st = la_handle(ctx, &hv);
+    if (st) return st;
+    rocblas_handle h = (rocblas_handle)hv;
+    const double one = 1.0;
+    // B(xy, K): X L^T = M  (right side, lower L, transposed)
+    if (rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose, rocblas_diagonal_non_unit,
+                      n * n, naux, &one, ctx->d_dfV.p, naux, M.p, n * n) != rocblas_status_success) {
+        ctx->err = "m2_df_setup: rocblas_dtrsm failed";
+        return M2_ERR_HIP;
+    }
+    HIPCHK(ctx->d_B64.reserve(nB));
+    HIPCHK(ctx->d_B32.reserve(nB));
+    hipLaunchKernelGGL(k_df_reorder, dim3(8192), dim3(256), 0, ctx->stream, M.p, ctx->d_B64.p, ctx->d_B32.p, n, naux);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    M.release();
+    dt.release();
+    info.release();
+    ctx->df_ready = true;
+    return M2_OK;
+}
+
+template <typename R>
+struct Blas;
+template <>
+struct Blas<float> {
+    static rocblas_status gemm(rocblas_handle h, rocblas_operation a, rocblas_operation b, int m, int n, int k,
+                               const float* al, const float* A, int lda, const float* B, int ldb, const float* be,
+                               float* C, int ldc) {
+        return rocblas_sgemm(h, a, b, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+    }
+    static rocblas_status gemv_sb(rocblas_handle h, int m, int n, const float* al, const float* A, int lda,
+                                  rocblas_stride sA, const float* x, rocblas_stride sx, const float* be, float* y,
+                                  rocblas_stride sy, int batch) {
+        return rocblas_sgemv_strided_batched(h, rocblas_operation_none, m, n, al, A, lda, sA, x, 1, sx, be, y, 1, sy,
+                                             batch);
+    }
+};
+template <>
+struct Blas<double> {
+    static rocblas_status gemm(rocblas_handle h, rocblas_operation a, rocblas_operation b, int m, int n, int k,
+                               const double* al, const double* A, int lda, const double* B, int ldb, const double* be,
+                               double* C, int ldc) {
+        return rocblas_dgemm(h, a, b, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+    }
+    static rocblas_status gemv_sb(rocblas_handle h, int m, int n, const double* al, const double* A, int lda,
+                                  rocblas_stride sA, const double* x, rocblas_stride sx, const double* be, double* y,
+                                  rocblas_stride sy, int batch) {
+        return rocblas_dgemv_strided_batched(h, rocblas_operation_none, m, n, al, A, lda, sA, x, 1, sx, be, y, 1, sy,
+                                             batch);
+    }
+};
+
+template <typename R>
+m2_status jk_df_core(m2_ctx* ctx, const R* B, const R* C, int nocc, R* X, R* g, R* J, R* K) {
+    const int n = ctx->obs.nbf, naux = ctx->aux.nbf;
+    void* hv = nullptr;
+    m2_status st = la_handle(ctx, &hv);
+    if (st) return st;
+    rocblas_handle h = (rocblas_handle)hv;
+    const R one = R(1), zero = R(0);
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (ctx->prof)
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    if (ctx->prof) HIPCHK(hipEventRecord(ev[0], ctx->stream));
+    // X = C^T B : (nocc x n) (n x n*naux)
+    if (Blas<R>::gemm(h, rocblas_operation_transpose, rocblas_operation_none, nocc, n * naux, n, &one, C, n, B, n, &zero,
+                      X, nocc) != rocblas_status_success) {
+        ctx->err = "m2_jk_df: GEMM X = C^T B failed";
+        return M2_ERR_HIP;
+    }
+    if (ctx->prof) HIPCHK(hipEventRecord(ev[1], ctx->stream));
+    // K = Z^T Z with Z = X as (naux*nocc) x n
+    const int ld = naux * nocc;
+    if (Blas<R>::gemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, ld, &one, X, ld, X, ld, &zero, K, n) !=
+        rocblas_status_success) {
+        ctx->err = "m2_jk_df: GEMM K = X^T X failed";
+        return M2_ERR_HIP;
+    }
+    if (ctx->prof) HIPCHK(hipEventRecord(ev[2], ctx->stream));
+    hipLaunchKernelGGL(k_df_gamma<R>, dim3(naux), dim3(256), 0, ctx->stream, X, C, g, n, naux, nocc);
+    HIPCHK(hipGetLastError());
+    // J(:, x) = B(:, x*naux .. x*naux+naux) g
+    if (Blas<R>::gemv_sb(h, n, naux, &one, B, n, (rocblas_stride)n * naux, g, 0, &zero, J, n, n) !=
+        rocblas_status_success) {
+        ctx->err = "m2_jk_df: strided GEMV for J failed";
+        return M2_ERR_HIP;
+    }
+    if (ctx->prof) {
+        HIPCHK(hipEventRecord(ev[3], ctx->stream));
+        HIPCHK(hipEventSynchronize(ev[3]));
+        float ms[3];
+        for (int i = 0; i < 3; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        ctx->df_last_ms[0] = ms[0];
+        ctx->df_last_ms[1] = ms[1];
+        ctx->df_last_ms[2] = ms[2];
+        ctx->df_last_ms[3] = ms[0] + ms[1] + ms[2];
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    return M2_OK;
+}
+
+m2_status jk_df_device(m2_ctx* ctx, const double* dC, int nocc, int fp32, double* dJ, double* dK) {
+    const int n = ctx->obs.nbf, naux = ctx->aux.nbf;
+    if (!ctx->df_ready) {
+        m2_status st = df_setup(ctx);
+        if (st) return st;
+    }
+    const size_t nx = (size_t)nocc * n * naux;
+    if (fp32) {
+        HIPCHK(ctx->d_dfC32.reserve((size_t)n * nocc));
+        HIPCHK(ctx->d_dfX32.reserve(nx));
+        HIPCHK(ctx->d_dfG32.reserve(naux));
+        HIPCHK(ctx->d_dfJ32.reserve((size_t)n * n));
+        HIPCHK(ctx->d_dfK32.reserve((size_t)n * n));
+        hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, ctx->stream, dC, ctx->d_dfC32.p, (size_t)n * nocc);
+        m2_status st = jk_df_core<float>(ctx, ctx->d_B32.p, ctx->d_dfC32.p, nocc, ctx->d_dfX32.p, ctx->d_dfG32.p,
+                                         ctx->d_dfJ32.p, ctx->d_dfK32.p);
+        if (st) return st;
+        hipLaunchKernelGGL(k_to_double<float>, dim3(256), dim3(256), 0, ctx->stream, ctx->d_dfJ32.p, dJ, (size_t)n * n);
+        hipLaunchKernelGGL(k_to_double<float>, dim3(256), dim3(256), 0, ctx->stream, ctx->d_dfK32.p, dK, (size_t)n * n);
+    } else {
+        HIPCHK(ctx->d_dfX.reserve(nx));
+        HIPCHK(ctx->d_dfG.reserve(naux));
+        m2_status st = jk_df_core<double>(ctx, ctx->d_B64.p, dC, nocc, ctx->d_dfX.p, ctx->d_dfG.p, dJ, dK);
+        if (st) return st;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return M2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+m2_status m2_df_setup(m2_ctx* ctx) {
+    if (!ctx) return M2_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    return df_setup(ctx);
+}
+
+m2_status m2_jk_df(m2_ctx* ctx, const double* Cocc, int nocc, int fp32, double* J, double* K) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(Cocc && J && K && nocc > 0, "m2_jk_df: bad arguments");
+    ARGCHK(ctx->obs.nbf > 0 && ctx->aux.nbf > 0, "m2_jk_df: orbital and auxiliary bases must be set");
+    ARGCHK(nocc <= ctx->obs.nbf, "m2_jk_df: nocc exceeds the number of basis functions");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = ctx->obs.nbf;
+    HIPCHK(ctx->d_dfC.reserve((size_t)n * nocc));
+    HIPCHK(ctx->d_dfJ.reserve((size_t)n * n));
+    HIPCHK(ctx->d_dfK.reserve((size_t)n * n));
+    HIPCHK(hipMemcpyAsync(ctx->d_dfC.p, Cocc, sizeof(double) * n * nocc, hipMemcpyHostToDevice, ctx->stream));
+    m2_status st = jk_df_device(ctx, ctx->d_dfC.p, nocc, fp32, ctx->d_dfJ.p, ctx->d_dfK.p);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(J, ctx->d_dfJ.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(K, ctx->d_dfK.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return M2_OK;
+}
+
+m2_status m2_jk_df_device(m2_ctx* ctx, const double* dCocc, int nocc, int fp32, double* dJ, double* dK) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(dCocc && dJ && dK && nocc > 0, "m2_jk_df_device: bad arguments");
+    ARGCHK(ctx->obs.nbf > 0 && ctx->aux.nbf > 0, "m2_jk_df_device: orbital and auxiliary bases must be set");
+    HIPCHK(hipSetDevice(ctx->device));
+    return jk_df_device(ctx, dCocc, nocc, fp32, dJ, dK);
+}
+
+m2_status m2_df_timing(m2_ctx* ctx, double* ms4) {
+    if (!ctx || !ms4) return M2_ERR_ARG;
+    for (int i = 0; i < 4; ++i) ms4[i] = ctx->df_last_ms[i];
+    return M2_OK;
+}
+
+}  // extern "C"

@@ -253,31 +253,35 @@ def gen_class(la, lb, lc, ld, deriv=False):
     for e in E:
         for f in F:
             em.out("    R a%s_%s = R(0);" % (cname(e), cname(f)))
-    em.out("    for (int ib = 0; ib < nbra; ++ib) {")
-    em.out("      const PrimPair bp = bra[ib];")
-    em.out("      const R p = R(bp.p), inv2p = R(bp.inv2p), Kab = R(bp.K);")
-    em.out("      const R Px = R(bp.P[0]), Py = R(bp.P[1]), Pz = R(bp.P[2]);")
+    # ket primitive pairs (different per lane) outside, bra primitive pairs
+    # (the same for the whole wavefront: scalar loads) inside
+    em.out("    for (int ik = 0; ik < nket; ++ik) {")
+    em.out("      const PrimPair kp = ket[ik];")
+    em.out("      const R q = R(kp.p), Kcd = R(kp.K);")
+    em.out("      const R Qx = R(kp.P[0]), Qy = R(kp.P[1]), Qz = R(kp.P[2]);")
+    if lc + ld > 0:
+        em.out("      const R inv2q = R(kp.inv2p);")
+        em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
+    em.out("      for (int ib = 0; ib < nbra; ++ib) {")
+    em.out("        const PrimPair bp = bra[ib];")
+    em.out("        const R p = R(bp.p), Kab = R(bp.K);")
     if la + lb > 0:
-        em.out("      const R PAx = R(bp.PA[0]), PAy = R(bp.PA[1]), PAz = R(bp.PA[2]);")
-    em.out("      for (int ik = 0; ik < nket; ++ik) {")
-    em.out("        const PrimPair kp = ket[ik];")
-    em.out("        const R q = R(kp.p), Kcd = R(kp.K);")
-    em.out("        const R dx = Px - R(kp.P[0]), dy = Py - R(kp.P[1]), dz = Pz - R(kp.P[2]);")
-    em.out("        const R inv_pq = R(1) / (p + q);")
+        em.out("        const R inv2p = R(bp.inv2p);")
+        em.out("        const R PAx = R(bp.PA[0]), PAy = R(bp.PA[1]), PAz = R(bp.PA[2]);")
+    em.out("        const R dx = R(bp.P[0]) - Qx, dy = R(bp.P[1]) - Qy, dz = R(bp.P[2]) - Qz;")
+    em.out("        const R rs_pq = sqrt_inv<R>(p + q);")
+    em.out("        const R inv_pq = rs_pq * rs_pq;")
     em.out("        const R rho = p * q * inv_pq;")
     em.out("        const R T = rho * (dx * dx + dy * dy + dz * dz);")
     em.out("        R Fm[%d];" % (L + 1))
     em.out("        BoysEval<R, %d>::run(T, Fm, tab);" % L)
-    em.out("        const R pref = Kab * Kcd * sqrt_inv<R>(p + q);")
+    em.out("        const R pref = Kab * Kcd * rs_pq;")
     em.out("#pragma unroll")
     em.out("        for (int m = 0; m <= %d; ++m) Fm[m] *= pref;" % L)
     if L > 0:
         em.out("        const R rho_p = q * inv_pq, rho_q = p * inv_pq, inv2pq = R(0.5) * inv_pq;")
         em.out("        const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
         em.out("        const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
-        if lc + ld > 0:
-            em.out("        const R inv2q = R(kp.inv2p);")
-            em.out("        const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
     emit_vrr(em, order, nodes, "        ")
     for e in E:
         for f in F:
@@ -412,11 +416,123 @@ def main(outdir):
                 fh.write(text)
         written.append(path)
     written += write_tus(outdir)
+    written += write_df_tus(outdir)
     written.append(write_sph_tables(outdir))
+    written.append(write_programs(outdir))
+    written.append(write_flop_model(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                 "flop_model.json")))
     return written
 
 
 PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+
+
+def _ops(expr):
+    """arithmetic operations of one emitted right-hand side (binary + - *)."""
+    n = 0
+    depth = 0
+    prev = ""
+    for ch in expr:
+        if ch in "*+" or (ch == "-" and prev not in "(e"):
+            n += 1
+        if not ch.isspace():
+            prev = ch
+    return n
+
+
+def flop_model(la, lb, lc, ld):
+    """Algorithmic operation counts of the textbook OS-VRR + HGP-HRR scheme
+    for one class, read off the generated recurrence DAG:
+      f_prim: per primitive quartet (setup, Boys, VRR, accumulation),
+      f_cont: per contracted quartet (HRR, solid-harmonic transform),
+      n_eri:  spherical integrals per quartet."""
+    L = la + lb + lc + ld
+    text = gen_class(la, lb, lc, ld)
+    lines = text.splitlines()
+    i0 = next(i for i, t in enumerate(lines) if "for (int ib = 0" in t)
+    i1 = next(i for i in range(i0, len(lines)) if lines[i].strip() == "}")
+    prim = 0
+    for t in lines[i0 + 1:i1]:
+        t = t.strip()
+        if t.startswith("const R v") or t.startswith("a") and "+=" in t:
+            prim += _ops(t.split("=", 1)[1]) + (1 if "+=" in t else 0)
+    # setup (distance, 1/(p+q), rho, T, prefactor, W-P, W-Q, scaling of F_m)
+    # and Boys (8-term Horner + downward recursion)
+    prim += 30 + (L + 1) + 21 + 3 * L
+    cont_pure = cont_cart = 0
+    in_pure = False
+    for t in lines[i1:]:
+        st = t.strip()
+        if st.startswith("} else {"):
+            in_pure = True
+        if st.startswith("const R k") or st.startswith("const R h"):
+            c = _ops(st.split("=", 1)[1])
+            cont_pure += c
+            cont_cart += c
+        elif st.startswith("const R s") and in_pure:
+            cont_pure += _ops(st.split("=", 1)[1])
+    n_pure = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1)
+    n_cart = ((la + 1) * (la + 2) * (lb + 1) * (lb + 2) * (lc + 1) * (lc + 2) * (ld + 1) * (ld + 2)) // 16
+    return {"f_prim": prim, "f_cont_pure": cont_pure, "f_cont_cart": cont_cart,
+            "n_eri_pure": n_pure, "n_eri_cart": n_cart}
+
+
+def write_flop_model(path):
+    import json
+    model = {}
+    for X, (la, lb) in enumerate(PAIR_CLASSES):
+        for Y in range(X + 1):
+            lc, ld = PAIR_CLASSES[Y]
+            model["%d,%d" % (X, Y)] = dict(flop_model(la, lb, lc, ld), cls="(%d%d|%d%d)" % (la, lb, lc, ld))
+    text = json.dumps({"digest_flops_per_eri": 12, "classes": model}, indent=1, sort_keys=True) + "\n"
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as fh:
+            fh.write(text)
+    return path
+
+
+def write_df_tus(outdir):
+    """3-centre (P s|c d) launchers per aux l (one TU per aux l) and 2-centre
+    (P s|Q s) launchers (one TU)."""
+    paths = []
+
+    def put(name, lines):
+        text = "\n".join(lines) + "\n"
+        path = os.path.join(outdir, name)
+        if not os.path.exists(path) or open(path).read() != text:
+            with open(path, "w") as fh:
+                fh.write(text)
+        paths.append(path)
+
+    head = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.",
+            '#include "../df_kernel.hpp"', "namespace m2 {"]
+    for lp in range(4):
+        lines = list(head)
+        for Y, (lc, ld) in enumerate(PAIR_CLASSES):
+            lines.append('#include "eri_%d0%d%d.inc"' % (lp, lc, ld))
+        lines.append("void launch_df3_%d(int Y, int pure, const DF3Args& a, const BoysTables& tab, const Task* t, int nt, hipStream_t st) {" % lp)
+        lines.append("  switch (Y) {")
+        for Y, (lc, ld) in enumerate(PAIR_CLASSES):
+            lines.append("    case %d: if (pure) launch_df3<%d, %d, %d, true>(a, tab, t, nt, st); else launch_df3<%d, %d, %d, false>(a, tab, t, nt, st); break;"
+                         % (Y, lp, lc, ld, lp, lc, ld))
+        lines.append("    default: break;")
+        lines.append("  }")
+        lines.append("}")
+        lines.append("}  // namespace m2")
+        put("df3_%d.hip" % lp, lines)
+    lines = list(head)
+    for lp in range(4):
+        for lq in range(lp + 1):
+            lines.append('#include "eri_%d0%d0.inc"' % (lp, lq))
+    lines.append("void launch_df2(int lp, int lq, int pure, const DF2Args& a, const BoysTables& tab, hipStream_t st) {")
+    for lp in range(4):
+        for lq in range(lp + 1):
+            lines.append("  if (lp == %d && lq == %d) { if (pure) launch_df2_t<%d, %d, true>(a, tab, st); else launch_df2_t<%d, %d, false>(a, tab, st); return; }"
+                         % (lp, lq, lp, lq, lp, lq))
+    lines.append("}")
+    lines.append("}  // namespace m2")
+    put("df2.hip", lines)
+    return paths
 
 
 def write_sph_tables(outdir, lmax=4):
@@ -461,6 +577,8 @@ def write_tus(outdir):
     for X, (la, lb) in enumerate(PAIR_CLASSES):
         for Y in range(X + 1):
             lc, ld = PAIR_CLASSES[Y]
+            if (la, lb, lc, ld) in HEAVY:
+                continue
             lines = list(head)
             lines.append('#include "eri_%d%d%d%d.inc"' % (la, lb, lc, ld))
             lines.append("void launch_jk_%d_%d(int pure, const JKArgs& a, const BoysTables& tab, const Task* t, int nt, int mode, hipStream_t st) {" % (X, Y))
@@ -476,18 +594,271 @@ def write_tus(outdir):
     # dispatch table
     lines = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.",
              '#include "../m2_common.hpp"', '#include "../boys.hpp"', "namespace m2 {"]
+    def heavy(X, Y):
+        return PAIR_CLASSES[X] + PAIR_CLASSES[Y] in HEAVY
+
     for X in range(len(PAIR_CLASSES)):
         for Y in range(X + 1):
-            lines.append("void launch_jk_%d_%d(int, const JKArgs&, const BoysTables&, const Task*, int, int, hipStream_t);" % (X, Y))
+            if not heavy(X, Y):
+                lines.append("void launch_jk_%d_%d(int, const JKArgs&, const BoysTables&, const Task*, int, int, hipStream_t);" % (X, Y))
     lines.append("typedef void (*jk_combo_launcher)(int, const JKArgs&, const BoysTables&, const Task*, int, int, hipStream_t);")
     lines.append("const jk_combo_launcher JK_COMBO[%d][%d] = {" % (len(PAIR_CLASSES), len(PAIR_CLASSES)))
     for X in range(len(PAIR_CLASSES)):
-        row = ["launch_jk_%d_%d" % (X, Y) if Y <= X else "nullptr" for Y in range(len(PAIR_CLASSES))]
+        row = ["launch_jk_%d_%d" % (X, Y) if Y <= X and not heavy(X, Y) else "nullptr" for Y in range(len(PAIR_CLASSES))]
         lines.append("  {" + ", ".join(row) + "},")
     lines.append("};")
+    lines.append("typedef void (*sw_launcher)(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);")
+    for X in range(len(PAIR_CLASSES)):
+        if not heavy(X, X):
+            lines.append("void launch_schwarz%d(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);" % X)
+    lines.append("const sw_launcher SW_LAUNCH[%d] = {%s};" % (len(PAIR_CLASSES), ", ".join(
+        "nullptr" if heavy(X, X) else "launch_schwarz%d" % X for X in range(len(PAIR_CLASSES)))))
     lines.append("}  // namespace m2")
     put("jk_dispatch.inc", lines)
     return paths
+
+
+
+
+# ---------------------------------------------------------------------------
+# Table-driven programs for heavy classes (one wavefront per quartet, LDS)
+# ---------------------------------------------------------------------------
+# Scalar ids available to program terms (filled per primitive quartet in LDS)
+SID = {"one": 0, "PAx": 1, "PAy": 2, "PAz": 3, "WPx": 4, "WPy": 5, "WPz": 6,
+       "QCx": 7, "QCy": 8, "QCz": 9, "WQx": 10, "WQy": 11, "WQz": 12,
+       "inv2p": 13, "inv2q": 14, "inv2pq": 15, "inv2p_rho_p": 16, "inv2q_rho_q": 17,
+       "ABx": 18, "ABy": 19, "ABz": 20, "CDx": 21, "CDy": 22, "CDz": 23}
+NSCALAR = 24
+MAXTERMS = 6
+
+
+def gen_program(la, lb, lc, ld, pure):
+    """Recurrence DAG of one class as level-ordered instructions
+    dst = sum_t coef_t * S[sidx_t] * X[src_t].  Returns a dict:
+      nslots, base (slots base..base+L hold [00|00]^(m)), acc (list of
+      (acc_slot, vrr_slot)), vrr_levels / post_levels (lists of instruction
+      lists), out (output slots in (a,b,c,d) order), sizes."""
+    L = la + lb + lc + ld
+    E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
+    F = [c for Lf in range(lc, lc + ld + 1) for c in comps(Lf)]
+    order, nodes = build_vrr([(e, f) for e in E for f in F], L)
+    nacc = len(E) * len(F)
+    acc_slot = {}
+    k = 0
+    for e in E:
+        for f in F:
+            acc_slot[(e, f)] = k
+            k += 1
+    nxt = [nacc]
+    slot = {}
+
+    def alloc():
+        s = nxt[0]
+        nxt[0] += 1
+        return s
+
+    base = nxt[0]
+    for m in range(L + 1):
+        slot[((0, 0, 0), (0, 0, 0), m)] = alloc()
+    vrr_levels = [[] for _ in range(L + 1)]
+    for key in order:
+        e, f, m = key
+        kind = nodes[key]
+        if kind[0] == "base":
+            continue
+        _, terms, extra = kind
+        t = []
+        for c, src in terms:
+            t.append((slot[src], SID[c], 1.0))
+        for x in extra:
+            if x[0] == "ff":
+                t.append((slot[x[2]], SID["inv2q"], float(x[1])))
+                t.append((slot[x[3]], SID["inv2q_rho_q"], -float(x[1])))
+            elif x[0] == "ee":
+                t.append((slot[x[2]], SID["inv2p"], float(x[1])))
+                t.append((slot[x[3]], SID["inv2p_rho_p"], -float(x[1])))
+            elif x[0] == "ef":
+                t.append((slot[x[2]], SID["inv2pq"], float(x[1])))
+        assert len(t) <= MAXTERMS
+        d = alloc()
+        slot[key] = d
+        vrr_levels[sum(e) + sum(f)].append((d, t))
+    vrr_levels = [lv for lv in vrr_levels if lv]
+    vrr_end = nxt[0]
+    acc = [(acc_slot[(e, f)], slot[(e, f, 0)]) for e in E for f in F]
+    # post phase built on symbolic values, then slot-allocated with reuse:
+    # ("acc", e, f) are the accumulators, other keys are HRR / transform nodes
+    Cc, Dc, Ac, Bc = comps(lc), comps(ld), comps(la), comps(lb)
+    sym_levels = []
+
+    def kget(e, c, d):
+        return ("acc", e, c) if sum(d) == 0 else ("k", e, c, d)
+
+    for lev in range(1, ld + 1):
+        ins = []
+        for e in E:
+            for Lc in range(lc, lc + ld - lev + 1):
+                for c in comps(Lc):
+                    for d in comps(lev):
+                        i = next(q for q in range(3) if d[q] > 0)
+                        ins.append((("k", e, c, d), [(kget(e, add(c, i), sub(d, i)), SID["one"], 1.0),
+                                                     (kget(e, c, sub(d, i)), SID["CD" + AX[i]], 1.0)]))
+        sym_levels.append(ins)
+
+    def bget(a, b, c, d):
+        return kget(a, c, d) if sum(b) == 0 else ("b", a, b, c, d)
+
+    for lev in range(1, lb + 1):
+        ins = []
+        for La in range(la, la + lb - lev + 1):
+            for a in comps(La):
+                for b in comps(lev):
+                    for c in Cc:
+                        for d in Dc:
+                            i = next(q for q in range(3) if b[q] > 0)
+                            ins.append((("b", a, b, c, d), [(bget(add(a, i), sub(b, i), c, d), SID["one"], 1.0),
+                                                            (bget(a, sub(b, i), c, d), SID["AB" + AX[i]], 1.0)]))
+        sym_levels.append(ins)
+    import itertools
+    cur = {}
+    for ia, a in enumerate(Ac):
+        for ib, b in enumerate(Bc):
+            for ic, c in enumerate(Cc):
+                for idd, d in enumerate(Dc):
+                    cur[(ia, ib, ic, idd)] = bget(a, b, c, d)
+    sizes = [len(Ac), len(Bc), len(Cc), len(Dc)]
+    dims = [la, lb, lc, ld]
+    if pure:
+        for dim in range(4):
+            if dims[dim] < 2:
+                continue
+            rows = solid_harmonics(dims[dim])
+            nsz = list(sizes)
+            nsz[dim] = len(rows)
+            new = {}
+            ins = []
+            for idx in itertools.product(*[range(s_) for s_ in nsz]):
+                t = []
+                for ci, coef in enumerate(rows[idx[dim]]):
+                    if abs(coef) < 1e-15:
+                        continue
+                    src = list(idx)
+                    src[dim] = ci
+                    t.append((cur[tuple(src)], SID["one"], coef))
+                assert len(t) <= MAXTERMS
+                key = ("t", dim, idx)
+                new[idx] = key
+                ins.append((key, t))
+            sym_levels.append(ins)
+            cur = new
+            sizes = nsz
+    out_sym = [cur[idx] for idx in itertools.product(*[range(s_) for s_ in sizes])]
+    # allocation with reuse: a slot is freed after the last level reading it
+    last = {}
+    for li, ins in enumerate(sym_levels):
+        for _, t in ins:
+            for src, _, _ in t:
+                last[src] = li
+    keep = set(out_sym)
+    where = {("acc",) + ef: acc_slot[ef] for ef in acc_slot}
+    free = list(range(nacc, vrr_end))
+    top = vrr_end
+    post = []
+    import heapq
+    heapq.heapify(free)
+    for li, ins in enumerate(sym_levels):
+        rows = []
+        for key, t in ins:
+            if free:
+                d = heapq.heappop(free)
+            else:
+                d = top
+                top += 1
+            where[key] = d
+            rows.append((d, [(where[src], sid, c) for src, sid, c in t]))
+        post.append(rows)
+        for v, lu in last.items():
+            if lu == li and v not in keep and v in where:
+                heapq.heappush(free, where[v])
+    out = [where[k] for k in out_sym]
+    nxt[0] = top
+    return {"L": L, "nslots": max(vrr_end, nxt[0]), "base": base, "acc": acc, "nacc": nacc,
+            "vrr_levels": vrr_levels, "post_levels": post, "out": out, "sizes": sizes}
+
+
+HEAVY = [(2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]
+
+
+def write_programs(outdir, classes=None):
+    """Emit all heavy-class programs (pure and Cartesian) as flat tables:
+    PROG_INS[i] = {dst, nterm, src0..5, sid0..5} (14 ints), PROG_COEF[i][6],
+    PROG_DESC[k] describes class k (see ClassProgram in heavy_kernel.hpp)."""
+    classes = classes or HEAVY
+    ins_i, ins_c, lev, accs, outs, desc = [], [], [], [], [], []
+    for (la, lb, lc, ld) in classes:
+        for pure in (0, 1):
+            P = gen_program(la, lb, lc, ld, pure)
+            d = {"la": la, "lb": lb, "lc": lc, "ld": ld, "pure": pure, "L": P["L"], "nslots": P["nslots"],
+                 "base": P["base"], "nacc": P["nacc"], "acc_off": len(accs) // 2}
+            for a_, v_ in P["acc"]:
+                accs += [a_, v_]
+            d["vrr_lev_off"] = len(lev)
+            d["n_vrr_lev"] = len(P["vrr_levels"])
+            for lvl in P["vrr_levels"]:
+                lev.append(len(ins_c) // MAXTERMS)
+                for dst, t in lvl:
+                    row = [dst, len(t)] + [x[0] for x in t] + [0] * (MAXTERMS - len(t)) + \
+                        [x[1] for x in t] + [0] * (MAXTERMS - len(t))
+                    ins_i += row
+                    ins_c += [x[2] for x in t] + [0.0] * (MAXTERMS - len(t))
+            lev.append(len(ins_c) // MAXTERMS)
+            d["post_lev_off"] = len(lev)
+            d["n_post_lev"] = len(P["post_levels"])
+            for lvl in P["post_levels"]:
+                lev.append(len(ins_c) // MAXTERMS)
+                for dst, t in lvl:
+                    row = [dst, len(t)] + [x[0] for x in t] + [0] * (MAXTERMS - len(t)) + \
+                        [x[1] for x in t] + [0] * (MAXTERMS - len(t))
+                    ins_i += row
+                    ins_c += [x[2] for x in t] + [0.0] * (MAXTERMS - len(t))
+            lev.append(len(ins_c) // MAXTERMS)
+            d["out_off"] = len(outs)
+            d["nout"] = len(P["out"])
+            outs += P["out"]
+            d["na"], d["nb"], d["nc"], d["nd"] = P["sizes"]
+            desc.append(d)
+    keys = ["la", "lb", "lc", "ld", "pure", "L", "nslots", "base", "nacc", "acc_off", "vrr_lev_off",
+            "n_vrr_lev", "post_lev_off", "n_post_lev", "out_off", "nout", "na", "nb", "nc", "nd"]
+    lines = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.", "#pragma once", "namespace m2 {",
+             "constexpr int PROG_MAXTERMS = %d;" % MAXTERMS,
+             "constexpr int PROG_NCLASS = %d;" % len(desc),
+             "constexpr int PROG_DESC_FIELDS = %d;" % len(keys),
+             "// fields: " + " ".join(keys),
+             "constexpr int PROG_DESC[%d] = {" % (len(desc) * len(keys))]
+    for d in desc:
+        lines.append("  " + ", ".join(str(d[k]) for k in keys) + ",")
+    lines.append("};")
+
+    def arr(name, typ, vals, per=16):
+        lines.append("constexpr %s %s[%d] = {" % (typ, name, max(1, len(vals))))
+        for i in range(0, len(vals), per):
+            lines.append("  " + ", ".join(repr(v) if typ == "double" else str(v) for v in vals[i:i + per]) + ",")
+        if not vals:
+            lines.append("  0")
+        lines.append("};")
+
+    arr("PROG_INS", "int", ins_i)
+    arr("PROG_COEF", "double", [float(x) for x in ins_c], 8)
+    arr("PROG_LEV", "int", lev)
+    arr("PROG_ACC", "int", accs)
+    arr("PROG_OUT", "int", outs)
+    lines.append("}  // namespace m2")
+    text = "\n".join(lines) + "\n"
+    path = os.path.join(outdir, "programs.inc")
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as fh:
+            fh.write(text)
+    return path
 
 
 if __name__ == "__main__":

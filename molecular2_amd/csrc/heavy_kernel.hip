@@ -1,0 +1,344 @@
+// Heavy shell-quartet classes ((dp|dp), (dd|pp), (dd|ds), (dd|dp), (dd|dd)):
+// one wavefront per quartet executing the class's generated recurrence
+// program out of LDS.
+//
+// The program (csrc/gen_eri.py: gen_program) is the same Obara-Saika VRR +
+// HGP HRR DAG the register-resident classes use, flattened into levels of
+// independent instructions dst = sum_t coef_t * S[sid_t] * X[src_t].  The 64
+// lanes execute one level at a time (barrier between levels): per primitive
+// quartet lane m <= L evaluates F_m(T) straight from the Boys table, lane 0
+// writes the per-primitive scalars (PA, WP, QC, WQ, 1/2p, ...), the VRR
+// levels run, the [e0|f0] are accumulated; after the primitive loops the
+// HRR / solid-harmonic levels run and the quartet is digested into J and K
+// with the same screening, degeneracy and factors as the register kernels.
+// Slots are reused by liveness, so (dd|dd) needs 3179 slots (25 KB FP64).
+#include "m2_common.hpp"
+#include "boys.hpp"
+#include "generated/programs.inc"
+
+namespace m2 {
+
+constexpr int NSCALAR = 24;
+
+struct HeavyProg {
+    int L, nslots, base, nacc;
+    const int* acc;      // nacc pairs (acc slot, vrr slot)
+    const int* vlev;     // n_vrr_lev + 1 instruction offsets
+    int nvlev;
+    const int* plev;     // n_post_lev + 1 offsets
+    int nplev;
+    const int* out;      // nout slots, (a,b,c,d) order
+    int na, nb, nc, nd;
+    const int* ins;      // 14 ints per instruction
+    const double* coef;  // PROG_MAXTERMS per instruction
+};
+
+template <typename R>
+__device__ __forceinline__ R boys_one(double T, int m, const BoysTables& tab);
+
+template <>
+__device__ __forceinline__ double boys_one<double>(double T, int m, const BoysTables& tab) {
+    if (T < BOYS_TMAX) {
+        const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
+        const double d = k * BOYS_STEP - T;
+        const double* r = tab.d + k * BOYS_MTAB + m;
+        double s = r[7];
+        s = r[6] + d * (1.0 / 7.0) * s;
+        s = r[5] + d * (1.0 / 6.0) * s;
+        s = r[4] + d * (1.0 / 5.0) * s;
+        s = r[3] + d * (1.0 / 4.0) * s;
+        s = r[2] + d * (1.0 / 3.0) * s;
+        s = r[1] + d * (1.0 / 2.0) * s;
+        return r[0] + d * s;
+    }
+    const double rs = rsqrt(T);
+    double F = 0.88622692545275801365 * rs;
+    const double h = 0.5 * rs * rs;
+    for (int i = 0; i < m; ++i) F = (2 * i + 1) * F * h;
+    return F;
+}
+
+template <>
+__device__ __forceinline__ float boys_one<float>(double T, int m, const BoysTables& tab) {
+    if (T < BOYS_TMAX) {
+        const float Tf = float(T);
+        const int k = __float2int_rn(Tf * float(1.0 / BOYS_STEP));
+        const float d = k * float(BOYS_STEP) - Tf;
+        const float* r = tab.f + k * BOYS_MTAB + m;
+        float s = r[4];
+        s = r[3] + d * (1.0f / 4.0f) * s;
+        s = r[2] + d * (1.0f / 3.0f) * s;
+        s = r[1] + d * (1.0f / 2.0f) * s;
+        return r[0] + d * s;
+    }
+    return float(boys_one<double>(T, m, tab));
+}
+
+template <typename R>
+__device__ __forceinline__ void run_level(const HeavyProg& P, int i0, int i1, R* __restrict__ X, const R* __restrict__ S) {
+    for (int j = i0 + (int)threadIdx.x; j < i1; j += 64) {
+        const int* in = P.ins + 14 * j;
+        const double* cf = P.coef + PROG_MAXTERMS * j;
+        const int nt = in[1];
+        R v = R(0);
+        for (int t = 0; t < nt; ++t) v += R(cf[t]) * S[in[8 + t]] * X[in[2 + t]];
+        X[in[0]] = v;
+    }
+}
+
+// computes the quartet (bra pair | ket pair) into X; returns with all lanes
+// synchronised and the outputs at X[P.out[o]]
+template <typename R>
+__device__ void heavy_quartet(const HeavyProg& P, const JKArgs& a, const BoysTables& tab, int bra, int ket,
+                              double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,
+                              R* __restrict__ X, R* __restrict__ S) {
+    const int lane = threadIdx.x;
+    for (int t = lane; t < P.nacc; t += 64) X[P.acc[2 * t]] = R(0);
+    const PrimPair* bp = a.prims + a.pair_off[bra];
+    const PrimPair* kp = a.prims + a.pair_off[ket];
+    const int nbp = a.pair_np[bra], nkp = a.pair_np[ket];
+    for (int ib = 0; ib < nbp; ++ib) {
+        const PrimPair B = bp[ib];
+        for (int ik = 0; ik < nkp; ++ik) {
+            const PrimPair K = kp[ik];
+            const double p = B.p, q = K.p;
+            const double dx = B.P[0] - K.P[0], dy = B.P[1] - K.P[1], dz = B.P[2] - K.P[2];
+            const double ipq = 1.0 / (p + q);
+            const double T = p * q * ipq * (dx * dx + dy * dy + dz * dz);
+            __syncthreads();   // previous primitive's readers are done
+            if (lane <= P.L) {
+                const double pref = B.K * K.K * rsqrt(p + q);
+                X[P.base + lane] = R(pref) * boys_one<R>(T, lane, tab);
+            }
+            if (lane == 0) {
+                const double rp = q * ipq, rq = p * ipq;
+                S[0] = R(1);
+                S[1] = R(B.PA[0]); S[2] = R(B.PA[1]); S[3] = R(B.PA[2]);
+                S[4] = R(-rp * dx); S[5] = R(-rp * dy); S[6] = R(-rp * dz);
+                S[7] = R(K.PA[0]); S[8] = R(K.PA[1]); S[9] = R(K.PA[2]);
+                S[10] = R(rq * dx); S[11] = R(rq * dy); S[12] = R(rq * dz);
+                S[13] = R(B.inv2p); S[14] = R(K.inv2p); S[15] = R(0.5 * ipq);
+                S[16] = R(B.inv2p * rp); S[17] = R(K.inv2p * rq);
+                S[18] = R(ABx); S[19] = R(ABy); S[20] = R(ABz);
+                S[21] = R(CDx); S[22] = R(CDy); S[23] = R(CDz);
+            }
+            __syncthreads();
+            for (int l = 0; l < P.nvlev; ++l) {
+                run_level<R>(P, P.vlev[l], P.vlev[l + 1], X, S);
+                __syncthreads();
+            }
+            for (int t = lane; t < P.nacc; t += 64) X[P.acc[2 * t]] += X[P.acc[2 * t + 1]];
+        }
+    }
+    __syncthreads();
+    for (int l = 0; l < P.nplev; ++l) {
+        run_level<R>(P, P.plev[l], P.plev[l + 1], X, S);
+        __syncthreads();
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R outv(const HeavyProg& P, const R* X, int a_, int b_, int c_, int d_) {
+    return X[P.out[((a_ * P.nb + b_) * P.nc + c_) * P.nd + d_]];
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
+                                                 double lo, double hi, HeavyProg P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    R* X = reinterpret_cast<R*>(smem);
+    R* S = X + P.nslots;
+    const int w = blockIdx.x;
+    if (w >= ntasks) return;
+    const int lane = threadIdx.x;
+    const Task tk = tasks[w];
+    const int nsh = a.nsh, n = a.n;
+    const int2 sab = a.pair_sh[tk.bra];
+    const int sa = sab.x, sb = sab.y;
+    const double* O = a.sh_O;
+    const double ABx = O[3 * sa] - O[3 * sb], ABy = O[3 * sa + 1] - O[3 * sb + 1], ABz = O[3 * sa + 2] - O[3 * sb + 2];
+    const int fa = a.sh_bf[sa], fb = a.sh_bf[sb];
+    const int NA = P.na, NB = P.nb, NC = P.nc, ND = P.nd;
+    for (int kk = 0; kk < tk.nket; ++kk) {
+        const int ket = tk.ket0 + kk;
+        const int2 scd = a.pair_sh[ket];
+        const int sc = scd.x, sd = scd.y;
+        const double* Dn = a.Dn;
+        const double dn = fmax(fmax(Dn[sa * nsh + sb], Dn[sc * nsh + sd]),
+                               fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+        const double bound = dn * a.pair_Q[tk.bra] * a.pair_Q[ket];
+        if (!(bound >= lo && bound < hi)) continue;
+        if (a.counters && lane == 0) {
+            atomicAdd(&a.counters[2 * a.combo], 1ull);
+            atomicAdd(&a.counters[2 * a.combo + 1], (unsigned long long)a.pair_np[tk.bra] * a.pair_np[ket]);
+        }
+        const double CDx = O[3 * sc] - O[3 * sd], CDy = O[3 * sc + 1] - O[3 * sd + 1], CDz = O[3 * sc + 2] - O[3 * sd + 2];
+        heavy_quartet<R>(P, a, tab, tk.bra, ket, ABx, ABy, ABz, CDx, CDy, CDz, X, S);
+        const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);
+        const double f = deg * a.scale, h = 0.5 * f;
+        const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+        const double* D = a.D;
+        for (int i = lane; i < NA * NB; i += 64) {   // J(ab)
+            const int x = i / NB, y = i % NB;
+            double s = 0.0;
+            for (int c = 0; c < NC; ++c)
+                for (int d = 0; d < ND; ++d) s += D[(fc + c) * n + fd + d] * double(outv(P, X, x, y, c, d));
+            atomicAdd(&a.J[(fa + x) * n + fb + y], f * s);
+        }
+        for (int i = lane; i < NC * ND; i += 64) {   // J(cd)
+            const int c = i / ND, d = i % ND;
+            double s = 0.0;
+            for (int x = 0; x < NA; ++x)
+                for (int y = 0; y < NB; ++y) s += D[(fa + x) * n + fb + y] * double(outv(P, X, x, y, c, d));
+            atomicAdd(&a.J[(fc + c) * n + fd + d], f * s);
+        }
+        for (int i = lane; i < NA * NC; i += 64) {   // K(ac) += D(bd)
+            const int x = i / NC, c = i % NC;
+            double s = 0.0;
+            for (int y = 0; y < NB; ++y)
+                for (int d = 0; d < ND; ++d) s += D[(fb + y) * n + fd + d] * double(outv(P, X, x, y, c, d));
+            atomicAdd(&a.K[(fa + x) * n + fc + c], h * s);
+        }
+        for (int i = lane; i < NB * ND; i += 64) {   // K(bd) += D(ac)
+            const int y = i / ND, d = i % ND;
+            double s = 0.0;
+            for (int x = 0; x < NA; ++x)
+                for (int c = 0; c < NC; ++c) s += D[(fa + x) * n + fc + c] * double(outv(P, X, x, y, c, d));
+            atomicAdd(&a.K[(fb + y) * n + fd + d], h * s);
+        }
+        for (int i = lane; i < NA * ND; i += 64) {   // K(ad) += D(bc)
+            const int x = i / ND, d = i % ND;
+            double s = 0.0;
+            for (int y = 0; y < NB; ++y)
+                for (int c = 0; c < NC; ++c) s += D[(fb + y) * n + fc + c] * double(outv(P, X, x, y, c, d));
+            atomicAdd(&a.K[(fa + x) * n + fd + d], h * s);
+        }
+        for (int i = lane; i < NB * NC; i += 64) {   // K(bc) += D(ad)
+            const int y = i / NC, c = i % NC;
+            double s = 0.0;
+            for (int x = 0; x < NA; ++x)
+                for (int d = 0; d < ND; ++d) s += D[(fa + x) * n + fd + d] * double(outv(P, X, x, y, c, d));
+            atomicAdd(&a.K[(fb + y) * n + fc + c], h * s);
+        }
+        __syncthreads();   // outputs consumed before the next quartet overwrites X
+    }
+}
+
+// Schwarz of heavy pair classes: one wavefront per pair, (ab|ab) in FP64
+__global__ __launch_bounds__(64) void k_schwarz_heavy(JKArgs a, BoysTables tab, int p0, int np, HeavyProg P,
+                                                      double* __restrict__ Qout) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* X = reinterpret_cast<double*>(smem);
+    double* S = X + P.nslots;
+    const int w = blockIdx.x;
+    if (w >= np) return;
+    const int pr = p0 + w;
+    const int2 s = a.pair_sh[pr];
+    const double* O = a.sh_O;
+    const double ABx = O[3 * s.x] - O[3 * s.y], ABy = O[3 * s.x + 1] - O[3 * s.y + 1], ABz = O[3 * s.x + 2] - O[3 * s.y + 2];
+    heavy_quartet<double>(P, a, tab, pr, pr, ABx, ABy, ABz, ABx, ABy, ABz, X, S);
+    if (threadIdx.x == 0) {
+        double mx = 0.0;
+        for (int x = 0; x < P.na; ++x)
+            for (int y = 0; y < P.nb; ++y) mx = fmax(mx, fabs(outv(P, X, x, y, x, y)));
+        Qout[pr] = sqrt(mx);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct HeavyTables {
+    int* ins = nullptr;
+    double* coef = nullptr;
+    int* lev = nullptr;
+    int* acc = nullptr;
+    int* out = nullptr;
+};
+
+static HeavyTables g_heavy[64];   // per device
+
+hipError_t heavy_upload(int device) {
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    HeavyTables& t = g_heavy[device];
+    if (t.ins) return hipSuccess;
+    hipError_t e;
+    if ((e = hipMalloc((void**)&t.ins, sizeof(PROG_INS)))) return e;
+    if ((e = hipMalloc((void**)&t.coef, sizeof(PROG_COEF)))) return e;
+    if ((e = hipMalloc((void**)&t.lev, sizeof(PROG_LEV)))) return e;
+    if ((e = hipMalloc((void**)&t.acc, sizeof(PROG_ACC)))) return e;
+    if ((e = hipMalloc((void**)&t.out, sizeof(PROG_OUT)))) return e;
+    if ((e = hipMemcpy(t.ins, PROG_INS, sizeof(PROG_INS), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.coef, PROG_COEF, sizeof(PROG_COEF), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.lev, PROG_LEV, sizeof(PROG_LEV), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.acc, PROG_ACC, sizeof(PROG_ACC), hipMemcpyHostToDevice))) return e;
+    return hipMemcpy(t.out, PROG_OUT, sizeof(PROG_OUT), hipMemcpyHostToDevice);
+}
+
+static bool find_prog(int device, int la, int lb, int lc, int ld, int pure, HeavyProg& P) {
+    const HeavyTables& t = g_heavy[device];
+    for (int k = 0; k < PROG_NCLASS; ++k) {
+        const int* d = PROG_DESC + k * PROG_DESC_FIELDS;
+        if (d[0] == la && d[1] == lb && d[2] == lc && d[3] == ld && d[4] == pure) {
+            // fields: la lb lc ld pure L nslots base nacc acc_off vrr_lev_off n_vrr_lev post_lev_off
+            //         n_post_lev out_off nout na nb nc nd
+            P.L = d[5];
+            P.nslots = d[6];
+            P.base = d[7];
+            P.nacc = d[8];
+            P.acc = t.acc + 2 * d[9];
+            P.vlev = t.lev + d[10];
+            P.nvlev = d[11];
+            P.plev = t.lev + d[12];
+            P.nplev = d[13];
+            P.out = t.out + d[14];
+            P.na = d[16];
+            P.nb = d[17];
+            P.nc = d[18];
+            P.nd = d[19];
+            P.ins = t.ins;
+            P.coef = t.coef;
+            return true;
+        }
+    }
+    return false;
+}
+
+bool heavy_has(int la, int lb, int lc, int ld) {
+    for (int k = 0; k < PROG_NCLASS; ++k) {
+        const int* d = PROG_DESC + k * PROG_DESC_FIELDS;
+        if (d[0] == la && d[1] == lb && d[2] == lc && d[3] == ld) return true;
+    }
+    return false;
+}
+
+hipError_t launch_jk_heavy(int device, int la, int lb, int lc, int ld, int pure, const JKArgs& a,
+                           const BoysTables& tab, const Task* tasks, int ntasks, int mode, hipStream_t st) {
+    if (ntasks <= 0) return hipSuccess;
+    HeavyProg P;
+    if (!find_prog(device, la, lb, lc, ld, pure, P)) return hipErrorInvalidValue;
+    const double inf = __builtin_huge_val();
+    if (mode == 2) {
+        const size_t sm = sizeof(float) * (P.nslots + NSCALAR);
+        hipLaunchKernelGGL(k_jk_heavy<float>, dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks, a.precision,
+                           inf, P);
+    } else {
+        const size_t sm = sizeof(double) * (P.nslots + NSCALAR);
+        hipLaunchKernelGGL(k_jk_heavy<double>, dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks, a.precision,
+                           inf, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_schwarz_heavy(int device, int la, int lb, int pure, const JKArgs& a, const BoysTables& tab, int p0,
+                                int np, double* Q, hipStream_t st) {
+    if (np <= 0) return hipSuccess;
+    HeavyProg P;
+    if (!find_prog(device, la, lb, la, lb, pure, P)) return hipErrorInvalidValue;
+    const size_t sm = sizeof(double) * (P.nslots + NSCALAR);
+    hipLaunchKernelGGL(k_schwarz_heavy, dim3(np), dim3(64), sm, st, a, tab, p0, np, P, Q);
+    return hipGetLastError();
+}
+
+}  // namespace m2

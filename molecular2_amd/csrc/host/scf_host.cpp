@@ -1,0 +1,548 @@
+// Host C++ SCF over the MI355X engine: restatement of the reference's
+// SCF::rhf loop (src/scf.cpp:90-185), Fock incremental build logic
+// (src/fockbuild.cpp:176-222), Fock set-up / transform / diagonalise / makeDens
+// (src/fock.cpp:32-73,114-244) and DIISEngine (src/diis.cpp:15-116).
+// All O(n^3) work runs on the device (m2_gemm / m2_eigh), all two-electron
+// work through m2_jk_direct / m2_jk_df.
+#include "scf_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace molecular {
+
+// ---------------------------------------------------------------- Matrix
+Matrix Matrix::transpose() const {
+    Matrix t(c_, r_);
+    for (int j = 0; j < c_; ++j)
+        for (int i = 0; i < r_; ++i) t(j, i) = (*this)(i, j);
+    return t;
+}
+Matrix Matrix::operator+(const Matrix& o) const {
+    Matrix t(*this);
+    t += o;
+    return t;
+}
+Matrix Matrix::operator-(const Matrix& o) const {
+    Matrix t(*this);
+    t -= o;
+    return t;
+}
+Matrix& Matrix::operator+=(const Matrix& o) {
+    if (o.r_ != r_ || o.c_ != c_) throw Error("MATRIX", "size mismatch in +=");
+    for (size_t k = 0; k < a_.size(); ++k) a_[k] += o.a_[k];
+    return *this;
+}
+Matrix& Matrix::operator-=(const Matrix& o) {
+    if (o.r_ != r_ || o.c_ != c_) throw Error("MATRIX", "size mismatch in -=");
+    for (size_t k = 0; k < a_.size(); ++k) a_[k] -= o.a_[k];
+    return *this;
+}
+Matrix Matrix::operator*(double s) const {
+    Matrix t(*this);
+    for (auto& x : t.a_) x *= s;
+    return t;
+}
+double Matrix::norm() const {
+    double s = 0;
+    for (double x : a_) s += x * x;
+    return std::sqrt(s);
+}
+double Matrix::dot(const Matrix& o) const {
+    double s = 0;
+    for (size_t k = 0; k < a_.size(); ++k) s += a_[k] * o.a_[k];
+    return s;
+}
+Matrix Matrix::block(int r0, int c0, int nr, int nc) const {
+    Matrix b(nr, nc);
+    for (int j = 0; j < nc; ++j)
+        for (int i = 0; i < nr; ++i) b(i, j) = (*this)(r0 + i, c0 + j);
+    return b;
+}
+
+static void check(m2_ctx* ctx, m2_status st, const char* what) {
+    if (st != M2_OK) throw Error("M2", std::string(what) + ": " + m2_last_error(ctx));
+}
+
+Matrix matmul(m2_ctx* ctx, const Matrix& A, const Matrix& B, bool tA, bool tB) {
+    const int m = tA ? A.cols() : A.rows();
+    const int k = tA ? A.rows() : A.cols();
+    const int n = tB ? B.rows() : B.cols();
+    if ((tB ? B.cols() : B.rows()) != k) throw Error("MATRIX", "size mismatch in matmul");
+    Matrix C(m, n);
+    check(ctx, m2_gemm(ctx, tA ? 'T' : 'N', tB ? 'T' : 'N', m, n, k, 1.0, A.data(), A.rows(), B.data(), B.rows(), 0.0,
+                       C.data(), m),
+          "m2_gemm");
+    return C;
+}
+
+void eigh(m2_ctx* ctx, const Matrix& A, Vector& w, Matrix& V) {
+    const int n = A.rows();
+    w.assign(n, 0.0);
+    V = Matrix(n, n);
+    check(ctx, m2_eigh(ctx, n, A.data(), w.data(), V.data()), "m2_eigh");
+}
+
+// -------------------------------------------------------------- Molecule
+int Molecule::getNel() const {
+    int z = 0;
+    for (const auto& a : atoms) z += a.Z;
+    return z - charge;
+}
+
+double Molecule::getEnuc() const {
+    double e = 0.0;
+    for (size_t i = 0; i < atoms.size(); ++i)
+        for (size_t j = i + 1; j < atoms.size(); ++j) {
+            const double dx = atoms[i].x - atoms[j].x, dy = atoms[i].y - atoms[j].y, dz = atoms[i].z - atoms[j].z;
+            e += atoms[i].Z * atoms[j].Z / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    return e;
+}
+
+// -------------------------------------------------------- IntegralEngine
+static void set_basis(m2_ctx* ctx, int which, const std::vector<RawShell>& sh, bool pure) {
+    std::vector<int> l, np, at;
+    std::vector<double> ex, cf, cen;
+    for (const auto& s : sh) {
+        l.push_back(s.l);
+        np.push_back((int)s.exps.size());
+        at.push_back(s.atom);
+        ex.insert(ex.end(), s.exps.begin(), s.exps.end());
+        cf.insert(cf.end(), s.coefs.begin(), s.coefs.end());
+        cen.insert(cen.end(), s.O, s.O + 3);
+    }
+    check(ctx, m2_basis_set(ctx, which, (int)sh.size(), l.data(), np.data(), ex.data(), cf.data(), cen.data(),
+                            at.data(), pure ? 1 : 0),
+          "m2_basis_set");
+}
+
+IntegralEngine::IntegralEngine(m2_ctx* c, const Molecule& m) : ctx(c) {
+    set_basis(ctx, M2_ORBITAL, m.shells, m.pure);
+    check(ctx, m2_basis_info(ctx, M2_ORBITAL, &nbf, &nsh), "m2_basis_info");
+    if (!m.jkshells.empty()) {
+        set_basis(ctx, M2_AUXILIARY, m.jkshells, m.pure);
+        check(ctx, m2_basis_info(ctx, M2_AUXILIARY, &naux, nullptr), "m2_basis_info");
+    }
+    sints = tints = naints = Matrix(nbf, nbf);
+    std::vector<double> Z, xyz;
+    for (const auto& a : m.atoms) {
+        Z.push_back(a.Z);
+        xyz.push_back(a.x);
+        xyz.push_back(a.y);
+        xyz.push_back(a.z);
+    }
+    check(ctx, m2_one_body(ctx, (int)Z.size(), Z.data(), xyz.data(), sints.data(), tints.data(), naints.data()),
+          "m2_one_body");
+    prescreen = Matrix(nsh, nsh);
+    check(ctx, m2_schwarz(ctx, prescreen.data()), "m2_schwarz");
+}
+
+Matrix IntegralEngine::compute_shellblock_norm(const Matrix& A) {
+    Matrix out(nsh, nsh);
+    check(ctx, m2_shellblock_norm(ctx, A.data(), out.data()), "m2_shellblock_norm");
+    return out;
+}
+
+// ------------------------------------------------------------ DIISEngine
+void DIISEngine::init(int maxd, bool use, double damp) {
+    maxDiis = maxd > 0 ? maxd : 8;
+    useDiis = use;
+    damping_factor = damp;
+    lastB = Matrix::Zero(1, 1);
+    errs.clear();
+}
+
+static double vdot(const Vector& a, const Vector& b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+// Full-pivot LU of a small dense system; returns the rank like Eigen's
+// FullPivLU (pivots above eps * size * max pivot).
+static Vector lu_solve(Matrix A, Vector b, int& rank) {
+    const int n = A.rows();
+    std::vector<int> rp(n), cp(n);
+    for (int i = 0; i < n; ++i) rp[i] = cp[i] = i;
+    double maxpiv = 0.0;
+    rank = 0;
+    for (int k = 0; k < n; ++k) {
+        int pi = k, pj = k;
+        double best = -1.0;
+        for (int i = k; i < n; ++i)
+            for (int j = k; j < n; ++j)
+                if (std::fabs(A(i, j)) > best) {
+                    best = std::fabs(A(i, j));
+                    pi = i;
+                    pj = j;
+                }
+        if (k == 0) maxpiv = best;
+        if (best <= maxpiv * n * 2.220446049250313e-16 || best == 0.0) break;
+        ++rank;
+        if (pi != k) {
+            for (int j = 0; j < n; ++j) std::swap(A(pi, j), A(k, j));
+            std::swap(b[pi], b[k]);
+        }
+        if (pj != k) {
+            for (int i = 0; i < n; ++i) std::swap(A(i, pj), A(i, k));
+            std::swap(cp[pj], cp[k]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A(i, k) / A(k, k);
+            for (int j = k; j < n; ++j) A(i, j) -= f * A(k, j);
+            b[i] -= f * b[k];
+        }
+    }
+    Vector y(n, 0.0);
+    for (int i = rank - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int j = i + 1; j < rank; ++j) s -= A(i, j) * y[j];
+        y[i] = s / A(i, i);
+    }
+    Vector x(n, 0.0);
+    for (int i = 0; i < n; ++i) x[cp[i]] = y[i];
+    return x;
+}
+
+Vector DIISEngine::solve(const Matrix& B, int start) {
+    const int size = B.rows() - start;
+    double norm = 1.0;
+    const double scale = 1.0 + damping_factor;
+    if (B(start, start) > 1e-10) norm = 1.0 / B(start, start);
+    Matrix nB(size, size);
+    for (int i = 0; i < size - 1; ++i) {
+        nB(i, i) = B(i + start, i + start) * scale;
+        for (int j = i; j < size - 1; ++j) {   // j == i replaces the damped diagonal, as the reference does
+            nB(i, j) = norm * B(i + start, j + start);
+            nB(j, i) = nB(i, j);
+        }
+        nB(i, size - 1) = nB(size - 1, i) = -1.0;
+    }
+    nB(size - 1, size - 1) = 0.0;
+    Vector rhs(size, 0.0);
+    rhs[size - 1] = -1.0;
+    int rank = 0;
+    Vector w = lu_solve(nB, rhs, rank);
+    Vector out(w);
+    out.push_back((double)rank);
+    return out;
+}
+
+Vector DIISEngine::compute(std::vector<Vector>& errors) {
+    Vector weights;
+    const int it = (int)errs.size() + 1;
+    if (useDiis) {
+        if (it > maxDiis) errs.erase(errs.begin());
+        Vector ne;
+        for (const auto& e : errors) ne.insert(ne.end(), e.begin(), e.end());
+        errs.push_back(ne);
+        if (it == 1) lastB(0, 0) = vdot(ne, ne);
+        if (it > 1) {
+            const int lim = it < maxDiis ? it : maxDiis;
+            Matrix B(lim + 1, lim + 1);
+            B(lim, lim) = 0.0;
+            int start = it > lim ? 1 : 0;
+            for (int i = 0; i < lim - 1; ++i) {
+                for (int j = i; j < lim - 1; ++j) {
+                    B(i, j) = lastB(i + start, j + start);
+                    B(j, i) = B(i, j);
+                }
+                B(lim, i) = -1.0;
+                B(i, lim) = -1.0;
+            }
+            for (int i = 0; i < lim; ++i) {
+                B(lim - 1, i) = vdot(errs[i], errs[lim - 1]);
+                B(i, lim - 1) = B(lim - 1, i);
+            }
+            B(lim, lim - 1) = -1.0;
+            B(lim - 1, lim) = -1.0;
+            lastB = B;
+            weights = solve(B, 0);
+            double rank = weights.back();
+            start = 1;
+            while (rank < (lim - start + 1) && start < lim) {
+                weights = solve(B, start++);
+                rank = weights.back();
+            }
+            weights.resize(weights.size() - 2);
+        }
+    }
+    return weights;
+}
+
+// ------------------------------------------------------------------ Fock
+Fock::Fock(const Options& o, IntegralEngine& ints, const Molecule& m) : opt(o), integrals(ints), molecule(m) {
+    formHCore();
+    formOrthog();
+    direct = o.direct;
+    density_fitted = o.df;
+    diis = o.diis;
+    iter = 0;
+    nocc = molecule.getNel() / 2;
+    MAX = o.maxdiis;
+    precision = o.precision;
+    if (density_fitted && integrals.nauxiliary() == 0) throw Error("DF", "no auxiliary (JKFIT) basis set");
+}
+
+void Fock::formHCore() {
+    hcore = integrals.getKinetic() + integrals.getNucAttract();
+    nbfs = hcore.rows();
+}
+
+void Fock::formOrthog() {
+    Vector lambda;
+    Matrix U;
+    eigh(integrals.context(), integrals.getOverlap(), lambda, U);
+    Matrix Us(U);
+    for (int j = 0; j < nbfs; ++j) {
+        if (lambda[j] <= 0.0) throw Error("ORTHOG", "overlap matrix is not positive definite");
+        const double s = 1.0 / std::sqrt(lambda[j]);
+        for (int i = 0; i < nbfs; ++i) Us(i, j) *= s;
+    }
+    orthog = matmul(integrals.context(), Us, U, false, true);   // U lambda^-1/2 U^T
+}
+
+void Fock::makeJK(Matrix& P, double multiplier) {
+    if (density_fitted) formJKdf(P, multiplier);
+    else formJKdirect(integrals.getPrescreen(), P, multiplier);
+}
+
+void Fock::formJKdirect(const Matrix& Schwarz, Matrix& D, double multiplier) {
+    (void)Schwarz;   // the engine holds the device copy of the same Schwarz bounds
+    m2_ctx* ctx = integrals.context();
+    unsigned flags = M2_FP64;
+    if (opt.mixed && !fp64_forced) {
+        if (last_build_incremental) {
+            check(ctx, m2_set_mixed_threshold(ctx, 1e300), "m2_set_mixed_threshold");   // all FP32
+        } else {
+            check(ctx, m2_set_mixed_threshold(ctx, opt.mixed_threshold), "m2_set_mixed_threshold");
+        }
+        flags = M2_MIXED;
+        ++n_fp32_builds;
+    }
+    jints = Matrix(nbfs, nbfs);
+    kints = Matrix(nbfs, nbfs);
+    auto t0 = std::chrono::steady_clock::now();
+    check(ctx, m2_jk_direct(ctx, D.data(), multiplier, precision, flags, jints.data(), kints.data()), "m2_jk_direct");
+    jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    jkints = jints - 0.5 * kints;
+}
+
+void Fock::formJKdf(Matrix& Cocc, double multiplier) {
+    m2_ctx* ctx = integrals.context();
+    jints = Matrix(nbfs, nbfs);
+    kints = Matrix(nbfs, nbfs);
+    auto t0 = std::chrono::steady_clock::now();
+    check(ctx, m2_jk_df(ctx, Cocc.data(), Cocc.cols(), opt.mixed && !fp64_forced ? 1 : 0, jints.data(), kints.data()),
+          "m2_jk_df");
+    jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    jkints = multiplier * (2.0 * jints - kints);
+}
+
+// src/fockbuild.cpp:176-203
+void Fock::makeFock(Matrix& P, double multiplier) {
+    if (!density_fitted) {
+        focka = fockinc;
+        if (!started_incremental && rms_error < incremental_threshold) {
+            started_incremental = true;
+            reset_incremental = false;
+            next_reset = rms_error / 1e1;
+            last_reset = iter - 1;
+        }
+        bool full = reset_incremental || !started_incremental;
+        if (full) {
+            focka = hcore;
+            dens_diff = dens;
+        }
+        if (reset_incremental && started_incremental) {
+            reset_incremental = false;
+            last_reset = iter;
+            next_reset = rms_error / 1e1;
+        }
+        last_build_incremental = !full;
+        if (full) ++n_full;
+        else ++n_incremental;
+        // dens_diff may have been reassigned: build from it
+        makeJK(full ? dens_diff : P, multiplier);
+    } else {
+        focka = hcore;
+        makeJK(P, multiplier);
+    }
+    focka += jkints;
+    if (!density_fitted) fockinc = focka;
+    addDiis();
+}
+
+void Fock::addDiis() {
+    if (diis) {
+        if (iter >= MAX) focks.erase(focks.begin());
+        focks.push_back(focka);
+        iter++;
+    }
+}
+
+void Fock::makeFock() {
+    if (!density_fitted) makeFock(dens_diff);
+    else {
+        Matrix cocc = CP.block(0, 0, nbfs, nocc);
+        makeFock(cocc);
+    }
+}
+
+void Fock::average(const Vector& w) {
+    if (diis && iter > 2) {
+        focka = Matrix::Zero(nbfs, nbfs);
+        const int offset = (int)focks.size() - (int)w.size();
+        for (int i = offset; i < (int)focks.size(); ++i) focka += w[i - offset] * focks[i];
+    }
+}
+
+void Fock::transform(bool first) {
+    m2_ctx* ctx = integrals.context();
+    if (first) {
+        if (opt.guess != "hcore")
+            throw Error("GUESS", "guess '" + opt.guess + "' is not available in this engine (use hcore)");
+        fockm = matmul(ctx, orthog, matmul(ctx, hcore, orthog), true, false);
+        fockinc = hcore;
+    } else {
+        fockm = matmul(ctx, orthog, matmul(ctx, focka, orthog), true, false);
+    }
+}
+
+void Fock::diagonalise() {
+    Matrix U;
+    eigh(integrals.context(), fockm, eps, U);
+    CP = matmul(integrals.context(), orthog, U);
+}
+
+void Fock::makeDens() {
+    if (dens.rows() != 0) dens_diff = -dens;
+    Matrix Co = CP.block(0, 0, nbfs, nocc);
+    dens = 2.0 * matmul(integrals.context(), Co, Co, false, true);
+    if (dens_diff.rows() == 0) dens_diff = dens;
+    else dens_diff += dens;
+}
+
+// ------------------------------------------------------------------- SCF
+SCF::SCF(const Options& o, const Molecule& m, Fock& f) : opt(o), molecule(m), focker(f) {
+    diis.init(o.maxdiis, o.diis);
+}
+
+double SCF::calcE(const Matrix& hcore, const Matrix& dens, const Matrix& fock) {
+    // 0.5 tr(D H), 0.5 tr(D F): D, H, F symmetric so tr(D X) = sum D_ij X_ij
+    one_E = 0.5 * dens.dot(hcore);
+    two_E = 0.5 * dens.dot(fock) - one_E;
+    one_E *= 2.0;
+    return one_E + two_E;
+}
+
+void SCF::calcE() {
+    last_energy = energy;
+    energy = calcE(focker.getHCore(), focker.getDens(), focker.getFockAO()) + molecule.getEnuc();
+}
+
+// e = X^T (F D S - S D F) X, flattened column-major (src/scf.cpp:52-59)
+Vector SCF::calcErr(const Matrix& F, const Matrix& D, const Matrix& S, const Matrix& orthog) {
+    m2_ctx* ctx = focker.context();
+    Matrix DS = matmul(ctx, D, S);
+    Matrix FDS = matmul(ctx, F, DS);
+    // S D F = (F D S)^T for symmetric F, D, S
+    Matrix e = FDS - FDS.transpose();
+    e = matmul(ctx, orthog, matmul(ctx, e, orthog), true, false);
+    error = e.norm();
+    return Vector(e.data(), e.data() + e.size());
+}
+
+Vector SCF::calcErr() {
+    return calcErr(focker.getFockAO(), focker.getDens(), focker.getS(), focker.getOrthog());
+}
+
+bool SCF::testConvergence(double val) {
+    const bool r = val < opt.densconverge;
+    last_err = error;
+    return r;
+}
+
+// src/scf.cpp:90-185, plus the precision policy of the engine: with
+// Options::mixed the SCF first converges with mixed / FP32 builds, then
+// forces a full FP64 rebuild and keeps FP64 for at least fp64_final more
+// iterations before it may stop.
+void SCF::rhf(bool print) {
+    (void)print;
+    const int nel = molecule.getNel();
+    if (nel % 2 != 0) throw Error("RHF", "Molecule has an odd number of electrons.");
+    if (molecule.getMultiplicity() != 1) throw Error("RHF", "Molecule is not a singlet state.");
+    auto tic = std::chrono::steady_clock::now();
+    auto lap = [&]() {
+        auto t = std::chrono::steady_clock::now();
+        double s = std::chrono::duration<double>(t - tic).count();
+        tic = t;
+        return s;
+    };
+    converged = false;
+    log.clear();
+    focker.fp64_forced = !opt.mixed;
+    focker.transform(true);
+    focker.diagonalise();
+    focker.makeDens();
+    Matrix old_dens = focker.getDens();
+    focker.makeFock();
+    std::vector<Vector> errs;
+    errs.push_back(calcErr());
+    Vector weights = diis.compute(errs);
+    errs.clear();
+    calcE();
+    log.push_back({0, energy, 0.0, 0.0, lap(), 0, opt.mixed ? 1 : 0});
+    focker.average(weights);
+    focker.transform(false);
+    int iter = 1;
+    int fp64_iters = opt.mixed ? 0 : 1 << 30;
+    while (!converged && iter < opt.maxiter) {
+        focker.diagonalise();
+        focker.makeDens();
+        const double dd = (focker.getDens() - old_dens).norm();
+        old_dens = focker.getDens();
+        const int inc_before = focker.n_incremental;
+        focker.makeFock();
+        const int was_incremental = focker.n_incremental > inc_before;
+        errs.push_back(calcErr());
+        focker.rms_error = [&]() {
+            double s = 0;
+            for (double x : errs[0]) s += x * x;
+            return std::sqrt(s);
+        }() / ((double)old_dens.rows() * old_dens.cols());
+        if (focker.rms_error < focker.next_reset || iter - focker.last_reset > 8) focker.reset_incremental = true;
+        if (iter > 2) weights = diis.compute(errs);
+        errs.clear();
+        calcE();
+        const double delta = std::fabs(energy - last_energy);
+        log.push_back({iter, energy, delta, dd, lap(), was_incremental, focker.fp64_forced ? 0 : 1});
+        if (iter > 2) focker.average(weights);
+        focker.transform(false);
+        bool conv = testConvergence(dd) && delta < opt.enconverge;
+        if (focker.fp64_forced) ++fp64_iters;
+        // the mixed phase cannot resolve changes below FP32 noise: it hands
+        // over to FP64 at max(criterion, FP32 floor)
+        const bool mixed_done = (dd < std::max(opt.densconverge, opt.handover_dens) &&
+                                 delta < std::max(opt.enconverge, opt.handover_energy)) ||
+                                iter >= opt.maxiter / 2;
+        if (!focker.fp64_forced && mixed_done) conv = true;
+        if (conv && !focker.fp64_forced) {
+            // converged under mixed precision: switch to FP64, full rebuild next
+            focker.fp64_forced = true;
+            focker.reset_incremental = true;
+            fp64_iters = 0;
+            conv = false;
+        }
+        if (conv && fp64_iters < opt.fp64_final) conv = false;
+        converged = conv;
+        iter++;
+    }
+    focker.diagonalise();
+    iterations = iter;
+    // like the reference (src/scf.cpp:163-166) non-convergence is reported,
+    // not thrown: the caller reads `converged`
+}
+
+}  // namespace molecular

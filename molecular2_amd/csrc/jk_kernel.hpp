@@ -122,9 +122,8 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, bool active, int
         }
 }
 
-// Quartets are computed when lo <= bound < hi: FP64 builds use
-// [precision, inf); a mixed build runs the FP64 instantiation on
-// [max(precision, fp32_below), inf) and the FP32 one on [precision, fp32_below).
+// Quartets are computed when lo <= bound < hi (the launchers pass
+// [precision, inf); precision is chosen per task list).
 template <int LA, int LB, int LC, int LD, typename R, bool PURE>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
                                             double lo, double hi) {
@@ -148,6 +147,14 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a, BoysTables tab, const Task
     const double bound = dn * a.pair_Q[bra] * a.pair_Q[ket];
     const bool active = lane < nket && bound >= lo && bound < hi;
     if (!__any(active)) return;
+    if (a.counters) {
+        const int na = __popcll(__ballot(active));
+        const double nk = wave_sum(active ? double(a.pair_np[ket]) : 0.0);
+        if (lane == 0) {
+            atomicAdd(&a.counters[2 * a.combo], (unsigned long long)na);
+            atomicAdd(&a.counters[2 * a.combo + 1], (unsigned long long)(nk * a.pair_np[bra] + 0.5));
+        }
+    }
     const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
     const double* O = a.sh_O;
     const R ABx = R(O[3 * sa] - O[3 * sb]), ABy = R(O[3 * sa + 1] - O[3 * sb + 1]), ABz = R(O[3 * sa + 2] - O[3 * sb + 2]);
@@ -183,22 +190,20 @@ __global__ __launch_bounds__(256) void k_schwarz(JKArgs a, BoysTables tab, int p
     Qout[pr] = sqrt(mx);
 }
 
-// mode 0: FP64 on [precision, inf); 1: FP64 part of a mixed build; 2: FP32 part.
+// mode 0: FP64 kernel; mode 2: FP32 kernel (the caller hands each the tasks
+// of its precision); both compute every quartet with bound >= precision.
 template <int LA, int LB, int LC, int LD, bool PURE>
 void launch_jk(const JKArgs& a, const BoysTables& tab, const Task* tasks, int ntasks, int mode, hipStream_t st) {
     if (ntasks <= 0) return;
     const int waves_per_block = 4;
     dim3 grid((ntasks + waves_per_block - 1) / waves_per_block), block(64 * waves_per_block);
     const double inf = __builtin_huge_val();
-    if (mode == 0)
+    if (mode == 2)
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, block, 0, st, a, tab, tasks, ntasks,
+                           a.precision, inf);
+    else
         hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, block, 0, st, a, tab, tasks, ntasks,
                            a.precision, inf);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, block, 0, st, a, tab, tasks, ntasks,
-                           a.precision > a.fp32_below ? a.precision : a.fp32_below, inf);
-    else
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, block, 0, st, a, tab, tasks, ntasks,
-                           a.precision, a.fp32_below);
 }
 
 template <int LA, int LB, bool PURE>

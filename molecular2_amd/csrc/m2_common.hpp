@@ -22,9 +22,9 @@ static_assert(sizeof(PrimPair) == 80, "PrimPair layout");
 // Boys table: F_m(T_k) at T_k = k * BOYS_STEP, k = 0..BOYS_NGRID-1,
 // m = 0..BOYS_MTAB-1.  Beyond BOYS_TMAX the asymptotic branch is used.
 constexpr int BOYS_MTAB = 26;
-constexpr int BOYS_NGRID = 401;
+constexpr int BOYS_NGRID = 601;
 constexpr double BOYS_STEP = 0.1;
-constexpr double BOYS_TMAX = 40.0;
+constexpr double BOYS_TMAX = 60.0;
 
 struct JKArgs {
     const PrimPair* prims;
@@ -43,6 +43,8 @@ struct JKArgs {
     double precision;        // quartet screening threshold
     double fp32_below;       // quartets with bound < this run in FP32 (mixed mode)
     double scale;            // multiplier
+    unsigned long long* counters;  // profiling only: per class combo {quartets, primitive quartets}
+    int combo;               // 6 * X + Y of the launch
 };
 
 struct Task {                // one wavefront of quartets: bra pair x 1..64 ket pairs

@@ -58,8 +58,7 @@ __device__ void boys_rt(double T, int M, double* F, const BoysTables& tab) {
         for (int m = M - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
     } else {
         F[0] = 0.88622692545275801365 * rsqrt(T);
-        const double e = exp(-T);
-        for (int m = 0; m < M; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * (0.5 / T);
+        for (int m = 0; m < M; ++m) F[m + 1] = (2 * m + 1) * F[m] * (0.5 / T);
     }
 }
 

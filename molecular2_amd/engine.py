@@ -20,6 +20,21 @@ _ip = C.POINTER(C.c_int)
 _lib = None
 
 
+class SCFOptions(C.Structure):
+    """Options of the host C++ SCF (the reference's RHF defaults,
+    src/ProgramController.cpp:295-307, plus the precision policy)."""
+    _fields_ = [("diis", C.c_int), ("maxdiis", C.c_int), ("maxiter", C.c_int),
+                ("enconverge", C.c_double), ("densconverge", C.c_double), ("precision", C.c_double),
+                ("df", C.c_int), ("mixed", C.c_int), ("mixed_threshold", C.c_double), ("fp64_final", C.c_int)]
+
+
+class SCFResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("one_e", C.c_double), ("two_e", C.c_double), ("enuc", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int), ("n_full", C.c_int),
+                ("n_incremental", C.c_int), ("n_fp32", C.c_int),
+                ("jk_seconds", C.c_double), ("total_seconds", C.c_double)]
+
+
 class M2Error(RuntimeError):
     """Mirrors the reference's Error(code, msg) (inc/error.hpp:18-35)."""
 
@@ -52,7 +67,25 @@ def lib():
         L.m2_jk_direct_device.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint, vp, vp]
         L.m2_set_mixed_threshold.argtypes = [vp, C.c_double]
         L.m2_last_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
-        for name in ("m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
+        L.m2_set_profiling.argtypes = [vp, C.c_int]
+        _ll = C.POINTER(C.c_longlong)
+        L.m2_get_profile.argtypes = [vp, _dp, _ll, _ll, _ll]
+        L.m2_reset_profile.argtypes = [vp]
+        L.m2_df_setup.argtypes = [vp]
+        L.m2_jk_df.argtypes = [vp, _dp, C.c_int, C.c_int, _dp, _dp]
+        L.m2_jk_df_device.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp]
+        L.m2_df_timing.argtypes = [vp, _dp]
+        L.m2_eigh.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+        L.m2_gemm.argtypes = [vp, C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp,
+                              C.c_int, C.c_double, _dp, C.c_int]
+        L.m2_rhf_last_error.restype = C.c_char_p
+        L.m2_rhf.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
+                             C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                             C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                             C.POINTER(SCFOptions), C.POINTER(SCFResult), _dp, _dp, _dp, _dp]
+        for name in ("m2_df_setup", "m2_jk_df", "m2_jk_df_device", "m2_df_timing", "m2_eigh", "m2_gemm", "m2_rhf"):
+            getattr(L, name).restype = C.c_int
+        for name in ("m2_set_profiling", "m2_get_profile", "m2_reset_profile", "m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
                      "m2_shellblock_norm", "m2_one_body", "m2_jk_direct", "m2_jk_direct_device",
                      "m2_set_mixed_threshold", "m2_last_stats"):
             getattr(L, name).restype = C.c_int
@@ -156,6 +189,88 @@ class Engine:
 
     def set_mixed_threshold(self, t):
         self._chk(self.L.m2_set_mixed_threshold(self.h, t))
+
+    def set_profiling(self, on=True):
+        self._chk(self.L.m2_set_profiling(self.h, int(on)))
+
+    def reset_profile(self):
+        self._chk(self.L.m2_reset_profile(self.h))
+
+    def profile(self):
+        """dict combo 'X,Y' -> (ms, launches, quartets, prim_quartets)"""
+        ms = np.zeros(36)
+        la, q, pq = (np.zeros(36, np.int64) for _ in range(3))
+        P = C.POINTER(C.c_longlong)
+        self._chk(self.L.m2_get_profile(self.h, _p(ms), la.ctypes.data_as(P), q.ctypes.data_as(P),
+                                        pq.ctypes.data_as(P)))
+        return {(c // 6, c % 6): (ms[c], int(la[c]), int(q[c]), int(pq[c])) for c in range(36) if la[c] > 0}
+
+    def df_setup(self):
+        self._chk(self.L.m2_df_setup(self.h))
+
+    def jk_df(self, Cocc, fp32=True):
+        """J, K over Cocc Cocc^T (Cocc: n x nocc)."""
+        Cf = np.asfortranarray(Cocc, float)
+        n, nocc = Cf.shape
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        self._chk(self.L.m2_jk_df(self.h, Cf.ctypes.data_as(_dp), nocc, int(fp32), _p(J), _p(K)))
+        return J, K
+
+    def jk_df_device(self, dC_ptr, nocc, dJ_ptr, dK_ptr, fp32=True):
+        self._chk(self.L.m2_jk_df_device(self.h, C.c_void_p(dC_ptr), nocc, int(fp32), C.c_void_p(dJ_ptr),
+                                         C.c_void_p(dK_ptr)))
+
+    def df_timing(self):
+        out = np.zeros(4)
+        self._chk(self.L.m2_df_timing(self.h, _p(out)))
+        return out
+
+    def eigh(self, A):
+        A = np.asfortranarray(A, float)
+        n = A.shape[0]
+        w, V = np.zeros(n), np.zeros((n, n), order="F")
+        self._chk(self.L.m2_eigh(self.h, n, A.ctypes.data_as(_dp), _p(w), V.ctypes.data_as(_dp)))
+        return w, V
+
+    def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, **kw):
+        """Run the host C++ SCF (molecular::SCF::rhf) on `mol`.  Keyword
+        options as SCFOptions (defaults: the reference's RHF defaults)."""
+        o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
+                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        sh = mol.shells(basis)
+        l, npr, ex, cf, cen, at = shell_arrays(sh)
+        if aux:
+            ash = mol.shells(aux)
+            la, npa, exa, cfa, cena, ata = shell_arrays(ash)
+            nsa = len(ash)
+        else:
+            la = npa = ata = np.zeros(1, np.int32)
+            exa = cfa = cena = np.zeros(1)
+            nsa = 0
+        Z = np.asarray(mol.Z, float)
+        X = np.ascontiguousarray(mol.xyz, float)
+        nbf = sum((2 * s.l + 1) if pure else (s.l + 1) * (s.l + 2) // 2 for s in sh)
+        D = np.zeros((nbf, nbf))
+        Cm = np.zeros((nbf, nbf))
+        eps = np.zeros(nbf)
+        log = np.zeros((max(1, o.maxiter), 5))
+        res = SCFResult()
+        ip = lambda a: a.ctypes.data_as(_ip)
+        st = self.L.m2_rhf(self.h, mol.natom, _p(Z), _p(X), mol.charge, mol.multiplicity, int(pure),
+                           len(sh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
+                           nsa, ip(la), ip(npa), _p(exa), _p(cfa), _p(cena), ip(ata),
+                           C.byref(o), C.byref(res), _p(D), _p(Cm), _p(eps), _p(log))
+        if st != 0:
+            raise M2Error(st, self.L.m2_rhf_last_error().decode())
+        self.nbf = nbf
+        self.nsh = len(sh)
+        return {"energy": res.energy, "one_e": res.one_e, "two_e": res.two_e, "enuc": res.enuc,
+                "iterations": res.iterations, "converged": bool(res.converged), "n_full": res.n_full,
+                "n_incremental": res.n_incremental, "n_fp32": res.n_fp32, "jk_seconds": res.jk_seconds,
+                "total_seconds": res.total_seconds, "D": D, "C": Cm.T.copy(), "eps": eps,
+                "log": log[:res.iterations]}
 
     def last_stats(self):
         a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()

@@ -529,21 +529,34 @@ void or_shellblock_norm(const or_basis* b, const double* A, double* out) {
  * reference does).  Q may be NULL (no Schwarz screening, as when the
  * reference's Schwarz matrix is empty).  nthreads <= 0: all.
  * Returns the number of quartets computed. */
-long or_jk_direct(const or_basis* b, const double* D, const double* Q, double precision,
-                  double multiplier, double* J, double* K, int nthreads) {
+/* Quartets with s1 in [s1lo, s1hi) only (a bounded sample of one build when
+ * the range is partial); counts[0] += quartets, counts[1] += primitive
+ * quartets.  OpenMP over the (s1, s2) pairs of the range. */
+long or_jk_direct_range(const or_basis* b, const double* D, const double* Q, double precision,
+                        double multiplier, double* J, double* K, int nthreads, int s1lo, int s1hi,
+                        long* counts) {
     const int nsh = b->nsh, n = b->nbf;
     const int do_screen = (Q != NULL);
     double* Dn = (double*)malloc(sizeof(double) * nsh * nsh);
     or_shellblock_norm(b, D, Dn);
-    long count = 0;
+    long count = 0, pcount = 0;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #else
     nthreads = 1;
 #endif
+    if (s1lo < 0) s1lo = 0;
+    if (s1hi > nsh) s1hi = nsh;
+    /* flat list of (s1, s2) pairs of the range, heaviest first */
+    long npairs = 0;
+    for (int s1 = s1lo; s1 < s1hi; ++s1) npairs += s1 + 1;
+    int* pl = (int*)malloc(sizeof(int) * 2 * (npairs > 0 ? npairs : 1));
+    long k = 0;
+    for (int s1 = s1hi - 1; s1 >= s1lo; --s1)
+        for (int s2 = 0; s2 <= s1; ++s2) { pl[2 * k] = s1; pl[2 * k + 1] = s2; ++k; }
     double* Jt = (double*)calloc((size_t)nthreads * n * n, sizeof(double));
     double* Kt = (double*)calloc((size_t)nthreads * n * n, sizeof(double));
-#pragma omp parallel num_threads(nthreads) reduction(+ : count)
+#pragma omp parallel num_threads(nthreads) reduction(+ : count, pcount)
     {
         int tid = 0;
 #ifdef _OPENMP
@@ -555,41 +568,41 @@ long or_jk_direct(const or_basis* b, const double* D, const double* Q, double pr
         double* buf = (double*)malloc(sizeof(double) * 50625 * 4);
         double* tmp = (double*)malloc(sizeof(double) * 50625 * 4);
 #pragma omp for schedule(dynamic)
-        for (int s1 = nsh - 1; s1 >= 0; --s1) {
+        for (long pi = 0; pi < npairs; ++pi) {
+            const int s1 = pl[2 * pi], s2 = pl[2 * pi + 1];
             const or_shell* S1 = &b->sh[s1];
-            for (int s2 = 0; s2 <= s1; ++s2) {
-                const or_shell* S2 = &b->sh[s2];
-                double Dn12 = do_screen ? Dn[s1 * nsh + s2] : 0.0;
-                for (int s3 = 0; s3 <= s1; ++s3) {
-                    const or_shell* S3 = &b->sh[s3];
-                    double Dn123 = do_screen ? fmax(Dn[s1 * nsh + s3], fmax(Dn[s2 * nsh + s3], Dn12)) : 0.0;
-                    int s4max = (s1 == s3) ? s2 : s3;
-                    for (int s4 = 0; s4 <= s4max; ++s4) {
-                        double Dn1234 = do_screen ? fmax(Dn[s1 * nsh + s4], fmax(Dn[s2 * nsh + s4],
-                                                         fmax(Dn[s3 * nsh + s4], Dn123))) : 0.0;
-                        if (do_screen && Dn1234 * Q[s1 * nsh + s2] * Q[s3 * nsh + s4] < precision) continue;
-                        const or_shell* S4 = &b->sh[s4];
-                        double deg = (s1 == s2 ? 1.0 : 2.0) * (s3 == s4 ? 1.0 : 2.0) *
-                                     ((s1 == s3) ? (s2 == s4 ? 1.0 : 2.0) : 2.0);
-                        eri_quartet(S1, S2, S3, S4, &w, buf, tmp);
-                        ++count;
-                        int n1 = S1->nbf, n2 = S2->nbf, n3 = S3->nbf, n4 = S4->nbf;
-                        for (int f1 = 0, f = 0; f1 < n1; ++f1) {
-                            int b1 = S1->bf + f1;
-                            for (int f2 = 0; f2 < n2; ++f2) {
-                                int b2 = S2->bf + f2;
-                                for (int f3 = 0; f3 < n3; ++f3) {
-                                    int b3 = S3->bf + f3;
-                                    for (int f4 = 0; f4 < n4; ++f4, ++f) {
-                                        int b4 = S4->bf + f4;
-                                        double v = buf[f] * deg;
-                                        Jl[b1 * n + b2] += multiplier * D[b3 * n + b4] * v;
-                                        Jl[b3 * n + b4] += multiplier * D[b1 * n + b2] * v;
-                                        Kl[b1 * n + b3] += 0.5 * multiplier * D[b2 * n + b4] * v;
-                                        Kl[b2 * n + b4] += 0.5 * multiplier * D[b1 * n + b3] * v;
-                                        Kl[b1 * n + b4] += 0.5 * multiplier * D[b2 * n + b3] * v;
-                                        Kl[b2 * n + b3] += 0.5 * multiplier * D[b1 * n + b4] * v;
-                                    }
+            const or_shell* S2 = &b->sh[s2];
+            double Dn12 = do_screen ? Dn[s1 * nsh + s2] : 0.0;
+            for (int s3 = 0; s3 <= s1; ++s3) {
+                const or_shell* S3 = &b->sh[s3];
+                double Dn123 = do_screen ? fmax(Dn[s1 * nsh + s3], fmax(Dn[s2 * nsh + s3], Dn12)) : 0.0;
+                int s4max = (s1 == s3) ? s2 : s3;
+                for (int s4 = 0; s4 <= s4max; ++s4) {
+                    double Dn1234 = do_screen ? fmax(Dn[s1 * nsh + s4], fmax(Dn[s2 * nsh + s4],
+                                                     fmax(Dn[s3 * nsh + s4], Dn123))) : 0.0;
+                    if (do_screen && Dn1234 * Q[s1 * nsh + s2] * Q[s3 * nsh + s4] < precision) continue;
+                    const or_shell* S4 = &b->sh[s4];
+                    double deg = (s1 == s2 ? 1.0 : 2.0) * (s3 == s4 ? 1.0 : 2.0) *
+                                 ((s1 == s3) ? (s2 == s4 ? 1.0 : 2.0) : 2.0);
+                    eri_quartet(S1, S2, S3, S4, &w, buf, tmp);
+                    ++count;
+                    pcount += (long)S1->nprim * S2->nprim * S3->nprim * S4->nprim;
+                    int n1 = S1->nbf, n2 = S2->nbf, n3 = S3->nbf, n4 = S4->nbf;
+                    for (int f1 = 0, f = 0; f1 < n1; ++f1) {
+                        int b1 = S1->bf + f1;
+                        for (int f2 = 0; f2 < n2; ++f2) {
+                            int b2 = S2->bf + f2;
+                            for (int f3 = 0; f3 < n3; ++f3) {
+                                int b3 = S3->bf + f3;
+                                for (int f4 = 0; f4 < n4; ++f4, ++f) {
+                                    int b4 = S4->bf + f4;
+                                    double v = buf[f] * deg;
+                                    Jl[b1 * n + b2] += multiplier * D[b3 * n + b4] * v;
+                                    Jl[b3 * n + b4] += multiplier * D[b1 * n + b2] * v;
+                                    Kl[b1 * n + b3] += 0.5 * multiplier * D[b2 * n + b4] * v;
+                                    Kl[b2 * n + b4] += 0.5 * multiplier * D[b1 * n + b3] * v;
+                                    Kl[b1 * n + b4] += 0.5 * multiplier * D[b2 * n + b3] * v;
+                                    Kl[b2 * n + b3] += 0.5 * multiplier * D[b1 * n + b4] * v;
                                 }
                             }
                         }
@@ -607,8 +620,14 @@ long or_jk_direct(const or_basis* b, const double* D, const double* Q, double pr
             J[i * n + j] = 0.25 * (Jt[i * n + j] + Jt[j * n + i]);
             K[i * n + j] = 0.25 * (Kt[i * n + j] + Kt[j * n + i]);
         }
-    free(Jt); free(Kt); free(Dn);
+    free(Jt); free(Kt); free(Dn); free(pl);
+    if (counts) { counts[0] += count; counts[1] += pcount; }
     return count;
+}
+
+long or_jk_direct(const or_basis* b, const double* D, const double* Q, double precision,
+                  double multiplier, double* J, double* K, int nthreads) {
+    return or_jk_direct_range(b, D, Q, precision, multiplier, J, K, nthreads, 0, b->nsh, NULL);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -56,6 +56,9 @@ def lib():
         L.or_shellblock_norm.argtypes = [C.c_void_p, _dp, _dp]
         L.or_jk_direct.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.c_int]
         L.or_jk_direct.restype = C.c_long
+        L.or_jk_direct_range.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                         C.c_int, C.c_int, C.POINTER(C.c_long)]
+        L.or_jk_direct_range.restype = C.c_long
         L.or_eri_full.argtypes = [C.c_void_p, _dp]
         L.or_jk_incore.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp]
         L.or_one_body.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -140,6 +143,16 @@ class Basis:
         n = lib().or_jk_direct(self.h, _p(D), Qp, precision, multiplier, _p(J), _p(K), nthreads)
         self.last_quartets = n
         return J, K
+
+    def jk_direct_range(self, D, Q, s1lo, s1hi, precision=1e-12, nthreads=0):
+        """Bounded sample of one build: quartets with s1 in [s1lo, s1hi).
+        Returns (quartets, primitive quartets)."""
+        D = np.ascontiguousarray(D, float)
+        Qa = np.ascontiguousarray(Q, float)
+        J, K = np.zeros_like(D), np.zeros_like(D)
+        cnt = (C.c_long * 2)(0, 0)
+        lib().or_jk_direct_range(self.h, _p(D), _p(Qa), precision, 1.0, _p(J), _p(K), nthreads, s1lo, s1hi, cnt)
+        return int(cnt[0]), int(cnt[1])
 
     def eri_full(self) -> np.ndarray:
         n = self.nbf
