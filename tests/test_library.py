@@ -1,0 +1,125 @@
+"""CPU-side checks of the native library and its host logic (no GPU calls)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "molecular2_amd", "csrc"))
+
+
+def _declared_symbols():
+    syms = set()
+    for name in os.listdir(os.path.join(ROOT, "include")):
+        if name.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", name)).read()
+            syms |= set(re.findall(r"\b(m2_[a-z0-9_]+)\s*\(", text))
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    from molecular2_amd import engine
+    if not os.path.exists(engine.LIBPATH):
+        pytest.skip("libm2.so not built")
+    lib = ctypes.CDLL(engine.LIBPATH)          # loads without touching the GPU
+    syms = _declared_symbols()
+    assert len(syms) >= 20
+    for s in sorted(syms):
+        assert hasattr(lib, s), s
+    assert hasattr(lib, "m2_rhf")
+
+
+def test_engine_fails_loudly_without_library(monkeypatch):
+    from molecular2_amd import engine
+    monkeypatch.setattr(engine, "LIBPATH", "/nonexistent/libm2.so")
+    monkeypatch.setattr(engine, "_lib", None)
+    with pytest.raises(RuntimeError, match="not built"):
+        engine.lib()
+
+
+def test_solid_harmonics_orthonormal_generator():
+    import gen_eri
+    for l in range(5):
+        rows = np.array(gen_eri.solid_harmonics(l))
+        cs = gen_eri.comps(l)
+        G = np.zeros((len(cs), len(cs)))
+        for i, a in enumerate(cs):
+            for j, b in enumerate(cs):
+                if all((a[k] + b[k]) % 2 == 0 for k in range(3)):
+                    G[i, j] = np.prod([gen_eri.dfact(a[k] + b[k] - 1) for k in range(3)]) / gen_eri.dfact(2 * l - 1)
+        np.testing.assert_allclose(rows @ G @ rows.T, np.eye(2 * l + 1), atol=1e-13)
+
+
+def test_generator_programs_are_level_consistent():
+    """Every instruction of a heavy-class program reads only slots written at
+    an earlier level (or the base / accumulator slots), and no level writes a
+    slot it also reads."""
+    import gen_eri
+    for cls in gen_eri.HEAVY:
+        for pure in (0, 1):
+            P = gen_eri.gen_program(*cls, pure)
+            written = set(range(P["base"], P["base"] + P["L"] + 1))
+            for lvl in P["vrr_levels"]:
+                dsts = {d for d, _ in lvl}
+                for d, t in lvl:
+                    for src, sid, c in t:
+                        assert src in written and src not in dsts
+                written |= dsts
+            live = {a for a, _ in P["acc"]}
+            for lvl in P["post_levels"]:
+                dsts = {d for d, _ in lvl}
+                for d, t in lvl:
+                    for src, sid, c in t:
+                        assert src not in dsts
+                live |= dsts
+            assert len(P["out"]) == np.prod(P["sizes"])
+            assert P["nslots"] * 8 <= 64 * 1024
+
+
+def test_flop_model_committed_matches_generator():
+    import json
+    import gen_eri
+    with open(os.path.join(ROOT, "molecular2_amd", "flop_model.json")) as fh:
+        model = json.load(fh)["classes"]
+    for key, v in model.items():
+        X, Y = map(int, key.split(","))
+        la, lb = gen_eri.PAIR_CLASSES[X]
+        lc, ld = gen_eri.PAIR_CLASSES[Y]
+        f = gen_eri.flop_model(la, lb, lc, ld)
+        assert f["f_prim"] == v["f_prim"] and f["f_cont_pure"] == v["f_cont_pure"]
+
+
+def test_water_cluster_generator():
+    from molecular2_amd import molecules as M
+    a = M.water_cluster(20, seed=20)
+    b = M.water_cluster(20, seed=20)
+    np.testing.assert_array_equal(a.xyz, b.xyz)
+    assert a.natom == 60 and a.nel == 200
+    shells = a.shells("cc-pvdz")
+    assert len(shells) == 240 and M.nbf_of(shells, True) == 480
+    assert M.nbf_of(a.shells("cc-pvdz-jkfit"), True) == 2320
+    # intermolecular O..H, H..H >= 1 A
+    xyz = a.xyz / M.TOBOHR
+    for i in range(60):
+        for j in range(i + 1, 60):
+            if i // 3 != j // 3 and not (a.Z[i] == 8 and a.Z[j] == 8):
+                assert np.linalg.norm(xyz[i] - xyz[j]) >= 1.0
+
+
+def test_random_molecules_are_closed_shell():
+    from molecular2_amd import molecules as M
+    for m in M.random_batch(20, seed=3):
+        assert m.nel % 2 == 0 and 3 <= m.natom <= 30
+    c = M.random_chain(12, seed=12)
+    assert c.natom == 12 and c.nel % 2 == 0
+
+
+def test_missing_basis_raises_basislib():
+    from molecular2_amd import molecules as M
+    mol = M.Molecule([8], np.zeros((1, 3)))
+    with pytest.raises(KeyError, match="BASISLIB"):
+        mol.shells("nonexistent-basis")
