@@ -111,6 +111,17 @@ m2_status m2_jk_df_device(m2_ctx* ctx, const double* dCocc, int nocc, int fp32, 
  * K = X^T X, of the J part, and their sum */
 m2_status m2_df_timing(m2_ctx* ctx, double* ms4);
 
+/* RHF nuclear gradient dE/dR (natom x 3, Eh/bohr): Fock::compute_forces
+ * (src/rhfgrad.cpp:15-89) = D:(dT + dV) - 2 W:dS + 1/4 G1[D]:D + dE_nuc,
+ * with the two-electron part of compute_2body_fock_deriv<1>
+ * (src/rhfgrad.cpp:418-598) computed as a derivative-integral adjoint on the
+ * device (integrals recomputed, never stored).  D = 2 C_occ C_occ^T and
+ * W = C_occ diag(eps_occ) C_occ^T (n x n); quartets screened like the
+ * reference's gradient loop (overlap shell-pair list, Schwarz x |D| against
+ * `precision`).  Z / xyz are the nuclear charges and positions (bohr). */
+m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, const double* D,
+                  const double* W, double precision, double* g);
+
 /* Dense FP64 linear algebra on the device for the SCF driver (host buffers,
  * column-major).  m2_eigh replaces Eigen::SelfAdjointEigenSolver in
  * Fock::formOrthog / Fock::diagonalise (src/fock.cpp:124-142,228-233):

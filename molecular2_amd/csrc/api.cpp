@@ -163,6 +163,7 @@ m2_status build_pairs(m2_ctx* ctx) {
                         q.P[k] = (a * A.O[k] + bb * B.O[k]) / p;
                         q.PA[k] = q.P[k] - A.O[k];
                     }
+                    q.a = a;
                     pr.pp.push_back(q);
                 }
             pairs.push_back(std::move(pr));
@@ -285,7 +286,7 @@ struct ComboRange {
 // heavy classes take HEAVY_CHUNK (one wavefront walks them in turn).
 constexpr int HEAVY_CHUNK = 4;
 
-std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed) {
+std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed, bool gradient = false) {
     std::vector<ComboRange> ranges;
     auto& T = ctx->h_tasks;
     T.clear();
@@ -295,7 +296,10 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
         const PairClassSeg& sx = ctx->cls[X];
         for (int Y = 0; Y <= X; ++Y) {
             const PairClassSeg& sy = ctx->cls[Y];
-            const int chunk = JK_COMBO[X][Y] ? 64 : HEAVY_CHUNK;
+            // gradient: the five smallest classes run one quartet per lane
+            const bool reg = gradient ? ((X <= 1 && Y <= X) || (X == 2 && Y == 0) || (X == 3 && Y == 0))
+                                      : JK_COMBO[X][Y] != nullptr;
+            const int chunk = reg ? 64 : HEAVY_CHUNK;
             ComboRange r{X, Y, (int)T.size(), 0, 0};
             t32.clear();
             for (int i = sx.begin; i < sx.end; ++i) {
@@ -410,6 +414,15 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
 }  // namespace
 
 namespace m2 {
+struct ComboRangeG {
+    int X, Y, t0, tmid, t1;
+};
+std::vector<ComboRangeG> grad_tasks(m2_ctx* ctx, double precision, double Dmax) {
+    std::vector<ComboRange> r = build_tasks(ctx, precision, Dmax, false, true);
+    std::vector<ComboRangeG> out;
+    for (const auto& x : r) out.push_back(ComboRangeG{x.X, x.Y, x.t0, x.tmid, x.t1});
+    return out;
+}
 struct OneBodyArgs {
     const double* sh_O;
     const int* sh_l;

@@ -110,6 +110,7 @@ m2_status df_setup(m2_ctx* ctx) {
                 q.P[d] = ab.sh[s].O[d];
                 q.PA[d] = 0.0;
             }
+            q.a = a;
             pp.push_back(q);
         }
     }

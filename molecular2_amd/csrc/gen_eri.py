@@ -419,6 +419,8 @@ def main(outdir):
     written += write_df_tus(outdir)
     written.append(write_sph_tables(outdir))
     written.append(write_programs(outdir))
+    written.append(write_deriv_programs(outdir))
+    written.append(write_deriv_register(outdir))
     written.append(write_flop_model(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                                  "flop_model.json")))
     return written
@@ -855,6 +857,494 @@ def write_programs(outdir, classes=None):
     lines.append("}  // namespace m2")
     text = "\n".join(lines) + "\n"
     path = os.path.join(outdir, "programs.inc")
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as fh:
+            fh.write(text)
+    return path
+
+
+# ---------------------------------------------------------------------------
+# Derivative (gradient adjoint) programs
+# ---------------------------------------------------------------------------
+DSID = dict(SID, wA=24, wB=25, wC=26)
+NSCALAR_D = 27
+
+
+def _alloc_levels(levels, reads_after, pinned, first_free, preset=None):
+    """Slot allocation with reuse for instruction levels.
+    levels: list of lists of (key, [(src_key, sid, coef)])
+    reads_after: extra {key: last level index} uses outside the instruction
+    lists (e.g. accumulation right after the level that defines the key)
+    pinned: keys whose slots are never freed
+    preset: {key: slot} already allocated (inputs)
+    Returns (slot map, allocated instruction levels, top slot)."""
+    import heapq
+    last = dict(reads_after)
+    for li, ins in enumerate(levels):
+        for _, t in ins:
+            for src, _, _ in t:
+                last[src] = max(last.get(src, -1), li)
+    where = dict(preset or {})
+    free = []
+    top = first_free
+    out = []
+    for li, ins in enumerate(levels):
+        rows = []
+        for key, t in ins:
+            if free:
+                d = heapq.heappop(free)
+            else:
+                d = top
+                top += 1
+            where[key] = d
+            rows.append((d, [(where[src], sid, c) for src, sid, c in t]))
+        out.append(rows)
+        for v, lu in list(last.items()):
+            if lu == li and v not in pinned and v in where:
+                heapq.heappush(free, where[v])
+                del last[v]
+    return where, out, top
+
+
+def gen_deriv_program(la, lb, lc, ld):
+    """Cartesian derivative-adjoint program of class (la lb|lc ld).
+
+    VRR per primitive quartet to the nodes needed by
+      S0 = sum w [e|f]       e in [la-1, la+lb],   f in [lc-1, lc+ld]
+      SA = sum 2a w [e|f]    e in [la+1, la+lb+1], f in [lc, lc+ld]
+      SB = sum 2b w [e|f]    e in [la, la+lb+1],   f in [lc, lc+ld]
+      SC = sum 2c w [e|f]    e in [la, la+lb],     f in [lc+1, lc+ld+1]
+    (accumulated right after the VRR level that produces each node), then
+    HRR to the shifted Cartesian blocks (la+-1, ...), (la, lb+-1, ...),
+    (..., lc+-1, ld), and a contraction list
+      g[3*X + i] += coef * Gamma[abcd] * X[slot]
+    with dX_i (ab|cd) = 2x (..x+1_i..) - x_i (..x-1_i..)."""
+    Lp = la + lb + lc + ld + 1
+
+    def rng(lo, hi):
+        return [c for L_ in range(max(lo, 0), hi + 1) for c in comps(L_)]
+
+    sets = {
+        "S0": (rng(la - 1, la + lb), rng(lc - 1, lc + ld), "one"),
+        "SA": (rng(la + 1, la + lb + 1), rng(lc, lc + ld), "wA"),
+        "SB": (rng(la, la + lb + 1), rng(lc, lc + ld), "wB"),
+        "SC": (rng(la, la + lb), rng(lc + 1, lc + ld + 1), "wC"),
+    }
+    targets = set()
+    for E, F, _ in sets.values():
+        for e in E:
+            for f in F:
+                targets.add((e, f))
+    order, nodes = build_vrr(sorted(targets), Lp)
+    # accumulator slots first (pinned)
+    acc_slot = {}
+    k = 0
+    for name, (E, F, _) in sets.items():
+        for e in E:
+            for f in F:
+                acc_slot[(name, e, f)] = k
+                k += 1
+    nacc = k
+    base = nacc
+    preset = {((0, 0, 0), (0, 0, 0), m): base + m for m in range(Lp + 1)}
+    levels = [[] for _ in range(Lp + 1)]
+    for key in order:
+        e, f, m = key
+        kind = nodes[key]
+        if kind[0] == "base":
+            continue
+        _, terms, extra = kind
+        t = [(src, DSID[c], 1.0) for c, src in terms]
+        for x in extra:
+            if x[0] == "ff":
+                t += [(x[2], DSID["inv2q"], float(x[1])), (x[3], DSID["inv2q_rho_q"], -float(x[1]))]
+            elif x[0] == "ee":
+                t += [(x[2], DSID["inv2p"], float(x[1])), (x[3], DSID["inv2p_rho_p"], -float(x[1]))]
+            elif x[0] == "ef":
+                t.append((x[2], DSID["inv2pq"], float(x[1])))
+        levels[sum(e) + sum(f)].append((key, t))
+    # accumulation after the level of each target node
+    acc_lists = [[] for _ in range(Lp + 1)]
+    reads_after = {}
+    for name, (E, F, w) in sets.items():
+        for e in E:
+            for f in F:
+                lv = sum(e) + sum(f)
+                acc_lists[lv].append((acc_slot[(name, e, f)], (e, f, 0), DSID[w]))
+                reads_after[(e, f, 0)] = max(reads_after.get((e, f, 0), -1), lv)
+    where, vl, top = _alloc_levels(levels, reads_after, set(), base + Lp + 1, preset)
+    acc_ins = [[(a, where[key], sid) for a, key, sid in lst] for lst in acc_lists]
+    # drop empty leading levels but keep level/acc pairing
+    vrr_levels, acc_levels = [], []
+    for li in range(Lp + 1):
+        if vl[li] or acc_ins[li]:
+            vrr_levels.append(vl[li])
+            acc_levels.append(acc_ins[li])
+    vrr_top = top
+    # post phase: six shifted blocks
+    post_sym = []
+    blocks = {}
+
+    def hrr_block(name, sa, sb, sc, sd):
+        """symbolic HRR from accumulator set `name` to Cartesian (sa sb|sc sd)"""
+        E = rng(sa, sa + sb)
+        F = rng(sc, sc + sd)
+        lv_k = []
+
+        def kget(e, c, d):
+            return ("acc", name, e, c) if sum(d) == 0 else ("k", name, sa, sb, sc, sd, e, c, d)
+        for lev in range(1, sd + 1):
+            ins = []
+            for e in E:
+                for Lc in range(sc, sc + sd - lev + 1):
+                    for c in comps(Lc):
+                        for d in comps(lev):
+                            i = next(q for q in range(3) if d[q] > 0)
+                            ins.append((kget(e, c, d), [(kget(e, add(c, i), sub(d, i)), DSID["one"], 1.0),
+                                                        (kget(e, c, sub(d, i)), DSID["CD" + AX[i]], 1.0)]))
+            lv_k.append(ins)
+
+        def bget(a, b, c, d):
+            return kget(a, c, d) if sum(b) == 0 else ("b", name, sa, sb, sc, sd, a, b, c, d)
+        lv_b = []
+        for lev in range(1, sb + 1):
+            ins = []
+            for La in range(sa, sa + sb - lev + 1):
+                for a in comps(La):
+                    for b in comps(lev):
+                        for c in comps(sc):
+                            for d in comps(sd):
+                                i = next(q for q in range(3) if b[q] > 0)
+                                ins.append((bget(a, b, c, d), [(bget(add(a, i), sub(b, i), c, d), DSID["one"], 1.0),
+                                                               (bget(a, sub(b, i), c, d), DSID["AB" + AX[i]], 1.0)]))
+            lv_b.append(ins)
+        vals = {}
+        for a in comps(sa):
+            for b in comps(sb):
+                for c in comps(sc):
+                    for d in comps(sd):
+                        vals[(a, b, c, d)] = bget(a, b, c, d)
+        return lv_k, lv_b, vals
+
+    terms = [("A+", "SA", (la + 1, lb, lc, ld), 0, +1), ("B+", "SB", (la, lb + 1, lc, ld), 1, +1),
+             ("C+", "SC", (la, lb, lc + 1, ld), 2, +1)]
+    if la > 0:
+        terms.append(("A-", "S0", (la - 1, lb, lc, ld), 0, -1))
+    if lb > 0:
+        terms.append(("B-", "S0", (la, lb - 1, lc, ld), 1, -1))
+    if lc > 0:
+        terms.append(("C-", "S0", (la, lb, lc - 1, ld), 2, -1))
+    # terms one after another (each term's HRR levels, then the next term):
+    # only one term's intermediates are live at a time
+    seq = []
+    for nm, accn, cls, ctr, sgn in terms:
+        lv_k, lv_b, vals = hrr_block(accn, *cls)
+        blocks[nm] = (cls, ctr, sgn, vals)
+        seq.extend(lv_k)
+        seq.extend(lv_b)
+    # dedupe identical symbolic nodes (S0 blocks share intermediates)
+    post_sym = []
+    seen = set()
+    for lv in seq:
+        ins = []
+        for key, t in lv:
+            if key in seen:
+                continue
+            seen.add(key)
+            ins.append((key, t))
+        post_sym.append(ins)
+    # contraction list (reads block values after the post phase)
+    gam = {}
+    ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
+    for ia, a in enumerate(ca):
+        for ib, b in enumerate(cb):
+            for ic, c in enumerate(cc):
+                for idd, d in enumerate(cd):
+                    gam[(a, b, c, d)] = ((ia * len(cb) + ib) * len(cc) + ic) * len(cd) + idd
+    contr = []
+    for nm, (cls, ctr, sgn, vals) in blocks.items():
+        for (a, b, c, d), key in vals.items():
+            quad = [a, b, c, d]
+            for i in range(3):
+                q = list(quad[ctr])
+                if sgn > 0:
+                    if q[i] == 0:
+                        continue
+                    q[i] -= 1
+                    coef = 1.0
+                else:
+                    q[i] += 1
+                    coef = -float(q[i])
+                src = list(quad)
+                src[ctr] = tuple(q)
+                contr.append((key, gam[tuple(src)], coef, 3 * ctr + i))
+    # contract every block value right after the post level that produces
+    # it (accumulator-sourced values before the first level), so its slot
+    # can be reused by later levels
+    defined = {}
+    for li, lv in enumerate(post_sym):
+        for key, _ in lv:
+            defined[key] = li
+    preset_post = {("acc",) + tuple(k): v for k, v in acc_slot.items()}
+    reads = {}
+    for key, g, c, x in contr:
+        if key in defined:
+            reads[key] = max(reads.get(key, -1), defined[key])
+    where2, pl, top2 = _alloc_levels(post_sym, reads, set(preset_post.keys()), nacc, preset_post)
+    contr_levels = [[] for _ in range(len(post_sym) + 1)]   # [0]: before level 0
+    for key, g, c, x in contr:
+        li = defined.get(key, -1)
+        contr_levels[li + 1].append((where2[key], g, c, x))
+    return {"L": Lp, "nslots": max(vrr_top, top2), "base": base, "nacc": nacc,
+            "vrr_levels": vrr_levels, "acc_levels": acc_levels, "post_levels": pl,
+            "contr_levels": contr_levels, "contr": [e for lv in contr_levels for e in lv], "ngam": len(gam),
+            "sizes": [len(ca), len(cb), len(cc), len(cd)]}
+
+
+def write_deriv_programs(outdir):
+    """All 21 orbital class combinations (bra class >= ket class)."""
+    ins_i, ins_c, lev, accs, contr_i, contr_c, desc = [], [], [], [], [], [], []
+    for X, (la, lb) in enumerate(PAIR_CLASSES):
+        for Y in range(X + 1):
+            lc, ld = PAIR_CLASSES[Y]
+            P = gen_deriv_program(la, lb, lc, ld)
+            d = {"la": la, "lb": lb, "lc": lc, "ld": ld, "L": P["L"], "nslots": P["nslots"], "base": P["base"],
+                 "nacc": P["nacc"], "lev_off": len(lev), "nvlev": len(P["vrr_levels"])}
+            # per VRR level: instruction range, then accumulation range
+            for lv, al in zip(P["vrr_levels"], P["acc_levels"]):
+                lev.append(len(ins_c) // MAXTERMS)
+                for dst, t in lv:
+                    ins_i += [dst, len(t)] + [x[0] for x in t] + [0] * (MAXTERMS - len(t)) + \
+                        [x[1] for x in t] + [0] * (MAXTERMS - len(t))
+                    ins_c += [x[2] for x in t] + [0.0] * (MAXTERMS - len(t))
+                lev.append(len(ins_c) // MAXTERMS)
+                lev.append(len(accs) // 3)
+                for a_, v_, sid in al:
+                    accs += [a_, v_, sid]
+                lev.append(len(accs) // 3)
+            # post phase: contraction list before level 0, then for each
+            # level its instruction range and its contraction range
+            d["post_off"] = len(lev)
+            d["nplev"] = len(P["post_levels"])
+            cl = P["contr_levels"]
+            lev.append(len(contr_c))
+            for slot, g, c, x in cl[0]:
+                contr_i += [slot, g, x]
+                contr_c.append(c)
+            lev.append(len(contr_c))
+            for li, lv in enumerate(P["post_levels"]):
+                lev.append(len(ins_c) // MAXTERMS)
+                for dst, t in lv:
+                    ins_i += [dst, len(t)] + [x[0] for x in t] + [0] * (MAXTERMS - len(t)) + \
+                        [x[1] for x in t] + [0] * (MAXTERMS - len(t))
+                    ins_c += [x[2] for x in t] + [0.0] * (MAXTERMS - len(t))
+                lev.append(len(ins_c) // MAXTERMS)
+                lev.append(len(contr_c))
+                for slot, g, c, x in cl[li + 1]:
+                    contr_i += [slot, g, x]
+                    contr_c.append(c)
+                lev.append(len(contr_c))
+            d["contr_off"] = 0
+            d["ncontr"] = len(P["contr"])
+            d["na"], d["nb"], d["nc"], d["nd"] = P["sizes"]
+            desc.append(d)
+    keys = ["la", "lb", "lc", "ld", "L", "nslots", "base", "nacc", "lev_off", "nvlev", "post_off", "nplev",
+            "contr_off", "ncontr", "na", "nb", "nc", "nd"]
+    lines = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.", "#pragma once", "namespace m2 {",
+             "constexpr int DPROG_NCLASS = %d;" % len(desc),
+             "constexpr int DPROG_DESC_FIELDS = %d;" % len(keys),
+             "// fields: " + " ".join(keys),
+             "constexpr int DPROG_DESC[%d] = {" % (len(desc) * len(keys))]
+    for d in desc:
+        lines.append("  " + ", ".join(str(d[k]) for k in keys) + ",")
+    lines.append("};")
+
+    def arr(name, typ, vals, per=16):
+        lines.append("constexpr %s %s[%d] = {" % (typ, name, max(1, len(vals))))
+        for i in range(0, len(vals), per):
+            lines.append("  " + ", ".join(repr(v) if typ == "double" else str(v) for v in vals[i:i + per]) + ",")
+        if not vals:
+            lines.append("  0")
+        lines.append("};")
+
+    arr("DPROG_INS", "int", ins_i)
+    arr("DPROG_COEF", "double", [float(x) for x in ins_c], 8)
+    arr("DPROG_LEV", "int", lev)
+    arr("DPROG_ACC", "int", accs)
+    arr("DPROG_CONTR", "int", contr_i)
+    arr("DPROG_CCOEF", "double", [float(x) for x in contr_c], 8)
+    lines.append("}  // namespace m2")
+    text = "\n".join(lines) + "\n"
+    path = os.path.join(outdir, "deriv_programs.inc")
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as fh:
+            fh.write(text)
+    return path
+
+
+DERIV_REGISTER = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0)]
+
+
+def gen_deriv_class(la, lb, lc, ld):
+    """Register-resident derivative adjoint of one small class:
+    EriDeriv<la,lb,lc,ld>::run<R>(..., Gam, g) adds to g[0..8] the
+    contraction of d/dA, d/dB, d/dC (ab|cd) (Cartesian) with Gam."""
+    Lp = la + lb + lc + ld + 1
+
+    def rng(lo, hi):
+        return [c for L_ in range(max(lo, 0), hi + 1) for c in comps(L_)]
+
+    sets = {
+        "s0": (rng(la - 1, la + lb), rng(lc - 1, lc + ld), None),
+        "sa": (rng(la + 1, la + lb + 1), rng(lc, lc + ld), "wA"),
+        "sb": (rng(la, la + lb + 1), rng(lc, lc + ld), "wB"),
+        "sc": (rng(la, la + lb), rng(lc + 1, lc + ld + 1), "wC"),
+    }
+    targets = sorted({(e, f) for E, F, _ in sets.values() for e in E for f in F})
+    order, nodes = build_vrr(targets, Lp)
+    em = Emitter()
+    em.out("template <> struct EriDeriv<%d, %d, %d, %d> {" % (la, lb, lc, ld))
+    em.out("  template <typename R>")
+    em.out("  static __device__ __forceinline__ void run(const PrimPair* __restrict__ bra, int nbra,")
+    em.out("      const PrimPair* __restrict__ ket, int nket, const R ABx, const R ABy, const R ABz,")
+    em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, const R* __restrict__ Gam, R* __restrict__ g) {")
+    for nm, (E, F, _) in sets.items():
+        for e in E:
+            for f in F:
+                em.out("    R %s_%s_%s = R(0);" % (nm, cname(e), cname(f)))
+    em.out("    for (int ik = 0; ik < nket; ++ik) {")
+    em.out("      const PrimPair kp = ket[ik];")
+    em.out("      const R q = R(kp.p), Kcd = R(kp.K), wC = R(2) * R(kp.a);")
+    em.out("      const R Qx = R(kp.P[0]), Qy = R(kp.P[1]), Qz = R(kp.P[2]);")
+    em.out("      const R inv2q = R(kp.inv2p);")
+    em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
+    em.out("      for (int ib = 0; ib < nbra; ++ib) {")
+    em.out("        const PrimPair bp = bra[ib];")
+    em.out("        const R p = R(bp.p), Kab = R(bp.K), wA = R(2) * R(bp.a), wB = R(2) * (R(bp.p) - R(bp.a));")
+    em.out("        const R inv2p = R(bp.inv2p);")
+    em.out("        const R PAx = R(bp.PA[0]), PAy = R(bp.PA[1]), PAz = R(bp.PA[2]);")
+    em.out("        const R dx = R(bp.P[0]) - Qx, dy = R(bp.P[1]) - Qy, dz = R(bp.P[2]) - Qz;")
+    em.out("        const R rs_pq = sqrt_inv<R>(p + q);")
+    em.out("        const R inv_pq = rs_pq * rs_pq;")
+    em.out("        const R rho = p * q * inv_pq;")
+    em.out("        const R T = rho * (dx * dx + dy * dy + dz * dz);")
+    em.out("        R Fm[%d];" % (Lp + 1))
+    em.out("        BoysEval<R, %d>::run(T, Fm, tab);" % Lp)
+    em.out("        const R pref = Kab * Kcd * rs_pq;")
+    em.out("#pragma unroll")
+    em.out("        for (int m = 0; m <= %d; ++m) Fm[m] *= pref;" % Lp)
+    em.out("        const R rho_p = q * inv_pq, rho_q = p * inv_pq, inv2pq = R(0.5) * inv_pq;")
+    em.out("        const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
+    em.out("        const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
+    emit_vrr(em, order, nodes, "        ")
+    for nm, (E, F, w) in sets.items():
+        for e in E:
+            for f in F:
+                v = vrr_name(e, f, 0)
+                em.out("        %s_%s_%s += %s;" % (nm, cname(e), cname(f), v if w is None else "%s * %s" % (w, v)))
+    em.out("      }")
+    em.out("    }")
+    # post: HRR per term, then contraction
+    ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
+    gidx = {}
+    for ia, a in enumerate(ca):
+        for ib, b in enumerate(cb):
+            for ic, c in enumerate(cc):
+                for idd, d in enumerate(cd):
+                    gidx[(a, b, c, d)] = ((ia * len(cb) + ib) * len(cc) + ic) * len(cd) + idd
+    defined = set()
+
+    def hrr(acc, sa, sb, sc, sd):
+        vals = {}
+        memo = {}
+
+        def kget(e, c, d):
+            if sum(d) == 0:
+                return "%s_%s_%s" % (acc, cname(e), cname(c))
+            key = ("k", e, c, d)
+            if key in memo:
+                return memo[key]
+            i = next(q for q in range(3) if d[q] > 0)
+            n1 = kget(e, add(c, i), sub(d, i))
+            n0 = kget(e, c, sub(d, i))
+            nm = "k%s_%d%d%d%d_%s_%s_%s" % (acc, sa, sb, sc, sd, cname(e), cname(c), cname(d))
+            if nm not in defined:
+                em.out("    const R %s = %s + CD%s * %s;" % (nm, n1, AX[i], n0))
+                defined.add(nm)
+            memo[key] = nm
+            return nm
+
+        def bget(a, b, c, d):
+            if sum(b) == 0:
+                return kget(a, c, d)
+            key = ("b", a, b, c, d)
+            if key in memo:
+                return memo[key]
+            i = next(q for q in range(3) if b[q] > 0)
+            n1 = bget(add(a, i), sub(b, i), c, d)
+            n0 = bget(a, sub(b, i), c, d)
+            nm = "h%s_%d%d%d%d_%s_%s_%s_%s" % (acc, sa, sb, sc, sd, cname(a), cname(b), cname(c), cname(d))
+            if nm not in defined:
+                em.out("    const R %s = %s + AB%s * %s;" % (nm, n1, AX[i], n0))
+                defined.add(nm)
+            memo[key] = nm
+            return nm
+
+        for a in comps(sa):
+            for b in comps(sb):
+                for c in comps(sc):
+                    for d in comps(sd):
+                        vals[(a, b, c, d)] = bget(a, b, c, d)
+        return vals
+
+    terms = [("sa", (la + 1, lb, lc, ld), 0, +1), ("sb", (la, lb + 1, lc, ld), 1, +1),
+             ("sc", (la, lb, lc + 1, ld), 2, +1)]
+    if la > 0:
+        terms.append(("s0", (la - 1, lb, lc, ld), 0, -1))
+    if lb > 0:
+        terms.append(("s0", (la, lb - 1, lc, ld), 1, -1))
+    if lc > 0:
+        terms.append(("s0", (la, lb, lc - 1, ld), 2, -1))
+    sums = {k: [] for k in range(9)}
+    for acc, cls, ctr, sgn in terms:
+        vals = hrr(acc, *cls)
+        for (a, b, c, d), nm in vals.items():
+            quad = [a, b, c, d]
+            for i in range(3):
+                q = list(quad[ctr])
+                if sgn > 0:
+                    if q[i] == 0:
+                        continue
+                    q[i] -= 1
+                    coef = None
+                else:
+                    q[i] += 1
+                    coef = float(q[i])
+                src = list(quad)
+                src[ctr] = tuple(q)
+                gi = gidx[tuple(src)]
+                sums[3 * ctr + i].append(("+" if coef is None else "-", coef, gi, nm))
+    for k in range(9):
+        if not sums[k]:
+            continue
+        parts = []
+        for sg, coef, gi, nm in sums[k]:
+            if coef is None:
+                parts.append("+ Gam[%d] * %s" % (gi, nm))
+            else:
+                parts.append("- %s * Gam[%d] * %s" % (lit(coef), gi, nm))
+        em.out("    g[%d] += R(0) %s;" % (k, " ".join(parts)))
+    em.out("  }")
+    em.out("};")
+    return "\n".join(em.lines)
+
+
+def write_deriv_register(outdir):
+    text = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.", "#pragma once"]
+    for c in DERIV_REGISTER:
+        text.append(gen_deriv_class(*c))
+    text = "\n".join(text) + "\n"
+    path = os.path.join(outdir, "deriv_register.inc")
     if not os.path.exists(path) or open(path).read() != text:
         with open(path, "w") as fh:
             fh.write(text)

@@ -1,0 +1,681 @@
+// RHF nuclear gradient on the device (the reference's Fock::compute_forces,
+// src/rhfgrad.cpp:15-89, with compute_2body_fock_deriv<1>, :418-598).
+//
+// Two-electron part, adjoint form: for every unique shell quartet that the
+// reference's gradient loop visits (pairs in the overlap shell-pair list,
+// src/integrals.cpp:515-566; Schwarz x density screen against precision) the
+// derivative integrals d/dA, d/dB, d/dC (ab|cd) are recomputed and contracted
+// at once with
+//   Gamma_abcd = deg (1/2 D_ab D_cd - 1/8 (D_ac D_bd + D_ad D_bc)),
+// which equals 1/4 G1:D of src/rhfgrad.cpp:49-57 per quartet; d/dD follows
+// from translational invariance.  Nothing is stored: small classes run one
+// quartet per lane from generated register code, the others one quartet per
+// wavefront from the generated LDS program.  The contraction is done in the
+// Cartesian basis with the density blocks transformed to Cartesian.
+//
+// One-electron part: one thread per ordered shell pair (s1, s2) adds twice
+// d/dA of D:(T+V) - 2 W:S (A = centre of s1; the (s2, s1) thread supplies the
+// other centre) and the Hellmann-Feynman term of each nucleus (dC = -(dA + dB)).
+#include "m2_common.hpp"
+#include "boys.hpp"
+#include "generated/deriv_programs.inc"
+
+namespace m2 {
+
+template <int LA, int LB, int LC, int LD>
+struct EriDeriv;
+#include "generated/deriv_register.inc"
+
+constexpr int NSCALAR_D = 27;
+
+struct GradArgs {
+    const PrimPair* prims;
+    const int* pair_off;
+    const int* pair_np;
+    const int2* pair_sh;
+    const double* pair_Q;
+    const unsigned char* pair_sig;   // overlap-significant pairs
+    const double* sh_O;
+    const int* sh_bf;
+    const int* sh_atom;
+    const double* Dn;
+    const double* D;
+    const double* sph2;              // 5 x 6 solid harmonics of l = 2
+    int n, nsh, pure;
+    double precision;
+    double* grad;                    // natom x 3
+};
+
+__device__ __forceinline__ int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Cartesian density block Dt (na x nb) of shells (s1 with l1, s2 with l2)
+template <int L1, int L2>
+__device__ __forceinline__ void cart_block(const GradArgs& a, int f1, int f2, double* Dt) {
+    constexpr int N1 = (L1 + 1) * (L1 + 2) / 2, N2 = (L2 + 1) * (L2 + 2) / 2;
+    const bool p1 = a.pure && L1 >= 2, p2 = a.pure && L2 >= 2;
+    const int m1 = p1 ? 2 * L1 + 1 : N1, m2 = p2 ? 2 * L2 + 1 : N2;
+#pragma unroll
+    for (int i = 0; i < N1; ++i)
+#pragma unroll
+        for (int j = 0; j < N2; ++j) {
+            double s = 0.0;
+            for (int u = 0; u < m1; ++u) {
+                const double cu = p1 ? a.sph2[u * 6 + i] : (u == i ? 1.0 : 0.0);
+                if (cu == 0.0) continue;
+                for (int v = 0; v < m2; ++v) {
+                    const double cv = p2 ? a.sph2[v * 6 + j] : (v == j ? 1.0 : 0.0);
+                    if (cv == 0.0) continue;
+                    s += cu * cv * a.D[(size_t)(f1 + u) * a.n + f2 + v];
+                }
+            }
+            Dt[i * N2 + j] = s;
+        }
+}
+
+__device__ __forceinline__ bool grad_screen(const GradArgs& a, int bra, int ket, int sa, int sb, int sc, int sd) {
+    if (!a.pair_sig[bra] || !a.pair_sig[ket]) return false;
+    const int nsh = a.nsh;
+    const double* Dn = a.Dn;
+    const double dn = fmax(fmax(Dn[sa * nsh + sb], Dn[sc * nsh + sd]),
+                           fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+    return dn * a.pair_Q[bra] * a.pair_Q[ket] >= a.precision;
+}
+
+__device__ __forceinline__ double wave_sum_g(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// register classes: one quartet per lane, per-block LDS bins for the atoms
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, const Task* __restrict__ tasks,
+                                                  int ntasks, int natom) {
+    constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
+    constexpr int NC = (LC + 1) * (LC + 2) / 2, ND = (LD + 1) * (LD + 2) / 2;
+    extern __shared__ double bins[];   // natom * 3
+    for (int i = threadIdx.x; i < natom * 3; i += blockDim.x) bins[i] = 0.0;
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (w < ntasks) {
+        const int lane = __lane_id();
+        const int bra = __builtin_amdgcn_readfirstlane(tasks[w].bra);
+        const int ket0 = __builtin_amdgcn_readfirstlane(tasks[w].ket0);
+        const int nket = __builtin_amdgcn_readfirstlane(tasks[w].nket);
+        const int ket = ket0 + (lane < nket ? lane : 0);
+        const int2 sab = a.pair_sh[bra], scd = a.pair_sh[ket];
+        const int sa = sab.x, sb = sab.y, sc = scd.x, sd = scd.y;
+        const bool active = lane < nket && grad_screen(a, bra, ket, sa, sb, sc, sd);
+        if (__any(active)) {
+            double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (active) {
+                const int fa = a.sh_bf[sa], fb = a.sh_bf[sb], fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+                double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dbd[NB * ND], Dad[NA * ND], Dbc[NB * NC];
+                cart_block<LA, LB>(a, fa, fb, Dab);
+                cart_block<LC, LD>(a, fc, fd, Dcd);
+                cart_block<LA, LC>(a, fa, fc, Dac);
+                cart_block<LB, LD>(a, fb, fd, Dbd);
+                cart_block<LA, LD>(a, fa, fd, Dad);
+                cart_block<LB, LC>(a, fb, fc, Dbc);
+                const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
+                double Gam[NA * NB * NC * ND];
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+#pragma unroll
+                    for (int j = 0; j < NB; ++j)
+#pragma unroll
+                        for (int k = 0; k < NC; ++k)
+#pragma unroll
+                            for (int l = 0; l < ND; ++l)
+                                Gam[((i * NB + j) * NC + k) * ND + l] =
+                                    deg * (0.5 * Dab[i * NB + j] * Dcd[k * ND + l] -
+                                           0.125 * (Dac[i * NC + k] * Dbd[j * ND + l] + Dad[i * ND + l] * Dbc[j * NC + k]));
+                const double* O = a.sh_O;
+                EriDeriv<LA, LB, LC, LD>::template run<double>(
+                    a.prims + a.pair_off[bra], a.pair_np[bra], a.prims + a.pair_off[ket], a.pair_np[ket],
+                    O[3 * sa] - O[3 * sb], O[3 * sa + 1] - O[3 * sb + 1], O[3 * sa + 2] - O[3 * sb + 2],
+                    O[3 * sc] - O[3 * sd], O[3 * sc + 1] - O[3 * sd + 1], O[3 * sc + 2] - O[3 * sd + 2], tab, Gam, g);
+            }
+            const int at_a = a.sh_atom[sa], at_b = a.sh_atom[sb];
+            // A and B are the same for the whole wavefront
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double ga = wave_sum_g(g[i]), gb = wave_sum_g(g[3 + i]);
+                const double gc = wave_sum_g(g[6 + i]);
+                (void)gc;
+                if (lane == 0) {
+                    atomicAdd(&bins[at_a * 3 + i], ga);
+                    atomicAdd(&bins[at_b * 3 + i], gb);
+                }
+            }
+            if (active) {
+                const int at_c = a.sh_atom[sc], at_d = a.sh_atom[sd];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    atomicAdd(&bins[at_c * 3 + i], g[6 + i]);
+                    atomicAdd(&bins[at_d * 3 + i], -(g[i] + g[3 + i] + g[6 + i]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < natom * 3; i += blockDim.x)
+        if (bins[i] != 0.0) atomicAdd(&a.grad[i], bins[i]);
+}
+
+// ------------------------------------------------------- program classes
+struct DerivProg {
+    int L, nslots, base, nacc;
+    const int* lev;        // per VRR level: ins0, ins1, acc0, acc1
+    int nvlev;
+    const int* plev;       // c0, c1, then per post level: ins0, ins1, c0, c1
+    int nplev;
+    const int* ins;
+    const double* coef;
+    const int* acc;        // (acc slot, vrr slot, scalar id)
+    const int* contr;      // (slot, gamma index, g index)
+    const double* ccoef;
+    int la, lb, lc, ld;
+};
+
+__device__ __forceinline__ void prog_level(const DerivProg& P, int i0, int i1, double* X, const double* S) {
+    for (int j = i0 + (int)threadIdx.x; j < i1; j += 64) {
+        const int* in = P.ins + 14 * j;
+        const double* cf = P.coef + 6 * j;
+        const int nt = in[1];
+        double v = 0.0;
+        for (int t = 0; t < nt; ++t) v += cf[t] * S[in[8 + t]] * X[in[2 + t]];
+        X[in[0]] = v;
+    }
+}
+
+__device__ __forceinline__ void prog_contract(const DerivProg& P, int c0, int c1, const double* X, const double* Gam,
+                                              double* g) {
+    for (int j = c0 + (int)threadIdx.x; j < c1; j += 64) {
+        const int* c = P.contr + 3 * j;
+        g[c[2]] += P.ccoef[j] * Gam[c[1]] * X[c[0]];
+    }
+}
+
+__global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
+                                                  DerivProg P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int NA = ncart_d(P.la), NB = ncart_d(P.lb), NC = ncart_d(P.lc), ND = ncart_d(P.ld);
+    const int ngam = NA * NB * NC * ND;
+    double* X = reinterpret_cast<double*>(smem);
+    double* S = X + P.nslots;
+    double* Gam = S + NSCALAR_D;
+    double* Dt = Gam + ngam;   // 6 Cartesian density blocks, 36 each
+    const int w = blockIdx.x;
+    if (w >= ntasks) return;
+    const int lane = threadIdx.x;
+    const Task tk = tasks[w];
+    const int2 sab = a.pair_sh[tk.bra];
+    const int sa = sab.x, sb = sab.y;
+    const double* O = a.sh_O;
+    const double ABx = O[3 * sa] - O[3 * sb], ABy = O[3 * sa + 1] - O[3 * sb + 1], ABz = O[3 * sa + 2] - O[3 * sb + 2];
+    for (int kk = 0; kk < tk.nket; ++kk) {
+        const int ket = tk.ket0 + kk;
+        const int2 scd = a.pair_sh[ket];
+        const int sc = scd.x, sd = scd.y;
+        if (!grad_screen(a, tk.bra, ket, sa, sb, sc, sd)) continue;
+        const double CDx = O[3 * sc] - O[3 * sd], CDy = O[3 * sc + 1] - O[3 * sd + 1], CDz = O[3 * sc + 2] - O[3 * sd + 2];
+        const int sh[4] = {sa, sb, sc, sd};
+        const int ls[4] = {P.la, P.lb, P.lc, P.ld};
+        // Cartesian density blocks ab, cd, ac, bd, ad, bc (lanes 0..5)
+        const int bx[6] = {0, 2, 0, 1, 0, 1}, by[6] = {1, 3, 2, 3, 3, 2};
+        if (lane < 6) {
+            const int x = bx[lane], y = by[lane];
+            const int lx = ls[x], ly = ls[y], nx = ncart_d(lx), ny = ncart_d(ly);
+            const bool px = a.pure && lx >= 2, py = a.pure && ly >= 2;
+            const int mx = px ? 2 * lx + 1 : nx, my = py ? 2 * ly + 1 : ny;
+            const int fx = a.sh_bf[sh[x]], fy = a.sh_bf[sh[y]];
+            for (int i = 0; i < nx; ++i)
+                for (int j = 0; j < ny; ++j) {
+                    double s = 0.0;
+                    for (int u = 0; u < mx; ++u) {
+                        const double cu = px ? a.sph2[u * 6 + i] : (u == i ? 1.0 : 0.0);
+                        if (cu == 0.0) continue;
+                        for (int v = 0; v < my; ++v) {
+                            const double cv = py ? a.sph2[v * 6 + j] : (v == j ? 1.0 : 0.0);
+                            if (cv == 0.0) continue;
+                            s += cu * cv * a.D[(size_t)(fx + u) * a.n + fy + v];
+                        }
+                    }
+                    Dt[lane * 36 + i * ny + j] = s;
+                }
+        }
+        __syncthreads();
+        const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);
+        for (int t = lane; t < ngam; t += 64) {
+            const int l = t % ND, k = (t / ND) % NC, j = (t / (ND * NC)) % NB, i = t / (ND * NC * NB);
+            Gam[t] = deg * (0.5 * Dt[0 * 36 + i * NB + j] * Dt[1 * 36 + k * ND + l] -
+                            0.125 * (Dt[2 * 36 + i * NC + k] * Dt[3 * 36 + j * ND + l] +
+                                     Dt[4 * 36 + i * ND + l] * Dt[5 * 36 + j * NC + k]));
+        }
+        for (int t = lane; t < P.nacc; t += 64) X[t] = 0.0;
+        const PrimPair* bp = a.prims + a.pair_off[tk.bra];
+        const PrimPair* kp = a.prims + a.pair_off[ket];
+        const int nbp = a.pair_np[tk.bra], nkp = a.pair_np[ket];
+        for (int ib = 0; ib < nbp; ++ib) {
+            const PrimPair B = bp[ib];
+            for (int ik = 0; ik < nkp; ++ik) {
+                const PrimPair K = kp[ik];
+                const double p = B.p, q = K.p;
+                const double dx = B.P[0] - K.P[0], dy = B.P[1] - K.P[1], dz = B.P[2] - K.P[2];
+                const double rs = rsqrt(p + q);
+                const double ipq = rs * rs;
+                const double T = p * q * ipq * (dx * dx + dy * dy + dz * dz);
+                __syncthreads();
+                if (lane <= P.L) {
+                    double F;
+                    if (T < BOYS_TMAX) {
+                        const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
+                        const double d = k * BOYS_STEP - T;
+                        const double* r = tab.d + k * BOYS_MTAB + lane;
+                        double s = r[7];
+                        for (int j = 6; j >= 0; --j) s = r[j] + d * s / (j + 1);
+                        F = s;
+                    } else {
+                        F = 0.88622692545275801365 * rsqrt(T);
+                        const double h = 0.5 / T;
+                        for (int i = 0; i < lane; ++i) F = (2 * i + 1) * F * h;
+                    }
+                    X[P.base + lane] = B.K * K.K * rs * F;
+                }
+                if (lane == 0) {
+                    const double rp = q * ipq, rq = p * ipq;
+                    S[0] = 1.0;
+                    S[1] = B.PA[0]; S[2] = B.PA[1]; S[3] = B.PA[2];
+                    S[4] = -rp * dx; S[5] = -rp * dy; S[6] = -rp * dz;
+                    S[7] = K.PA[0]; S[8] = K.PA[1]; S[9] = K.PA[2];
+                    S[10] = rq * dx; S[11] = rq * dy; S[12] = rq * dz;
+                    S[13] = B.inv2p; S[14] = K.inv2p; S[15] = 0.5 * ipq;
+                    S[16] = B.inv2p * rp; S[17] = K.inv2p * rq;
+                    S[18] = ABx; S[19] = ABy; S[20] = ABz;
+                    S[21] = CDx; S[22] = CDy; S[23] = CDz;
+                    S[24] = 2.0 * B.a; S[25] = 2.0 * (B.p - B.a); S[26] = 2.0 * K.a;
+                }
+                __syncthreads();
+                for (int l = 0; l < P.nvlev; ++l) {
+                    const int* L4 = P.lev + 4 * l;
+                    prog_level(P, L4[0], L4[1], X, S);
+                    __syncthreads();
+                    for (int j = L4[2] + lane; j < L4[3]; j += 64) {
+                        const int* ac = P.acc + 3 * j;
+                        X[ac[0]] += S[ac[2]] * X[ac[1]];
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        __syncthreads();
+        double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        prog_contract(P, P.plev[0], P.plev[1], X, Gam, g);
+        for (int l = 0; l < P.nplev; ++l) {
+            const int* L4 = P.plev + 2 + 4 * l;
+            prog_level(P, L4[0], L4[1], X, S);
+            __syncthreads();
+            prog_contract(P, L4[2], L4[3], X, Gam, g);
+            __syncthreads();
+        }
+        double tot[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) tot[i] = wave_sum_g(g[i]);
+        if (lane == 0) {
+            const int at[4] = {a.sh_atom[sa], a.sh_atom[sb], a.sh_atom[sc], a.sh_atom[sd]};
+            for (int i = 0; i < 3; ++i) {
+                atomicAdd(&a.grad[at[0] * 3 + i], tot[i]);
+                atomicAdd(&a.grad[at[1] * 3 + i], tot[3 + i]);
+                atomicAdd(&a.grad[at[2] * 3 + i], tot[6 + i]);
+                atomicAdd(&a.grad[at[3] * 3 + i], -(tot[i] + tot[3 + i] + tot[6 + i]));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ one-electron
+struct Grad1Args {
+    const double* sh_O;
+    const int* sh_l;
+    const int* sh_bf;
+    const int* sh_np;
+    const int* sh_poff;
+    const int* sh_atom;
+    const double* exps;
+    const double* coefs;
+    const double* D;
+    const double* W;
+    const double* sph2;
+    const double* Z;
+    const double* xyz;
+    int nsh, n, pure, natom;
+    BoysTables tab;
+    double* grad;
+};
+
+__device__ __forceinline__ void comp_of(int l, int i, int* c) {
+    int k = 0;
+    for (int x = l; x >= 0; --x)
+        for (int y = l - x; y >= 0; --y) {
+            if (k == i) {
+                c[0] = x;
+                c[1] = y;
+                c[2] = l - x - y;
+                return;
+            }
+            ++k;
+        }
+}
+__device__ __forceinline__ int cum_of(int l) { return l * (l + 1) * (l + 2) / 6; }
+__device__ __forceinline__ int cidx_of(int x, int y, int z) {
+    const int L = x + y + z;
+    return (L - x) * (L - x + 1) / 2 + (L - x - y);
+}
+
+// Cartesian block of D (or W): rows components of s1 (l1), cols of s2 (l2)
+__device__ void cart_block_rt(const double* M, int n, const double* sph2, int pure, int l1, int f1, int l2, int f2,
+                              double* out) {
+    const int n1 = ncart_d(l1), n2 = ncart_d(l2);
+    const bool p1 = pure && l1 >= 2, p2 = pure && l2 >= 2;
+    const int m1 = p1 ? 2 * l1 + 1 : n1, m2 = p2 ? 2 * l2 + 1 : n2;
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j) {
+            double s = 0.0;
+            for (int u = 0; u < m1; ++u) {
+                const double cu = p1 ? sph2[u * 6 + i] : (u == i ? 1.0 : 0.0);
+                if (cu == 0.0) continue;
+                for (int v = 0; v < m2; ++v) {
+                    const double cv = p2 ? sph2[v * 6 + j] : (v == j ? 1.0 : 0.0);
+                    if (cv == 0.0) continue;
+                    s += cu * cv * M[(size_t)(f1 + u) * n + f2 + v];
+                }
+            }
+            out[i * n2 + j] = s;
+        }
+}
+
+constexpr int G1_LA = 3;   // l of the bra after the +1 shift
+constexpr int G1_CUMA = (G1_LA + 1) * (G1_LA + 2) * (G1_LA + 3) / 6;   // 20
+constexpr int G1_CUMB = 10;                                            // |b| <= 2
+
+__global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nsh * a.nsh) return;
+    const int s1 = t / a.nsh, s2 = t % a.nsh;
+    const int la = a.sh_l[s1], lb = a.sh_l[s2];
+    const int na = ncart_d(la), nb = ncart_d(lb);
+    double Dt[36], Wt[36];
+    cart_block_rt(a.D, a.n, a.sph2, a.pure, la, a.sh_bf[s1], lb, a.sh_bf[s2], Dt);
+    cart_block_rt(a.W, a.n, a.sph2, a.pure, la, a.sh_bf[s1], lb, a.sh_bf[s2], Wt);
+    const double* A = a.sh_O + 3 * s1;
+    const double* B = a.sh_O + 3 * s2;
+    double AB2 = 0.0;
+    for (int k = 0; k < 3; ++k) AB2 += (A[k] - B[k]) * (A[k] - B[k]);
+    double gA[3] = {0, 0, 0};   // T, Pulay and V (basis-centre) part on atom(s1)
+    const int Lab = la + 1 + lb;
+    for (int ip = 0; ip < a.sh_np[s1]; ++ip)
+        for (int jp = 0; jp < a.sh_np[s2]; ++jp) {
+            const double al = a.exps[a.sh_poff[s1] + ip], be = a.exps[a.sh_poff[s2] + jp];
+            const double p = al + be, ip2 = 0.5 / p;
+            const double cc = a.coefs[a.sh_poff[s1] + ip] * a.coefs[a.sh_poff[s2] + jp];
+            const double Kab = exp(-al * be / p * AB2);
+            double P[3], PA[3], PB[3];
+            for (int k = 0; k < 3; ++k) {
+                P[k] = (al * A[k] + be * B[k]) / p;
+                PA[k] = P[k] - A[k];
+                PB[k] = P[k] - B[k];
+            }
+            double O1[3][G1_LA + 1][5];
+            for (int k = 0; k < 3; ++k)
+                for (int ii = 0; ii <= la + 1; ++ii)
+                    for (int jj = 0; jj <= lb + 2; ++jj) {
+                        double v;
+                        if (ii == 0 && jj == 0) v = 1.0;
+                        else if (ii > 0) {
+                            v = PA[k] * O1[k][ii - 1][jj];
+                            if (ii > 1) v += (ii - 1) * ip2 * O1[k][ii - 2][jj];
+                            if (jj > 0) v += jj * ip2 * O1[k][ii - 1][jj - 1];
+                        } else {
+                            v = PB[k] * O1[k][0][jj - 1];
+                            if (jj > 1) v += (jj - 1) * ip2 * O1[k][0][jj - 2];
+                        }
+                        O1[k][ii][jj] = v;
+                    }
+            const double spref = pow(M_PI / p, 1.5) * Kab * cc;
+            // S and T of a Cartesian pair (exponents ex of a, eb of b)
+            auto ST = [&](const int* ex, const int* eb, double& s, double& tk) {
+                const double sx = O1[0][ex[0]][eb[0]], sy = O1[1][ex[1]][eb[1]], sz = O1[2][ex[2]][eb[2]];
+                double t3[3];
+                for (int k = 0; k < 3; ++k) {
+                    const int ii = ex[k], jj = eb[k];
+                    double v = -2.0 * be * (2 * jj + 1) * O1[k][ii][jj] + 4.0 * be * be * O1[k][ii][jj + 2];
+                    if (jj >= 2) v += jj * (jj - 1) * O1[k][ii][jj - 2];
+                    t3[k] = -0.5 * v;
+                }
+                s = spref * sx * sy * sz;
+                tk = spref * (t3[0] * sy * sz + sx * t3[1] * sz + sx * sy * t3[2]);
+            };
+            for (int ia = 0; ia < na; ++ia) {
+                int ea[3];
+                comp_of(la, ia, ea);
+                for (int ib = 0; ib < nb; ++ib) {
+                    int eb[3];
+                    comp_of(lb, ib, eb);
+                    const double dd = Dt[ia * nb + ib], ww = Wt[ia * nb + ib];
+                    for (int ax = 0; ax < 3; ++ax) {
+                        int ep[3] = {ea[0], ea[1], ea[2]};
+                        ep[ax]++;
+                        double sp, tp;
+                        ST(ep, eb, sp, tp);
+                        double ds = 2.0 * al * sp, dt = 2.0 * al * tp;
+                        if (ea[ax] > 0) {
+                            int em[3] = {ea[0], ea[1], ea[2]};
+                            em[ax]--;
+                            double sm, tm;
+                            ST(em, eb, sm, tm);
+                            ds -= ea[ax] * sm;
+                            dt -= ea[ax] * tm;
+                        }
+                        gA[ax] += dd * dt - 2.0 * ww * ds;
+                    }
+                }
+            }
+            // nuclear attraction: [a|b]^(m) with |a| <= la+1, |b| <= lb
+            const double vpref = 2.0 * M_PI / p * Kab * cc;
+            for (int c = 0; c < a.natom; ++c) {
+                const double* Cc = a.xyz + 3 * c;
+                double PC[3], PC2 = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    PC[k] = P[k] - Cc[k];
+                    PC2 += PC[k] * PC[k];
+                }
+                double F[G1_LA + 3];
+                {
+                    const double T = p * PC2;
+                    if (T < BOYS_TMAX) {
+                        const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
+                        const double d = k * BOYS_STEP - T;
+                        const double* r = a.tab.d + k * BOYS_MTAB + Lab;
+                        double s = r[7];
+                        for (int j = 6; j >= 0; --j) s = r[j] + d * s / (j + 1);
+                        F[Lab] = s;
+                        const double e = exp(-T);
+                        for (int m = Lab - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
+                    } else {
+                        F[0] = 0.88622692545275801365 * rsqrt(T);
+                        for (int m = 0; m < Lab; ++m) F[m + 1] = (2 * m + 1) * F[m] * (0.5 / T);
+                    }
+                }
+                double Wv[G1_CUMA][G1_CUMB][G1_LA + 3];
+                for (int m = 0; m <= Lab; ++m) Wv[0][0][m] = vpref * F[m];
+                for (int lbb = 0; lbb <= lb; ++lbb)
+                    for (int ib = cum_of(lbb); ib < cum_of(lbb + 1); ++ib) {
+                        int eb[3];
+                        comp_of(lbb, ib - cum_of(lbb), eb);
+                        for (int laa = 0; laa <= la + 1; ++laa)
+                            for (int ia = cum_of(laa); ia < cum_of(laa + 1); ++ia) {
+                                if (ia == 0 && ib == 0) continue;
+                                int ea[3];
+                                comp_of(laa, ia - cum_of(laa), ea);
+                                const int mmax = Lab - laa - lbb;
+                                if (lbb > 0) {
+                                    const int ax = eb[0] > 0 ? 0 : (eb[1] > 0 ? 1 : 2);
+                                    int bm[3] = {eb[0], eb[1], eb[2]};
+                                    bm[ax]--;
+                                    const int b1 = cum_of(lbb - 1) + cidx_of(bm[0], bm[1], bm[2]);
+                                    int b2 = -1, a1 = -1;
+                                    if (bm[ax] > 0) {
+                                        int t2[3] = {bm[0], bm[1], bm[2]};
+                                        t2[ax]--;
+                                        b2 = cum_of(lbb - 2) + cidx_of(t2[0], t2[1], t2[2]);
+                                    }
+                                    if (ea[ax] > 0) {
+                                        int t2[3] = {ea[0], ea[1], ea[2]};
+                                        t2[ax]--;
+                                        a1 = cum_of(laa - 1) + cidx_of(t2[0], t2[1], t2[2]);
+                                    }
+                                    for (int m = 0; m <= mmax; ++m) {
+                                        double v = PB[ax] * Wv[ia][b1][m] - PC[ax] * Wv[ia][b1][m + 1];
+                                        if (b2 >= 0) v += bm[ax] * ip2 * (Wv[ia][b2][m] - Wv[ia][b2][m + 1]);
+                                        if (a1 >= 0) v += ea[ax] * ip2 * (Wv[a1][b1][m] - Wv[a1][b1][m + 1]);
+                                        Wv[ia][ib][m] = v;
+                                    }
+                                } else {
+                                    const int ax = ea[0] > 0 ? 0 : (ea[1] > 0 ? 1 : 2);
+                                    int am[3] = {ea[0], ea[1], ea[2]};
+                                    am[ax]--;
+                                    const int a1 = cum_of(laa - 1) + cidx_of(am[0], am[1], am[2]);
+                                    int a2 = -1;
+                                    if (am[ax] > 0) {
+                                        int t2[3] = {am[0], am[1], am[2]};
+                                        t2[ax]--;
+                                        a2 = cum_of(laa - 2) + cidx_of(t2[0], t2[1], t2[2]);
+                                    }
+                                    for (int m = 0; m <= mmax; ++m) {
+                                        double v = PA[ax] * Wv[a1][0][m] - PC[ax] * Wv[a1][0][m + 1];
+                                        if (a2 >= 0) v += am[ax] * ip2 * (Wv[a2][0][m] - Wv[a2][0][m + 1]);
+                                        Wv[ia][0][m] = v;
+                                    }
+                                }
+                            }
+                    }
+                double gv[3] = {0, 0, 0};
+                for (int ia = 0; ia < na; ++ia) {
+                    int ea[3];
+                    comp_of(la, ia, ea);
+                    for (int ib = 0; ib < nb; ++ib) {
+                        const double dd = Dt[ia * nb + ib];
+                        const int bpos = cum_of(lb) + ib;
+                        for (int ax = 0; ax < 3; ++ax) {
+                            int ep[3] = {ea[0], ea[1], ea[2]};
+                            ep[ax]++;
+                            double dv = 2.0 * al * Wv[cum_of(la + 1) + cidx_of(ep[0], ep[1], ep[2])][bpos][0];
+                            if (ea[ax] > 0) {
+                                int em[3] = {ea[0], ea[1], ea[2]};
+                                em[ax]--;
+                                dv -= ea[ax] * Wv[cum_of(la - 1) + cidx_of(em[0], em[1], em[2])][bpos][0];
+                            }
+                            gv[ax] += dd * dv;
+                        }
+                    }
+                }
+                for (int ax = 0; ax < 3; ++ax) {
+                    gA[ax] -= a.Z[c] * gv[ax];
+                    atomicAdd(&a.grad[c * 3 + ax], 2.0 * a.Z[c] * gv[ax]);   // -(dA) of the point-charge term
+                }
+            }
+        }
+    // the ket-centre derivative of block (s1, s2) is the bra-centre derivative
+    // of block (s2, s1) (symmetric D and W): every ordered pair counts twice
+    const int at = a.sh_atom[s1];
+    for (int ax = 0; ax < 3; ++ax) atomicAdd(&a.grad[at * 3 + ax], 2.0 * gA[ax]);
+}
+
+// --------------------------------------------------------------- host side
+struct DerivTables {
+    int* ins = nullptr;
+    double* coef = nullptr;
+    int* lev = nullptr;
+    int* acc = nullptr;
+    int* contr = nullptr;
+    double* ccoef = nullptr;
+};
+static DerivTables g_dt[64];
+
+hipError_t deriv_upload(int device) {
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    DerivTables& t = g_dt[device];
+    if (t.ins) return hipSuccess;
+    hipError_t e;
+    if ((e = hipMalloc((void**)&t.ins, sizeof(DPROG_INS)))) return e;
+    if ((e = hipMalloc((void**)&t.coef, sizeof(DPROG_COEF)))) return e;
+    if ((e = hipMalloc((void**)&t.lev, sizeof(DPROG_LEV)))) return e;
+    if ((e = hipMalloc((void**)&t.acc, sizeof(DPROG_ACC)))) return e;
+    if ((e = hipMalloc((void**)&t.contr, sizeof(DPROG_CONTR)))) return e;
+    if ((e = hipMalloc((void**)&t.ccoef, sizeof(DPROG_CCOEF)))) return e;
+    if ((e = hipMemcpy(t.ins, DPROG_INS, sizeof(DPROG_INS), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.coef, DPROG_COEF, sizeof(DPROG_COEF), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.lev, DPROG_LEV, sizeof(DPROG_LEV), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.acc, DPROG_ACC, sizeof(DPROG_ACC), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.contr, DPROG_CONTR, sizeof(DPROG_CONTR), hipMemcpyHostToDevice))) return e;
+    return hipMemcpy(t.ccoef, DPROG_CCOEF, sizeof(DPROG_CCOEF), hipMemcpyHostToDevice);
+}
+
+template <int LA, int LB, int LC, int LD>
+static void launch_reg(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom, hipStream_t st) {
+    hipLaunchKernelGGL((k_grad_reg<LA, LB, LC, LD>), dim3((nt + 3) / 4), dim3(256), sizeof(double) * natom * 3, st, a,
+                       tab, t, nt, natom);
+}
+
+hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,
+                          const Task* t, int nt, int natom, hipStream_t st) {
+    if (nt <= 0) return hipSuccess;
+    if (la == 0 && lb == 0 && lc == 0 && ld == 0) launch_reg<0, 0, 0, 0>(a, tab, t, nt, natom, st);
+    else if (la == 1 && lb == 0 && lc == 0 && ld == 0) launch_reg<1, 0, 0, 0>(a, tab, t, nt, natom, st);
+    else if (la == 1 && lb == 0 && lc == 1 && ld == 0) launch_reg<1, 0, 1, 0>(a, tab, t, nt, natom, st);
+    else if (la == 1 && lb == 1 && lc == 0 && ld == 0) launch_reg<1, 1, 0, 0>(a, tab, t, nt, natom, st);
+    else if (la == 2 && lb == 0 && lc == 0 && ld == 0) launch_reg<2, 0, 0, 0>(a, tab, t, nt, natom, st);
+    else {
+        const DerivTables& T = g_dt[device];
+        int k = 0;
+        for (; k < DPROG_NCLASS; ++k) {
+            const int* d = DPROG_DESC + k * DPROG_DESC_FIELDS;
+            if (d[0] == la && d[1] == lb && d[2] == lc && d[3] == ld) break;
+        }
+        if (k == DPROG_NCLASS) return hipErrorInvalidValue;
+        // fields: la lb lc ld L nslots base nacc lev_off nvlev post_off nplev contr_off ncontr na nb nc nd
+        const int* d = DPROG_DESC + k * DPROG_DESC_FIELDS;
+        DerivProg P;
+        P.la = la;
+        P.lb = lb;
+        P.lc = lc;
+        P.ld = ld;
+        P.L = d[4];
+        P.nslots = d[5];
+        P.base = d[6];
+        P.nacc = d[7];
+        P.lev = T.lev + d[8];
+        P.nvlev = d[9];
+        P.plev = T.lev + d[10];
+        P.nplev = d[11];
+        P.ins = T.ins;
+        P.coef = T.coef;
+        P.acc = T.acc;
+        P.contr = T.contr;
+        P.ccoef = T.ccoef;
+        const int ngam = d[14] * d[15] * d[16] * d[17];
+        const size_t sm = sizeof(double) * (P.nslots + NSCALAR_D + ngam + 6 * 36);
+        hipLaunchKernelGGL(k_grad_prog, dim3(nt), dim3(64), sm, st, a, tab, t, nt, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_1e(const Grad1Args& a, hipStream_t st) {
+    const int nt = a.nsh * a.nsh;
+    hipLaunchKernelGGL(k_grad_1e, dim3((nt + 63) / 64), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace m2

@@ -17,7 +17,7 @@ struct m2_scf_options_c {
 struct m2_scf_result_c {
     double energy, one_e, two_e, enuc;
     int iterations, converged, n_full, n_incremental, n_fp32;
-    double jk_seconds, total_seconds;
+    double jk_seconds, total_seconds, grad_seconds;
 };
 
 static thread_local std::string g_rhf_error;
@@ -33,7 +33,8 @@ int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
            int nsh, const int* l, const int* nprim, const double* exps, const double* coefs, const double* centers,
            const int* atoms, int nsh_aux, const int* l_aux, const int* nprim_aux, const double* exps_aux,
            const double* coefs_aux, const double* centers_aux, const int* atoms_aux, const m2_scf_options_c* o,
-           m2_scf_result_c* res, double* D, double* C, double* eps, double* log5 /* maxiter x 5 or NULL */) {
+           m2_scf_result_c* res, double* D, double* C, double* eps, double* log5 /* maxiter x 5 or NULL */,
+           double* grad /* natom x 3 or NULL: also compute Fock::compute_forces */) {
     using namespace molecular;
     g_rhf_error.clear();
     try {
@@ -75,6 +76,11 @@ int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
         SCF scf(opt, mol, fock);
         scf.rhf(false);
         const int n = ints.nbasis();
+        if (grad) {
+            fock.compute_forces(mol.atoms, fock.getNocc());
+            for (int a = 0; a < natom; ++a)
+                for (int k = 0; k < 3; ++k) grad[3 * a + k] = fock.getForces()(a, k);
+        }
         if (res) {
             res->energy = scf.getEnergy();
             res->one_e = scf.getOneE();
@@ -87,6 +93,7 @@ int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
             res->n_fp32 = fock.n_fp32_builds;
             res->jk_seconds = fock.jk_seconds;
             res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            res->grad_seconds = fock.grad_seconds;
         }
         if (D) std::memcpy(D, fock.getDens().data(), sizeof(double) * n * n);
         if (C) std::memcpy(C, fock.getCP().data(), sizeof(double) * n * n);

@@ -424,6 +424,32 @@ void Fock::makeDens() {
     else dens_diff += dens;
 }
 
+// W = C_occ eps_occ C_occ^T, then the device gradient (one-electron, Pulay,
+// two-electron adjoint and nuclear terms)
+void Fock::compute_forces(const std::vector<Atom>& atoms, int nocc_) {
+    m2_ctx* ctx = integrals.context();
+    auto t0 = std::chrono::steady_clock::now();
+    Matrix Co = CP.block(0, 0, nbfs, nocc_);
+    Matrix Ce(Co);
+    for (int j = 0; j < nocc_; ++j)
+        for (int i = 0; i < nbfs; ++i) Ce(i, j) *= eps[j];
+    Matrix W = matmul(ctx, Ce, Co, false, true);
+    std::vector<double> Z, xyz;
+    for (const auto& a : atoms) {
+        Z.push_back(a.Z);
+        xyz.push_back(a.x);
+        xyz.push_back(a.y);
+        xyz.push_back(a.z);
+    }
+    std::vector<double> g(3 * atoms.size());
+    check(ctx, m2_grad(ctx, (int)atoms.size(), Z.data(), xyz.data(), dens.data(), W.data(), precision, g.data()),
+          "m2_grad");
+    forces = Matrix((int)atoms.size(), 3);
+    for (size_t a = 0; a < atoms.size(); ++a)
+        for (int k = 0; k < 3; ++k) forces((int)a, k) = g[3 * a + k];
+    grad_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // ------------------------------------------------------------------- SCF
 SCF::SCF(const Options& o, const Molecule& m, Fock& f) : opt(o), molecule(m), focker(f) {
     diis.init(o.maxdiis, o.diis);

@@ -173,6 +173,10 @@ public:
     Vector& getEps() { return eps; }
     int getNocc() const { return nocc; }
     m2_ctx* context() const { return integrals.context(); }
+    // Fock::compute_forces (src/rhfgrad.cpp:15-89): dE/dR, natom x 3
+    void compute_forces(const std::vector<Atom>& atoms, int nocc);
+    Matrix& getForces() { return forces; }
+    double grad_seconds = 0.0;
 
     // incremental Fock state (inc/fock.hpp:127-129)
     bool reset_incremental = false, started_incremental = false;
@@ -187,7 +191,7 @@ private:
     const Options opt;
     IntegralEngine& integrals;
     const Molecule& molecule;
-    Matrix hcore, orthog, fockm, focka, fockinc, CP, dens, dens_diff, jints, kints, jkints;
+    Matrix hcore, orthog, fockm, focka, fockinc, CP, dens, dens_diff, jints, kints, jkints, forces;
     Vector eps;
     std::vector<Matrix> focks;
     int nbfs = 0, nocc = 0, iter = 0, MAX = 8;

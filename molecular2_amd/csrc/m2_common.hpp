@@ -15,7 +15,7 @@ struct alignas(16) PrimPair {
     double K;
     double P[3];    // product centre
     double PA[3];   // P - A (A = centre of the pair's first shell)
-    double pad;
+    double a;       // exponent of the first shell's primitive (derivatives)
 };
 static_assert(sizeof(PrimPair) == 80, "PrimPair layout");
 

@@ -32,7 +32,7 @@ class SCFResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("one_e", C.c_double), ("two_e", C.c_double), ("enuc", C.c_double),
                 ("iterations", C.c_int), ("converged", C.c_int), ("n_full", C.c_int),
                 ("n_incremental", C.c_int), ("n_fp32", C.c_int),
-                ("jk_seconds", C.c_double), ("total_seconds", C.c_double)]
+                ("jk_seconds", C.c_double), ("total_seconds", C.c_double), ("grad_seconds", C.c_double)]
 
 
 class M2Error(RuntimeError):
@@ -82,8 +82,9 @@ def lib():
         L.m2_rhf.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
                              C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                              C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
-                             C.POINTER(SCFOptions), C.POINTER(SCFResult), _dp, _dp, _dp, _dp]
-        for name in ("m2_df_setup", "m2_jk_df", "m2_jk_df_device", "m2_df_timing", "m2_eigh", "m2_gemm", "m2_rhf"):
+                             C.POINTER(SCFOptions), C.POINTER(SCFResult), _dp, _dp, _dp, _dp, _dp]
+        L.m2_grad.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
+        for name in ("m2_grad", "m2_df_setup", "m2_jk_df", "m2_jk_df_device", "m2_df_timing", "m2_eigh", "m2_gemm", "m2_rhf"):
             getattr(L, name).restype = C.c_int
         for name in ("m2_set_profiling", "m2_get_profile", "m2_reset_profile", "m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
                      "m2_shellblock_norm", "m2_one_body", "m2_jk_direct", "m2_jk_direct_device",
@@ -232,7 +233,17 @@ class Engine:
         self._chk(self.L.m2_eigh(self.h, n, A.ctypes.data_as(_dp), _p(w), V.ctypes.data_as(_dp)))
         return w, V
 
-    def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, **kw):
+    def grad(self, mol, D, W, precision=1e-12):
+        """dE/dR (natom x 3) for density D and energy-weighted density W."""
+        Z = np.asarray(mol.Z, float)
+        X = np.ascontiguousarray(mol.xyz, float)
+        D = np.ascontiguousarray(D, float)
+        W = np.ascontiguousarray(W, float)
+        g = np.zeros(mol.natom * 3)
+        self._chk(self.L.m2_grad(self.h, mol.natom, _p(Z), _p(X), _p(D), _p(W), precision, _p(g)))
+        return g.reshape(mol.natom, 3)
+
+    def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, gradient=False, **kw):
         """Run the host C++ SCF (molecular::SCF::rhf) on `mol`.  Keyword
         options as SCFOptions (defaults: the reference's RHF defaults)."""
         o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
@@ -256,12 +267,14 @@ class Engine:
         Cm = np.zeros((nbf, nbf))
         eps = np.zeros(nbf)
         log = np.zeros((max(1, o.maxiter), 5))
+        g = np.zeros(mol.natom * 3)
         res = SCFResult()
         ip = lambda a: a.ctypes.data_as(_ip)
         st = self.L.m2_rhf(self.h, mol.natom, _p(Z), _p(X), mol.charge, mol.multiplicity, int(pure),
                            len(sh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
                            nsa, ip(la), ip(npa), _p(exa), _p(cfa), _p(cena), ip(ata),
-                           C.byref(o), C.byref(res), _p(D), _p(Cm), _p(eps), _p(log))
+                           C.byref(o), C.byref(res), _p(D), _p(Cm), _p(eps), _p(log),
+                           _p(g) if gradient else None)
         if st != 0:
             raise M2Error(st, self.L.m2_rhf_last_error().decode())
         self.nbf = nbf
@@ -270,7 +283,8 @@ class Engine:
                 "iterations": res.iterations, "converged": bool(res.converged), "n_full": res.n_full,
                 "n_incremental": res.n_incremental, "n_fp32": res.n_fp32, "jk_seconds": res.jk_seconds,
                 "total_seconds": res.total_seconds, "D": D, "C": Cm.T.copy(), "eps": eps,
-                "log": log[:res.iterations]}
+                "log": log[:res.iterations], "grad": g.reshape(mol.natom, 3) if gradient else None,
+                "grad_seconds": res.grad_seconds}
 
     def last_stats(self):
         a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()

@@ -1126,11 +1126,14 @@ void or_grad_1e(const or_basis* b, int natom, const double* Z, const double* xyz
                             accT += D[I * n + Jj] * dt[i * B->nbf + j];
                             accV += D[I * n + Jj] * dv[i * B->nbf + j];
                         }
+                    /* ordered pairs with the bra-centre derivative only: the
+                     * ket-centre derivative of the (s1,s2) block is the bra
+                     * derivative of the (s2,s1) block (symmetric D, W), hence 2x */
                     if (c < 0) {
-                        grad[A->atom * 3 + ax] += accT - 2.0 * accS;
+                        grad[A->atom * 3 + ax] += 2.0 * (accT - 2.0 * accS);
                     } else {
-                        grad[A->atom * 3 + ax] += accV;
-                        grad[c * 3 + ax] -= accV;
+                        grad[A->atom * 3 + ax] += 2.0 * accV;
+                        grad[c * 3 + ax] -= 2.0 * accV;
                     }
                 }
             }
