@@ -365,6 +365,11 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
             ctx->ev.push_back(e);
         }
     }
+    int maxnp = 1;
+    for (int x : ctx->pair_np) maxnp = std::max(maxnp, x);
+    ARGCHK(maxnp <= BRA_LDS_MAX, "a shell pair has more primitive pairs than the J/K kernels stage in LDS (" +
+                                     std::to_string(BRA_LDS_MAX) + ")");
+    a.bra_lds = maxnp;
     a.D = dD;
     a.J = ctx->d_J.p;
     a.K = ctx->d_K.p;

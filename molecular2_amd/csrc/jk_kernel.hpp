@@ -156,13 +156,34 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a, BoysTables tab, const Task
         }
     }
     const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
+    // the bra's primitive pairs are the same for the whole wavefront: stage
+    // them in LDS once (converted to R), so the inner primitive loop reads
+    // LDS broadcasts
+    const int nbp = a.pair_np[bra];
+    const PrimPair* gb = a.prims + a.pair_off[bra];
+    extern __shared__ float4 s_raw[];
+    BraPair<R>* bprims = reinterpret_cast<BraPair<R>*>(s_raw) + (threadIdx.x >> 6) * a.bra_lds;
+    for (int i = lane; i < nbp; i += 64) {
+        const PrimPair g = gb[i];
+        BraPair<R> b;
+        b.p = R(g.p);
+        b.inv2p = R(g.inv2p);
+        b.K = R(g.K);
+        for (int k = 0; k < 3; ++k) {
+            b.P[k] = R(g.P[k]);
+            b.PA[k] = R(g.PA[k]);
+        }
+        bprims[i] = b;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const double* O = a.sh_O;
     const R ABx = R(O[3 * sa] - O[3 * sb]), ABy = R(O[3 * sa + 1] - O[3 * sb + 1]), ABz = R(O[3 * sa + 2] - O[3 * sb + 2]);
     const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
     R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
     const int nkp = active ? a.pair_np[ket] : 0;
-    EriClass<LA, LB, LC, LD>::template run<R, PURE>(a.prims + a.pair_off[bra], a.pair_np[bra],
-                                                    a.prims + a.pair_off[ket], nkp,
+    EriClass<LA, LB, LC, LD>::template run<R, PURE>(bprims, nbp, a.prims + a.pair_off[ket], nkp,
                                                     ABx, ABy, ABz, CDx, CDy, CDz, tab, v);
     digest<NA, NB, NC, ND, R>(v, active, a.sh_bf[sa], a.sh_bf[sb], a.sh_bf[sc], a.sh_bf[sd],
                               deg * a.scale, a.D, a.n, a.J, a.K);
@@ -199,10 +220,12 @@ void launch_jk(const JKArgs& a, const BoysTables& tab, const Task* tasks, int nt
     dim3 grid((ntasks + waves_per_block - 1) / waves_per_block), block(64 * waves_per_block);
     const double inf = __builtin_huge_val();
     if (mode == 2)
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, block, 0, st, a, tab, tasks, ntasks,
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, block,
+                           sizeof(BraPair<float>) * (size_t)a.bra_lds * waves_per_block, st, a, tab, tasks, ntasks,
                            a.precision, inf);
     else
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, block, 0, st, a, tab, tasks, ntasks,
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, block,
+                           sizeof(BraPair<double>) * (size_t)a.bra_lds * waves_per_block, st, a, tab, tasks, ntasks,
                            a.precision, inf);
 }
 

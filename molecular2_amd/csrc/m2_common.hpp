@@ -19,6 +19,18 @@ struct alignas(16) PrimPair {
 };
 static_assert(sizeof(PrimPair) == 80, "PrimPair layout");
 
+// The fields of a PrimPair that the J/K kernels read, in the kernel's
+// arithmetic type; the bra pairs of a wavefront are staged in LDS in this form.
+template <typename R>
+struct alignas(16) BraPair {
+    R p, inv2p, K;
+    R P[3];
+    R PA[3];
+};
+static_assert(sizeof(BraPair<double>) == 80 && sizeof(BraPair<float>) == 48, "BraPair layout");
+// LDS budget of the bra staging: primitive pairs per shell pair
+constexpr int BRA_LDS_MAX = 500;
+
 // Boys table: F_m(T_k) at T_k = k * BOYS_STEP, k = 0..BOYS_NGRID-1,
 // m = 0..BOYS_MTAB-1.  Beyond BOYS_TMAX the asymptotic branch is used.
 constexpr int BOYS_MTAB = 26;
@@ -45,6 +57,7 @@ struct JKArgs {
     double scale;            // multiplier
     unsigned long long* counters;  // profiling only: per class combo {quartets, primitive quartets}
     int combo;               // 6 * X + Y of the launch
+    int bra_lds;             // LDS capacity per wavefront in primitive pairs (>= every pair_np)
 };
 
 struct Task {                // one wavefront of quartets: bra pair x 1..64 ket pairs
