@@ -1,17 +1,29 @@
 #!/usr/bin/env python3
-"""bench.py -- direct J+K builds/s on one MI355X.
+"""bench.py -- direct J+K builds/s and RHF gradient seconds on one MI355X,
+FP64 and mixed precision.
 
 Workload: (H2O)20 water cluster (seeded, 3.0 A lattice, +-0.3 A jitter, Haar
-rotations), cc-pVDZ spherical (480 basis functions, 240 shells), Schwarz x
-density screening at precision 1e-12, one full J+K build per step
-(Fock::formJKdirect semantics, src/fockbuild.cpp:64-164).  The density is
-the idempotent core-Hamiltonian-guess density 2 C_occ C_occ^T of the same
-molecule (no converged density is needed to exercise every quartet class).
-Inputs are resident in HBM before timing starts.
+rotations), cc-pVDZ spherical (480 basis functions, 240 shells).
 
-Prints ONE JSON line.  Also reports the FP64 and mixed-precision build rates,
-the roofline fraction of the dominant class kernel (FP64 VALU peak) and the
-oracle's CPU FP64 restatement timed on a bounded sample of the same build.
+1. Two RHF runs through the C++ SCF driver (molecular::SCF::rhf over the HIP
+   engine): plain FP64, then mixed precision (FP32 for quartets with
+   Q_ab Q_cd max|D| below the mixed threshold and for the incremental
+   Delta-D builds, FP64 for the last iterations).  Each is followed by the
+   nuclear gradient (Fock::compute_forces).  Reports seconds, iterations,
+   |dE| and max|dg| between the two.
+2. The timed step: one full J+K build (Fock::formJKdirect semantics,
+   src/fockbuild.cpp:64-164, precision 1e-12) at the converged FP64 density,
+   with D, J and K resident in HBM.  `value` is FP64 builds/s; the mixed
+   build rate and its error are reported beside it.
+3. Roofline of the dominant class kernel of the FP64 build: algorithmic
+   FLOPs from the committed op-count model (molecular2_amd/flop_model.json)
+   over the kernel's average launch time, measured with HIP events recorded
+   on the engine's stream around each launch.  HBM traffic per launch comes
+   from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present.
+4. cpu_baseline: the oracle's OpenMP FP64 restatement of formJKdirect timed on
+   a bounded sample of the same build.
+
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -28,16 +40,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "J+K builds/s and RHF grad s: mixed vs FP64 GPU at |ΔE|<1µEh"
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (= FP64 matrix) peak
-FP32_PEAK_TFLOPS = 157.3
-
-
-def core_guess_density(S, H, nocc):
-    w, U = np.linalg.eigh(S)
-    X = U @ np.diag(w ** -0.5) @ U.T
-    e, C = np.linalg.eigh(X.T @ H @ X)
-    C = X @ C
-    Co = C[:, :nocc]
-    return 2.0 * Co @ Co.T
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "jk_traffic_h2o20.json")
 
 
 def class_flops(model, prof, pure=True):
@@ -78,9 +81,20 @@ def cpu_baseline(mol, D, total_primq, budget_s=15.0):
     rate = done_pq / t_used if t_used > 0 else 0.0
     return {"value": rate / total_primq if total_primq else None, "unit": "J+K builds/s",
             "cores": nthreads, "kind": "port",
-            "sample": "oracle formJKdirect restatement (own OS/HGP ERIs, not libint2) over outer shells "
-                      "s1 in [%d, %d) of the same build: %d quartets / %d primitive quartets in %.1f s, "
+            "sample": "oracle formJKdirect restatement (own OS/HGP ERIs, not libint2), OpenMP over s1, on outer "
+                      "shells s1 in [%d, %d) of the same build: %d quartets / %d primitive quartets in %.1f s, "
                       "extrapolated by primitive-quartet count" % (hi, nsh, done_q, done_pq, t_used)}
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
 
 
 def main():
@@ -89,8 +103,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nwater", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-scf", action="store_true",
+                    help="skip the SCF/gradient runs (J/K timing at the core-guess density)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB device copy (HBM byte-counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -101,17 +119,51 @@ def main():
     from molecular2_amd import molecules as M
     from molecular2_amd.engine import Engine
 
+    dev = torch.device("cuda:0")
+    if a.copy_calibration:
+        src = torch.ones(32 << 20, dtype=torch.float64, device=dev)   # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     mol = M.water_cluster(a.nwater, seed=20)
     shells = mol.shells("cc-pvdz")
     eng = Engine(0)
+    out_scf = None
+    if not a.no_scf:
+        kw = dict(enconverge=1e-8, densconverge=1e-5, maxiter=60)
+        r64 = eng.rhf(mol, "cc-pvdz", pure=True, gradient=True, **kw)
+        rmx = eng.rhf(mol, "cc-pvdz", pure=True, gradient=True, mixed=1, **kw)
+        D = r64["D"]
+        out_scf = {
+            "fp64": {"energy_Eh": r64["energy"], "iterations": r64["iterations"], "converged": r64["converged"],
+                     "scf_s": round(r64["total_seconds"] - r64["grad_seconds"], 3),
+                     "jk_s": round(r64["jk_seconds"], 3), "grad_s": round(r64["grad_seconds"], 3)},
+            "mixed": {"energy_Eh": rmx["energy"], "iterations": rmx["iterations"], "converged": rmx["converged"],
+                      "fp32_builds": rmx["n_fp32"],
+                      "scf_s": round(rmx["total_seconds"] - rmx["grad_seconds"], 3),
+                      "jk_s": round(rmx["jk_seconds"], 3), "grad_s": round(rmx["grad_seconds"], 3)},
+            "abs_dE_Eh": abs(rmx["energy"] - r64["energy"]),
+            "max_abs_dgrad_Eh_per_bohr": float(np.max(np.abs(rmx["grad"] - r64["grad"]))),
+            "scf_speedup": round((r64["total_seconds"] - r64["grad_seconds"]) /
+                                 max(1e-9, rmx["total_seconds"] - rmx["grad_seconds"]), 3),
+            "grad_speedup": round(r64["grad_seconds"] / max(1e-9, rmx["grad_seconds"]), 3),
+            "note": "the gradient runs the FP64 derivative kernels in both cases (on each SCF's density)",
+        }
     t = time.perf_counter()
     eng.set_basis(shells, pure=True)
     t_setup = time.perf_counter() - t
-    S, T, V = eng.one_body(mol.Z, mol.xyz)
-    D = core_guess_density(S, T + V, mol.nel // 2)
+    if a.no_scf:
+        S, T, V = eng.one_body(mol.Z, mol.xyz)
+        w, U = np.linalg.eigh(S)
+        X = U @ np.diag(w ** -0.5) @ U.T
+        _, C = np.linalg.eigh(X.T @ (T + V) @ X)
+        Co = (X @ C)[:, :mol.nel // 2]
+        D = 2.0 * Co @ Co.T
     n = eng.nbf
-    dev = torch.device("cuda:0")
-    dD = torch.from_numpy(D).to(dev)
+    dD = torch.from_numpy(np.ascontiguousarray(D)).to(dev)
     dJ = torch.empty_like(dD)
     dK = torch.empty_like(dD)
     torch.cuda.synchronize()
@@ -120,7 +172,7 @@ def main():
         for _ in range(steps):
             eng.jk_device(dD.data_ptr(), dJ.data_ptr(), dK.data_ptr(), 1.0, 1e-12, mixed)
 
-    # FP64 path
+    # FP64 path (timed region: K builds, HIP events per class launch)
     run(a.warmup, False)
     eng.reset_profile()
     eng.set_profiling(True)
@@ -133,7 +185,8 @@ def main():
     prof = eng.profile()
     J64 = dJ.cpu().numpy()
     K64 = dK.cpu().numpy()
-    # mixed path
+    # mixed path (FP32 below the default split Q_ab Q_cd max|D| < 1e-4)
+    eng.set_mixed_threshold(1e-4)
     run(a.warmup, True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -154,10 +207,11 @@ def main():
     d = cf[dom]
     ms_launch = d["ms"] / max(1, d["launches"])
     achieved = d["flops"] / max(1, d["launches"]) / (ms_launch * 1e-3) / 1e12
+    kname = "k_jk%s double" % d["cls"]
 
     builds64 = a.steps / t64
     buildsmx = a.steps / tmx
-    # energy-type error of the mixed build at this density: E2 = 1/2 tr(D (J - K/2))
+    # two-electron energy of the build at this density: E2 = 1/2 tr(D (J - K/2))
     e64 = 0.5 * np.sum(D * (J64 - 0.5 * K64))
     emx = 0.5 * np.sum(D * (Jm - 0.5 * Km))
     out = {
@@ -173,18 +227,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "(H2O)%d cc-pVDZ direct J+K build (FP64), core-guess density" % a.nwater,
+        "config": {"workload": "(H2O)%d cc-pVDZ direct J+K build (FP64), %s density" %
+                               (a.nwater, "core-guess" if a.no_scf else "converged RHF"),
                    "molecule": mol.name, "nbf": n, "nshells": len(shells), "precision": 1e-12},
-        "roofline": {"bound": "valu", "kernel": "k_jk %s (FP64)" % d["cls"],
+        "roofline": {"bound": "valu", "kernel": kname,
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "ms_per_launch": round(ms_launch, 4)},
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic_for(kname),
+                     "ms_per_launch": round(ms_launch, 4),
+                     "flops_per_launch": d["flops"] / max(1, d["launches"])},
         "fp64": {"builds_per_s": round(builds64, 4), "ms": round(1e3 * t64 / a.steps, 3),
                  "algorithmic_tflops": round(tot_flops / (t64 / a.steps) / 1e12, 3),
                  "quartets": int(tot_q), "prim_quartets": int(tot_primq), "eris": int(tot_eri),
                  "eris_per_s": round(tot_eri / (t64 / a.steps), 1)},
         "mixed": {"builds_per_s": round(buildsmx, 4), "ms": round(1e3 * tmx / a.steps, 3),
-                  "speedup_vs_fp64": round(builds64 and buildsmx / builds64, 3),
+                  "speedup_vs_fp64": round(buildsmx / builds64, 3),
                   "max_abs_dJ": float(np.max(np.abs(Jm - J64))), "max_abs_dK": float(np.max(np.abs(Km - K64))),
                   "abs_dE2_Eh": float(abs(emx - e64))},
         "classes": {v["cls"]: {"ms_per_build": round(v["ms"] / a.steps, 3),
@@ -192,6 +248,8 @@ def main():
                                "quartets": v["quartets"] // a.steps} for v in cf.values()},
         "setup_s": round(t_setup, 3),
     }
+    if out_scf is not None:
+        out["scf"] = out_scf
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol, D, tot_primq, a.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -112,6 +112,7 @@ int np_bucket(int np) {
 JKArgs jk_args(m2_ctx* ctx) {
     JKArgs a{};
     a.prims = ctx->d_prims.p;
+    a.prims_f = ctx->d_prims_f.p;
     a.pair_off = ctx->d_pair_off.p;
     a.pair_np = ctx->d_pair_np.p;
     a.pair_sh = ctx->d_pair_sh.p;
@@ -187,8 +188,23 @@ m2_status build_pairs(m2_ctx* ctx) {
         HIPCHK(ctx->d_pair_np.reserve(npair));
         HIPCHK(ctx->d_pair_sh.reserve(npair));
         HIPCHK(ctx->d_pair_Q.reserve(npair));
-        if (!prims.empty())
+        HIPCHK(ctx->d_prims_f.reserve(prims.size()));
+        if (!prims.empty()) {
             HIPCHK(hipMemcpy(ctx->d_prims.p, prims.data(), sizeof(PrimPair) * prims.size(), hipMemcpyHostToDevice));
+            std::vector<BraPair<float>> pf(prims.size());
+            for (size_t i = 0; i < prims.size(); ++i) {
+                const PrimPair& q = prims[i];
+                BraPair<float>& f = pf[i];
+                f.p = (float)q.p;
+                f.inv2p = (float)q.inv2p;
+                f.K = (float)q.K;
+                for (int k = 0; k < 3; ++k) {
+                    f.P[k] = (float)q.P[k];
+                    f.PA[k] = (float)q.PA[k];
+                }
+            }
+            HIPCHK(hipMemcpy(ctx->d_prims_f.p, pf.data(), sizeof(BraPair<float>) * pf.size(), hipMemcpyHostToDevice));
+        }
         HIPCHK(hipMemcpy(ctx->d_pair_off.p, ctx->pair_off.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_pair_np.p, ctx->pair_np.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_pair_sh.p, ctx->pair_sh.data(), sizeof(int2) * npair, hipMemcpyHostToDevice));
@@ -274,7 +290,20 @@ int count_ge(const std::vector<double>& Q, int b, int e, double thr) {
 
 struct ComboRange {
     int X, Y, t0, tmid, t1;   // FP64 tasks [t0, tmid), FP32 tasks [tmid, t1)
+    int b0, bmid, b1;         // JKBlocks of the FP64 / FP32 tasks
 };
+
+// Group consecutive Tasks of one bra pair into workgroup units of at most
+// JK_BLOCK_TASKS Tasks (the kernel keeps the bra's K rows in LDS across them).
+static void append_blocks(const std::vector<Task>& T, int t0, int t1, std::vector<int2>& B) {
+    int i = t0;
+    while (i < t1) {
+        int j = i + 1;
+        while (j < t1 && j - i < JK_BLOCK_TASKS && T[j].bra == T[i].bra) ++j;
+        B.push_back(make_int2(i, j));
+        i = j;
+    }
+}
 
 // Work list: for every bra class X >= ket class Y, every bra pair i and every
 // primitive-count group of Y, the ket pairs j whose Q_j clears
@@ -290,6 +319,8 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
     std::vector<ComboRange> ranges;
     auto& T = ctx->h_tasks;
     T.clear();
+    auto& B = ctx->h_blocks;
+    B.clear();
     const auto& Q = ctx->pair_Q;
     std::vector<Task> t32;
     for (int X = 0; X < NPAIRCLASS; ++X) {
@@ -300,7 +331,7 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
             const bool reg = gradient ? ((X <= 1 && Y <= X) || (X == 2 && Y == 0) || (X == 3 && Y == 0))
                                       : JK_COMBO[X][Y] != nullptr;
             const int chunk = reg ? 64 : HEAVY_CHUNK;
-            ComboRange r{X, Y, (int)T.size(), 0, 0};
+            ComboRange r{X, Y, (int)T.size(), 0, 0, 0, 0, 0};
             t32.clear();
             for (int i = sx.begin; i < sx.end; ++i) {
                 const double qi = Q[i];
@@ -322,6 +353,13 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
             r.tmid = (int)T.size();
             T.insert(T.end(), t32.begin(), t32.end());
             r.t1 = (int)T.size();
+            if (reg && !gradient) {
+                r.b0 = (int)B.size();
+                append_blocks(T, r.t0, r.tmid, B);
+                r.bmid = (int)B.size();
+                append_blocks(T, r.tmid, r.t1, B);
+                r.b1 = (int)B.size();
+            }
             if (r.t1 > r.t0) ranges.push_back(r);
         }
     }
@@ -351,6 +389,11 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
         HIPCHK(hipMemcpyAsync(ctx->d_tasks.p, ctx->h_tasks.data(), sizeof(Task) * ctx->h_tasks.size(),
                               hipMemcpyHostToDevice, ctx->stream));
     }
+    if (!ctx->h_blocks.empty()) {
+        HIPCHK(ctx->d_blocks.reserve(ctx->h_blocks.size()));
+        HIPCHK(hipMemcpyAsync(ctx->d_blocks.p, ctx->h_blocks.data(), sizeof(int2) * ctx->h_blocks.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+    }
     launch_zero(ctx->d_J.p, (size_t)n * n, ctx->stream);
     launch_zero(ctx->d_K.p, (size_t)n * n, ctx->stream);
     JKArgs a = jk_args(ctx);
@@ -365,11 +408,6 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
             ctx->ev.push_back(e);
         }
     }
-    int maxnp = 1;
-    for (int x : ctx->pair_np) maxnp = std::max(maxnp, x);
-    ARGCHK(maxnp <= BRA_LDS_MAX, "a shell pair has more primitive pairs than the J/K kernels stage in LDS (" +
-                                     std::to_string(BRA_LDS_MAX) + ")");
-    a.bra_lds = maxnp;
     a.D = dD;
     a.J = ctx->d_J.p;
     a.K = ctx->d_K.p;
@@ -386,7 +424,10 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
             if (b1 <= b0) continue;
             const Task* tp = ctx->d_tasks.p + b0;
             if (JK_COMBO[r.X][r.Y]) {
-                JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), tp, b1 - b0, mode, ctx->stream);
+                // block ranges index the whole task array
+                const int k0 = mode == 0 ? r.b0 : r.bmid, k1 = mode == 0 ? r.bmid : r.b1;
+                a.blocks = ctx->d_blocks.p + k0;
+                JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p, k1 - k0, mode, ctx->stream);
                 HIPCHK(hipGetLastError());
             } else {
                 HIPCHK(launch_jk_heavy(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1],
@@ -490,6 +531,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
         b->d_nbf.release();
     }
     ctx->d_prims.release();
+    ctx->d_prims_f.release();
     ctx->d_pair_off.release();
     ctx->d_pair_np.release();
     ctx->d_pair_sh.release();
@@ -503,6 +545,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_Ko.release();
     ctx->d_Dn.release();
     ctx->d_tasks.release();
+    ctx->d_blocks.release();
     ctx->d_counters.release();
     for (auto* x : {&ctx->d_B64, &ctx->d_dfV, &ctx->d_dfC, &ctx->d_dfX, &ctx->d_dfG, &ctx->d_dfJ, &ctx->d_dfK}) x->release();
     for (auto* x : {&ctx->d_B32, &ctx->d_dfC32, &ctx->d_dfX32, &ctx->d_dfG32, &ctx->d_dfJ32, &ctx->d_dfK32}) x->release();

@@ -74,6 +74,7 @@ struct m2_ctx {
     std::vector<double> Qsh;   // nsh x nsh
     m2::PairClassSeg cls[m2::NPAIRCLASS];
     m2::DevBuf<m2::PrimPair> d_prims;
+    m2::DevBuf<m2::BraPair<float>> d_prims_f;
     m2::DevBuf<int> d_pair_off, d_pair_np;
     m2::DevBuf<int2> d_pair_sh;
     m2::DevBuf<double> d_pair_Q;
@@ -84,6 +85,8 @@ struct m2_ctx {
     m2::DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
     m2::DevBuf<m2::Task> d_tasks;
     std::vector<m2::Task> h_tasks;
+    m2::DevBuf<int2> d_blocks;   // JKBlocks of the register-class launches
+    std::vector<int2> h_blocks;
     std::vector<double> h_Dn;
     double fp32_below = 1e-4;
     long long last_tasks = 0;

@@ -246,9 +246,9 @@ def gen_class(la, lb, lc, ld, deriv=False):
     em.out("template <> struct EriClass<%d, %d, %d, %d> {" % (la, lb, lc, ld))
     em.out("  static constexpr int L = %d;" % L)
     em.out("  static constexpr int NCART = %d;" % (na * nb * nc * nd))
-    em.out("  template <typename R, bool PURE, typename BP>")
-    em.out("  static __device__ __forceinline__ void run(const BP* __restrict__ bra, int nbra,")
-    em.out("      const PrimPair* __restrict__ ket, int nket, const R ABx, const R ABy, const R ABz,")
+    em.out("  template <typename R, bool PURE, typename BPtr, typename KPtr>")
+    em.out("  static __device__ __forceinline__ void run(BPtr bra, int nbra,")
+    em.out("      KPtr ket, int nket, const R ABx, const R ABy, const R ABz,")
     em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, R* __restrict__ out) {")
     for e in E:
         for f in F:
@@ -256,14 +256,14 @@ def gen_class(la, lb, lc, ld, deriv=False):
     # ket primitive pairs (different per lane) outside, bra primitive pairs
     # (the same for the whole wavefront: scalar loads) inside
     em.out("    for (int ik = 0; ik < nket; ++ik) {")
-    em.out("      const PrimPair kp = ket[ik];")
+    em.out("      const auto kp = ket[ik];")
     em.out("      const R q = R(kp.p), Kcd = R(kp.K);")
     em.out("      const R Qx = R(kp.P[0]), Qy = R(kp.P[1]), Qz = R(kp.P[2]);")
     if lc + ld > 0:
         em.out("      const R inv2q = R(kp.inv2p);")
         em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
     em.out("      for (int ib = 0; ib < nbra; ++ib) {")
-    em.out("        const BP bp = bra[ib];")
+    em.out("        const auto& bp = bra[ib];")
     em.out("        const R p = R(bp.p), Kab = R(bp.K);")
     if la + lb > 0:
         em.out("        const R inv2p = R(bp.inv2p);")

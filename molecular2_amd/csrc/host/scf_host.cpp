@@ -326,6 +326,8 @@ void Fock::formJKdirect(const Matrix& Schwarz, Matrix& D, double multiplier) {
     auto t0 = std::chrono::steady_clock::now();
     check(ctx, m2_jk_direct(ctx, D.data(), multiplier, precision, flags, jints.data(), kints.data()), "m2_jk_direct");
     jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // leave the context's split where the caller set it (an all-FP32 Delta-D build changes it)
+    if (flags == M2_MIXED) check(ctx, m2_set_mixed_threshold(ctx, opt.mixed_threshold), "m2_set_mixed_threshold");
     jkints = jints - 0.5 * kints;
 }
 

@@ -1,16 +1,21 @@
 // Direct J/K shell-quartet kernels (templates; instantiated per class in the
 // generated translation units).
 //
-// Work unit: one Task = one wavefront = one bra shell pair (uniform across the
-// wave, read into SGPRs) x up to 64 consecutive ket shell pairs of one class
-// and one primitive-count group (so every lane runs the same primitive loop).
+// Work unit: one Task = one bra shell pair x up to 64 consecutive ket shell
+// pairs of one class and one primitive-count group (so every lane runs the
+// same primitive loop); one lane per ket pair.  A workgroup owns one bra
+// pair and a run of its Tasks (a JKBlock), which its waves take in turn.
 // Each lane applies the exact screening of Fock::formJKdirect
 // (src/fockbuild.cpp:98-115): max of the six shell-block norms of D times
 // Q_ab Q_cd against `precision`, computes its contracted quartet with the
-// generated OS/HGP code, and digests it into unsymmetrised J and K with the
-// degeneracy factors of src/fockbuild.cpp:121-149.  The bra-pair J block is
-// summed across the wavefront before its atomics; the other five blocks are
-// per-lane FP64 atomics (hardware global_atomic_add_f64).
+// generated OS/HGP code, and digests it with the degeneracy factors of
+// src/fockbuild.cpp:121-149 into unsymmetrised J and K:
+//   J(ab)  per-lane register accumulators over every Task of the wave,
+//          summed across the wave once at the end;
+//   K(a.), K(b.)  (every K contribution has a bra shell as its row) LDS rows
+//          of the workgroup (ds_add_f64), flushed once to HBM with global
+//          FP64 atomics; when the rows do not fit in LDS, direct atomics;
+//   J(cd)  per-lane global FP64 atomics (hardware global_atomic_add_f64).
 #pragma once
 #include "m2_common.hpp"
 #include "boys.hpp"
@@ -36,30 +41,33 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// K-row accumulation: LDS row (row < NA: shell a, else shell b) or HBM
+__device__ __forceinline__ void krow_add(bool lds, double* __restrict__ sK, double* __restrict__ K, int n, int row,
+                                         int grow, int col, double v) {
+    if (lds)
+        __hip_atomic_fetch_add((__attribute__((address_space(3))) double*)(sK + row * n + col), v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);   // ds_add_f64
+    else
+        atomic_add_f64(&K[grow * n + col], v);
+}
+
 template <int NA, int NB, int NC, int ND, typename R>
-__device__ __forceinline__ void digest(const R* __restrict__ v, bool active, int fa, int fb, int fc, int fd,
-                                       double f, const double* __restrict__ D, int n,
-                                       double* __restrict__ J, double* __restrict__ K) {
+__device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, int fc, int fd, double f,
+                                       const double* __restrict__ Dab, double* __restrict__ jab,
+                                       const double* __restrict__ D, int n, double* __restrict__ J,
+                                       double* __restrict__ K, bool lds, double* __restrict__ sK) {
     const double h = 0.5 * f;
-    // J(ab) += f sum_cd D(cd) v : bra block shared by the whole wave
+    // J(ab) += f sum_cd D(cd) v  (lane registers)
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            double s = 0.0;
-            if (active) {
+        for (int d = 0; d < ND; ++d) {
+            const double dcd = f * D[(fc + c) * n + fd + d];
 #pragma unroll
-                for (int c = 0; c < NC; ++c)
+            for (int a = 0; a < NA; ++a)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d)
-                        s += D[(fc + c) * n + fd + d] * double(v[((a * NB + b) * NC + c) * ND + d]);
-                s *= f;
-            }
-            s = wave_sum(s);
-            if (__lane_id() == 0 && s != 0.0) atomic_add_f64(&J[(fa + a) * n + fb + b], s);
+                for (int b = 0; b < NB; ++b) jab[a * NB + b] += dcd * double(v[((a * NB + b) * NC + c) * ND + d]);
         }
-    }
-    if (!active) return;
     // J(cd) += f sum_ab D(ab) v
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -69,7 +77,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, bool active, int
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
-                for (int b = 0; b < NB; ++b) s += D[(fa + a) * n + fb + b] * double(v[((a * NB + b) * NC + c) * ND + d]);
+                for (int b = 0; b < NB; ++b) s += Dab[a * NB + b] * double(v[((a * NB + b) * NC + c) * ND + d]);
             atomic_add_f64(&J[(fc + c) * n + fd + d], f * s);
         }
     // K(ac) += h sum_bd D(bd) v
@@ -82,7 +90,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, bool active, int
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) s += D[(fb + b) * n + fd + d] * double(v[((a * NB + b) * NC + c) * ND + d]);
-            atomic_add_f64(&K[(fa + a) * n + fc + c], h * s);
+            krow_add(lds, sK, K, n, a, fa + a, fc + c, h * s);
         }
     // K(bd) += h sum_ac D(ac) v
 #pragma unroll
@@ -94,7 +102,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, bool active, int
             for (int a = 0; a < NA; ++a)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) s += D[(fa + a) * n + fc + c] * double(v[((a * NB + b) * NC + c) * ND + d]);
-            atomic_add_f64(&K[(fb + b) * n + fd + d], h * s);
+            krow_add(lds, sK, K, n, NA + b, fb + b, fd + d, h * s);
         }
     // K(ad) += h sum_bc D(bc) v
 #pragma unroll
@@ -106,7 +114,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, bool active, int
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) s += D[(fb + b) * n + fc + c] * double(v[((a * NB + b) * NC + c) * ND + d]);
-            atomic_add_f64(&K[(fa + a) * n + fd + d], h * s);
+            krow_add(lds, sK, K, n, a, fa + a, fd + d, h * s);
         }
     // K(bc) += h sum_ad D(ad) v
 #pragma unroll
@@ -118,75 +126,113 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, bool active, int
             for (int a = 0; a < NA; ++a)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) s += D[(fa + a) * n + fd + d] * double(v[((a * NB + b) * NC + c) * ND + d]);
-            atomic_add_f64(&K[(fb + b) * n + fc + c], h * s);
+            krow_add(lds, sK, K, n, NA + b, fb + b, fc + c, h * s);
         }
 }
 
+template <typename R>
+struct PrimPairOf;
+template <>
+struct PrimPairOf<double> {
+    using type = PrimPair;
+    static __device__ __forceinline__ const PrimPair* get(const JKArgs& a) { return a.prims; }
+};
+template <>
+struct PrimPairOf<float> {
+    using type = BraPair<float>;
+    static __device__ __forceinline__ const BraPair<float>* get(const JKArgs& a) { return a.prims_f; }
+};
+
 // Quartets are computed when lo <= bound < hi (the launchers pass
 // [precision, inf); precision is chosen per task list).
+// Dynamic LDS (when a.k_lds): the (NA + NB) x n FP64 K rows of shells a and b.
 template <int LA, int LB, int LC, int LD, typename R, bool PURE>
-__global__ __launch_bounds__(256) void k_jk(JKArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
-                                            double lo, double hi) {
+__global__ __launch_bounds__(JK_WAVES * 64) void k_jk(JKArgs a, BoysTables tab, const Task* __restrict__ tasks,
+                                                      double lo, double hi) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NC = NFun<LC, PURE>::value, ND = NFun<LD, PURE>::value;
-    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    if (w >= ntasks) return;
+    const int2 rng = a.blocks[blockIdx.x];
+    const int t0 = __builtin_amdgcn_readfirstlane(rng.x), t1 = __builtin_amdgcn_readfirstlane(rng.y);
     const int lane = __lane_id();
-    const int bra = __builtin_amdgcn_readfirstlane(tasks[w].bra);
-    const int ket0 = __builtin_amdgcn_readfirstlane(tasks[w].ket0);
-    const int nket = __builtin_amdgcn_readfirstlane(tasks[w].nket);
-    const int ket = ket0 + (lane < nket ? lane : 0);
+    const int wave = threadIdx.x >> 6;
+    const int bra = __builtin_amdgcn_readfirstlane(tasks[t0].bra);
     const int2 sab = a.pair_sh[bra];
-    const int2 scd = a.pair_sh[ket];
-    const int sa = sab.x, sb = sab.y, sc = scd.x, sd = scd.y;
-    const int nsh = a.nsh;
-    // exact screening of src/fockbuild.cpp:98-115
-    const double* Dn = a.Dn;
-    double dn = fmax(fmax(Dn[sa * nsh + sb], Dn[sc * nsh + sd]),
-                     fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
-    const double bound = dn * a.pair_Q[bra] * a.pair_Q[ket];
-    const bool active = lane < nket && bound >= lo && bound < hi;
-    if (!__any(active)) return;
-    if (a.counters) {
-        const int na = __popcll(__ballot(active));
-        const double nk = wave_sum(active ? double(a.pair_np[ket]) : 0.0);
-        if (lane == 0) {
-            atomicAdd(&a.counters[2 * a.combo], (unsigned long long)na);
-            atomicAdd(&a.counters[2 * a.combo + 1], (unsigned long long)(nk * a.pair_np[bra] + 0.5));
-        }
-    }
-    const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
-    // the bra's primitive pairs are the same for the whole wavefront: stage
-    // them in LDS once (converted to R), so the inner primitive loop reads
-    // LDS broadcasts
+    const int sa = __builtin_amdgcn_readfirstlane(sab.x), sb = __builtin_amdgcn_readfirstlane(sab.y);
+    const int n = a.n, nsh = a.nsh;
+    const int fa = a.sh_bf[sa], fb = a.sh_bf[sb];
+    const bool lds = a.k_lds != 0;
+    // primitive pairs in the kernel's arithmetic type; the bra's are read
+    // through the constant address space (never written by the kernel), so
+    // the uniform bra loop loads them with scalar loads into SGPRs
+    using PP = typename PrimPairOf<R>::type;
+    const PP* gp = PrimPairOf<R>::get(a);
+    typedef const __attribute__((address_space(4))) PP cPP;
     const int nbp = a.pair_np[bra];
-    const PrimPair* gb = a.prims + a.pair_off[bra];
-    extern __shared__ float4 s_raw[];
-    BraPair<R>* bprims = reinterpret_cast<BraPair<R>*>(s_raw) + (threadIdx.x >> 6) * a.bra_lds;
-    for (int i = lane; i < nbp; i += 64) {
-        const PrimPair g = gb[i];
-        BraPair<R> b;
-        b.p = R(g.p);
-        b.inv2p = R(g.inv2p);
-        b.K = R(g.K);
-        for (int k = 0; k < 3; ++k) {
-            b.P[k] = R(g.P[k]);
-            b.PA[k] = R(g.PA[k]);
+    cPP* bprims = (cPP*)(gp + a.pair_off[bra]);
+    extern __shared__ double sK[];
+    if (lds)
+        for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
+    __syncthreads();
+    const double* D = a.D;
+    double Dab[NA * NB], jab[NA * NB];
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            Dab[x * NB + y] = D[(fa + x) * n + fb + y];
+            jab[x * NB + y] = 0.0;
         }
-        bprims[i] = b;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const double* O = a.sh_O;
     const R ABx = R(O[3 * sa] - O[3 * sb]), ABy = R(O[3 * sa + 1] - O[3 * sb + 1]), ABz = R(O[3 * sa + 2] - O[3 * sb + 2]);
-    const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
-    R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
-    const int nkp = active ? a.pair_np[ket] : 0;
-    EriClass<LA, LB, LC, LD>::template run<R, PURE>(bprims, nbp, a.prims + a.pair_off[ket], nkp,
-                                                    ABx, ABy, ABz, CDx, CDy, CDz, tab, v);
-    digest<NA, NB, NC, ND, R>(v, active, a.sh_bf[sa], a.sh_bf[sb], a.sh_bf[sc], a.sh_bf[sd],
-                              deg * a.scale, a.D, a.n, a.J, a.K);
+    const double* Dn = a.Dn;
+    const double dnab = Dn[sa * nsh + sb];
+    const double qab = a.pair_Q[bra];
+    const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
+    for (int t = t0 + wave; t < t1; t += JK_WAVES) {
+        const int ket0 = __builtin_amdgcn_readfirstlane(tasks[t].ket0);
+        const int nket = __builtin_amdgcn_readfirstlane(tasks[t].nket);
+        const int ket = ket0 + (lane < nket ? lane : 0);
+        const int2 scd = a.pair_sh[ket];
+        const int sc = scd.x, sd = scd.y;
+        // exact screening of src/fockbuild.cpp:98-115
+        const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]),
+                               fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+        const double bound = dn * qab * a.pair_Q[ket];
+        const bool active = lane < nket && bound >= lo && bound < hi;
+        if (!__any(active)) continue;
+        if (a.counters) {
+            const int na = __popcll(__ballot(active));
+            const double nk = wave_sum(active ? double(a.pair_np[ket]) : 0.0);
+            if (lane == 0) {
+                atomicAdd(&a.counters[2 * a.combo], (unsigned long long)na);
+                atomicAdd(&a.counters[2 * a.combo + 1], (unsigned long long)(nk * nbp + 0.5));
+            }
+        }
+        const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
+        R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
+        const int nkp = active ? a.pair_np[ket] : 0;
+        EriClass<LA, LB, LC, LD>::template run<R, PURE>(bprims, nbp, gp + a.pair_off[ket], nkp,
+                                                        ABx, ABy, ABz, CDx, CDy, CDz, tab, v);
+        if (active) {
+            const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
+            digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK);
+        }
+    }
+    // J(ab): one wave sum per function
+#pragma unroll
+    for (int x = 0; x < NA * NB; ++x) {
+        const double s = wave_sum(jab[x]);
+        if (lane == 0 && s != 0.0) atomic_add_f64(&a.J[(fa + x / NB) * n + fb + x % NB], s);
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) {
+        const double s = sK[i];
+        if (s != 0.0) {
+            const int row = i / n, col = i - row * n;
+            atomic_add_f64(&a.K[(row < NA ? fa + row : fb + row - NA) * n + col], s);
+        }
+    }
 }
 
 // Schwarz: Q(pair) = sqrt(max_ab |(ab|ab)|), one lane per pair
@@ -213,20 +259,21 @@ __global__ __launch_bounds__(256) void k_schwarz(JKArgs a, BoysTables tab, int p
 
 // mode 0: FP64 kernel; mode 2: FP32 kernel (the caller hands each the tasks
 // of its precision); both compute every quartet with bound >= precision.
+// `nblocks` JKBlocks in a.blocks index into `tasks`.
 template <int LA, int LB, int LC, int LD, bool PURE>
-void launch_jk(const JKArgs& a, const BoysTables& tab, const Task* tasks, int ntasks, int mode, hipStream_t st) {
-    if (ntasks <= 0) return;
-    const int waves_per_block = 4;
-    dim3 grid((ntasks + waves_per_block - 1) / waves_per_block), block(64 * waves_per_block);
+void launch_jk(const JKArgs& a0, const BoysTables& tab, const Task* tasks, int nblocks, int mode, hipStream_t st) {
+    if (nblocks <= 0) return;
+    constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
+    JKArgs a = a0;
+    const size_t kbytes = sizeof(double) * (size_t)(NA + NB) * a.n;
+    a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
     const double inf = __builtin_huge_val();
+    const dim3 grid(nblocks), block(64 * JK_WAVES);
+    const size_t lds = a.k_lds ? kbytes : 0;
     if (mode == 2)
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, block,
-                           sizeof(BraPair<float>) * (size_t)a.bra_lds * waves_per_block, st, a, tab, tasks, ntasks,
-                           a.precision, inf);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, block, lds, st, a, tab, tasks, a.precision, inf);
     else
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, block,
-                           sizeof(BraPair<double>) * (size_t)a.bra_lds * waves_per_block, st, a, tab, tasks, ntasks,
-                           a.precision, inf);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, block, lds, st, a, tab, tasks, a.precision, inf);
 }
 
 template <int LA, int LB, bool PURE>

@@ -20,7 +20,7 @@ struct alignas(16) PrimPair {
 static_assert(sizeof(PrimPair) == 80, "PrimPair layout");
 
 // The fields of a PrimPair that the J/K kernels read, in the kernel's
-// arithmetic type; the bra pairs of a wavefront are staged in LDS in this form.
+// arithmetic type (the FP32 kernels read an FP32 copy of the pair records).
 template <typename R>
 struct alignas(16) BraPair {
     R p, inv2p, K;
@@ -28,8 +28,6 @@ struct alignas(16) BraPair {
     R PA[3];
 };
 static_assert(sizeof(BraPair<double>) == 80 && sizeof(BraPair<float>) == 48, "BraPair layout");
-// LDS budget of the bra staging: primitive pairs per shell pair
-constexpr int BRA_LDS_MAX = 500;
 
 // Boys table: F_m(T_k) at T_k = k * BOYS_STEP, k = 0..BOYS_NGRID-1,
 // m = 0..BOYS_MTAB-1.  Beyond BOYS_TMAX the asymptotic branch is used.
@@ -40,6 +38,7 @@ constexpr double BOYS_TMAX = 60.0;
 
 struct JKArgs {
     const PrimPair* prims;
+    const BraPair<float>* prims_f;   // the same records in FP32
     const int* pair_off;     // first PrimPair of each shell pair
     const int* pair_np;      // number of PrimPairs of each shell pair
     const int2* pair_sh;     // (a, b) shell indices, la >= lb
@@ -57,8 +56,15 @@ struct JKArgs {
     double scale;            // multiplier
     unsigned long long* counters;  // profiling only: per class combo {quartets, primitive quartets}
     int combo;               // 6 * X + Y of the launch
-    int bra_lds;             // LDS capacity per wavefront in primitive pairs (>= every pair_np)
+    const int2* blocks;      // JKBlocks: [first, last) Task of one workgroup, all with the same bra
+    int k_lds;               // set by the launcher: K rows of the bra shells accumulated in LDS
 };
+
+// workgroup shape of the direct J/K class kernels, tasks per workgroup, and
+// the LDS budget of the K-row accumulators
+constexpr int JK_WAVES = 8;
+constexpr int JK_BLOCK_TASKS = 64;
+constexpr size_t JK_KROW_LDS_MAX = 96 * 1024;
 
 struct Task {                // one wavefront of quartets: bra pair x 1..64 ket pairs
     int bra;
