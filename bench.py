@@ -172,7 +172,7 @@ def main():
         for _ in range(steps):
             eng.jk_device(dD.data_ptr(), dJ.data_ptr(), dK.data_ptr(), 1.0, 1e-12, mixed)
 
-    # FP64 path (timed region: K builds, HIP events per class launch)
+    # FP64 path (timed region: K builds, HIP events around each class launch)
     run(a.warmup, False)
     eng.reset_profile()
     eng.set_profiling(True)
@@ -182,9 +182,17 @@ def main():
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t0
     eng.set_profiling(False)
-    prof = eng.profile()
+    prof_t = eng.profile()
     J64 = dJ.cpu().numpy()
     K64 = dK.cpu().numpy()
+    # computed quartets / primitive quartets per class: device counters on one
+    # extra (untimed) build of the same density
+    eng.reset_profile()
+    eng.set_profiling(True, counters=True)
+    run(1, False)
+    eng.set_profiling(False)
+    prof_c = eng.profile()
+    prof = {k: (ms, nl, prof_c[k][2] * a.steps, prof_c[k][3] * a.steps) for k, (ms, nl, _, _) in prof_t.items()}
     # mixed path (FP32 below the default split Q_ab Q_cd max|D| < 1e-4)
     eng.set_mixed_threshold(1e-4)
     run(a.warmup, True)

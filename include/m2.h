@@ -138,7 +138,8 @@ m2_status m2_gemm(m2_ctx* ctx, char ta, char tb, int m, int n, int k, double alp
  * classes ss, ps, pp, ds, dp, dd) is bracketed by HIP events on the context
  * stream and its computed quartets / primitive quartets are counted on the
  * device.  m2_get_profile fills 36-entry arrays accumulated since the last
- * reset. */
+ * reset.  on = 1: events only (timings undisturbed); on = 2: events and the
+ * device quartet counters (these add atomics and distort the timings). */
 m2_status m2_set_profiling(m2_ctx* ctx, int on);
 m2_status m2_get_profile(m2_ctx* ctx, double* ms, long long* launches, long long* quartets,
                          long long* prim_quartets);

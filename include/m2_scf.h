@@ -1,0 +1,61 @@
+/*
+ * m2_scf.h -- extern "C" driver of the host C++ RHF (molecular::SCF::rhf over
+ * the HIP engine), for callers that cannot link C++ (ctypes, other FFIs).
+ *
+ * C++ callers use the classes in molecular2_amd/csrc/host/scf_host.hpp
+ * directly (IntegralEngine, Fock, SCF), which keep the reference's call
+ * surface (inc/integrals.hpp:108, inc/fock.hpp:131-188, inc/scf.hpp:36-57).
+ */
+#ifndef M2_SCF_H
+#define M2_SCF_H
+
+#include "m2.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Options of SCF::rhf and Fock (defaults of the reference in
+ * src/ProgramController.cpp:295-307 and src/fock.cpp:32-73):
+ * diis 1, maxdiis 8, maxiter 40, enconverge 1e-7, densconverge 1e-4,
+ * precision 1e-12 (formJKdirect screening), df 0 (1: density-fitted J/K with
+ * the auxiliary shells), mixed 0 (1: FP32 quartets below mixed_threshold and
+ * FP32 incremental builds, FP64 for the last fp64_final iterations). */
+typedef struct {
+    int diis, maxdiis, maxiter;
+    double enconverge, densconverge, precision;
+    int df, mixed;
+    double mixed_threshold;
+    int fp64_final;
+} m2_scf_options_c;
+
+/* energy = one_e + two_e + enuc (SCF::calcE, src/scf.cpp:29-48); counts of
+ * full / incremental / FP32 J/K builds; wall seconds of the J/K builds, the
+ * whole call, and the gradient. */
+typedef struct {
+    double energy, one_e, two_e, enuc;
+    int iterations, converged, n_full, n_incremental, n_fp32;
+    double jk_seconds, total_seconds, grad_seconds;
+} m2_scf_result_c;
+
+/* SCF::rhf (src/scf.cpp:90-185) on a molecule given as natom nuclear charges
+ * Z and positions xyz (bohr, 3*natom), charge and multiplicity, and its
+ * orbital shells (nsh, l, nprim, exps, raw coefs, centers 3*nsh, atoms) plus
+ * optional auxiliary shells (nsh_aux may be 0).  Outputs may be NULL: D
+ * (n x n density), C (n x n MO coefficients, column-major), eps (n orbital
+ * energies), log5 (maxiter x 5: energy, dE, dD, incremental, fp32 per
+ * iteration), grad (natom x 3: Fock::compute_forces, src/rhfgrad.cpp:15-89).
+ * Returns 0 on success; otherwise m2_rhf_last_error() says why.  A run that
+ * does not converge returns 0 with res->converged = 0, as the reference logs
+ * and continues. */
+int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charge, int multiplicity, int pure,
+           int nsh, const int* l, const int* nprim, const double* exps, const double* coefs, const double* centers,
+           const int* atoms, int nsh_aux, const int* l_aux, const int* nprim_aux, const double* exps_aux,
+           const double* coefs_aux, const double* centers_aux, const int* atoms_aux, const m2_scf_options_c* o,
+           m2_scf_result_c* res, double* D, double* C, double* eps, double* log5, double* grad);
+const char* m2_rhf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

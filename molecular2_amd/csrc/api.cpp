@@ -401,7 +401,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     if (ctx->prof) {
         HIPCHK(ctx->d_counters.reserve(72));
         launch_zero((double*)ctx->d_counters.p, 72, ctx->stream);
-        a.counters = ctx->d_counters.p;
+        if (ctx->prof_counters) a.counters = ctx->d_counters.p;
         while (ctx->ev.size() < 2 * ranges.size() + 2) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
@@ -746,6 +746,7 @@ m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below) {
 m2_status m2_set_profiling(m2_ctx* ctx, int on) {
     if (!ctx) return M2_ERR_ARG;
     ctx->prof = on != 0;
+    ctx->prof_counters = on >= 2;
     return M2_OK;
 }
 

@@ -91,7 +91,8 @@ struct m2_ctx {
     double fp32_below = 1e-4;
     long long last_tasks = 0;
     // profiling (m2_set_profiling): per class combo kernel time and counters
-    bool prof = false;
+    bool prof = false;            // HIP events around each class launch
+    bool prof_counters = false;   // also count computed quartets on the device (slows the kernels)
     std::vector<hipEvent_t> ev;
     double prof_ms[36] = {0};
     long long prof_launch[36] = {0};

@@ -246,6 +246,7 @@ def gen_class(la, lb, lc, ld, deriv=False):
     em.out("template <> struct EriClass<%d, %d, %d, %d> {" % (la, lb, lc, ld))
     em.out("  static constexpr int L = %d;" % L)
     em.out("  static constexpr int NCART = %d;" % (na * nb * nc * nd))
+    em.out("  static constexpr int NACC = %d;   // VRR accumulators per lane" % (len(E) * len(F)))
     em.out("  template <typename R, bool PURE, typename BPtr, typename KPtr>")
     em.out("  static __device__ __forceinline__ void run(BPtr bra, int nbra,")
     em.out("      KPtr ket, int nket, const R ABx, const R ABy, const R ABz,")

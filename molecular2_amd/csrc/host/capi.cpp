@@ -1,24 +1,12 @@
 // extern "C" driver of the host C++ SCF (molecular::SCF::rhf), for callers
 // that cannot link C++ (the Python tests and bench use it through ctypes).
+#include <chrono>
 #include <cstring>
 #include <exception>
 #include <string>
 
+#include "m2_scf.h"
 #include "scf_host.hpp"
-
-struct m2_scf_options_c {
-    int diis, maxdiis, maxiter;
-    double enconverge, densconverge, precision;
-    int df, mixed;
-    double mixed_threshold;
-    int fp64_final;
-};
-
-struct m2_scf_result_c {
-    double energy, one_e, two_e, enuc;
-    int iterations, converged, n_full, n_incremental, n_fp32;
-    double jk_seconds, total_seconds, grad_seconds;
-};
 
 static thread_local std::string g_rhf_error;
 

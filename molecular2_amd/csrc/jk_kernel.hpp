@@ -143,11 +143,20 @@ struct PrimPairOf<float> {
     static __device__ __forceinline__ const BraPair<float>* get(const JKArgs& a) { return a.prims_f; }
 };
 
+// Waves per workgroup: 8 (two per SIMD, <= 256 registers per wave) for
+// classes whose VRR accumulators fit comfortably; 4 (one per SIMD, so a wave
+// may use the 256 AGPRs as well) for the register-heavy ones.
+template <int LA, int LB, int LC, int LD, typename R>
+struct JKWaves {
+    static constexpr int value = EriClass<LA, LB, LC, LD>::NACC * (int)sizeof(R) <= 280 ? 8 : 4;
+};
+
 // Quartets are computed when lo <= bound < hi (the launchers pass
 // [precision, inf); precision is chosen per task list).
 // Dynamic LDS (when a.k_lds): the (NA + NB) x n FP64 K rows of shells a and b.
 template <int LA, int LB, int LC, int LD, typename R, bool PURE>
-__global__ __launch_bounds__(JK_WAVES * 64) void k_jk(JKArgs a, BoysTables tab, const Task* __restrict__ tasks,
+__global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk(JKArgs a, BoysTables tab,
+                                                                        const Task* __restrict__ tasks,
                                                       double lo, double hi) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NC = NFun<LC, PURE>::value, ND = NFun<LD, PURE>::value;
@@ -188,7 +197,8 @@ __global__ __launch_bounds__(JK_WAVES * 64) void k_jk(JKArgs a, BoysTables tab, 
     const double dnab = Dn[sa * nsh + sb];
     const double qab = a.pair_Q[bra];
     const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
-    for (int t = t0 + wave; t < t1; t += JK_WAVES) {
+    unsigned long long cnt_q = 0, cnt_pq = 0;   // profiling counters of this wave
+    for (int t = t0 + wave; t < t1; t += JKWaves<LA, LB, LC, LD, R>::value) {
         const int ket0 = __builtin_amdgcn_readfirstlane(tasks[t].ket0);
         const int nket = __builtin_amdgcn_readfirstlane(tasks[t].nket);
         const int ket = ket0 + (lane < nket ? lane : 0);
@@ -201,12 +211,8 @@ __global__ __launch_bounds__(JK_WAVES * 64) void k_jk(JKArgs a, BoysTables tab, 
         const bool active = lane < nket && bound >= lo && bound < hi;
         if (!__any(active)) continue;
         if (a.counters) {
-            const int na = __popcll(__ballot(active));
-            const double nk = wave_sum(active ? double(a.pair_np[ket]) : 0.0);
-            if (lane == 0) {
-                atomicAdd(&a.counters[2 * a.combo], (unsigned long long)na);
-                atomicAdd(&a.counters[2 * a.combo + 1], (unsigned long long)(nk * nbp + 0.5));
-            }
+            cnt_q += __popcll(__ballot(active));
+            cnt_pq += (unsigned long long)(wave_sum(active ? double(a.pair_np[ket]) : 0.0) * nbp + 0.5);
         }
         const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
         R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
@@ -217,6 +223,10 @@ __global__ __launch_bounds__(JK_WAVES * 64) void k_jk(JKArgs a, BoysTables tab, 
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
             digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK);
         }
+    }
+    if (a.counters && lane == 0 && cnt_q) {
+        atomicAdd(&a.counters[2 * a.combo], cnt_q);
+        atomicAdd(&a.counters[2 * a.combo + 1], cnt_pq);
     }
     // J(ab): one wave sum per function
 #pragma unroll
@@ -268,12 +278,14 @@ void launch_jk(const JKArgs& a0, const BoysTables& tab, const Task* tasks, int n
     const size_t kbytes = sizeof(double) * (size_t)(NA + NB) * a.n;
     a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
     const double inf = __builtin_huge_val();
-    const dim3 grid(nblocks), block(64 * JK_WAVES);
+    const dim3 grid(nblocks);
     const size_t lds = a.k_lds ? kbytes : 0;
     if (mode == 2)
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, block, lds, st, a, tab, tasks, a.precision, inf);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, dim3(64 * JKWaves<LA, LB, LC, LD, float>::value),
+                           lds, st, a, tab, tasks, a.precision, inf);
     else
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, block, lds, st, a, tab, tasks, a.precision, inf);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, dim3(64 * JKWaves<LA, LB, LC, LD, double>::value),
+                           lds, st, a, tab, tasks, a.precision, inf);
 }
 
 template <int LA, int LB, bool PURE>

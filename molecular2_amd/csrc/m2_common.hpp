@@ -60,9 +60,8 @@ struct JKArgs {
     int k_lds;               // set by the launcher: K rows of the bra shells accumulated in LDS
 };
 
-// workgroup shape of the direct J/K class kernels, tasks per workgroup, and
-// the LDS budget of the K-row accumulators
-constexpr int JK_WAVES = 8;
+// tasks per workgroup of the direct J/K class kernels, and the LDS budget of
+// the K-row accumulators
 constexpr int JK_BLOCK_TASKS = 64;
 constexpr size_t JK_KROW_LDS_MAX = 96 * 1024;
 

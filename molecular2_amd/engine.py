@@ -191,8 +191,10 @@ class Engine:
     def set_mixed_threshold(self, t):
         self._chk(self.L.m2_set_mixed_threshold(self.h, t))
 
-    def set_profiling(self, on=True):
-        self._chk(self.L.m2_set_profiling(self.h, int(on)))
+    def set_profiling(self, on=True, counters=False):
+        """HIP events around every class launch; counters=True also counts
+        computed (primitive) quartets on the device, which slows the kernels."""
+        self._chk(self.L.m2_set_profiling(self.h, (2 if counters else 1) if on else 0))
 
     def reset_profile(self):
         self._chk(self.L.m2_reset_profile(self.h))
