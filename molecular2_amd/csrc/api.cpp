@@ -1,6 +1,7 @@
 // C ABI of the engine (include/m2.h): context, basis and shell-pair set-up,
 // Schwarz bounds, one-electron integrals and the direct J/K driver.
 #include "ctx.hpp"
+#include "jk_lists.hpp"
 #include "generated/sph_tables.inc"
 #include "generated/jk_dispatch.inc"
 
@@ -274,6 +275,36 @@ m2_status build_pairs(m2_ctx* ctx) {
         seg.end = i1;
         i0 = i1;
     }
+    // device work-list buffers of the register-class combinations (sizes
+    // depend on the basis only)
+    for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
+        for (int Y = 0; Y <= X; ++Y, ++ci) {
+            JKList& L = ctx->lists[ci];
+            L = JKList{};
+            const PairClassSeg &sx = ctx->cls[X], &sy = ctx->cls[Y];
+            if (!JK_COMBO[X][Y] || sx.end <= sx.begin || sy.end <= sy.begin) continue;
+            ARGCHK((int)sy.groups.size() <= JK_MAXG, "m2_basis_set: too many primitive-count groups");
+            L.bra0 = sx.begin;
+            L.nbra = sx.end - sx.begin;
+            L.same = X == Y;
+            L.ngrp = (int)sy.groups.size();
+            int maxch = 0;
+            for (int g = 0; g < L.ngrp; ++g) {
+                L.gbeg[g] = sy.groups[g].begin;
+                L.gend[g] = sy.groups[g].end;
+                maxch += (sy.groups[g].end - sy.groups[g].begin + 63) / 64;
+            }
+            L.cap = L.nbra * ((maxch + JK_BLOCK_TASKS - 1) / JK_BLOCK_TASKS);
+            DevBuf<int2>& cb = ctx->list_cut[ci];
+            DevBuf<int>& ob = ctx->list_off[ci];
+            DevBuf<int>& bb = ctx->list_bra[ci];
+            HIPCHK(cb.reserve((size_t)L.nbra * JK_MAXG));
+            HIPCHK(ob.reserve(2 * (size_t)(L.nbra + 1)));
+            HIPCHK(bb.reserve(2 * (size_t)std::max(L.cap, 1)));
+            L.cut = cb.p;
+            L.off = ob.p;
+            L.bra_of = bb.p;
+        }
     return M2_OK;
 }
 
@@ -290,20 +321,7 @@ int count_ge(const std::vector<double>& Q, int b, int e, double thr) {
 
 struct ComboRange {
     int X, Y, t0, tmid, t1;   // FP64 tasks [t0, tmid), FP32 tasks [tmid, t1)
-    int b0, bmid, b1;         // JKBlocks of the FP64 / FP32 tasks
 };
-
-// Group consecutive Tasks of one bra pair into workgroup units of at most
-// JK_BLOCK_TASKS Tasks (the kernel keeps the bra's K rows in LDS across them).
-static void append_blocks(const std::vector<Task>& T, int t0, int t1, std::vector<int2>& B) {
-    int i = t0;
-    while (i < t1) {
-        int j = i + 1;
-        while (j < t1 && j - i < JK_BLOCK_TASKS && T[j].bra == T[i].bra) ++j;
-        B.push_back(make_int2(i, j));
-        i = j;
-    }
-}
 
 // Work list: for every bra class X >= ket class Y, every bra pair i and every
 // primitive-count group of Y, the ket pairs j whose Q_j clears
@@ -315,12 +333,13 @@ static void append_blocks(const std::vector<Task>& T, int t0, int t1, std::vecto
 // heavy classes take HEAVY_CHUNK (one wavefront walks them in turn).
 constexpr int HEAVY_CHUNK = 4;
 
-std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed, bool gradient = false) {
+// heavy_only: the J/K path builds host tasks only for the table-driven heavy
+// classes (the register classes use the device work lists of tasks.hip).
+std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed, bool gradient,
+                                    bool heavy_only) {
     std::vector<ComboRange> ranges;
     auto& T = ctx->h_tasks;
     T.clear();
-    auto& B = ctx->h_blocks;
-    B.clear();
     const auto& Q = ctx->pair_Q;
     std::vector<Task> t32;
     for (int X = 0; X < NPAIRCLASS; ++X) {
@@ -330,8 +349,9 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
             // gradient: the five smallest classes run one quartet per lane
             const bool reg = gradient ? ((X <= 1 && Y <= X) || (X == 2 && Y == 0) || (X == 3 && Y == 0))
                                       : JK_COMBO[X][Y] != nullptr;
+            if (heavy_only && reg) continue;
             const int chunk = reg ? 64 : HEAVY_CHUNK;
-            ComboRange r{X, Y, (int)T.size(), 0, 0, 0, 0, 0};
+            ComboRange r{X, Y, (int)T.size(), 0, 0};
             t32.clear();
             for (int i = sx.begin; i < sx.end; ++i) {
                 const double qi = Q[i];
@@ -353,13 +373,6 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
             r.tmid = (int)T.size();
             T.insert(T.end(), t32.begin(), t32.end());
             r.t1 = (int)T.size();
-            if (reg && !gradient) {
-                r.b0 = (int)B.size();
-                append_blocks(T, r.t0, r.tmid, B);
-                r.bmid = (int)B.size();
-                append_blocks(T, r.tmid, r.t1, B);
-                r.b1 = (int)B.size();
-            }
             if (r.t1 > r.t0) ranges.push_back(r);
         }
     }
@@ -372,28 +385,53 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     const int n = b.nbf, nsh = b.nsh;
     ARGCHK(n > 0, "m2_jk_direct: no orbital basis set");
     ARGCHK(flags <= M2_MIXED, "m2_jk_direct: unknown flags");
+    const bool mixed = flags == M2_MIXED;
     HIPCHK(ctx->d_Dn.reserve((size_t)nsh * nsh));
     HIPCHK(ctx->d_J.reserve((size_t)n * n));
     HIPCHK(ctx->d_K.reserve((size_t)n * n));
+    HIPCHK(ctx->d_lists.reserve(2 * NCOMBO + 2));
+    // shell-block norms, max|D| and the register-class work lists, on the device
     launch_blocknorm(dD, n, nsh, b.d_bf.p, b.d_nbf.p, ctx->d_Dn.p, ctx->stream);
+    double* d_dmax = reinterpret_cast<double*>(ctx->d_lists.p + 2 * NCOMBO);
+    launch_dmax(ctx->d_Dn.p, nsh * nsh, d_dmax, ctx->stream);
+    for (int ci = 0; ci < NCOMBO; ++ci) {
+        const JKList& L = ctx->lists[ci];
+        if (L.nbra > 0)
+            launch_task_lists(L, ctx->d_pair_Q.p, d_dmax, precision, ctx->fp32_below, mixed ? 1 : 0,
+                              ctx->d_lists.p + 2 * ci, ctx->stream);
+    }
     HIPCHK(hipGetLastError());
-    ctx->h_Dn.resize((size_t)nsh * nsh);
-    HIPCHK(hipMemcpyAsync(ctx->h_Dn.data(), ctx->d_Dn.p, sizeof(double) * nsh * nsh, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_lists, ctx->d_lists.p, sizeof(int) * (2 * NCOMBO + 2), hipMemcpyDeviceToHost,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    double Dmax = 0.0;
-    for (double x : ctx->h_Dn) Dmax = std::max(Dmax, x);
-    std::vector<ComboRange> ranges = build_tasks(ctx, precision, Dmax, flags == M2_MIXED);
-    ctx->last_tasks = (long long)ctx->h_tasks.size();
+    double Dmax;
+    std::memcpy(&Dmax, ctx->h_lists + 2 * NCOMBO, sizeof(double));
+    // host tasks of the heavy classes
+    std::vector<ComboRange> heavy = build_tasks(ctx, precision, Dmax, mixed, false, true);
+    long long units = (long long)ctx->h_tasks.size();
     if (!ctx->h_tasks.empty()) {
         HIPCHK(ctx->d_tasks.reserve(ctx->h_tasks.size()));
         HIPCHK(hipMemcpyAsync(ctx->d_tasks.p, ctx->h_tasks.data(), sizeof(Task) * ctx->h_tasks.size(),
                               hipMemcpyHostToDevice, ctx->stream));
     }
-    if (!ctx->h_blocks.empty()) {
-        HIPCHK(ctx->d_blocks.reserve(ctx->h_blocks.size()));
-        HIPCHK(hipMemcpyAsync(ctx->d_blocks.p, ctx->h_blocks.data(), sizeof(int2) * ctx->h_blocks.size(),
-                              hipMemcpyHostToDevice, ctx->stream));
-    }
+    // launch plan in class order: register combos from their lists, heavy ones from host tasks
+    struct Launch {
+        int X, Y, ci, hv;   // ci: list index (register) or -1; hv: index into `heavy` or -1
+    };
+    std::vector<Launch> plan;
+    for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
+        for (int Y = 0; Y <= X; ++Y, ++ci) {
+            if (JK_COMBO[X][Y]) {
+                const int t64 = ctx->h_lists[2 * ci], t32 = ctx->h_lists[2 * ci + 1];
+                ARGCHK(t64 <= ctx->lists[ci].cap && t32 <= ctx->lists[ci].cap, "m2_jk_direct: work list overflow");
+                units += t64 + t32;
+                if (t64 + t32 > 0) plan.push_back(Launch{X, Y, ci, -1});
+            } else {
+                for (size_t h = 0; h < heavy.size(); ++h)
+                    if (heavy[h].X == X && heavy[h].Y == Y) plan.push_back(Launch{X, Y, -1, (int)h});
+            }
+        }
+    ctx->last_tasks = units;
     launch_zero(ctx->d_J.p, (size_t)n * n, ctx->stream);
     launch_zero(ctx->d_K.p, (size_t)n * n, ctx->stream);
     JKArgs a = jk_args(ctx);
@@ -402,7 +440,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
         HIPCHK(ctx->d_counters.reserve(72));
         launch_zero((double*)ctx->d_counters.p, 72, ctx->stream);
         if (ctx->prof_counters) a.counters = ctx->d_counters.p;
-        while (ctx->ev.size() < 2 * ranges.size() + 2) {
+        while (ctx->ev.size() < 2 * plan.size() + 2) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
             ctx->ev.push_back(e);
@@ -414,27 +452,36 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     a.precision = precision;
     a.fp32_below = ctx->fp32_below;
     a.scale = multiplier;
-    for (size_t ri = 0; ri < ranges.size(); ++ri) {
-        const ComboRange& r = ranges[ri];
-        a.combo = 6 * r.X + r.Y;
-        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * ri], ctx->stream));
-        // mode 0: FP64 kernel on [t0, tmid); mode 2: FP32 kernel on [tmid, t1)
-        for (int mode : {0, 2}) {
-            const int b0 = mode == 0 ? r.t0 : r.tmid, b1 = mode == 0 ? r.tmid : r.t1;
-            if (b1 <= b0) continue;
-            const Task* tp = ctx->d_tasks.p + b0;
-            if (JK_COMBO[r.X][r.Y]) {
-                // block ranges index the whole task array
-                const int k0 = mode == 0 ? r.b0 : r.bmid, k1 = mode == 0 ? r.bmid : r.b1;
-                a.blocks = ctx->d_blocks.p + k0;
-                JK_COMBO[r.X][r.Y](b.pure, a, tables(ctx), ctx->d_tasks.p, k1 - k0, mode, ctx->stream);
+    for (size_t pi = 0; pi < plan.size(); ++pi) {
+        const Launch& pl = plan[pi];
+        a.combo = 6 * pl.X + pl.Y;
+        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi], ctx->stream));
+        if (pl.ci >= 0) {
+            const JKList& L = ctx->lists[pl.ci];
+            a.cut = L.cut;
+            a.bra0 = L.bra0;
+            a.ngrp = L.ngrp;
+            for (int g = 0; g < JK_MAXG; ++g) a.gbeg[g] = L.gbeg[g];
+            // FP64 heads, then FP32 tails
+            for (int tail = 0; tail < 2; ++tail) {
+                const int nblk = ctx->h_lists[2 * pl.ci + tail];
+                if (nblk <= 0) continue;
+                a.tail = tail;
+                a.blk_off = L.off + tail * (L.nbra + 1);
+                a.blk_bra = L.bra_of + (size_t)tail * L.cap;
+                JK_COMBO[pl.X][pl.Y](b.pure, a, tables(ctx), nblk, tail ? 2 : 0, ctx->stream);
                 HIPCHK(hipGetLastError());
-            } else {
+            }
+        } else {
+            const ComboRange& r = heavy[pl.hv];
+            for (int mode : {0, 2}) {
+                const int b0 = mode == 0 ? r.t0 : r.tmid, b1 = mode == 0 ? r.tmid : r.t1;
+                if (b1 <= b0) continue;
                 HIPCHK(launch_jk_heavy(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1],
-                                       b.pure, a, tables(ctx), tp, b1 - b0, mode, ctx->stream));
+                                       b.pure, a, tables(ctx), ctx->d_tasks.p + b0, b1 - b0, mode, ctx->stream));
             }
         }
-        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * ri + 1], ctx->stream));
+        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi + 1], ctx->stream));
     }
     launch_symmetrize(ctx->d_J.p, ctx->d_K.p, dJ, dK, n, ctx->stream);
     HIPCHK(hipGetLastError());
@@ -442,10 +489,10 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     if (ctx->prof) {
         unsigned long long cnt[72];
         HIPCHK(hipMemcpy(cnt, ctx->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
-        for (size_t ri = 0; ri < ranges.size(); ++ri) {
+        for (size_t pi = 0; pi < plan.size(); ++pi) {
             float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * ri], ctx->ev[2 * ri + 1]));
-            const int c = 6 * ranges[ri].X + ranges[ri].Y;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * pi], ctx->ev[2 * pi + 1]));
+            const int c = 6 * plan[pi].X + plan[pi].Y;
             ctx->prof_ms[c] += ms;
             ctx->prof_launch[c] += 1;
         }
@@ -464,7 +511,7 @@ struct ComboRangeG {
     int X, Y, t0, tmid, t1;
 };
 std::vector<ComboRangeG> grad_tasks(m2_ctx* ctx, double precision, double Dmax) {
-    std::vector<ComboRange> r = build_tasks(ctx, precision, Dmax, false, true);
+    std::vector<ComboRange> r = build_tasks(ctx, precision, Dmax, false, true, false);
     std::vector<ComboRangeG> out;
     for (const auto& x : r) out.push_back(ComboRangeG{x.X, x.Y, x.t0, x.tmid, x.t1});
     return out;
@@ -498,6 +545,11 @@ m2_status m2_ctx_create(m2_ctx** out, int device) {
     if (e != hipSuccess) {
         std::fprintf(stderr, "m2_ctx_create: %s\n", hipGetErrorString(e));
         delete ctx;
+        return M2_ERR_HIP;
+    }
+    if (hipHostMalloc((void**)&ctx->h_lists, sizeof(int) * (2 * NCOMBO + 2)) != hipSuccess) {
+        std::fprintf(stderr, "m2_ctx_create: pinned allocation failed\n");
+        m2_ctx_destroy(ctx);
         return M2_ERR_HIP;
     }
     if (heavy_upload(device) != hipSuccess) {
@@ -545,7 +597,15 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_Ko.release();
     ctx->d_Dn.release();
     ctx->d_tasks.release();
-    ctx->d_blocks.release();
+    for (int ci = 0; ci < NCOMBO; ++ci) {
+        ctx->list_cut[ci].release();
+        ctx->list_off[ci].release();
+        ctx->list_bra[ci].release();
+        ctx->lists[ci] = JKList{};
+    }
+    ctx->d_lists.release();
+    if (ctx->h_lists) (void)hipHostFree(ctx->h_lists);
+    ctx->h_lists = nullptr;
     ctx->d_counters.release();
     for (auto* x : {&ctx->d_B64, &ctx->d_dfV, &ctx->d_dfC, &ctx->d_dfX, &ctx->d_dfG, &ctx->d_dfJ, &ctx->d_dfK}) x->release();
     for (auto* x : {&ctx->d_B32, &ctx->d_dfC32, &ctx->d_dfX32, &ctx->d_dfG32, &ctx->d_dfJ32, &ctx->d_dfK32}) x->release();

@@ -3,6 +3,7 @@
 #pragma once
 #include "m2.h"
 #include "m2_common.hpp"
+#include "jk_lists.hpp"
 #include "boys.hpp"
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -14,6 +15,7 @@ struct rocblas_handle_opaque;   // rocBLAS handle (typedef'd pointer in rocblas 
 namespace m2 {
 
 constexpr int NPAIRCLASS = 6;
+constexpr int NCOMBO = NPAIRCLASS * (NPAIRCLASS + 1) / 2;
 const int PAIR_L[NPAIRCLASS][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 struct HostShell {
     int l = 0, atom = -1, bf = 0, nbf = 0;
@@ -84,9 +86,13 @@ struct m2_ctx {
     // J/K work buffers
     m2::DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
     m2::DevBuf<m2::Task> d_tasks;
-    std::vector<m2::Task> h_tasks;
-    m2::DevBuf<int2> d_blocks;   // JKBlocks of the register-class launches
-    std::vector<int2> h_blocks;
+    std::vector<m2::Task> h_tasks;   // heavy-class (and gradient) tasks
+    // device work lists of the register-class combinations (index over X >= Y)
+    m2::JKList lists[m2::NCOMBO];
+    m2::DevBuf<int2> list_cut[m2::NCOMBO];
+    m2::DevBuf<int> list_off[m2::NCOMBO], list_bra[m2::NCOMBO];
+    m2::DevBuf<int> d_lists;          // per combination workgroup totals (FP64, FP32), then max|D|
+    int* h_lists = nullptr;           // pinned copy of d_lists
     std::vector<double> h_Dn;
     double fp32_below = 1e-4;
     long long last_tasks = 0;

@@ -584,9 +584,9 @@ def write_tus(outdir):
                 continue
             lines = list(head)
             lines.append('#include "eri_%d%d%d%d.inc"' % (la, lb, lc, ld))
-            lines.append("void launch_jk_%d_%d(int pure, const JKArgs& a, const BoysTables& tab, const Task* t, int nt, int mode, hipStream_t st) {" % (X, Y))
-            lines.append("  if (pure) launch_jk<%d, %d, %d, %d, true>(a, tab, t, nt, mode, st);" % (la, lb, lc, ld))
-            lines.append("  else launch_jk<%d, %d, %d, %d, false>(a, tab, t, nt, mode, st);" % (la, lb, lc, ld))
+            lines.append("void launch_jk_%d_%d(int pure, const JKArgs& a, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {" % (X, Y))
+            lines.append("  if (pure) launch_jk<%d, %d, %d, %d, true>(a, tab, nblocks, mode, st);" % (la, lb, lc, ld))
+            lines.append("  else launch_jk<%d, %d, %d, %d, false>(a, tab, nblocks, mode, st);" % (la, lb, lc, ld))
             lines.append("}")
             if X == Y:
                 lines.append("void launch_schwarz%d(int pure, const JKArgs& a, const BoysTables& tab, int p0, int np, double* Q, hipStream_t st) {" % X)
@@ -603,8 +603,8 @@ def write_tus(outdir):
     for X in range(len(PAIR_CLASSES)):
         for Y in range(X + 1):
             if not heavy(X, Y):
-                lines.append("void launch_jk_%d_%d(int, const JKArgs&, const BoysTables&, const Task*, int, int, hipStream_t);" % (X, Y))
-    lines.append("typedef void (*jk_combo_launcher)(int, const JKArgs&, const BoysTables&, const Task*, int, int, hipStream_t);")
+                lines.append("void launch_jk_%d_%d(int, const JKArgs&, const BoysTables&, int, int, hipStream_t);" % (X, Y))
+    lines.append("typedef void (*jk_combo_launcher)(int, const JKArgs&, const BoysTables&, int, int, hipStream_t);")
     lines.append("const jk_combo_launcher JK_COMBO[%d][%d] = {" % (len(PAIR_CLASSES), len(PAIR_CLASSES)))
     for X in range(len(PAIR_CLASSES)):
         row = ["launch_jk_%d_%d" % (X, Y) if Y <= X and not heavy(X, Y) else "nullptr" for Y in range(len(PAIR_CLASSES))]

@@ -1,10 +1,11 @@
 // Direct J/K shell-quartet kernels (templates; instantiated per class in the
 // generated translation units).
 //
-// Work unit: one Task = one bra shell pair x up to 64 consecutive ket shell
+// Work unit: one chunk = one bra shell pair x up to 64 consecutive ket shell
 // pairs of one class and one primitive-count group (so every lane runs the
 // same primitive loop); one lane per ket pair.  A workgroup owns one bra
-// pair and a run of its Tasks (a JKBlock), which its waves take in turn.
+// pair and up to JK_BLOCK_TASKS of its chunks, which its waves take in turn;
+// the chunks come from the device work list of tasks.hip.
 // Each lane applies the exact screening of Fock::formJKdirect
 // (src/fockbuild.cpp:98-115): max of the six shell-block norms of D times
 // Q_ab Q_cd against `precision`, computes its contracted quartet with the
@@ -155,16 +156,25 @@ struct JKWaves {
 // [precision, inf); precision is chosen per task list).
 // Dynamic LDS (when a.k_lds): the (NA + NB) x n FP64 K rows of shells a and b.
 template <int LA, int LB, int LC, int LD, typename R, bool PURE>
-__global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk(JKArgs a, BoysTables tab,
-                                                                        const Task* __restrict__ tasks,
-                                                      double lo, double hi) {
+__global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk(JKArgs a, BoysTables tab, double lo,
+                                                                                 double hi) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NC = NFun<LC, PURE>::value, ND = NFun<LD, PURE>::value;
-    const int2 rng = a.blocks[blockIdx.x];
-    const int t0 = __builtin_amdgcn_readfirstlane(rng.x), t1 = __builtin_amdgcn_readfirstlane(rng.y);
+    constexpr int W = JKWaves<LA, LB, LC, LD, R>::value;
+    typedef const __attribute__((address_space(4))) int cint;
+    // workgroup -> bra pair and its ket segments (written by the list kernels)
+    const int bl = ((cint*)a.blk_bra)[blockIdx.x];
+    const int lb = blockIdx.x - ((cint*)a.blk_off)[bl];
+    const int bra = a.bra0 + bl;
+    cint* cut = (cint*)a.cut + 2 * bl * JK_MAXG;   // (mid, end) per group
+    int nch = 0;
+    for (int g = 0; g < a.ngrp; ++g) {
+        const int mid = cut[2 * g], end = cut[2 * g + 1];
+        nch += ((a.tail ? end - mid : mid - a.gbeg[g]) + 63) >> 6;
+    }
+    const int c0 = lb * JK_BLOCK_TASKS, c1 = min(c0 + JK_BLOCK_TASKS, nch);
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
-    const int bra = __builtin_amdgcn_readfirstlane(tasks[t0].bra);
     const int2 sab = a.pair_sh[bra];
     const int sa = __builtin_amdgcn_readfirstlane(sab.x), sb = __builtin_amdgcn_readfirstlane(sab.y);
     const int n = a.n, nsh = a.nsh;
@@ -198,9 +208,20 @@ __global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk
     const double qab = a.pair_Q[bra];
     const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
     unsigned long long cnt_q = 0, cnt_pq = 0;   // profiling counters of this wave
-    for (int t = t0 + wave; t < t1; t += JKWaves<LA, LB, LC, LD, R>::value) {
-        const int ket0 = __builtin_amdgcn_readfirstlane(tasks[t].ket0);
-        const int nket = __builtin_amdgcn_readfirstlane(tasks[t].nket);
+    for (int ch = c0 + wave; ch < c1; ch += W) {
+        // chunk -> ket group segment
+        int ket0 = 0, nket = 0, rem = ch;
+        for (int g = 0; g < a.ngrp; ++g) {
+            const int mid = cut[2 * g], end = cut[2 * g + 1];
+            const int s0 = a.tail ? mid : a.gbeg[g], s1 = a.tail ? end : mid;
+            const int nc = (s1 - s0 + 63) >> 6;
+            if (rem < nc) {
+                ket0 = s0 + 64 * rem;
+                nket = min(64, s1 - ket0);
+                break;
+            }
+            rem -= nc;
+        }
         const int ket = ket0 + (lane < nket ? lane : 0);
         const int2 scd = a.pair_sh[ket];
         const int sc = scd.x, sd = scd.y;
@@ -267,11 +288,11 @@ __global__ __launch_bounds__(256) void k_schwarz(JKArgs a, BoysTables tab, int p
     Qout[pr] = sqrt(mx);
 }
 
-// mode 0: FP64 kernel; mode 2: FP32 kernel (the caller hands each the tasks
-// of its precision); both compute every quartet with bound >= precision.
-// `nblocks` JKBlocks in a.blocks index into `tasks`.
+// mode 0: FP64 kernel over the FP64 heads of the ket groups; mode 2: FP32
+// kernel over the FP32 tails (a.tail set by the caller).  `nblocks`
+// workgroups of the work list in a.blk_bra.
 template <int LA, int LB, int LC, int LD, bool PURE>
-void launch_jk(const JKArgs& a0, const BoysTables& tab, const Task* tasks, int nblocks, int mode, hipStream_t st) {
+void launch_jk(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
     if (nblocks <= 0) return;
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     JKArgs a = a0;
@@ -282,10 +303,10 @@ void launch_jk(const JKArgs& a0, const BoysTables& tab, const Task* tasks, int n
     const size_t lds = a.k_lds ? kbytes : 0;
     if (mode == 2)
         hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, dim3(64 * JKWaves<LA, LB, LC, LD, float>::value),
-                           lds, st, a, tab, tasks, a.precision, inf);
+                           lds, st, a, tab, a.precision, inf);
     else
         hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, dim3(64 * JKWaves<LA, LB, LC, LD, double>::value),
-                           lds, st, a, tab, tasks, a.precision, inf);
+                           lds, st, a, tab, a.precision, inf);
 }
 
 template <int LA, int LB, bool PURE>

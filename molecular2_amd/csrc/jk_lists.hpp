@@ -1,0 +1,22 @@
+// Device work lists of one direct J/K class combination (tasks.hip).
+#pragma once
+#include "m2_common.hpp"
+
+namespace m2 {
+
+struct JKList {
+    int bra0, nbra;           // bra pairs [bra0, bra0 + nbra) of class X
+    int same;                 // X == Y: only kets j <= i
+    int ngrp;                 // primitive-count groups of the ket class Y
+    int gbeg[JK_MAXG], gend[JK_MAXG];
+    int2* cut;                // [nbra][JK_MAXG] (mid, end) of every group
+    int* off;                 // [2][nbra + 1] workgroups per bra -> exclusive offsets (FP64 head, FP32 tail)
+    int* bra_of;              // [2][cap] workgroup -> bra (local index)
+    int cap;                  // capacity of each half of bra_of
+};
+
+void launch_dmax(const double* Dn, int m, double* out, hipStream_t st);
+void launch_task_lists(const JKList& c, const double* Q, const double* dmax, double precision, double fp32_below,
+                       int mixed, int* tot, hipStream_t st);
+
+}  // namespace m2

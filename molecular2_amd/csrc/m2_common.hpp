@@ -36,6 +36,13 @@ constexpr int BOYS_NGRID = 601;
 constexpr double BOYS_STEP = 0.1;
 constexpr double BOYS_TMAX = 60.0;
 
+// ket chunks (of <= 64 kets) per workgroup of the direct J/K class kernels,
+// primitive-count groups per pair class, and the LDS budget of the K-row
+// accumulators
+constexpr int JK_BLOCK_TASKS = 64;
+constexpr int JK_MAXG = 10;
+constexpr size_t JK_KROW_LDS_MAX = 96 * 1024;
+
 struct JKArgs {
     const PrimPair* prims;
     const BraPair<float>* prims_f;   // the same records in FP32
@@ -56,14 +63,19 @@ struct JKArgs {
     double scale;            // multiplier
     unsigned long long* counters;  // profiling only: per class combo {quartets, primitive quartets}
     int combo;               // 6 * X + Y of the launch
-    const int2* blocks;      // JKBlocks: [first, last) Task of one workgroup, all with the same bra
     int k_lds;               // set by the launcher: K rows of the bra shells accumulated in LDS
+    // device work list of the launch (tasks.hip): workgroup -> bra, and the
+    // bra's ket segments per primitive-count group of the ket class
+    const int2* cut;         // [bra][JK_MAXG] (mid, end)
+    const int* blk_off;      // [bra] first workgroup of the bra
+    const int* blk_bra;      // [workgroup] bra (index local to the class)
+    int bra0;                // first pair of the bra class
+    int ngrp;                // ket groups
+    int tail;                // 0: FP64 head [gbeg, mid) of each group; 1: FP32 tail [mid, end)
+    int gbeg[JK_MAXG];
 };
 
-// tasks per workgroup of the direct J/K class kernels, and the LDS budget of
-// the K-row accumulators
-constexpr int JK_BLOCK_TASKS = 64;
-constexpr size_t JK_KROW_LDS_MAX = 96 * 1024;
+
 
 struct Task {                // one wavefront of quartets: bra pair x 1..64 ket pairs
     int bra;

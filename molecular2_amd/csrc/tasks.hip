@@ -1,0 +1,144 @@
+// Device-side work lists of the direct J/K register-class kernels.
+//
+// For one class combination (bra pair class X, ket pair class Y <= X) and
+// every bra pair i of class X, the kets of class Y are kept in
+// primitive-count groups, each sorted by Schwarz Q descending.  The
+// quartets that can pass the screening of Fock::formJKdirect
+// (src/fockbuild.cpp:98-115), max|D_blocks| Q_i Q_j >= precision, form a
+// prefix of each group: Q_j >= precision / (Q_i max|D|).  Under mixed
+// precision the prefix splits again at max(precision, fp32_below) / (Q_i
+// max|D|) into an FP64 head and an FP32 tail.  With X == Y only kets j <= i
+// are unique quartets (src/fockbuild.cpp:88-109).
+//
+//   k_task_count: per bra, the (mid, end) cut of every group (two binary
+//                 searches) and the number of workgroups per precision;
+//   k_task_scan:  exclusive prefix sums of the workgroup counts;
+//   k_task_fill:  workgroup -> bra map.
+// The class kernels then take their ket chunks of 64 straight from the cuts.
+#include "jk_lists.hpp"
+
+namespace m2 {
+
+__device__ __forceinline__ int count_ge_dev(const double* __restrict__ Q, int b, int e, double thr) {
+    int lo = b, hi = e;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (Q[mid] >= thr) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - b;
+}
+
+// max over the shell-block norms of D (one workgroup)
+__global__ __launch_bounds__(1024) void k_dmax(const double* __restrict__ Dn, int m, double* __restrict__ out) {
+    __shared__ double s[16];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) v = fmax(v, Dn[i]);
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, s[w]);
+        out[0] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_task_count(JKList c, const double* __restrict__ Q,
+                                                    const double* __restrict__ dmax, double precision,
+                                                    double fp32_below, int mixed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.nbra) return;
+    const int bra = c.bra0 + i;
+    const double qi = Q[bra], Dmax = dmax[0];
+    int ch64 = 0, ch32 = 0;
+    const bool any = qi > 0.0 && Dmax > 0.0;
+    const double thr = any ? precision / (qi * Dmax) : 0.0;
+    const double thr64 = any ? (mixed ? fmax(precision, fp32_below) / (qi * Dmax) : thr) : 0.0;
+    for (int g = 0; g < c.ngrp; ++g) {
+        const int gb = c.gbeg[g], ge = c.gend[g];
+        int end = gb, mid = gb;
+        if (any) {
+            end = gb + count_ge_dev(Q, gb, ge, thr);
+            mid = gb + count_ge_dev(Q, gb, ge, thr64);
+            if (c.same) {
+                end = min(end, bra + 1);
+                mid = min(mid, bra + 1);
+            }
+            end = max(end, gb);
+            mid = max(min(mid, end), gb);
+        }
+        c.cut[i * JK_MAXG + g] = make_int2(mid, end);
+        ch64 += (mid - gb + 63) >> 6;
+        ch32 += (end - mid + 63) >> 6;
+    }
+    c.off[i] = (ch64 + JK_BLOCK_TASKS - 1) / JK_BLOCK_TASKS;
+    c.off[c.nbra + 1 + i] = (ch32 + JK_BLOCK_TASKS - 1) / JK_BLOCK_TASKS;
+}
+
+// exclusive scan in place of off[mode * (nbra + 1) + 0 .. nbra - 1]; the total
+// goes to off[mode * (nbra + 1) + nbra] and tot[mode].  blockIdx.x = mode.
+__global__ __launch_bounds__(1024) void k_task_scan(JKList c, int* __restrict__ tot) {
+    __shared__ int s_w[16];
+    __shared__ int s_carry;
+    int* a = c.off + blockIdx.x * (c.nbra + 1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < c.nbra; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        const int v = i < c.nbra ? a[i] : 0;
+        int x = v;   // inclusive scan within the wave
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_w[w] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int k = 0; k < nw; ++k) {
+                const int t = s_w[k];
+                s_w[k] = run;
+                run += t;
+            }
+        }
+        __syncthreads();
+        const int carry = s_carry;
+        if (i < c.nbra) a[i] = carry + s_w[w] + x - v;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) s_carry = carry + s_w[w] + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a[c.nbra] = s_carry;
+        tot[blockIdx.x] = s_carry;
+    }
+}
+
+// workgroup -> bra (local index); blockIdx.y = mode.  Entries beyond the
+// capacity are not written (the host checks the totals against it).
+__global__ __launch_bounds__(256) void k_task_fill(JKList c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.nbra) return;
+    const int mode = blockIdx.y;
+    const int* off = c.off + mode * (c.nbra + 1);
+    int* dst = c.bra_of + (size_t)mode * c.cap;
+    const int e = min(off[i + 1], c.cap);
+    for (int b = off[i]; b < e; ++b) dst[b] = i;
+}
+
+void launch_dmax(const double* Dn, int m, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_dmax, dim3(1), dim3(1024), 0, st, Dn, m, out);
+}
+
+void launch_task_lists(const JKList& c, const double* Q, const double* dmax, double precision, double fp32_below,
+                       int mixed, int* tot, hipStream_t st) {
+    if (c.nbra <= 0) return;
+    const int nb = (c.nbra + 255) / 256;
+    hipLaunchKernelGGL(k_task_count, dim3(nb), dim3(256), 0, st, c, Q, dmax, precision, fp32_below, mixed);
+    hipLaunchKernelGGL(k_task_scan, dim3(2), dim3(1024), 0, st, c, tot);
+    hipLaunchKernelGGL(k_task_fill, dim3(nb, 2), dim3(256), 0, st, c);
+}
+
+}  // namespace m2
