@@ -4,6 +4,7 @@
 #include "jk_lists.hpp"
 #include "generated/sph_tables.inc"
 #include "generated/jk_dispatch.inc"
+#include "generated/deriv_classes.inc"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -333,6 +334,14 @@ struct ComboRange {
 // heavy classes take HEAVY_CHUNK (one wavefront walks them in turn).
 constexpr int HEAVY_CHUNK = 4;
 
+// gradient classes with generated register code (one quartet per lane)
+static bool deriv_register(int la, int lb, int lc, int ld) {
+    for (int k = 0; k < DERIV_REG_N; ++k)
+        if (DERIV_REG_L[k][0] == la && DERIV_REG_L[k][1] == lb && DERIV_REG_L[k][2] == lc && DERIV_REG_L[k][3] == ld)
+            return true;
+    return false;
+}
+
 // heavy_only: the J/K path builds host tasks only for the table-driven heavy
 // classes (the register classes use the device work lists of tasks.hip).
 std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed, bool gradient,
@@ -347,7 +356,7 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
         for (int Y = 0; Y <= X; ++Y) {
             const PairClassSeg& sy = ctx->cls[Y];
             // gradient: the five smallest classes run one quartet per lane
-            const bool reg = gradient ? ((X <= 1 && Y <= X) || (X == 2 && Y == 0) || (X == 3 && Y == 0))
+            const bool reg = gradient ? deriv_register(PAIR_L[X][0], PAIR_L[X][1], PAIR_L[Y][0], PAIR_L[Y][1])
                                       : JK_COMBO[X][Y] != nullptr;
             if (heavy_only && reg) continue;
             const int chunk = reg ? 64 : HEAVY_CHUNK;

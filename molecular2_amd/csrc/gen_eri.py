@@ -1103,11 +1103,13 @@ def gen_deriv_program(la, lb, lc, ld):
 
 
 def write_deriv_programs(outdir):
-    """All 21 orbital class combinations (bra class >= ket class)."""
+    """Orbital class combinations (bra class >= ket class) not in DERIV_REGISTER."""
     ins_i, ins_c, lev, accs, contr_i, contr_c, desc = [], [], [], [], [], [], []
     for X, (la, lb) in enumerate(PAIR_CLASSES):
         for Y in range(X + 1):
             lc, ld = PAIR_CLASSES[Y]
+            if (la, lb, lc, ld) in DERIV_REGISTER:
+                continue   # register code (deriv_register.inc)
             P = gen_deriv_program(la, lb, lc, ld)
             d = {"la": la, "lb": lb, "lc": lc, "ld": ld, "L": P["L"], "nslots": P["nslots"], "base": P["base"],
                  "nacc": P["nacc"], "lev_off": len(lev), "nvlev": len(P["vrr_levels"])}
@@ -1183,36 +1185,55 @@ def write_deriv_programs(outdir):
     return path
 
 
-DERIV_REGISTER = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0)]
+DERIV_REGISTER = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0), (1, 1, 1, 0), (2, 0, 1, 0),
+                  (2, 1, 0, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0)]
+# classes whose three derivative centres run as three lighter passes
+DERIV_SPLIT = {(1, 1, 1, 0), (2, 0, 1, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0)}
 
 
-def gen_deriv_class(la, lb, lc, ld):
+def gen_deriv_class(la, lb, lc, ld, mask=7):
     """Register-resident derivative adjoint of one small class:
-    EriDeriv<la,lb,lc,ld>::run<R>(..., Gam, g) adds to g[0..8] the
-    contraction of d/dA, d/dB, d/dC (ab|cd) (Cartesian) with Gam."""
+    EriDeriv<la,lb,lc,ld,mask>::run<R>(..., gam_fill, g) adds to g[0..8] the
+    contraction of d/dA (mask & 1), d/dB (mask & 2), d/dC (mask & 4) of
+    (ab|cd) (Cartesian) with Gamma; the other entries of g are left alone."""
     Lp = la + lb + lc + ld + 1
 
     def rng(lo, hi):
         return [c for L_ in range(max(lo, 0), hi + 1) for c in comps(L_)]
 
-    sets = {
-        "s0": (rng(la - 1, la + lb), rng(lc - 1, lc + ld), None),
-        "sa": (rng(la + 1, la + lb + 1), rng(lc, lc + ld), "wA"),
-        "sb": (rng(la, la + lb + 1), rng(lc, lc + ld), "wB"),
-        "sc": (rng(la, la + lb), rng(lc + 1, lc + ld + 1), "wC"),
-    }
-    targets = sorted({(e, f) for E, F, _ in sets.values() for e in E for f in F})
+    def rect(E, F):
+        return [(e, f) for e in E for f in F]
+
+    # unweighted targets: only those the lowered-index terms -a(a-1), -b(b-1),
+    # -c(c-1) reach through the HRR
+    s0 = []
+    if la > 0 and mask & 1:
+        s0 += rect(rng(la - 1, la - 1 + lb), rng(lc, lc + ld))
+    if lb > 0 and mask & 2:
+        s0 += rect(rng(la, la + lb - 1), rng(lc, lc + ld))
+    if lc > 0 and mask & 4:
+        s0 += rect(rng(la, la + lb), rng(lc - 1, lc - 1 + ld))
+    s0 = sorted(set(s0))
+    sets = {"s0": (s0, None)}
+    if mask & 1:
+        sets["sa"] = (rect(rng(la + 1, la + lb + 1), rng(lc, lc + ld)), "wA")
+    if mask & 2:
+        sets["sb"] = (rect(rng(la, la + lb + 1), rng(lc, lc + ld)), "wB")
+    if mask & 4:
+        sets["sc"] = (rect(rng(la, la + lb), rng(lc + 1, lc + ld + 1)), "wC")
+    targets = sorted({t for T_, _ in sets.values() for t in T_})
     order, nodes = build_vrr(targets, Lp)
     em = Emitter()
-    em.out("template <> struct EriDeriv<%d, %d, %d, %d> {" % (la, lb, lc, ld))
-    em.out("  template <typename R>")
+    em.out("template <> struct EriDeriv<%d, %d, %d, %d, %d> {" % (la, lb, lc, ld, mask))
+    ngam = len(comps(la)) * len(comps(lb)) * len(comps(lc)) * len(comps(ld))
+    em.out("  static constexpr int NACC = %d;" % sum(len(T_) for T_, _ in sets.values()))
+    em.out("  template <typename R, typename GamFill>")
     em.out("  static __device__ __forceinline__ void run(const PrimPair* __restrict__ bra, int nbra,")
     em.out("      const PrimPair* __restrict__ ket, int nket, const R ABx, const R ABy, const R ABz,")
-    em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, const R* __restrict__ Gam, R* __restrict__ g) {")
-    for nm, (E, F, _) in sets.items():
-        for e in E:
-            for f in F:
-                em.out("    R %s_%s_%s = R(0);" % (nm, cname(e), cname(f)))
+    em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, const GamFill& gam_fill, R* __restrict__ g) {")
+    for nm, (T_, _) in sets.items():
+        for e, f in T_:
+            em.out("    R %s_%s_%s = R(0);" % (nm, cname(e), cname(f)))
     em.out("    for (int ik = 0; ik < nket; ++ik) {")
     em.out("      const PrimPair kp = ket[ik];")
     em.out("      const R q = R(kp.p), Kcd = R(kp.K), wC = R(2) * R(kp.a);")
@@ -1238,13 +1259,15 @@ def gen_deriv_class(la, lb, lc, ld):
     em.out("        const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
     em.out("        const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
     emit_vrr(em, order, nodes, "        ")
-    for nm, (E, F, w) in sets.items():
-        for e in E:
-            for f in F:
-                v = vrr_name(e, f, 0)
-                em.out("        %s_%s_%s += %s;" % (nm, cname(e), cname(f), v if w is None else "%s * %s" % (w, v)))
+    for nm, (T_, w) in sets.items():
+        for e, f in T_:
+            v = vrr_name(e, f, 0)
+            em.out("        %s_%s_%s += %s;" % (nm, cname(e), cname(f), v if w is None else "%s * %s" % (w, v)))
     em.out("      }")
     em.out("    }")
+    # Gamma is formed only now, so that it does not hold registers in the loop
+    em.out("    R Gam[%d];" % ngam)
+    em.out("    gam_fill(Gam);")
     # post: HRR per term, then contraction
     ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
     gidx = {}
@@ -1298,13 +1321,18 @@ def gen_deriv_class(la, lb, lc, ld):
                         vals[(a, b, c, d)] = bget(a, b, c, d)
         return vals
 
-    terms = [("sa", (la + 1, lb, lc, ld), 0, +1), ("sb", (la, lb + 1, lc, ld), 1, +1),
-             ("sc", (la, lb, lc + 1, ld), 2, +1)]
-    if la > 0:
+    terms = []
+    if mask & 1:
+        terms.append(("sa", (la + 1, lb, lc, ld), 0, +1))
+    if mask & 2:
+        terms.append(("sb", (la, lb + 1, lc, ld), 1, +1))
+    if mask & 4:
+        terms.append(("sc", (la, lb, lc + 1, ld), 2, +1))
+    if la > 0 and mask & 1:
         terms.append(("s0", (la - 1, lb, lc, ld), 0, -1))
-    if lb > 0:
+    if lb > 0 and mask & 2:
         terms.append(("s0", (la, lb - 1, lc, ld), 1, -1))
-    if lc > 0:
+    if lc > 0 and mask & 4:
         terms.append(("s0", (la, lb, lc - 1, ld), 2, -1))
     sums = {k: [] for k in range(9)}
     for acc, cls, ctr, sgn in terms:
@@ -1343,12 +1371,31 @@ def gen_deriv_class(la, lb, lc, ld):
 def write_deriv_register(outdir):
     text = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.", "#pragma once"]
     for c in DERIV_REGISTER:
-        text.append(gen_deriv_class(*c))
+        for m in ((1, 2, 4) if c in DERIV_SPLIT else (7,)):
+            text.append(gen_deriv_class(*c, mask=m))
+    # launch dispatch of the register classes (LAUNCH(la, lb, lc, ld, mask))
+    text.append("#define DERIV_REGISTER_DISPATCH(la, lb, lc, ld, LAUNCH) \\")
+    for i, (la, lb, lc, ld) in enumerate(DERIV_REGISTER):
+        calls = " ".join("LAUNCH(%d, %d, %d, %d, %d)" % (la, lb, lc, ld, m)
+                         for m in ((1, 2, 4) if (la, lb, lc, ld) in DERIV_SPLIT else (7,)))
+        text.append("  %sif (la == %d && lb == %d && lc == %d && ld == %d) { %s } \\" %
+                    ("" if i == 0 else "else ", la, lb, lc, ld, calls))
+    text.append("  else")
     text = "\n".join(text) + "\n"
     path = os.path.join(outdir, "deriv_register.inc")
     if not os.path.exists(path) or open(path).read() != text:
         with open(path, "w") as fh:
             fh.write(text)
+    # which classes are register code (host side)
+    cls = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.", "#pragma once",
+           "constexpr int DERIV_REG_N = %d;" % len(DERIV_REGISTER),
+           "constexpr int DERIV_REG_L[%d][4] = {%s};" % (len(DERIV_REGISTER), ", ".join(
+               "{%d, %d, %d, %d}" % c for c in DERIV_REGISTER))]
+    ctext = "\n".join(cls) + "\n"
+    cpath = os.path.join(outdir, "deriv_classes.inc")
+    if not os.path.exists(cpath) or open(cpath).read() != ctext:
+        with open(cpath, "w") as fh:
+            fh.write(ctext)
     return path
 
 

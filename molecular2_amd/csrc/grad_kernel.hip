@@ -13,16 +13,17 @@
 // wavefront from the generated LDS program.  The contraction is done in the
 // Cartesian basis with the density blocks transformed to Cartesian.
 //
-// One-electron part: one thread per ordered shell pair (s1, s2) adds twice
-// d/dA of D:(T+V) - 2 W:S (A = centre of s1; the (s2, s1) thread supplies the
-// other centre) and the Hellmann-Feynman term of each nucleus (dC = -(dA + dB)).
+// One-electron part: one wavefront per ordered shell pair (s1, s2), one lane
+// per nucleus, adds twice d/dA of D:(T+V) - 2 W:S (A = centre of s1; the
+// (s2, s1) wavefront supplies the other centre) and the Hellmann-Feynman term
+// of each nucleus (dC = -(dA + dB)).
 #include "m2_common.hpp"
 #include "boys.hpp"
 #include "generated/deriv_programs.inc"
 
 namespace m2 {
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int MASK>
 struct EriDeriv;
 #include "generated/deriv_register.inc"
 
@@ -88,7 +89,7 @@ __device__ __forceinline__ double wave_sum_g(double v) {
 }
 
 // register classes: one quartet per lane, per-block LDS bins for the atoms
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int MASK>
 __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, const Task* __restrict__ tasks,
                                                   int ntasks, int natom) {
     constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
@@ -110,31 +111,33 @@ __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, co
             double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             if (active) {
                 const int fa = a.sh_bf[sa], fb = a.sh_bf[sb], fc = a.sh_bf[sc], fd = a.sh_bf[sd];
-                double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dbd[NB * ND], Dad[NA * ND], Dbc[NB * NC];
-                cart_block<LA, LB>(a, fa, fb, Dab);
-                cart_block<LC, LD>(a, fc, fd, Dcd);
-                cart_block<LA, LC>(a, fa, fc, Dac);
-                cart_block<LB, LD>(a, fb, fd, Dbd);
-                cart_block<LA, LD>(a, fa, fd, Dad);
-                cart_block<LB, LC>(a, fb, fc, Dbc);
                 const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
-                double Gam[NA * NB * NC * ND];
+                // Gamma (Cartesian) is formed after the primitive loop
+                auto gam_fill = [&](double* Gam) {
+                    double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dbd[NB * ND], Dad[NA * ND], Dbc[NB * NC];
+                    cart_block<LA, LB>(a, fa, fb, Dab);
+                    cart_block<LC, LD>(a, fc, fd, Dcd);
+                    cart_block<LA, LC>(a, fa, fc, Dac);
+                    cart_block<LB, LD>(a, fb, fd, Dbd);
+                    cart_block<LA, LD>(a, fa, fd, Dad);
+                    cart_block<LB, LC>(a, fb, fc, Dbc);
 #pragma unroll
-                for (int i = 0; i < NA; ++i)
+                    for (int i = 0; i < NA; ++i)
 #pragma unroll
-                    for (int j = 0; j < NB; ++j)
+                        for (int j = 0; j < NB; ++j)
 #pragma unroll
-                        for (int k = 0; k < NC; ++k)
+                            for (int k = 0; k < NC; ++k)
 #pragma unroll
-                            for (int l = 0; l < ND; ++l)
-                                Gam[((i * NB + j) * NC + k) * ND + l] =
-                                    deg * (0.5 * Dab[i * NB + j] * Dcd[k * ND + l] -
-                                           0.125 * (Dac[i * NC + k] * Dbd[j * ND + l] + Dad[i * ND + l] * Dbc[j * NC + k]));
+                                for (int l = 0; l < ND; ++l)
+                                    Gam[((i * NB + j) * NC + k) * ND + l] =
+                                        deg * (0.5 * Dab[i * NB + j] * Dcd[k * ND + l] -
+                                               0.125 * (Dac[i * NC + k] * Dbd[j * ND + l] + Dad[i * ND + l] * Dbc[j * NC + k]));
+                };
                 const double* O = a.sh_O;
-                EriDeriv<LA, LB, LC, LD>::template run<double>(
+                EriDeriv<LA, LB, LC, LD, MASK>::template run<double>(
                     a.prims + a.pair_off[bra], a.pair_np[bra], a.prims + a.pair_off[ket], a.pair_np[ket],
                     O[3 * sa] - O[3 * sb], O[3 * sa + 1] - O[3 * sb + 1], O[3 * sa + 2] - O[3 * sb + 2],
-                    O[3 * sc] - O[3 * sd], O[3 * sc + 1] - O[3 * sd + 1], O[3 * sc + 2] - O[3 * sd + 2], tab, Gam, g);
+                    O[3 * sc] - O[3 * sd], O[3 * sc + 1] - O[3 * sd + 1], O[3 * sc + 2] - O[3 * sd + 2], tab, gam_fill, g);
             }
             const int at_a = a.sh_atom[sa], at_b = a.sh_atom[sb];
             // A and B are the same for the whole wavefront
@@ -400,10 +403,13 @@ constexpr int G1_LA = 3;   // l of the bra after the +1 shift
 constexpr int G1_CUMA = (G1_LA + 1) * (G1_LA + 2) * (G1_LA + 3) / 6;   // 20
 constexpr int G1_CUMB = 10;                                            // |b| <= 2
 
+// One wavefront per ordered shell pair (s1, s2); lane c takes the nuclei
+// c, c + 64, ...  Lane 0 also does the kinetic / Pulay part.
 __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x;
     if (t >= a.nsh * a.nsh) return;
     const int s1 = t / a.nsh, s2 = t % a.nsh;
+    const int lane = threadIdx.x;
     const int la = a.sh_l[s1], lb = a.sh_l[s2];
     const int na = ncart_d(la), nb = ncart_d(lb);
     double Dt[36], Wt[36];
@@ -414,6 +420,7 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
     double AB2 = 0.0;
     for (int k = 0; k < 3; ++k) AB2 += (A[k] - B[k]) * (A[k] - B[k]);
     double gA[3] = {0, 0, 0};   // T, Pulay and V (basis-centre) part on atom(s1)
+    double gC[3] = {0, 0, 0};   // Hellmann-Feynman part on the lane's nucleus (natom <= 64)
     const int Lab = la + 1 + lb;
     for (int ip = 0; ip < a.sh_np[s1]; ++ip)
         for (int jp = 0; jp < a.sh_np[s2]; ++jp) {
@@ -457,7 +464,7 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
                 s = spref * sx * sy * sz;
                 tk = spref * (t3[0] * sy * sz + sx * t3[1] * sz + sx * sy * t3[2]);
             };
-            for (int ia = 0; ia < na; ++ia) {
+            for (int ia = 0; ia < (lane == 0 ? na : 0); ++ia) {
                 int ea[3];
                 comp_of(la, ia, ea);
                 for (int ib = 0; ib < nb; ++ib) {
@@ -484,7 +491,7 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
             }
             // nuclear attraction: [a|b]^(m) with |a| <= la+1, |b| <= lb
             const double vpref = 2.0 * M_PI / p * Kab * cc;
-            for (int c = 0; c < a.natom; ++c) {
+            for (int c = lane; c < a.natom; c += 64) {
                 const double* Cc = a.xyz + 3 * c;
                 double PC[3], PC2 = 0.0;
                 for (int k = 0; k < 3; ++k) {
@@ -583,14 +590,22 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
                 }
                 for (int ax = 0; ax < 3; ++ax) {
                     gA[ax] -= a.Z[c] * gv[ax];
-                    atomicAdd(&a.grad[c * 3 + ax], 2.0 * a.Z[c] * gv[ax]);   // -(dA) of the point-charge term
+                    // -(dA) of the point-charge term, on nucleus c
+                    if (a.natom <= 64) gC[ax] += 2.0 * a.Z[c] * gv[ax];
+                    else atomicAdd(&a.grad[c * 3 + ax], 2.0 * a.Z[c] * gv[ax]);
                 }
             }
         }
+    if (a.natom <= 64 && lane < a.natom)
+        for (int ax = 0; ax < 3; ++ax)
+            if (gC[ax] != 0.0) atomicAdd(&a.grad[lane * 3 + ax], gC[ax]);
     // the ket-centre derivative of block (s1, s2) is the bra-centre derivative
     // of block (s2, s1) (symmetric D and W): every ordered pair counts twice
     const int at = a.sh_atom[s1];
-    for (int ax = 0; ax < 3; ++ax) atomicAdd(&a.grad[at * 3 + ax], 2.0 * gA[ax]);
+    for (int ax = 0; ax < 3; ++ax) {
+        const double v = wave_sum_g(gA[ax]);
+        if (lane == 0 && v != 0.0) atomicAdd(&a.grad[at * 3 + ax], 2.0 * v);
+    }
 }
 
 // --------------------------------------------------------------- host side
@@ -623,21 +638,18 @@ hipError_t deriv_upload(int device) {
     return hipMemcpy(t.ccoef, DPROG_CCOEF, sizeof(DPROG_CCOEF), hipMemcpyHostToDevice);
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int MASK>
 static void launch_reg(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom, hipStream_t st) {
-    hipLaunchKernelGGL((k_grad_reg<LA, LB, LC, LD>), dim3((nt + 3) / 4), dim3(256), sizeof(double) * natom * 3, st, a,
-                       tab, t, nt, natom);
+    hipLaunchKernelGGL((k_grad_reg<LA, LB, LC, LD, MASK>), dim3((nt + 3) / 4), dim3(256), sizeof(double) * natom * 3,
+                       st, a, tab, t, nt, natom);
 }
 
 hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,
                           const Task* t, int nt, int natom, hipStream_t st) {
     if (nt <= 0) return hipSuccess;
-    if (la == 0 && lb == 0 && lc == 0 && ld == 0) launch_reg<0, 0, 0, 0>(a, tab, t, nt, natom, st);
-    else if (la == 1 && lb == 0 && lc == 0 && ld == 0) launch_reg<1, 0, 0, 0>(a, tab, t, nt, natom, st);
-    else if (la == 1 && lb == 0 && lc == 1 && ld == 0) launch_reg<1, 0, 1, 0>(a, tab, t, nt, natom, st);
-    else if (la == 1 && lb == 1 && lc == 0 && ld == 0) launch_reg<1, 1, 0, 0>(a, tab, t, nt, natom, st);
-    else if (la == 2 && lb == 0 && lc == 0 && ld == 0) launch_reg<2, 0, 0, 0>(a, tab, t, nt, natom, st);
-    else {
+#define M2_LAUNCH_REG(A, B, C, D, M) launch_reg<A, B, C, D, M>(a, tab, t, nt, natom, st);
+    DERIV_REGISTER_DISPATCH(la, lb, lc, ld, M2_LAUNCH_REG) {
+#undef M2_LAUNCH_REG
         const DerivTables& T = g_dt[device];
         int k = 0;
         for (; k < DPROG_NCLASS; ++k) {
@@ -674,7 +686,7 @@ hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const Grad
 
 hipError_t launch_grad_1e(const Grad1Args& a, hipStream_t st) {
     const int nt = a.nsh * a.nsh;
-    hipLaunchKernelGGL(k_grad_1e, dim3((nt + 63) / 64), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_grad_1e, dim3(nt), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
