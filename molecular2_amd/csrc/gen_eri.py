@@ -132,6 +132,7 @@ def build_vrr(targets, L):
     """targets: list of (e, f) needed at m=0.  Returns ordered node list with
     expressions."""
     nodes = {}
+    post = []
 
     def need(e, f, m):
         key = (e, f, m)
@@ -161,6 +162,7 @@ def build_vrr(targets, L):
             nodes[key] = ("ket", terms, extra)
             for d in deps:
                 need(*d)
+            post.append(key)
         elif sum(e) > 0:
             best = None
             for i in range(3):
@@ -181,13 +183,15 @@ def build_vrr(targets, L):
             nodes[key] = ("bra", terms, extra)
             for d in deps:
                 need(*d)
+            post.append(key)
         else:
             nodes[key] = ("base", m)
+            post.append(key)
 
     for e, f in targets:
         need(e, f, 0)
-    order = sorted(nodes.keys(), key=lambda k: (sum(k[0]) + sum(k[1]), sum(k[1]), k[2]))
-    return order, nodes
+    # depth-first post-order: every value is emitted right before its first use
+    return post, nodes
 
 
 def emit_vrr(em, order, nodes, indent):
