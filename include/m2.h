@@ -55,6 +55,8 @@ void m2_ctx_destroy(m2_ctx* ctx);
 const char* m2_last_error(const m2_ctx* ctx);
 /* the HIP stream (hipStream_t) every call of this context runs on */
 void* m2_ctx_stream(m2_ctx* ctx);
+/* the device the context was created on */
+int m2_ctx_device(const m2_ctx* ctx);
 
 /* Basis.  Replaces Basis::addShell + libint2::Shell construction
  * (src/basis.cpp:33-66) and IntegralEngine::map_shell_to_basis_function /

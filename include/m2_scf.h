@@ -27,6 +27,8 @@ typedef struct {
     int df, mixed;
     double mixed_threshold;
     int fp64_final;
+    int guess;           /* 0: core Hamiltonian, 1: SOAD (needs the minimal basis) */
+    int diis_start;      /* DIIS from iteration diis_start + 1 on (reference source: 2) */
 } m2_scf_options_c;
 
 /* energy = one_e + two_e + enuc (SCF::calcE, src/scf.cpp:29-48); counts of
@@ -41,7 +43,9 @@ typedef struct {
 /* SCF::rhf (src/scf.cpp:90-185) on a molecule given as natom nuclear charges
  * Z and positions xyz (bohr, 3*natom), charge and multiplicity, and its
  * orbital shells (nsh, l, nprim, exps, raw coefs, centers 3*nsh, atoms) plus
- * optional auxiliary shells (nsh_aux may be 0).  Outputs may be NULL: D
+ * optional auxiliary shells (nsh_aux may be 0) and optional minimal-basis
+ * (STO-3G) shells for the SOAD guess, grouped by atom in atom order (nsh_min
+ * may be 0 unless o->guess == 1).  Outputs may be NULL: D
  * (n x n density), C (n x n MO coefficients, column-major), eps (n orbital
  * energies), log5 (maxiter x 5: energy, dE, dD, incremental, fp32 per
  * iteration), grad (natom x 3: Fock::compute_forces, src/rhfgrad.cpp:15-89).
@@ -51,9 +55,21 @@ typedef struct {
 int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charge, int multiplicity, int pure,
            int nsh, const int* l, const int* nprim, const double* exps, const double* coefs, const double* centers,
            const int* atoms, int nsh_aux, const int* l_aux, const int* nprim_aux, const double* exps_aux,
-           const double* coefs_aux, const double* centers_aux, const int* atoms_aux, const m2_scf_options_c* o,
-           m2_scf_result_c* res, double* D, double* C, double* eps, double* log5, double* grad);
+           const double* coefs_aux, const double* centers_aux, const int* atoms_aux, int nsh_min, const int* l_min,
+           const int* nprim_min, const double* exps_min, const double* coefs_min, const double* centers_min,
+           const int* atoms_min, const m2_scf_options_c* o, m2_scf_result_c* res, double* D, double* C, double* eps,
+           double* log5, double* grad);
 const char* m2_rhf_last_error(void);
+
+/* The two-electron part of the SOAD guess Fock matrix (Fock::compute_soad_guess,
+ * src/fock.cpp:174-225, with compute_2body_fock_general, src/fockbuild.cpp:
+ * 477-583): G (n x n, orbital basis) = sum_ls D(ls) [2 (uv|ls) - (ul|vs)]
+ * with D = diag(STO-3G ground-state occupations) / 2 over the minimal basis.
+ * Shells as for m2_rhf.  Returns 0 on success (m2_rhf_last_error otherwise). */
+int m2_soad_fock(int device, int natom, const double* Z, const double* xyz, int pure, int nsh, const int* l,
+                 const int* nprim, const double* exps, const double* coefs, const double* centers, const int* atoms,
+                 int nsh_min, const int* l_min, const int* nprim_min, const double* exps_min, const double* coefs_min,
+                 const double* centers_min, const int* atoms_min, double* G);
 
 #ifdef __cplusplus
 }

@@ -630,6 +630,8 @@ const char* m2_last_error(const m2_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 
 void* m2_ctx_stream(m2_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int m2_ctx_device(const m2_ctx* ctx) { return ctx ? ctx->device : -1; }
+
 m2_status m2_basis_set(m2_ctx* ctx, int which, int nsh, const int* l, const int* nprim, const double* exps,
                        const double* coefs, const double* centers, const int* atoms, int pure) {
     if (!ctx) return M2_ERR_ARG;

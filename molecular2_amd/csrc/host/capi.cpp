@@ -10,9 +10,45 @@
 
 static thread_local std::string g_rhf_error;
 
+static void load_shells(int ns, const int* ll, const int* np, const double* ex, const double* cf, const double* cen,
+                        const int* at, std::vector<molecular::RawShell>& out) {
+    int off = 0;
+    for (int s = 0; s < ns; ++s) {
+        molecular::RawShell r;
+        r.l = ll[s];
+        r.atom = at ? at[s] : -1;
+        for (int k = 0; k < 3; ++k) r.O[k] = cen[3 * s + k];
+        r.exps.assign(ex + off, ex + off + np[s]);
+        r.coefs.assign(cf + off, cf + off + np[s]);
+        off += np[s];
+        out.push_back(r);
+    }
+}
+
 extern "C" {
 
 const char* m2_rhf_last_error(void) { return g_rhf_error.c_str(); }
+
+int m2_soad_fock(int device, int natom, const double* Z, const double* xyz, int pure, int nsh, const int* l,
+                 const int* nprim, const double* exps, const double* coefs, const double* centers, const int* atoms,
+                 int nsh_min, const int* l_min, const int* nprim_min, const double* exps_min, const double* coefs_min,
+                 const double* centers_min, const int* atoms_min, double* G) {
+    using namespace molecular;
+    g_rhf_error.clear();
+    try {
+        Molecule mol;
+        mol.pure = pure != 0;
+        for (int a = 0; a < natom; ++a) mol.atoms.push_back(Atom{(int)Z[a], xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]});
+        load_shells(nsh, l, nprim, exps, coefs, centers, atoms, mol.shells);
+        load_shells(nsh_min, l_min, nprim_min, exps_min, coefs_min, centers_min, atoms_min, mol.minshells);
+        const Matrix g = soad_two_electron(device, mol, 1e-15);
+        std::memcpy(G, g.data(), sizeof(double) * g.rows() * g.cols());
+        return 0;
+    } catch (const std::exception& e) {
+        g_rhf_error = e.what();
+        return 1;
+    }
+}
 
 // shells: orbital (nsh, l, nprim, exps, coefs, centers, atoms), optional
 // auxiliary (nsh_aux may be 0).  Outputs D (n x n), C (n x n), eps (n) may
@@ -20,8 +56,10 @@ const char* m2_rhf_last_error(void) { return g_rhf_error.c_str(); }
 int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charge, int multiplicity, int pure,
            int nsh, const int* l, const int* nprim, const double* exps, const double* coefs, const double* centers,
            const int* atoms, int nsh_aux, const int* l_aux, const int* nprim_aux, const double* exps_aux,
-           const double* coefs_aux, const double* centers_aux, const int* atoms_aux, const m2_scf_options_c* o,
-           m2_scf_result_c* res, double* D, double* C, double* eps, double* log5 /* maxiter x 5 or NULL */,
+           const double* coefs_aux, const double* centers_aux, const int* atoms_aux, int nsh_min, const int* l_min,
+           const int* nprim_min, const double* exps_min, const double* coefs_min, const double* centers_min,
+           const int* atoms_min, const m2_scf_options_c* o, m2_scf_result_c* res, double* D, double* C, double* eps,
+           double* log5 /* maxiter x 5 or NULL */,
            double* grad /* natom x 3 or NULL: also compute Fock::compute_forces */) {
     using namespace molecular;
     g_rhf_error.clear();
@@ -48,6 +86,7 @@ int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
         };
         load(nsh, l, nprim, exps, coefs, centers, atoms, mol.shells);
         if (nsh_aux > 0) load(nsh_aux, l_aux, nprim_aux, exps_aux, coefs_aux, centers_aux, atoms_aux, mol.jkshells);
+        if (nsh_min > 0) load(nsh_min, l_min, nprim_min, exps_min, coefs_min, centers_min, atoms_min, mol.minshells);
         Options opt;
         opt.diis = o->diis != 0;
         opt.maxdiis = o->maxdiis;
@@ -59,6 +98,8 @@ int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
         opt.mixed = o->mixed != 0;
         opt.mixed_threshold = o->mixed_threshold;
         opt.fp64_final = o->fp64_final;
+        opt.guess = o->guess == 1 ? "soad" : "hcore";
+        opt.diis_start = o->diis_start;
         IntegralEngine ints(ctx, mol);
         Fock fock(opt, ints, mol);
         SCF scf(opt, mol, fock);

@@ -403,13 +403,128 @@ void Fock::average(const Vector& w) {
 void Fock::transform(bool first) {
     m2_ctx* ctx = integrals.context();
     if (first) {
-        if (opt.guess != "hcore")
-            throw Error("GUESS", "guess '" + opt.guess + "' is not available in this engine (use hcore)");
-        fockm = matmul(ctx, orthog, matmul(ctx, hcore, orthog), true, false);
-        fockinc = hcore;
+        if (opt.guess == "hcore") {
+            fockm = matmul(ctx, orthog, matmul(ctx, hcore, orthog), true, false);
+            fockinc = hcore;
+        } else if (opt.guess == "soad") {
+            compute_soad_guess();
+            fockm = matmul(ctx, orthog, matmul(ctx, focka, orthog), true, false);
+        } else {
+            throw Error("GUESS", "unknown guess '" + opt.guess + "' (hcore or soad)");
+        }
     } else {
         fockm = matmul(ctx, orthog, matmul(ctx, focka, orthog), true, false);
     }
+}
+
+Vector sto3g_occupations(int Z) {
+    if (Z < 1 || Z > 53) throw Error("SOAD", "STO-3G occupations are defined for 1 <= Z <= 53");
+    int nao;
+    if (Z <= 2) nao = 1;
+    else if (Z <= 10) nao = 5;
+    else if (Z <= 18) nao = 9;
+    else if (Z <= 20) nao = 13;
+    else if (Z <= 36) nao = 18;
+    else if (Z <= 38) nao = 22;
+    else nao = 27;
+    Vector occ(nao, 0.0);
+    size_t pos = 0;
+    // put min(ne, 2 size) electrons evenly into the next `size` AOs; AOs
+    // beyond the STO-3G set (3d of K and Ca) are dropped
+    auto fill = [&](int size, int& ne) {
+        const int n = std::min(ne, 2 * size);
+        ne -= n;
+        for (int f = 0; f < size; ++f)
+            if (pos + f < occ.size()) occ[pos + f] = double(n) / size;
+        pos += size;
+    };
+    int ne = Z;
+    fill(1, ne);                                        // 1s
+    if (Z > 2) { fill(1, ne); fill(3, ne); }            // 2s 2p
+    if (Z > 10) { fill(1, ne); fill(3, ne); }           // 3s 3p
+    if (Z > 18 && Z <= 36) {
+        int n4s = (Z == 19 || Z == 24 || Z == 29) ? 1 : 2;
+        ne -= n4s;
+        fill(1, n4s);                                   // 4s
+        int n4p = std::min(6, Z > 30 ? Z - 30 : 0);
+        ne -= n4p;
+        fill(3, n4p);                                   // 4p
+        fill(5, ne);                                    // 3d
+    }
+    if (Z > 36) {
+        fill(1, ne);                                    // 4s
+        fill(3, ne);                                    // 4p
+        int n5s = (Z == 37 || Z == 41 || Z == 42 || Z == 44 || Z == 45 || Z == 47) ? 1 : 2;
+        ne -= n5s;
+        fill(1, n5s);                                   // 5s
+        int n5p = std::min(6, Z > 48 ? Z - 48 : 0);
+        ne -= n5p;
+        fill(3, n5p);                                   // 5p
+        fill(5, ne);                                    // 3d
+        fill(5, ne);                                    // 4d
+    }
+    return occ;
+}
+
+Matrix soad_two_electron(int device, const Molecule& mol, double precision) {
+    if (mol.minshells.empty()) throw Error("SOAD", "no minimal (STO-3G) basis given for the SOAD guess");
+    // combined basis: orbital shells, then the minimal ones
+    std::vector<int> l, np, at;
+    std::vector<double> ex, cf, cen;
+    int nmin = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (const RawShell& r : pass == 0 ? mol.shells : mol.minshells) {
+            l.push_back(r.l);
+            np.push_back((int)r.exps.size());
+            at.push_back(r.atom);
+            ex.insert(ex.end(), r.exps.begin(), r.exps.end());
+            cf.insert(cf.end(), r.coefs.begin(), r.coefs.end());
+            for (int k = 0; k < 3; ++k) cen.push_back(r.O[k]);
+            if (pass == 1) {
+                if (r.l > 1) throw Error("SOAD", "minimal basis shells must have l <= 1");
+                nmin += 2 * r.l + 1;
+            }
+        }
+    // minimal-basis density: per atom, the occupations in shell order, halved
+    Vector dmin;
+    for (int a = 0; a < mol.getNAtoms(); ++a) {
+        int nao = 0;
+        for (const RawShell& r : mol.minshells)
+            if (r.atom == a) nao += 2 * r.l + 1;
+        const Vector occ = sto3g_occupations(mol.atoms[a].Z);
+        if ((int)occ.size() != nao)
+            throw Error("SOAD", "minimal basis of atom " + std::to_string(a) + " does not match STO-3G");
+        for (double o : occ) dmin.push_back(0.5 * o);
+    }
+    if ((int)dmin.size() != nmin) throw Error("SOAD", "minimal basis shells must be grouped by atom");
+    m2_ctx* tmp = nullptr;
+    if (m2_ctx_create(&tmp, device) != M2_OK) throw Error("SOAD", "m2_ctx_create failed");
+    struct Guard {
+        m2_ctx* c;
+        ~Guard() { m2_ctx_destroy(c); }
+    } guard{tmp};
+    if (m2_basis_set(tmp, M2_ORBITAL, (int)l.size(), l.data(), np.data(), ex.data(), cf.data(), cen.data(), at.data(),
+                     mol.pure ? 1 : 0) != M2_OK)
+        throw Error("SOAD", m2_last_error(tmp));
+    int ntot = 0;
+    m2_basis_info(tmp, M2_ORBITAL, &ntot, nullptr);
+    const int n = ntot - nmin;
+    Matrix D(ntot, ntot), J(ntot, ntot), K(ntot, ntot);
+    for (int i = 0; i < nmin; ++i) D(n + i, n + i) = dmin[i];
+    if (m2_jk_direct(tmp, D.data(), 1.0, precision, M2_FP64, J.data(), K.data()) != M2_OK)
+        throw Error("SOAD", m2_last_error(tmp));
+    Matrix G(n, n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) G(i, j) = 2.0 * J(i, j) - K(i, j);
+    return G;
+}
+
+void Fock::compute_soad_guess() {
+    // the minimal-basis J/K carries no screening worth the name in the
+    // reference (engine precision = machine epsilon)
+    Matrix G = soad_two_electron(m2_ctx_device(integrals.context()), molecule, 1e-15);
+    focka = hcore + G;
+    fockinc = focka;
 }
 
 void Fock::diagonalise() {
@@ -541,12 +656,12 @@ void SCF::rhf(bool print) {
             return std::sqrt(s);
         }() / ((double)old_dens.rows() * old_dens.cols());
         if (focker.rms_error < focker.next_reset || iter - focker.last_reset > 8) focker.reset_incremental = true;
-        if (iter > 2) weights = diis.compute(errs);
+        if (iter > opt.diis_start) weights = diis.compute(errs);
         errs.clear();
         calcE();
         const double delta = std::fabs(energy - last_energy);
         log.push_back({iter, energy, delta, dd, lap(), was_incremental, focker.fp64_forced ? 0 : 1});
-        if (iter > 2) focker.average(weights);
+        if (iter > opt.diis_start) focker.average(weights);
         focker.transform(false);
         bool conv = testConvergence(dd) && delta < opt.enconverge;
         if (focker.fp64_forced) ++fp64_iters;

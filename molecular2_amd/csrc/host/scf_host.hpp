@@ -78,6 +78,7 @@ public:
     std::vector<Atom> atoms;
     std::vector<RawShell> shells;      // orbital basis, reference shell order
     std::vector<RawShell> jkshells;    // DF auxiliary basis (may be empty)
+    std::vector<RawShell> minshells;   // STO-3G basis of the SOAD guess (may be empty)
     bool pure = true;
     int charge = 0, multiplicity = 1;
     int getNel() const;
@@ -97,7 +98,11 @@ struct Options {
     double precision = 1e-12;
     bool df = false;
     bool direct = true;
-    std::string guess = "hcore";
+    std::string guess = "hcore";      // or "soad" (superposition of atomic densities)
+    // DIIS extrapolation from SCF iteration diis_start + 1 on (2: the current
+    // reference source, src/scf.cpp:148,154; 0 reproduces the build that
+    // wrote the reference's recorded logs)
+    int diis_start = 2;
     // mixed precision: full builds with FP32 quartets below mixed_threshold,
     // incremental Delta-D builds entirely FP32, the last fp64_final
     // iterations full FP64 builds.
@@ -142,6 +147,19 @@ private:
     Matrix lastB;
 };
 
+// Average STO-3G AO occupations of the neutral ground-state atom Z (NIST
+// configurations, electrons smeared evenly over a subshell), in STO-3G shell
+// order: the restatement of sto3g_ao_occupation_vector
+// (inc/sto3g_atomic_density.hpp:56-110).  Z <= 53.
+Vector sto3g_occupations(int Z);
+
+// Two-electron SOAD Fock contribution in the orbital basis,
+// G(uv) = sum_ls D(ls) [2 (uv|ls) - (ul|vs)] with D = diag(occupations) / 2
+// over the minimal basis (compute_2body_fock_general, src/fockbuild.cpp:477-583,
+// with a shell-diagonal D).  Runs on a temporary context whose basis is the
+// orbital basis followed by the minimal one.
+Matrix soad_two_electron(int device, const Molecule& mol, double precision);
+
 // device linear algebra helpers
 Matrix matmul(m2_ctx* ctx, const Matrix& A, const Matrix& B, bool tA = false, bool tB = false);
 void eigh(m2_ctx* ctx, const Matrix& A, Vector& w, Matrix& V);
@@ -159,6 +177,8 @@ public:
     void addDiis();
     void average(const Vector& w);
     void transform(bool first = false);
+    // Fock::compute_soad_guess (src/fock.cpp:174-225): focka = H + G[D_SOAD]
+    void compute_soad_guess();
     void diagonalise();
     void makeDens();
     Matrix& getHCore() { return hcore; }

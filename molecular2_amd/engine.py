@@ -25,7 +25,8 @@ class SCFOptions(C.Structure):
     src/ProgramController.cpp:295-307, plus the precision policy)."""
     _fields_ = [("diis", C.c_int), ("maxdiis", C.c_int), ("maxiter", C.c_int),
                 ("enconverge", C.c_double), ("densconverge", C.c_double), ("precision", C.c_double),
-                ("df", C.c_int), ("mixed", C.c_int), ("mixed_threshold", C.c_double), ("fp64_final", C.c_int)]
+                ("df", C.c_int), ("mixed", C.c_int), ("mixed_threshold", C.c_double), ("fp64_final", C.c_int),
+                ("guess", C.c_int), ("diis_start", C.c_int)]
 
 
 class SCFResult(C.Structure):
@@ -82,7 +83,14 @@ def lib():
         L.m2_rhf.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
                              C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                              C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                             C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                              C.POINTER(SCFOptions), C.POINTER(SCFResult), _dp, _dp, _dp, _dp, _dp]
+        L.m2_soad_fock.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int,
+                                   C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                                   C.c_int, _ip, _ip, _dp, _dp, _dp, _ip, _dp]
+        L.m2_soad_fock.restype = C.c_int
+        L.m2_ctx_device.argtypes = [vp]
+        L.m2_ctx_device.restype = C.c_int
         L.m2_grad.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
         for name in ("m2_grad", "m2_df_setup", "m2_jk_df", "m2_jk_df_device", "m2_df_timing", "m2_eigh", "m2_gemm", "m2_rhf"):
             getattr(L, name).restype = C.c_int
@@ -245,13 +253,23 @@ class Engine:
         self._chk(self.L.m2_grad(self.h, mol.natom, _p(Z), _p(X), _p(D), _p(W), precision, _p(g)))
         return g.reshape(mol.natom, 3)
 
-    def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, gradient=False, **kw):
-        """Run the host C++ SCF (molecular::SCF::rhf) on `mol`.  Keyword
-        options as SCFOptions (defaults: the reference's RHF defaults)."""
+    def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, gradient=False, guess="hcore", **kw):
+        """Run the host C++ SCF (molecular::SCF::rhf) on `mol`.  guess:
+        "hcore" or "soad" (STO-3G superposition of atomic densities, the
+        reference's default).  Keyword options as SCFOptions (defaults: the
+        reference's RHF defaults)."""
+        if guess not in ("hcore", "soad"):
+            raise ValueError("guess must be 'hcore' or 'soad'")
         o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
-                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2)
+                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=1 if guess == "soad" else 0,
+                       diis_start=2)
         for k, v in kw.items():
             setattr(o, k, v)
+        if guess == "soad":
+            lm, npm, exm, cfm, cenm, atm = shell_arrays(mol.shells("sto-3g"))
+        else:
+            lm = npm = atm = np.zeros(1, np.int32)
+            exm = cfm = cenm = np.zeros(1)
         sh = mol.shells(basis)
         l, npr, ex, cf, cen, at = shell_arrays(sh)
         if aux:
@@ -275,6 +293,7 @@ class Engine:
         st = self.L.m2_rhf(self.h, mol.natom, _p(Z), _p(X), mol.charge, mol.multiplicity, int(pure),
                            len(sh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
                            nsa, ip(la), ip(npa), _p(exa), _p(cfa), _p(cena), ip(ata),
+                           len(lm) if guess == "soad" else 0, ip(lm), ip(npm), _p(exm), _p(cfm), _p(cenm), ip(atm),
                            C.byref(o), C.byref(res), _p(D), _p(Cm), _p(eps), _p(log),
                            _p(g) if gradient else None)
         if st != 0:
@@ -287,6 +306,25 @@ class Engine:
                 "total_seconds": res.total_seconds, "D": D, "C": Cm.T.copy(), "eps": eps,
                 "log": log[:res.iterations], "grad": g.reshape(mol.natom, 3) if gradient else None,
                 "grad_seconds": res.grad_seconds}
+
+    def soad_fock(self, mol, basis="cc-pvdz", pure=True):
+        """Two-electron part G of the SOAD guess Fock matrix in `basis`
+        (Fock::compute_soad_guess): sum_ls D(ls) [2 (uv|ls) - (ul|vs)] over the
+        STO-3G atomic occupations."""
+        sh = mol.shells(basis)
+        l, npr, ex, cf, cen, at = shell_arrays(sh)
+        lm, npm, exm, cfm, cenm, atm = shell_arrays(mol.shells("sto-3g"))
+        n = sum((2 * s.l + 1) if pure else (s.l + 1) * (s.l + 2) // 2 for s in sh)
+        G = np.zeros((n, n))
+        ip = lambda a: a.ctypes.data_as(_ip)
+        Z = np.asarray(mol.Z, float)
+        X = np.ascontiguousarray(mol.xyz, float)
+        st = self.L.m2_soad_fock(self.L.m2_ctx_device(self.h), mol.natom, _p(Z), _p(X), int(pure),
+                                 len(sh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
+                                 len(lm), ip(lm), ip(npm), _p(exm), _p(cfm), _p(cenm), ip(atm), _p(G))
+        if st != 0:
+            raise M2Error(st, self.L.m2_rhf_last_error().decode())
+        return G
 
     def last_stats(self):
         a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()

@@ -285,15 +285,84 @@ def _eigh_sorted(A):
     return w, U
 
 
+def sto3g_occupations(Z: int):
+    """Average STO-3G AO occupations of the neutral ground-state atom (NIST
+    configurations, electrons smeared evenly over each subshell), in STO-3G
+    shell order (sto3g_ao_occupation_vector, inc/sto3g_atomic_density.hpp:56-110)."""
+    if not 1 <= Z <= 53:
+        raise ValueError("STO-3G occupations are defined for 1 <= Z <= 53")
+    nao = 1 if Z <= 2 else 5 if Z <= 10 else 9 if Z <= 18 else 13 if Z <= 20 else 18 if Z <= 36 else \
+        22 if Z <= 38 else 27
+    occ = []
+
+    def fill(size, ne):
+        n = min(ne, 2 * size)
+        occ.extend([n / size] * size)
+        return ne - n
+
+    ne = fill(1, Z)                                        # 1s
+    if Z > 2:
+        ne = fill(3, fill(1, ne))                          # 2s 2p
+    if Z > 10:
+        ne = fill(3, fill(1, ne))                          # 3s 3p
+    if 18 < Z <= 36:
+        n4s = 1 if Z in (19, 24, 29) else 2
+        ne -= n4s
+        fill(1, n4s)                                       # 4s
+        n4p = min(6, Z - 30 if Z > 30 else 0)
+        ne -= n4p
+        fill(3, n4p)                                       # 4p
+        fill(5, ne)                                        # 3d
+    if Z > 36:
+        ne = fill(3, fill(1, ne))                          # 4s 4p
+        n5s = 1 if Z in (37, 41, 42, 44, 45, 47) else 2
+        ne -= n5s
+        fill(1, n5s)                                       # 5s
+        n5p = min(6, Z - 48 if Z > 48 else 0)
+        ne -= n5p
+        fill(3, n5p)                                       # 5p
+        ne = fill(5, ne)                                   # 3d
+        fill(5, ne)                                        # 4d
+    # K and Ca: the 3d entries fall outside the 13 STO-3G AOs and are dropped
+    return np.array(occ[:nao])
+
+
+def soad_two_electron(obs_shells, min_shells, Z, pure=True):
+    """G(uv) = sum_ls D(ls) [2 (uv|ls) - (ul|vs)] with D = diag(occ) / 2 over
+    the minimal basis: compute_2body_fock_general (src/fockbuild.cpp:477-583)
+    with the shell-diagonal SOAD density of Fock::compute_soad_guess
+    (src/fock.cpp:174-225).  Evaluated as J/K over the orbital basis followed
+    by the minimal one, with the density non-zero on the minimal block only,
+    and no screening (the reference runs at machine-epsilon precision)."""
+    B = Basis(list(obs_shells) + list(min_shells), pure)
+    n = sum((2 * s.l + 1) if pure else (s.l + 1) * (s.l + 2) // 2 for s in obs_shells)
+    dmin = np.concatenate([0.5 * sto3g_occupations(int(z)) for z in Z])
+    if n + len(dmin) != B.nbf:
+        raise ValueError("minimal basis does not match the STO-3G occupations")
+    D = np.zeros((B.nbf, B.nbf))
+    D[np.arange(n, B.nbf), np.arange(n, B.nbf)] = dmin
+    J, K = B.jk_direct(D, precision=0.0)
+    return 2.0 * J[:n, :n] - K[:n, :n]
+
+
 class RHF:
     """Closed-shell SCF restatement.  `jk(P, multiplier)` must return (J, K)
     for the direct/in-core paths (jkints = J - K/2) or, with df=True,
     `jk(Cocc)` returns (J, K) over C C^T (jkints = 2J - K)."""
 
     def __init__(self, S, H, enuc, nel, jk, df=False, maxdiis=8, diis=True, maxiter=40,
-                 enconverge=1e-7, densconverge=1e-4, guess="hcore"):
+                 enconverge=1e-7, densconverge=1e-4, guess="hcore", soad_G=None, diis_start=2):
         if nel % 2:
             raise ValueError("RHF: Molecule has an odd number of electrons.")
+        if guess not in ("hcore", "soad") or (guess == "soad" and soad_G is None):
+            raise ValueError("guess: 'hcore', or 'soad' with soad_G (soad_two_electron)")
+        self.guess = guess
+        self.soad_G = soad_G
+        # DIIS extrapolation from loop iteration diis_start + 1 on: 2 in the
+        # current source (src/scf.cpp:148,154); the build that wrote
+        # test/h2otest.out_1 and test/benzene.out_1 extrapolated from
+        # iteration 1 (diis_start = 0)
+        self.diis_start = diis_start
         self.S, self.H, self.enuc, self.nel = S, H, enuc, nel
         self.nocc = nel // 2
         self.n = S.shape[0]
@@ -386,9 +455,15 @@ class RHF:
             self.focka = F
 
     def transform(self, first=False):
-        if first:
+        # Fock::transform (src/fock.cpp:156-172)
+        if first and self.guess == "hcore":
             self.fockm = self.X.T @ self.H @ self.X
             self.fockinc = self.H.copy()
+        elif first:
+            # Fock::compute_soad_guess (src/fock.cpp:174-225)
+            self.focka = self.H + self.soad_G
+            self.fockinc = self.focka.copy()
+            self.fockm = self.X.T @ (self.focka @ self.X)
         else:
             self.fockm = self.X.T @ (self.focka @ self.X)
 
@@ -435,13 +510,13 @@ class RHF:
             self.rms_error = np.linalg.norm(errs[0]) / (old.shape[0] * old.shape[1])
             if self.rms_error < self.next_reset or it - self.last_reset > 8:
                 self.reset_incremental = True
-            if it > 2:
+            if it > self.diis_start:
                 w = diis.compute(errs)
             errs = []
             self.calcE()
             delta = abs(self.energy - self.last_energy)
             self.log.append((it, self.energy, delta, dd))
-            if it > 2:
+            if it > self.diis_start:
                 self.average(w)
             self.transform(False)
             converged = dd < self.dconv and delta < self.econv

@@ -58,3 +58,70 @@ def test_odd_electrons_rejected(engine):
     mol = M.Molecule([8, 1], np.array([[0, 0, 0], [0, 0, 1.8]]))
     with pytest.raises(Exception, match="odd number of electrons"):
         engine.rhf(mol, pure=True)
+
+
+def _golden_logs():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "scf_logs.json")) as fh:
+        return json.load(fh)
+
+
+def test_scf_trajectory_matches_recorded_water_log(engine):
+    """The device SCF reproduces every iteration of the reference's recorded
+    run test/h2otest.out_1 (energy, |dE|, dD), with that build's DIIS start
+    (diis_start = 0)."""
+    rows = _golden_logs()["h2otest"]["rows"]
+    print("case: C++ SCF trajectory vs test/h2otest.out_1", flush=True)
+    r = engine.rhf(M.water_h2otest(), pure=False, enconverge=1e-8, densconverge=1e-8, diis_start=0)
+    assert r["converged"] and r["iterations"] == len(rows)
+    log = r["log"]
+    for k, (it_r, E_r, dE_r, dD_r) in enumerate(rows):
+        E, dE, dD = log[k, 0], log[k, 1], log[k, 2]
+        assert abs(E - E_r) < 2e-9, (k, E, E_r)
+        assert abs(dE - dE_r) < 2e-9 + 1e-8 * dE_r, (k, dE, dE_r)
+        assert abs(dD - dD_r) < 1e-8 + 1e-8 * dD_r, (k, dD, dD_r)
+
+
+def test_scf_first_iterations_match_recorded_benzene_log(engine):
+    """test/benzene.out_1 (direct run): iterations 0-2, before any DIIS."""
+    rows = _golden_logs()["benzene"]["rows"]
+    print("case: C++ SCF benzene first iterations vs test/benzene.out_1", flush=True)
+    r = engine.rhf(M.benzene_dz_ref(), pure=False, maxiter=3)
+    log = r["log"]
+    assert len(log) == 3
+    for k, (it_r, E_r, dE_r, dD_r) in enumerate(rows[:3]):
+        assert abs(log[k, 0] - E_r) < 2e-6 and abs(log[k, 1] - dE_r) < 2e-6 and abs(log[k, 2] - dD_r) < 2e-6, k
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_soad_fock_matches_oracle(oracle, engine, pure):
+    """Two-electron SOAD guess Fock (Fock::compute_soad_guess) on the device
+    vs the oracle's restatement."""
+    mol = M.water()
+    print("case: SOAD G water pure=%s" % pure, flush=True)
+    G = engine.soad_fock(mol, pure=pure)
+    Go = oracle.soad_two_electron(mol.shells("cc-pvdz"), mol.shells("sto-3g"), mol.Z, pure)
+    assert np.max(np.abs(G - Go)) < 1e-10
+
+
+def test_soad_fock_benzene_matches_oracle(oracle, engine):
+    mol = M.benzene()
+    print("case: SOAD G benzene", flush=True)
+    G = engine.soad_fock(mol, pure=True)
+    Go = oracle.soad_two_electron(mol.shells("cc-pvdz"), mol.shells("sto-3g"), mol.Z, True)
+    assert np.max(np.abs(G - Go)) < 1e-10
+
+
+def test_scf_soad_guess(engine):
+    """SOAD start (the reference's default guess): same energy as the core
+    start, in fewer iterations."""
+    mol = M.water_h2otest()
+    print("case: C++ SCF water 6d, SOAD vs core guess", flush=True)
+    r1 = engine.rhf(mol, pure=False, guess="soad", enconverge=1e-10, densconverge=1e-8)
+    r0 = engine.rhf(mol, pure=False, guess="hcore", enconverge=1e-10, densconverge=1e-8)
+    assert r1["converged"] and r0["converged"]
+    assert abs(r1["energy"] - r0["energy"]) < 1e-9
+    assert abs(r1["energy"] - (-75.990178761701)) < 2e-9
+    assert r1["log"][0, 0] < -75.0
+    assert r1["iterations"] < r0["iterations"]
