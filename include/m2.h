@@ -91,6 +91,13 @@ m2_status m2_jk_direct(m2_ctx* ctx, const double* D, double multiplier, double p
 /* same on device buffers (n*n doubles each) */
 m2_status m2_jk_direct_device(m2_ctx* ctx, const double* dD, double multiplier, double precision,
                               unsigned flags, double* dJ, double* dK);
+/* UnrestrictedFock::formJKdirect (src/fockbuild.cpp:267-364): J and K of the
+ * alpha and beta densities (Da = Ca_occ Ca_occ^T, Db likewise), every quartet
+ * screened with the block norms of Da + Db; the unrestricted Fock matrices
+ * are F_a = H + Ja + Jb - Ka and F_b = H + Ja + Jb - Kb
+ * (UnrestrictedFock::makeFock, src/fockbuild.cpp:431-445). */
+m2_status m2_jk_direct_uhf(m2_ctx* ctx, const double* Da, const double* Db, double multiplier, double precision,
+                           unsigned flags, double* Ja, double* Ka, double* Jb, double* Kb);
 /* quartets with Schwarz bound x block norm below this run in FP32 under
  * M2_MIXED (default 1e-4) */
 m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below);

@@ -61,6 +61,20 @@ int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
            double* log5, double* grad);
 const char* m2_rhf_last_error(void);
 
+/* SCF::uhf (src/scf.cpp:187-272) with UnrestrictedFock: arguments as for
+ * m2_rhf (multiplicity = 2S + 1; n_alpha = (N - mult + 1) / 2 + mult - 1,
+ * src/molecule.cpp:425-439).  Outputs may be NULL: Da, Db (n x n spin
+ * densities C_s,occ C_s,occ^T), eps_a, eps_b (n), log5 (maxiter x 5:
+ * energy, dE, dD per iteration from iteration 1).  J/K are direct (FP64) or,
+ * with o->df, density fitted per spin. */
+int m2_uhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charge, int multiplicity, int pure,
+           int nsh, const int* l, const int* nprim, const double* exps, const double* coefs, const double* centers,
+           const int* atoms, int nsh_aux, const int* l_aux, const int* nprim_aux, const double* exps_aux,
+           const double* coefs_aux, const double* centers_aux, const int* atoms_aux, int nsh_min, const int* l_min,
+           const int* nprim_min, const double* exps_min, const double* coefs_min, const double* centers_min,
+           const int* atoms_min, const m2_scf_options_c* o, m2_scf_result_c* res, double* Da, double* Db,
+           double* eps_a, double* eps_b, double* log5);
+
 /* The two-electron part of the SOAD guess Fock matrix (Fock::compute_soad_guess,
  * src/fock.cpp:174-225, with compute_2body_fock_general, src/fockbuild.cpp:
  * 477-583): G (n x n, orbital basis) = sum_ls D(ls) [2 (uv|ls) - (ul|vs)]

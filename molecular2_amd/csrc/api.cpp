@@ -388,8 +388,10 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
     return ranges;
 }
 
+// dScreen (optional): the density whose shell-block norms screen the quartets
+// (UnrestrictedFock::formJKdirect screens both spin builds with Da + Db).
 m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, double precision, unsigned flags,
-                         double* dJ, double* dK) {
+                         double* dJ, double* dK, const double* dScreen = nullptr) {
     BasisData& b = ctx->obs;
     const int n = b.nbf, nsh = b.nsh;
     ARGCHK(n > 0, "m2_jk_direct: no orbital basis set");
@@ -400,7 +402,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     HIPCHK(ctx->d_K.reserve((size_t)n * n));
     HIPCHK(ctx->d_lists.reserve(2 * NCOMBO + 2));
     // shell-block norms, max|D| and the register-class work lists, on the device
-    launch_blocknorm(dD, n, nsh, b.d_bf.p, b.d_nbf.p, ctx->d_Dn.p, ctx->stream);
+    launch_blocknorm(dScreen ? dScreen : dD, n, nsh, b.d_bf.p, b.d_nbf.p, ctx->d_Dn.p, ctx->stream);
     double* d_dmax = reinterpret_cast<double*>(ctx->d_lists.p + 2 * NCOMBO);
     launch_dmax(ctx->d_Dn.p, nsh * nsh, d_dmax, ctx->stream);
     for (int ci = 0; ci < NCOMBO; ++ci) {
@@ -803,6 +805,34 @@ m2_status m2_jk_direct(m2_ctx* ctx, const double* D, double multiplier, double p
     if (st) return st;
     HIPCHK(hipMemcpyAsync(J, ctx->d_Jo.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(K, ctx->d_Ko.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return M2_OK;
+}
+
+m2_status m2_jk_direct_uhf(m2_ctx* ctx, const double* Da, const double* Db, double multiplier, double precision,
+                           unsigned flags, double* Ja, double* Ka, double* Jb, double* Kb) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(Da && Db && Ja && Ka && Jb && Kb, "m2_jk_direct_uhf: null buffer");
+    ARGCHK(ctx->obs.nbf > 0, "m2_jk_direct_uhf: no orbital basis set");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)ctx->obs.nbf * ctx->obs.nbf;
+    std::vector<double> Dt(n2);
+    for (size_t i = 0; i < n2; ++i) Dt[i] = Da[i] + Db[i];
+    HIPCHK(ctx->d_D.reserve(3 * n2));
+    HIPCHK(ctx->d_Jo.reserve(2 * n2));
+    HIPCHK(ctx->d_Ko.reserve(2 * n2));
+    double *dDa = ctx->d_D.p, *dDb = dDa + n2, *dDt = dDb + n2;
+    HIPCHK(hipMemcpyAsync(dDa, Da, sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dDb, Db, sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dDt, Dt.data(), sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
+    m2_status st = jk_device_core(ctx, dDa, multiplier, precision, flags, ctx->d_Jo.p, ctx->d_Ko.p, dDt);
+    if (st) return st;
+    st = jk_device_core(ctx, dDb, multiplier, precision, flags, ctx->d_Jo.p + n2, ctx->d_Ko.p + n2, dDt);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(Ja, ctx->d_Jo.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(Ka, ctx->d_Ko.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(Jb, ctx->d_Jo.p + n2, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(Kb, ctx->d_Ko.p + n2, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return M2_OK;
 }

@@ -688,4 +688,170 @@ void SCF::rhf(bool print) {
     // not thrown: the caller reads `converged`
 }
 
+// ---------------------------------------------------------------- UHF
+UnrestrictedFock::UnrestrictedFock(const Options& o, IntegralEngine& ints, const Molecule& m) : Fock(o, ints, m) {
+    nalpha = m.nalpha();
+    nbeta = m.nbeta();
+    if (nalpha < 0 || nbeta < 0 || nalpha > nbfs || (m.getNel() - m.multiplicity + 1) % 2 != 0)
+        throw Error("UHF", "electron count and multiplicity do not fit");
+}
+
+// src/fock.cpp:305-323
+void UnrestrictedFock::transform(bool first) {
+    m2_ctx* ctx = integrals.context();
+    if (first) {
+        if (opt.guess == "hcore") {
+            fock_alpha_mo = matmul(ctx, orthog, matmul(ctx, hcore, orthog), true, false);
+        } else if (opt.guess == "soad") {
+            compute_soad_guess();
+            fock_alpha_mo = matmul(ctx, orthog, matmul(ctx, focka, orthog), true, false);
+        } else {
+            throw Error("GUESS", "unknown guess '" + opt.guess + "' (hcore or soad)");
+        }
+        fock_beta_mo = fock_alpha_mo;
+    } else {
+        fock_alpha_mo = matmul(ctx, orthog, matmul(ctx, fock_alpha_ao, orthog), true, false);
+        fock_beta_mo = matmul(ctx, orthog, matmul(ctx, fock_beta_ao, orthog), true, false);
+    }
+}
+
+// src/fock.cpp:325-335
+void UnrestrictedFock::diagonalise() {
+    m2_ctx* ctx = integrals.context();
+    Matrix Ua, Ub;
+    eigh(ctx, fock_alpha_mo, eps_alpha, Ua);
+    eigh(ctx, fock_beta_mo, eps_beta, Ub);
+    CP_alpha = matmul(ctx, orthog, Ua);
+    CP_beta = matmul(ctx, orthog, Ub);
+}
+
+// src/fock.cpp:337-342: D_s = C_s,occ C_s,occ^T (one electron per orbital)
+void UnrestrictedFock::makeDens() {
+    m2_ctx* ctx = integrals.context();
+    Matrix Ca = CP_alpha.block(0, 0, nbfs, nalpha), Cb = CP_beta.block(0, 0, nbfs, nbeta);
+    dens_alpha = nalpha > 0 ? matmul(ctx, Ca, Ca, false, true) : Matrix(nbfs, nbfs);
+    dens_beta = nbeta > 0 ? matmul(ctx, Cb, Cb, false, true) : Matrix(nbfs, nbfs);
+}
+
+// src/fockbuild.cpp:412-429
+void UnrestrictedFock::makeJK() {
+    if (density_fitted) {
+        Matrix ca = CP_alpha.block(0, 0, nbfs, nalpha), cb = CP_beta.block(0, 0, nbfs, nbeta);
+        formJKdf(ca, cb);
+    } else {
+        formJKdirect(integrals.getPrescreen(), dens_alpha, dens_beta);
+    }
+}
+
+void UnrestrictedFock::formJKdirect(const Matrix& Schwarz, Matrix& Da, Matrix& Db, double multiplier) {
+    (void)Schwarz;   // the engine holds the device copy of the same Schwarz bounds
+    m2_ctx* ctx = integrals.context();
+    jints_alpha = Matrix(nbfs, nbfs);
+    kints_alpha = Matrix(nbfs, nbfs);
+    jints_beta = Matrix(nbfs, nbfs);
+    kints_beta = Matrix(nbfs, nbfs);
+    auto t0 = std::chrono::steady_clock::now();
+    check(ctx, m2_jk_direct_uhf(ctx, Da.data(), Db.data(), multiplier, precision, M2_FP64, jints_alpha.data(),
+                                kints_alpha.data(), jints_beta.data(), kints_beta.data()),
+          "m2_jk_direct_uhf");
+    jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    ++n_full;
+}
+
+// src/fockbuild.cpp:366-373 (J and K over C C^T of each spin)
+void UnrestrictedFock::formJKdf(Matrix& ca_occ, Matrix& cb_occ, double multiplier) {
+    m2_ctx* ctx = integrals.context();
+    auto one = [&](Matrix& c, Matrix& J, Matrix& K) {
+        J = Matrix(nbfs, nbfs);
+        K = Matrix(nbfs, nbfs);
+        if (c.cols() == 0) return;
+        check(ctx, m2_jk_df(ctx, c.data(), c.cols(), opt.mixed ? 1 : 0, J.data(), K.data()), "m2_jk_df");
+        J = multiplier * J;
+        K = multiplier * K;
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    one(ca_occ, jints_alpha, kints_alpha);
+    one(cb_occ, jints_beta, kints_beta);
+    jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    ++n_full;
+}
+
+// src/fockbuild.cpp:431-445
+void UnrestrictedFock::makeFock() {
+    fock_alpha_ao = hcore + jints_alpha + jints_beta;
+    fock_beta_ao = fock_alpha_ao - kints_beta;
+    fock_alpha_ao = fock_alpha_ao - kints_alpha;
+    if (diis) {
+        if (iter >= MAX) {
+            alpha_focks.erase(alpha_focks.begin());
+            beta_focks.erase(beta_focks.begin());
+        }
+        alpha_focks.push_back(fock_alpha_ao);
+        beta_focks.push_back(fock_beta_ao);
+        iter++;
+    }
+}
+
+// src/fock.cpp:344-355
+void UnrestrictedFock::average(const Vector& w) {
+    if (diis && iter > 2) {
+        fock_alpha_ao = Matrix(nbfs, nbfs);
+        fock_beta_ao = Matrix(nbfs, nbfs);
+        const int offset = (int)alpha_focks.size() - (int)w.size();
+        for (int i = offset; i < (int)alpha_focks.size(); ++i) {
+            fock_alpha_ao += w[i - offset] * alpha_focks[i];
+            fock_beta_ao += w[i - offset] * beta_focks[i];
+        }
+    }
+}
+
+// src/scf.cpp:193-272: DIIS from the first iteration over both spins' error
+// vectors; the energy uses the averaged Fock matrices.
+void SCF::uhf(bool print) {
+    (void)print;
+    auto* uf = dynamic_cast<UnrestrictedFock*>(&focker);
+    if (!uf) throw Error("UHF", "SCF::uhf needs an UnrestrictedFock");
+    UnrestrictedFock& U = *uf;
+    auto tic = std::chrono::steady_clock::now();
+    auto lap = [&]() {
+        auto t = std::chrono::steady_clock::now();
+        double sec = std::chrono::duration<double>(t - tic).count();
+        tic = t;
+        return sec;
+    };
+    converged = false;
+    log.clear();
+    U.transform(true);
+    U.diagonalise();
+    const int n = U.getHCore().rows();
+    Matrix old_a(n, n), old_b(n, n);
+    int iter = 1;
+    energy = 0.0;
+    while (!converged && iter < opt.maxiter) {
+        U.makeDens();
+        U.makeJK();
+        U.makeFock();
+        std::vector<Vector> errs;
+        errs.push_back(calcErr(U.getFockAlphaAO(), U.getDensAlpha(), U.getS(), U.getOrthog()));
+        errs.push_back(calcErr(U.getFockBetaAO(), U.getDensBeta(), U.getS(), U.getOrthog()));
+        Vector weights = diis.compute(errs);
+        U.average(weights);
+        const double ea = calcE(U.getHCore(), U.getDensAlpha(), U.getFockAlphaAO());
+        const double eb = calcE(U.getHCore(), U.getDensBeta(), U.getFockBetaAO());
+        U.transform(false);
+        U.diagonalise();
+        last_energy = energy;
+        energy = ea + eb + molecule.getEnuc();
+        const double delta = std::fabs(energy - last_energy);
+        const double dist = (U.getDensAlpha() + U.getDensBeta() - old_a - old_b).norm();
+        old_a = U.getDensAlpha();
+        old_b = U.getDensBeta();
+        log.push_back({iter, energy, delta, dist, lap(), 0, 0});
+        if (delta < opt.enconverge && dist < opt.densconverge) converged = true;
+        iter++;
+    }
+    U.diagonalise();
+    iterations = iter;
+}
+
 }  // namespace molecular

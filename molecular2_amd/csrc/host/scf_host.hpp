@@ -82,6 +82,9 @@ public:
     bool pure = true;
     int charge = 0, multiplicity = 1;
     int getNel() const;
+    // Molecule::nalpha / nbeta (src/molecule.cpp:425-439)
+    int nalpha() const { return (getNel() - multiplicity + 1) / 2 + multiplicity - 1; }
+    int nbeta() const { return (getNel() - multiplicity + 1) / 2; }
     int getNAtoms() const { return (int)atoms.size(); }
     int getMultiplicity() const { return multiplicity; }
     double getEnuc() const;            // Molecule::calcEnuc, src/molecule.cpp:443-457
@@ -167,6 +170,7 @@ void eigh(m2_ctx* ctx, const Matrix& A, Vector& w, Matrix& V);
 class Fock {
 public:
     Fock(const Options& opt, IntegralEngine& ints, const Molecule& m);
+    virtual ~Fock() = default;
     void formHCore();
     void formOrthog();
     void makeJK(Matrix& P, double multiplier = 1.0);
@@ -207,7 +211,7 @@ public:
     int n_full = 0, n_incremental = 0, n_fp32_builds = 0;
     double jk_seconds = 0.0;
 
-private:
+protected:
     const Options opt;
     IntegralEngine& integrals;
     const Molecule& molecule;
@@ -218,6 +222,43 @@ private:
     bool diis, density_fitted, direct;
     double precision;
     bool last_build_incremental = false;
+};
+
+// UnrestrictedFock (inc/fock.hpp:215-273): alpha and beta Fock matrices,
+// F_a = H + Ja + Jb - Ka, F_b = H + Ja + Jb - Kb (src/fockbuild.cpp:431-445),
+// no incremental builds.
+class UnrestrictedFock : public Fock {
+public:
+    UnrestrictedFock(const Options& opt, IntegralEngine& ints, const Molecule& m);
+    void transform(bool first = false);
+    void diagonalise();
+    void makeDens();
+    void makeJK();
+    void formJKdirect(const Matrix& Schwarz, Matrix& Da, Matrix& Db, double multiplier = 1.0);
+    void formJKdf(Matrix& ca_occ, Matrix& cb_occ, double multiplier = 1.0);
+    void makeFock();
+    void average(const Vector& w);
+    Matrix& getFockAlphaAO() { return fock_alpha_ao; }
+    Matrix& getFockBetaAO() { return fock_beta_ao; }
+    Matrix& getDensAlpha() { return dens_alpha; }
+    Matrix& getDensBeta() { return dens_beta; }
+    Matrix& getJAlpha() { return jints_alpha; }
+    Matrix& getJBeta() { return jints_beta; }
+    Matrix& getKAlpha() { return kints_alpha; }
+    Matrix& getKBeta() { return kints_beta; }
+    Matrix& getCPAlpha() { return CP_alpha; }
+    Matrix& getCPBeta() { return CP_beta; }
+    Vector& getEpsAlpha() { return eps_alpha; }
+    Vector& getEpsBeta() { return eps_beta; }
+    int getNalpha() const { return nalpha; }
+    int getNbeta() const { return nbeta; }
+
+protected:
+    Matrix fock_alpha_ao, fock_beta_ao, fock_alpha_mo, fock_beta_mo, dens_alpha, dens_beta;
+    Matrix kints_alpha, kints_beta, jints_alpha, jints_beta, CP_alpha, CP_beta;
+    Vector eps_alpha, eps_beta;
+    std::vector<Matrix> alpha_focks, beta_focks;
+    int nalpha = 0, nbeta = 0;
 };
 
 struct IterationRecord {
@@ -235,6 +276,9 @@ public:
     Vector calcErr(const Matrix& F, const Matrix& D, const Matrix& S, const Matrix& orthog);
     bool testConvergence(double val);
     void rhf(bool print = false);
+    // SCF::uhf / uhf_internal (src/scf.cpp:187-272); the Fock object must be
+    // an UnrestrictedFock
+    void uhf(bool print = false);
     double getEnergy() const { return energy; }
     double getOneE() const { return one_E; }
     double getTwoE() const { return two_E; }

@@ -85,6 +85,14 @@ def lib():
                              C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                              C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                              C.POINTER(SCFOptions), C.POINTER(SCFResult), _dp, _dp, _dp, _dp, _dp]
+        L.m2_uhf.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
+                             C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                             C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                             C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                             C.POINTER(SCFOptions), C.POINTER(SCFResult), _dp, _dp, _dp, _dp, _dp]
+        L.m2_uhf.restype = C.c_int
+        L.m2_jk_direct_uhf.argtypes = [vp, _dp, _dp, C.c_double, C.c_double, C.c_uint, _dp, _dp, _dp, _dp]
+        L.m2_jk_direct_uhf.restype = C.c_int
         L.m2_soad_fock.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int,
                                    C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                                    C.c_int, _ip, _ip, _dp, _dp, _dp, _ip, _dp]
@@ -306,6 +314,53 @@ class Engine:
                 "total_seconds": res.total_seconds, "D": D, "C": Cm.T.copy(), "eps": eps,
                 "log": log[:res.iterations], "grad": g.reshape(mol.natom, 3) if gradient else None,
                 "grad_seconds": res.grad_seconds}
+
+    def jk_uhf(self, Da, Db, multiplier=1.0, precision=1e-12, mixed=False):
+        """UnrestrictedFock::formJKdirect: (Ja, Ka, Jb, Kb), screened with
+        the block norms of Da + Db."""
+        Da = np.ascontiguousarray(Da, float)
+        Db = np.ascontiguousarray(Db, float)
+        out = [np.zeros_like(Da) for _ in range(4)]
+        self._chk(self.L.m2_jk_direct_uhf(self.h, _p(Da), _p(Db), multiplier, precision, M2_MIXED if mixed else 0,
+                                          *[_p(x) for x in out]))
+        return tuple(out)
+
+    def uhf(self, mol, basis="cc-pvdz", pure=True, aux=None, guess="hcore", **kw):
+        """Run the host C++ unrestricted SCF (molecular::SCF::uhf)."""
+        if guess not in ("hcore", "soad"):
+            raise ValueError("guess must be 'hcore' or 'soad'")
+        o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
+                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=1 if guess == "soad" else 0,
+                       diis_start=2)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        sh = mol.shells(basis)
+        l, npr, ex, cf, cen, at = shell_arrays(sh)
+        z1 = np.zeros(1, np.int32)
+        f1 = np.zeros(1)
+        la, npa, exa, cfa, cena, ata = shell_arrays(mol.shells(aux)) if aux else (z1, z1, f1, f1, f1, z1)
+        lm, npm, exm, cfm, cenm, atm = shell_arrays(mol.shells("sto-3g")) if guess == "soad" else \
+            (z1, z1, f1, f1, f1, z1)
+        n = sum((2 * s.l + 1) if pure else (s.l + 1) * (s.l + 2) // 2 for s in sh)
+        Da, Db = np.zeros((n, n)), np.zeros((n, n))
+        ea, eb = np.zeros(n), np.zeros(n)
+        log = np.zeros((max(1, o.maxiter), 5))
+        res = SCFResult()
+        ip = lambda a: a.ctypes.data_as(_ip)
+        st = self.L.m2_uhf(self.h, mol.natom, _p(np.asarray(mol.Z, float)), _p(np.ascontiguousarray(mol.xyz, float)),
+                           mol.charge, mol.multiplicity, int(pure),
+                           len(sh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
+                           len(la) if aux else 0, ip(la), ip(npa), _p(exa), _p(cfa), _p(cena), ip(ata),
+                           len(lm) if guess == "soad" else 0, ip(lm), ip(npm), _p(exm), _p(cfm), _p(cenm), ip(atm),
+                           C.byref(o), C.byref(res), _p(Da), _p(Db), _p(ea), _p(eb), _p(log))
+        if st != 0:
+            raise M2Error(st, self.L.m2_rhf_last_error().decode())
+        self.nbf = n
+        self.nsh = len(sh)
+        nrow = max(0, res.iterations - 1)
+        return {"energy": res.energy, "iterations": res.iterations, "converged": bool(res.converged),
+                "Da": Da, "Db": Db, "eps_a": ea, "eps_b": eb, "log": log[:nrow], "jk_seconds": res.jk_seconds,
+                "total_seconds": res.total_seconds}
 
     def soad_fock(self, mol, basis="cc-pvdz", pure=True):
         """Two-electron part G of the SOAD guess Fock matrix in `basis`

@@ -532,13 +532,17 @@ void or_shellblock_norm(const or_basis* b, const double* A, double* out) {
 /* Quartets with s1 in [s1lo, s1hi) only (a bounded sample of one build when
  * the range is partial); counts[0] += quartets, counts[1] += primitive
  * quartets.  OpenMP over the (s1, s2) pairs of the range. */
-long or_jk_direct_range(const or_basis* b, const double* D, const double* Q, double precision,
-                        double multiplier, double* J, double* K, int nthreads, int s1lo, int s1hi,
-                        long* counts) {
+/* Dscreen: the density whose shell-block norms screen the quartets (NULL:
+ * D itself).  UnrestrictedFock::formJKdirect (src/fockbuild.cpp:267-364)
+ * screens both spin builds with Da + Db and digests each density like the
+ * restricted loop. */
+long or_jk_direct_range_s(const or_basis* b, const double* D, const double* Dscreen, const double* Q,
+                          double precision, double multiplier, double* J, double* K, int nthreads, int s1lo,
+                          int s1hi, long* counts) {
     const int nsh = b->nsh, n = b->nbf;
     const int do_screen = (Q != NULL);
     double* Dn = (double*)malloc(sizeof(double) * nsh * nsh);
-    or_shellblock_norm(b, D, Dn);
+    or_shellblock_norm(b, Dscreen ? Dscreen : D, Dn);
     long count = 0, pcount = 0;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -623,6 +627,12 @@ long or_jk_direct_range(const or_basis* b, const double* D, const double* Q, dou
     free(Jt); free(Kt); free(Dn); free(pl);
     if (counts) { counts[0] += count; counts[1] += pcount; }
     return count;
+}
+
+long or_jk_direct_range(const or_basis* b, const double* D, const double* Q, double precision,
+                        double multiplier, double* J, double* K, int nthreads, int s1lo, int s1hi,
+                        long* counts) {
+    return or_jk_direct_range_s(b, D, NULL, Q, precision, multiplier, J, K, nthreads, s1lo, s1hi, counts);
 }
 
 long or_jk_direct(const or_basis* b, const double* D, const double* Q, double precision,

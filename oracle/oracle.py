@@ -55,6 +55,9 @@ def lib():
         L.or_schwarz.argtypes = [C.c_void_p, _dp]
         L.or_shellblock_norm.argtypes = [C.c_void_p, _dp, _dp]
         L.or_jk_direct.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.c_int]
+        L.or_jk_direct_range_s.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                           C.c_int, C.c_int, C.POINTER(C.c_long)]
+        L.or_jk_direct_range_s.restype = C.c_long
         L.or_jk_direct.restype = C.c_long
         L.or_jk_direct_range.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.c_int,
                                          C.c_int, C.c_int, C.POINTER(C.c_long)]
@@ -143,6 +146,24 @@ class Basis:
         n = lib().or_jk_direct(self.h, _p(D), Qp, precision, multiplier, _p(J), _p(K), nthreads)
         self.last_quartets = n
         return J, K
+
+    def jk_direct_uhf(self, Da, Db, Q=None, precision=1e-12, multiplier=1.0, nthreads=0):
+        """UnrestrictedFock::formJKdirect (src/fockbuild.cpp:267-364): (Ja, Ka,
+        Jb, Kb), every quartet screened with the block norms of Da + Db."""
+        Da = np.ascontiguousarray(Da, float)
+        Db = np.ascontiguousarray(Db, float)
+        Dt = np.ascontiguousarray(Da + Db)
+        Qp = _p(np.ascontiguousarray(Q, float)) if Q is not None else None
+        if Q is not None:
+            Qa = np.ascontiguousarray(Q, float)
+            Qp = _p(Qa)
+        out = []
+        for D in (Da, Db):
+            J, K = np.zeros_like(D), np.zeros_like(D)
+            lib().or_jk_direct_range_s(self.h, _p(D), _p(Dt), Qp, precision, multiplier, _p(J), _p(K), nthreads,
+                                       0, self.nsh, None)
+            out += [J, K]
+        return tuple(out)
 
     def jk_direct_range(self, D, Q, s1lo, s1hi, precision=1e-12, nthreads=0):
         """Bounded sample of one build: quartets with s1 in [s1lo, s1hi).
@@ -530,6 +551,91 @@ class RHF:
 # ----------------------------------------------------------------------------
 # DF-JK restated (src/fockbuild.cpp:585-660, src/eris.cpp:89-156,304-356)
 # ----------------------------------------------------------------------------
+
+class UHF:
+    """Unrestricted SCF restatement: SCF::uhf_internal (src/scf.cpp:193-272)
+    with UnrestrictedFock (src/fock.cpp:278-361, src/fockbuild.cpp:412-445).
+    `jk(Da, Db)` returns (Ja, Ka, Jb, Kb) (F_s = H + Ja + Jb - K_s).  DIIS runs
+    from the first iteration on the concatenated alpha and beta error
+    vectors; the energy is taken with the DIIS-averaged Fock matrices."""
+
+    def __init__(self, S, H, enuc, nalpha, nbeta, jk, maxdiis=8, diis=True, maxiter=40,
+                 enconverge=1e-7, densconverge=1e-4, guess="hcore", soad_G=None):
+        if guess not in ("hcore", "soad") or (guess == "soad" and soad_G is None):
+            raise ValueError("guess: 'hcore', or 'soad' with soad_G (soad_two_electron)")
+        self.S, self.H, self.enuc = S, H, enuc
+        self.na, self.nb = nalpha, nbeta
+        self.jk = jk
+        self.diis_on = diis
+        self.MAX = maxdiis
+        self.maxiter = maxiter
+        self.econv, self.dconv = enconverge, densconverge
+        self.guess, self.soad_G = guess, soad_G
+        lam, U = _eigh_sorted(S)
+        self.X = U @ np.diag(1.0 / np.sqrt(lam)) @ U.T
+        self.iter = 0
+        self.afocks, self.bfocks = [], []
+        self.log = []
+
+    def _calcE(self, D, F):
+        one = 0.5 * np.trace(D @ self.H)
+        two = 0.5 * np.trace(D @ F) - one
+        return 2.0 * one + two
+
+    def _calcErr(self, F, D):
+        e = self.X.T @ (F @ (D @ self.S) - self.S @ (D @ F)) @ self.X
+        return e.ravel(order="F")
+
+    def _diagonalise(self):
+        ea, Ua = _eigh_sorted(self.Fa_mo)
+        eb, Ub = _eigh_sorted(self.Fb_mo)
+        self.Ca, self.Cb = self.X @ Ua, self.X @ Ub
+        self.eps_a, self.eps_b = ea, eb
+
+    def run(self):
+        diis = DIIS(self.MAX, self.diis_on)
+        F0 = self.H if self.guess == "hcore" else self.H + self.soad_G
+        self.Fa_mo = self.Fb_mo = self.X.T @ (F0 @ self.X)
+        self._diagonalise()
+        n = self.S.shape[0]
+        old_a, old_b = np.zeros((n, n)), np.zeros((n, n))
+        self.energy = 0.0
+        it, converged = 1, False
+        while not converged and it < self.maxiter:
+            Ca, Cb = self.Ca[:, :self.na], self.Cb[:, :self.nb]
+            self.Da, self.Db = Ca @ Ca.T, Cb @ Cb.T
+            Ja, Ka, Jb, Kb = self.jk(self.Da, self.Db)
+            self.Fa = self.H + Ja + Jb - Ka
+            self.Fb = self.H + Ja + Jb - Kb
+            if self.diis_on:
+                if self.iter >= self.MAX:
+                    self.afocks.pop(0)
+                    self.bfocks.pop(0)
+                self.afocks.append(self.Fa.copy())
+                self.bfocks.append(self.Fb.copy())
+                self.iter += 1
+            w = diis.compute([self._calcErr(self.Fa, self.Da), self._calcErr(self.Fb, self.Db)])
+            if self.diis_on and self.iter > 2:
+                off = len(self.afocks) - len(w)
+                self.Fa = sum(w[i - off] * self.afocks[i] for i in range(off, len(self.afocks)))
+                self.Fb = sum(w[i - off] * self.bfocks[i] for i in range(off, len(self.bfocks)))
+            ea, eb = self._calcE(self.Da, self.Fa), self._calcE(self.Db, self.Fb)
+            self.Fa_mo = self.X.T @ (self.Fa @ self.X)
+            self.Fb_mo = self.X.T @ (self.Fb @ self.X)
+            self._diagonalise()
+            last = self.energy
+            self.energy = ea + eb + self.enuc
+            delta = abs(self.energy - last)
+            dist = np.linalg.norm(self.Da + self.Db - old_a - old_b)
+            old_a, old_b = self.Da, self.Db
+            self.log.append((it, self.energy, delta, dist))
+            converged = delta < self.econv and dist < self.dconv
+            it += 1
+        self._diagonalise()
+        self.converged = converged
+        self.iterations = it
+        return self.energy
+
 
 class DFJK:
     def __init__(self, obs: Basis, aux: Basis):

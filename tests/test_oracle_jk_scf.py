@@ -252,3 +252,37 @@ def test_scf_soad_guess_water(oracle):
     assert abs(E1 - E0) < 1e-9 and abs(E1 - (-75.990178761701)) < 2e-9
     assert s1.log[0][1] < -75.0 and s0.log[0][1] > -70.0
     assert len(s1.log) < len(s0.log)
+
+
+def _monomers():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "h2o20_monomers.json")) as fh:
+        return json.load(fh)["monomers"]
+
+
+def _monomer_mol(m):
+    return M.Molecule(list(m["Z"]), np.array(m["xyz_angstrom"]) * M.TOBOHR, 0, 1, "monomer")
+
+
+@pytest.mark.parametrize("k", [0, 7, 13])
+def test_df_scf_matches_recorded_monomer_energies(oracle, k):
+    """The reference's recorded density-fitted monomer energies of
+    test/h2oclusters/h2o_20.ref (printed to 1e-6; fixture made by
+    tests/golden/make_h2o20_monomers.py): cc-pVDZ with cc-pVDZ-JKFIT, both
+    spherical.  The exact-J/K energy differs by ~2e-5, so this pins the DF
+    path itself."""
+    m = _monomers()[k]
+    mol = _monomer_mol(m)
+    ob = oracle.Basis(mol.shells("cc-pvdz"), pure=True)
+    ab = oracle.Basis(mol.shells("cc-pvdz-jkfit"), pure=True)
+    S, T, V = ob.one_body(mol.Z, mol.xyz)
+    df = oracle.DFJK(ob, ab)
+    scf = oracle.RHF(S, T + V, mol.enuc(), mol.nel, df, df=True, enconverge=1e-10, densconverge=1e-8)
+    E = scf.run()
+    assert scf.converged
+    assert abs(E - m["energy"]) < 1e-6, (E, m["energy"])
+    Q = ob.schwarz()
+    exact = oracle.RHF(S, T + V, mol.enuc(), mol.nel, lambda P, mm: ob.jk_direct(P, Q, 1e-12, mm),
+                       enconverge=1e-10, densconverge=1e-8).run()
+    assert abs(exact - m["energy"]) > 5e-6
