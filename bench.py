@@ -86,6 +86,48 @@ def cpu_baseline(mol, D, total_primq, budget_s=15.0):
                       "extrapolated by primitive-quartet count" % (hi, nsh, done_q, done_pq, t_used)}
 
 
+def df_bench(eng, mol, Cocc, n, steps, torch, dev):
+    """Density-fitted J/K (compute_2body_fock_df, src/fockbuild.cpp:585-660)
+    with cc-pVDZ-JKFIT on the converged occupied orbitals: fitted-tensor setup
+    time, and per call the FP32 (MFMA) and FP64 GEMM paths.  FLOPs per call
+    4 naux n^2 nocc (X = C^T B and K = X^T X) + 4 naux n(n+1)/2 (J)."""
+    aux = mol.shells("cc-pvdz-jkfit")
+    eng.set_basis(aux, pure=True, which=1)
+    naux = sum(2 * s.l + 1 for s in aux)
+    nocc = Cocc.shape[1]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.df_setup()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t
+    dC = torch.from_numpy(np.ascontiguousarray(Cocc.T)).to(dev)   # column-major n x nocc
+    dJ = torch.empty((n, n), dtype=torch.float64, device=dev)
+    dK = torch.empty_like(dJ)
+    flops = 4.0 * naux * n * n * nocc + 4.0 * naux * n * (n + 1) / 2
+    res = {"naux": naux, "setup_s": round(t_setup, 3), "flops_per_call": flops}
+    for name, fp32, peak in (("fp32", True, 157.3), ("fp64", False, 78.6)):
+        eng.jk_df_device(dC.data_ptr(), nocc, dJ.data_ptr(), dK.data_ptr(), fp32)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            eng.jk_df_device(dC.data_ptr(), nocc, dJ.data_ptr(), dK.data_ptr(), fp32)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t) / steps
+        eng.set_profiling(True)
+        eng.jk_df_device(dC.data_ptr(), nocc, dJ.data_ptr(), dK.data_ptr(), fp32)
+        eng.set_profiling(False)
+        tx, tk, tj, tt = eng.df_timing()
+        res[name] = {"ms_per_call": round(ms, 3), "gemm_X_ms": round(tx, 3), "gemm_K_ms": round(tk, 3),
+                     "J_ms": round(tj, 3), "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                     "matrix_peak_tflops": peak}
+        res[name]["J"], res[name]["K"] = dJ.cpu().numpy(), dK.cpu().numpy()
+    dj = float(np.max(np.abs(res["fp32"].pop("J") - res["fp64"].pop("J"))))
+    dk = float(np.max(np.abs(res["fp32"].pop("K") - res["fp64"].pop("K"))))
+    res["fp32_vs_fp64"] = {"speedup": round(res["fp64"]["ms_per_call"] / res["fp32"]["ms_per_call"], 3),
+                           "max_abs_dJ": dj, "max_abs_dK": dk}
+    return res
+
+
 def traffic_for(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary."""
     try:
@@ -133,7 +175,8 @@ def main():
     eng = Engine(0)
     out_scf = None
     if not a.no_scf:
-        kw = dict(enconverge=1e-8, densconverge=1e-5, maxiter=60)
+        # SOAD start: the reference's default guess (src/ProgramController.cpp:307)
+        kw = dict(enconverge=1e-8, densconverge=1e-5, maxiter=60, guess="soad")
         r64 = eng.rhf(mol, "cc-pvdz", pure=True, gradient=True, **kw)
         rmx = eng.rhf(mol, "cc-pvdz", pure=True, gradient=True, mixed=1, **kw)
         D = r64["D"]
@@ -258,6 +301,7 @@ def main():
     }
     if out_scf is not None:
         out["scf"] = out_scf
+        out["df"] = df_bench(eng, mol, r64["C"][:, :mol.nel // 2], n, a.steps, torch, dev)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol, D, tot_primq, a.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -618,8 +618,10 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     if (ctx->h_lists) (void)hipHostFree(ctx->h_lists);
     ctx->h_lists = nullptr;
     ctx->d_counters.release();
-    for (auto* x : {&ctx->d_B64, &ctx->d_dfV, &ctx->d_dfC, &ctx->d_dfX, &ctx->d_dfG, &ctx->d_dfJ, &ctx->d_dfK}) x->release();
-    for (auto* x : {&ctx->d_B32, &ctx->d_dfC32, &ctx->d_dfX32, &ctx->d_dfG32, &ctx->d_dfJ32, &ctx->d_dfK32}) x->release();
+    for (auto* x : {&ctx->d_B64, &ctx->d_dfV, &ctx->d_dfC, &ctx->d_dfX, &ctx->d_dfG, &ctx->d_dfJ, &ctx->d_dfK, &ctx->d_dfKp})
+        x->release();
+    for (auto* x : {&ctx->d_B32, &ctx->d_dfC32, &ctx->d_dfX32, &ctx->d_dfG32, &ctx->d_dfJ32, &ctx->d_dfK32, &ctx->d_dfKp32})
+        x->release();
     for (auto* x : {&ctx->d_aux_off, &ctx->d_aux_np, &ctx->d_aux_lists}) x->release();
     ctx->d_aux_prims.release();
     la_release(ctx);

@@ -108,8 +108,8 @@ struct m2_ctx {
     std::vector<m2::PrimPair> h_aux_prims;
     m2::DevBuf<m2::PrimPair> d_aux_prims;
     m2::DevBuf<int> d_aux_off, d_aux_np, d_aux_lists;
-    m2::DevBuf<double> d_B64, d_dfV, d_dfC, d_dfX, d_dfG, d_dfJ, d_dfK;
-    m2::DevBuf<float> d_B32, d_dfC32, d_dfX32, d_dfG32, d_dfJ32, d_dfK32;
+    m2::DevBuf<double> d_B64, d_dfV, d_dfC, d_dfX, d_dfG, d_dfJ, d_dfK, d_dfKp;
+    m2::DevBuf<float> d_B32, d_dfC32, d_dfX32, d_dfG32, d_dfJ32, d_dfK32, d_dfKp32;   // Kp: split-K partial slices
     bool df_ready = false;
     double df_last_ms[4] = {0, 0, 0, 0};   // X = C^T B, K = X^T X, J, total (profiling)
     // dense linear algebra (rocBLAS / rocSOLVER)

@@ -67,6 +67,19 @@ __global__ __launch_bounds__(256) void k_df_gamma(const R* __restrict__ X, const
     if (threadIdx.x == 0) g[K] = R(red[0]);
 }
 
+// out[i] = sum_s part[s * m + i]
+template <typename R>
+__global__ void k_sum_slices(const R* __restrict__ part, R* __restrict__ out, size_t m, int ns) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    R v = R(0);
+    for (int s = 0; s < ns; ++s) v += part[(size_t)s * m + i];
+    out[i] = v;
+}
+
+// chunks of the auxiliary index of the exchange GEMM
+constexpr int DF_KSPLIT = 48;
+
 template <typename R>
 __global__ void k_to_double(const R* __restrict__ a, double* __restrict__ b, size_t n) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
@@ -234,6 +247,15 @@ struct Blas<float> {
                                float* C, int ldc) {
         return rocblas_sgemm(h, a, b, m, n, k, al, A, lda, B, ldb, be, C, ldc);
     }
+    static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation a, rocblas_operation b, int m, int n, int k,
+                                  const float* al, const float* A, int lda, rocblas_stride sA, const float* B, int ldb,
+                                  rocblas_stride sB, const float* be, float* C, int ldc, rocblas_stride sC, int batch) {
+        return rocblas_sgemm_strided_batched(h, a, b, m, n, k, al, A, lda, sA, B, ldb, sB, be, C, ldc, sC, batch);
+    }
+    static float* partial(m2_ctx* ctx, size_t cnt, hipError_t& e) {
+        e = ctx->d_dfKp32.reserve(cnt);
+        return ctx->d_dfKp32.p;
+    }
     static rocblas_status gemv_sb(rocblas_handle h, int m, int n, const float* al, const float* A, int lda,
                                   rocblas_stride sA, const float* x, rocblas_stride sx, const float* be, float* y,
                                   rocblas_stride sy, int batch) {
@@ -247,6 +269,16 @@ struct Blas<double> {
                                const double* al, const double* A, int lda, const double* B, int ldb, const double* be,
                                double* C, int ldc) {
         return rocblas_dgemm(h, a, b, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+    }
+    static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation a, rocblas_operation b, int m, int n, int k,
+                                  const double* al, const double* A, int lda, rocblas_stride sA, const double* B,
+                                  int ldb, rocblas_stride sB, const double* be, double* C, int ldc, rocblas_stride sC,
+                                  int batch) {
+        return rocblas_dgemm_strided_batched(h, a, b, m, n, k, al, A, lda, sA, B, ldb, sB, be, C, ldc, sC, batch);
+    }
+    static double* partial(m2_ctx* ctx, size_t cnt, hipError_t& e) {
+        e = ctx->d_dfKp.reserve(cnt);
+        return ctx->d_dfKp.p;
     }
     static rocblas_status gemv_sb(rocblas_handle h, int m, int n, const double* al, const double* A, int lda,
                                   rocblas_stride sA, const double* x, rocblas_stride sx, const double* be, double* y,
@@ -275,13 +307,35 @@ m2_status jk_df_core(m2_ctx* ctx, const R* B, const R* C, int nocc, R* X, R* g, 
         return M2_ERR_HIP;
     }
     if (ctx->prof) HIPCHK(hipEventRecord(ev[1], ctx->stream));
-    // K = Z^T Z with Z = X as (naux*nocc) x n
+    // K = Z^T Z with Z = X as (naux*nocc) x n, row i + nocc*K.  The output is
+    // only n x n while the inner dimension is naux*nocc, so the product is
+    // split over chunks of the auxiliary index (contiguous rows of Z): one
+    // strided-batched GEMM into partial K slices, then their sum.
     const int ld = naux * nocc;
-    if (Blas<R>::gemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, ld, &one, X, ld, X, ld, &zero, K, n) !=
-        rocblas_status_success) {
-        ctx->err = "m2_jk_df: GEMM K = X^T X failed";
+    const int nsplit = std::max(1, std::min(DF_KSPLIT, naux / 16));
+    const int ks = (naux + nsplit - 1) / nsplit;       // aux functions per chunk
+    const int nfull = naux / ks, krem = naux - nfull * ks;
+    const int nslice = nfull + (krem > 0 ? 1 : 0);
+    hipError_t pe;
+    R* Kp = Blas<R>::partial(ctx, (size_t)nslice * n * n, pe);
+    HIPCHK(pe);
+    if (Blas<R>::gemm_sb(h, rocblas_operation_transpose, rocblas_operation_none, n, n, ks * nocc, &one, X, ld,
+                         (rocblas_stride)ks * nocc, X, ld, (rocblas_stride)ks * nocc, &zero, Kp, n,
+                         (rocblas_stride)n * n, nfull) != rocblas_status_success) {
+        ctx->err = "m2_jk_df: split GEMM K = X^T X failed";
         return M2_ERR_HIP;
     }
+    if (krem > 0) {
+        const R* Xr = X + (size_t)nfull * ks * nocc;
+        if (Blas<R>::gemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, krem * nocc, &one, Xr, ld, Xr,
+                          ld, &zero, Kp + (size_t)nfull * n * n, n) != rocblas_status_success) {
+            ctx->err = "m2_jk_df: GEMM K = X^T X (last chunk) failed";
+            return M2_ERR_HIP;
+        }
+    }
+    hipLaunchKernelGGL(k_sum_slices<R>, dim3(((size_t)n * n + 255) / 256), dim3(256), 0, ctx->stream, Kp, K,
+                       (size_t)n * n, nslice);
+    HIPCHK(hipGetLastError());
     if (ctx->prof) HIPCHK(hipEventRecord(ev[2], ctx->stream));
     hipLaunchKernelGGL(k_df_gamma<R>, dim3(naux), dim3(256), 0, ctx->stream, X, C, g, n, naux, nocc);
     HIPCHK(hipGetLastError());
