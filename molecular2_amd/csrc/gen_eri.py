@@ -1190,9 +1190,11 @@ def write_deriv_programs(outdir):
 
 
 DERIV_REGISTER = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0), (1, 1, 1, 0), (2, 0, 1, 0),
-                  (2, 1, 0, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0)]
+                  (2, 1, 0, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0), (2, 2, 0, 0),
+                  (2, 2, 1, 0)]
 # classes whose three derivative centres run as three lighter passes
-DERIV_SPLIT = {(1, 1, 1, 0), (2, 0, 1, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0)}
+DERIV_SPLIT = {(1, 1, 1, 0), (2, 0, 1, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0),
+               (2, 2, 0, 0), (2, 2, 1, 0)}
 
 
 def gen_deriv_class(la, lb, lc, ld, mask=7):
@@ -1373,34 +1375,46 @@ def gen_deriv_class(la, lb, lc, ld, mask=7):
 
 
 def write_deriv_register(outdir):
-    text = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.", "#pragma once"]
+    """One translation unit per register derivative class (grad_reg_ABCD.hip,
+    compiled in parallel), the host-side class table (deriv_classes.inc) and
+    the dispatcher (deriv_dispatch.inc)."""
+    def put(path, text):
+        if not os.path.exists(path) or open(path).read() != text:
+            with open(path, "w") as fh:
+                fh.write(text)
+
+    head = "// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit."
+    names = []
     for c in DERIV_REGISTER:
-        for m in ((1, 2, 4) if c in DERIV_SPLIT else (7,)):
-            text.append(gen_deriv_class(*c, mask=m))
-    # launch dispatch of the register classes (LAUNCH(la, lb, lc, ld, mask))
-    text.append("#define DERIV_REGISTER_DISPATCH(la, lb, lc, ld, LAUNCH) \\")
-    for i, (la, lb, lc, ld) in enumerate(DERIV_REGISTER):
-        calls = " ".join("LAUNCH(%d, %d, %d, %d, %d)" % (la, lb, lc, ld, m)
-                         for m in ((1, 2, 4) if (la, lb, lc, ld) in DERIV_SPLIT else (7,)))
-        text.append("  %sif (la == %d && lb == %d && lc == %d && ld == %d) { %s } \\" %
-                    ("" if i == 0 else "else ", la, lb, lc, ld, calls))
-    text.append("  else")
-    text = "\n".join(text) + "\n"
-    path = os.path.join(outdir, "deriv_register.inc")
-    if not os.path.exists(path) or open(path).read() != text:
-        with open(path, "w") as fh:
-            fh.write(text)
-    # which classes are register code (host side)
-    cls = ["// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.", "#pragma once",
+        tag = "%d%d%d%d" % c
+        masks = (1, 2, 4) if c in DERIV_SPLIT else (7,)
+        body = [head, '#include "../grad_reg.hpp"', "namespace m2 {"]
+        for m in masks:
+            body.append(gen_deriv_class(*c, mask=m))
+        body.append("void launch_grad_reg_%s(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom,"
+                    " hipStream_t st) {" % tag)
+        for m in masks:
+            body.append("  launch_reg<%d, %d, %d, %d, %d>(a, tab, t, nt, natom, st);" % (c + (m,)))
+        body += ["}", "}  // namespace m2", ""]
+        put(os.path.join(outdir, "grad_reg_%s.hip" % tag), "\n".join(body))
+        names.append((c, tag))
+    disp = [head, "#pragma once"]
+    for c, tag in names:
+        disp.append("void launch_grad_reg_%s(const GradArgs&, const BoysTables&, const Task*, int, int, hipStream_t);"
+                    % tag)
+    disp.append("// true when (la, lb, lc, ld) has register code (and was launched)")
+    disp.append("inline bool launch_grad_reg(int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,"
+                " const Task* t, int nt, int natom, hipStream_t st) {")
+    for c, tag in names:
+        disp.append("  if (la == %d && lb == %d && lc == %d && ld == %d) { launch_grad_reg_%s(a, tab, t, nt, natom, st);"
+                    " return true; }" % (c + (tag,)))
+    disp += ["  return false;", "}", ""]
+    put(os.path.join(outdir, "deriv_dispatch.inc"), "\n".join(disp))
+    cls = [head, "#pragma once",
            "constexpr int DERIV_REG_N = %d;" % len(DERIV_REGISTER),
            "constexpr int DERIV_REG_L[%d][4] = {%s};" % (len(DERIV_REGISTER), ", ".join(
-               "{%d, %d, %d, %d}" % c for c in DERIV_REGISTER))]
-    ctext = "\n".join(cls) + "\n"
-    cpath = os.path.join(outdir, "deriv_classes.inc")
-    if not os.path.exists(cpath) or open(cpath).read() != ctext:
-        with open(cpath, "w") as fh:
-            fh.write(ctext)
-    return path
+               "{%d, %d, %d, %d}" % c for c in DERIV_REGISTER)), ""]
+    put(os.path.join(outdir, "deriv_classes.inc"), "\n".join(cls))
 
 
 if __name__ == "__main__":
