@@ -2,6 +2,7 @@
 // Schwarz bounds, one-electron integrals and the direct J/K driver.
 #include "ctx.hpp"
 #include "jk_lists.hpp"
+#include "one_body.hpp"
 #include "generated/sph_tables.inc"
 #include "generated/jk_dispatch.inc"
 #include "generated/deriv_classes.inc"
@@ -16,7 +17,7 @@
 #include <vector>
 
 namespace m2 {
-void launch_one_body(const struct OneBodyArgs&, hipStream_t);
+void launch_one_body(const OneBodyArgs&, hipStream_t);
 void launch_blocknorm(const double* A, int n, int nsh, const int* sh_bf, const int* sh_nbf, double* out, hipStream_t);
 void launch_symmetrize(double* J, double* K, double* Jo, double* Ko, int n, hipStream_t);
 void launch_zero(double* p, size_t n, hipStream_t);
@@ -527,21 +528,7 @@ std::vector<ComboRangeG> grad_tasks(m2_ctx* ctx, double precision, double Dmax) 
     for (const auto& x : r) out.push_back(ComboRangeG{x.X, x.Y, x.t0, x.tmid, x.t1});
     return out;
 }
-struct OneBodyArgs {
-    const double* sh_O;
-    const int* sh_l;
-    const int* sh_bf;
-    const int* sh_np;
-    const int* sh_poff;
-    const double* exps;
-    const double* coefs;
-    int nsh, n, pure, natom;
-    const double* Z;
-    const double* xyz;
-    const double* sph;
-    double *S, *T, *V;
-    BoysTables tab;
-};
+
 }  // namespace m2
 
 extern "C" {
@@ -771,6 +758,7 @@ m2_status m2_one_body(m2_ctx* ctx, int natom, const double* Z, const double* xyz
         a.S = d_S.p;
         a.T = d_T.p;
         a.V = d_V.p;
+        a.want_v = V ? 1 : 0;
         launch_one_body(a, ctx->stream);
         e = hipGetLastError();
         if (!e) e = hipStreamSynchronize(ctx->stream);

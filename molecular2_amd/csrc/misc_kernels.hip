@@ -1,25 +1,12 @@
 // Small device kernels: one-electron integrals, shell-block norms,
 // symmetrisation of J/K, fills.
 #include "m2_common.hpp"
+#include "one_body.hpp"
 #include "boys.hpp"
 
 namespace m2 {
 
-struct OneBodyArgs {
-    const double* sh_O;
-    const int* sh_l;
-    const int* sh_bf;
-    const int* sh_np;
-    const int* sh_poff;
-    const double* exps;
-    const double* coefs;
-    int nsh, n, pure, natom;
-    const double* Z;
-    const double* xyz;
-    const double* sph;
-    double *S, *T, *V;
-    BoysTables tab;
-};
+
 
 constexpr int OB_LMAX = 2;                 // orbital shells
 constexpr int OB_NC = (OB_LMAX + 1) * (OB_LMAX + 2) / 2;
@@ -65,8 +52,12 @@ __device__ void boys_rt(double T, int M, double* F, const BoysTables& tab) {
 // One thread per shell pair (s1 >= s2): S, T (1-D Obara-Saika overlaps) and
 // V (two-centre Obara-Saika nuclear-attraction recurrence), Cartesian, then
 // the real-solid-harmonic transform for l >= 2, written to both blocks.
+// One wavefront per shell pair s1 >= s2: lane 0 takes the overlap and kinetic
+// integrals, the lanes share the nuclei of the attraction integrals (c = lane,
+// lane + 64, ...) and the wavefront sums them.
 __global__ __launch_bounds__(64) void k_one_body(OneBodyArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
     const int npairs = a.nsh * (a.nsh + 1) / 2;
     if (t >= npairs) return;
     int s1 = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -94,6 +85,7 @@ __global__ __launch_bounds__(64) void k_one_body(OneBodyArgs a) {
                 PA[k] = P[k] - A[k];
                 PB[k] = P[k] - B[k];
             }
+            if (lane == 0) {
             // 1-D overlaps, i <= la, j <= lb + 2
             double O1[3][OB_LMAX + 1][OB_LMAX + 3];
             for (int k = 0; k < 3; ++k) {
@@ -131,9 +123,10 @@ __global__ __launch_bounds__(64) void k_one_body(OneBodyArgs a) {
                     sT[ia * nb + ib] += pref * (tk[0] * sy * sz + sx * tk[1] * sz + sx * sy * tk[2]);
                 }
             }
+            }
             // nuclear attraction: [a|b]^(m), |a| <= la, |b| <= lb
             const double vpref = 2.0 * M_PI / p * Kab * cc;
-            for (int c = 0; c < a.natom; ++c) {
+            for (int c = lane; c < (a.want_v ? a.natom : 0); c += 64) {
                 const double* C = a.xyz + 3 * c;
                 double PC[3], PC2 = 0.0;
                 for (int k = 0; k < 3; ++k) {
@@ -200,6 +193,13 @@ __global__ __launch_bounds__(64) void k_one_body(OneBodyArgs a) {
                         sV[ia * nb + ib] -= a.Z[c] * W[ob_cum(la) + ia][ob_cum(lb) + ib][0];
             }
         }
+    if (a.want_v)
+        for (int i = 0; i < na * nb; ++i) {
+            double v = sV[i];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            sV[i] = v;
+        }
+    if (lane != 0) return;
     // transform and store
     const int fa = a.pure && la >= 2 ? 2 * la + 1 : na;
     const int fb = a.pure && lb >= 2 ? 2 * lb + 1 : nb;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(64) void k_one_body(OneBodyArgs a) {
 
 void launch_one_body(const OneBodyArgs& a, hipStream_t st) {
     const int npairs = a.nsh * (a.nsh + 1) / 2;
-    hipLaunchKernelGGL(k_one_body, dim3((npairs + 63) / 64), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_one_body, dim3(npairs), dim3(64), 0, st, a);
 }
 
 // shell-block inf-norms: one thread per (s1, s2)
