@@ -1,0 +1,53 @@
+"""E_RHF(R) as a differentiable torch function.
+
+`rhf_energy(R, mol)` calls the libtorch operator torch.ops.m2.rhf_energy
+(csrc/autograd/rhf_energy.cpp, built into lib/libm2_torch.so): its forward
+runs SCF::rhf (src/scf.cpp:90-185) on the device at the nuclear positions R
+(natom x 3, bohr) and its backward returns dE/dR as Fock::compute_forces
+(src/rhfgrad.cpp:15-89), two-electron part from the derivative-integral
+adjoint kernel.  Shell centres move with their atoms.  There is no CPU
+fallback: without the built library this module raises.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libm2_torch.so")
+_loaded = False
+
+
+def load() -> None:
+    global _loaded
+    if not _loaded:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libm2_torch.so is not built (%s); run `python -m molecular2_amd.build`" % LIBPATH)
+        torch.ops.load_library(LIBPATH)
+        _loaded = True
+
+
+def basis_tensors(mol, basis: str = "cc-pvdz"):
+    """(l, nprim, exps, coefs, atoms) of mol's shells as CPU tensors."""
+    sh = mol.shells(basis)
+    l = torch.tensor([s.l for s in sh], dtype=torch.int32)
+    nprim = torch.tensor([len(s.exps) for s in sh], dtype=torch.int32)
+    exps = torch.tensor(np.concatenate([s.exps for s in sh]), dtype=torch.float64)
+    coefs = torch.tensor(np.concatenate([s.coefs for s in sh]), dtype=torch.float64)
+    atoms = torch.tensor([s.atom for s in sh], dtype=torch.int32)
+    return l, nprim, exps, coefs, atoms
+
+
+def rhf_energy(R: torch.Tensor, mol, basis: str = "cc-pvdz", pure: bool = True, precision: float = 1e-12,
+               enconverge: float = 1e-10, densconverge: float = 1e-8, maxiter: int = 80,
+               mixed: bool = False) -> torch.Tensor:
+    """E_RHF at positions R (natom x 3, bohr; FP64) for the atoms, charge and
+    basis of `mol` (a molecules.Molecule).  Differentiable in R (first order)."""
+    load()
+    Z = torch.tensor(np.asarray(mol.Z, float), dtype=torch.float64)
+    l, nprim, exps, coefs, atoms = basis_tensors(mol, basis)
+    return torch.ops.m2.rhf_energy(R, Z, l, nprim, exps, coefs, atoms, int(pure), int(mol.charge),
+                                   int(mol.multiplicity), float(precision), float(enconverge),
+                                   float(densconverge), int(maxiter), int(mixed))

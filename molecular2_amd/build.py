@@ -22,6 +22,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIBM2 = os.path.join(LIBDIR, "libm2.so")
+LIBM2_TORCH = os.path.join(LIBDIR, "libm2_torch.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -96,7 +97,33 @@ def build(force: bool = False, jobs: int = 0, verbose: bool = True) -> str:
         os.replace(LIBM2 + ".tmp", LIBM2)
     if verbose:
         print("libm2.so ready (%.1fs)" % (time.time() - t0), flush=True)
+    build_torch_op(force=force, verbose=verbose)
     return LIBM2
+
+
+def build_torch_op(force: bool = False, verbose: bool = True) -> str:
+    """libm2_torch.so: the libtorch autograd operator torch.ops.m2.rhf_energy
+    (csrc/autograd/rhf_energy.cpp), host C++ over libm2.so's C ABI."""
+    import torch
+    import torch.utils.cpp_extension as E
+    src = os.path.join(CSRC, "autograd", "rhf_energy.cpp")
+    if not force and os.path.exists(LIBM2_TORCH) and \
+            os.path.getmtime(LIBM2_TORCH) >= max(os.path.getmtime(src), os.path.getmtime(LIBM2), _headers()):
+        return LIBM2_TORCH
+    t = time.time()
+    tlib = E.library_paths()[0]
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared",
+           "-D_GLIBCXX_USE_CXX11_ABI=%d" % int(torch._C._GLIBCXX_USE_CXX11_ABI), "-I" + INCLUDE] + \
+        ["-I" + p for p in E.include_paths()] + \
+        [src, "-o", LIBM2_TORCH + ".tmp", "-L" + tlib, "-ltorch", "-ltorch_cpu", "-lc10", "-L" + LIBDIR, "-lm2",
+         "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + tlib]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for %s:\n%s" % (src, r.stderr[-6000:]))
+    os.replace(LIBM2_TORCH + ".tmp", LIBM2_TORCH)
+    if verbose:
+        print("  compiled %-28s %6.1fs" % ("rhf_energy.cpp", time.time() - t), flush=True)
+    return LIBM2_TORCH
 
 
 if __name__ == "__main__":

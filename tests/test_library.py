@@ -33,6 +33,19 @@ def test_library_exports_every_declared_symbol():
     assert hasattr(lib, "m2_rhf")
 
 
+def test_autograd_operator_registers():
+    """libm2_torch.so loads without a GPU and registers m2::rhf_energy with
+    the schema the Python wrapper calls."""
+    from molecular2_amd import autograd
+    if not os.path.exists(autograd.LIBPATH):
+        pytest.skip("libm2_torch.so not built")
+    import torch
+    autograd.load()
+    schema = str(torch.ops.m2.rhf_energy.default._schema)
+    assert schema.startswith("m2::rhf_energy(Tensor R, Tensor Z, Tensor l, Tensor nprim")
+    assert schema.endswith("-> Tensor")
+
+
 def test_engine_fails_loudly_without_library(monkeypatch):
     from molecular2_amd import engine
     monkeypatch.setattr(engine, "LIBPATH", "/nonexistent/libm2.so")
