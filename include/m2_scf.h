@@ -59,7 +59,7 @@ int m2_rhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
            const int* nprim_min, const double* exps_min, const double* coefs_min, const double* centers_min,
            const int* atoms_min, const m2_scf_options_c* o, m2_scf_result_c* res, double* D, double* C, double* eps,
            double* log5, double* grad);
-const char* m2_rhf_last_error(void);
+const char* m2_rhf_last_error(void);   /* of the calling thread */
 
 /* SCF::uhf (src/scf.cpp:187-272) with UnrestrictedFock: arguments as for
  * m2_rhf (multiplicity = 2S + 1; n_alpha = (N - mult + 1) / 2 + mult - 1,
@@ -74,6 +74,23 @@ int m2_uhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
            const int* nprim_min, const double* exps_min, const double* coefs_min, const double* centers_min,
            const int* atoms_min, const m2_scf_options_c* o, m2_scf_result_c* res, double* Da, double* Db,
            double* eps_a, double* eps_b, double* log5);
+
+/* Batched SCF::rhf over nmol closed-shell molecules on one device.  No
+ * reference counterpart: the reference runs one molecule per process
+ * (src/ProgramController.cpp:112-120, one SCF::rhf per geometry).  Molecule m
+ * has natom[m] atoms (Z, xyz concatenated over molecules) and nsh[m] shells (l,
+ * nprim, atoms — indices local to the molecule — and centers concatenated over
+ * molecules; exps/coefs over their primitives); charge may be NULL (all 0).
+ * nworkers host threads (0: 8) each own an engine context with its own HIP
+ * stream, so the SCFs of different molecules overlap on the GPU; molecules
+ * are taken largest basis first.  res[m] is m2_rhf's result for molecule m
+ * (iterations = -1 if its run failed).  Core-Hamiltonian guess only
+ * (o->guess must be 0).  Returns 0 when every molecule ran (converged or
+ * not); otherwise m2_rhf_last_error() names the first failure. */
+int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int* charge, const double* Z,
+                 const double* xyz, int pure, const int* nsh, const int* l, const int* nprim, const double* exps,
+                 const double* coefs, const double* centers, const int* atoms, const m2_scf_options_c* o,
+                 m2_scf_result_c* res);
 
 /* The two-electron part of the SOAD guess Fock matrix (Fock::compute_soad_guess,
  * src/fock.cpp:174-225, with compute_2body_fock_general, src/fockbuild.cpp:

@@ -17,6 +17,7 @@
 // per nucleus, adds twice d/dA of D:(T+V) - 2 W:S (A = centre of s1; the
 // (s2, s1) wavefront supplies the other centre) and the Hellmann-Feynman term
 // of each nucleus (dC = -(dA + dB)).
+#include <mutex>
 #include "grad_reg.hpp"
 #include "generated/deriv_programs.inc"
 
@@ -477,6 +478,8 @@ static DerivTables g_dt[64];
 
 hipError_t deriv_upload(int device) {
     if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    static std::mutex mu;   // contexts may be created from several host threads
+    std::lock_guard<std::mutex> lk(mu);
     DerivTables& t = g_dt[device];
     if (t.ins) return hipSuccess;
     hipError_t e;

@@ -12,6 +12,7 @@
 // HRR / solid-harmonic levels run and the quartet is digested into J and K
 // with the same screening, degeneracy and factors as the register kernels.
 // Slots are reused by liveness, so (dd|dd) needs 3179 slots (25 KB FP64).
+#include <mutex>
 #include "m2_common.hpp"
 #include "boys.hpp"
 #include "generated/programs.inc"
@@ -261,6 +262,8 @@ static HeavyTables g_heavy[64];   // per device
 
 hipError_t heavy_upload(int device) {
     if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    static std::mutex mu;   // contexts may be created from several host threads
+    std::lock_guard<std::mutex> lk(mu);
     HeavyTables& t = g_heavy[device];
     if (t.ins) return hipSuccess;
     hipError_t e;

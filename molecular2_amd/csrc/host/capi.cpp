@@ -1,9 +1,15 @@
 // extern "C" driver of the host C++ SCF (molecular::SCF::rhf), for callers
 // that cannot link C++ (the Python tests and bench use it through ctypes).
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <exception>
+#include <mutex>
+#include <numeric>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "m2_scf.h"
 #include "scf_host.hpp"
@@ -200,6 +206,85 @@ int m2_uhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
         g_rhf_error = e.what();
         return 1;
     }
+}
+
+// Batched RHF: molecule m runs m2_rhf on the context of whichever worker
+// takes it; workers take molecules from one queue ordered by basis size,
+// largest first, so the long runs start early and the short ones fill in.
+int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int* charge, const double* Z,
+                 const double* xyz, int pure, const int* nsh, const int* l, const int* nprim, const double* exps,
+                 const double* coefs, const double* centers, const int* atoms, const m2_scf_options_c* o,
+                 m2_scf_result_c* res) {
+    g_rhf_error.clear();
+    if (nmol < 0 || (nmol > 0 && (!natom || !Z || !xyz || !nsh || !l || !nprim || !exps || !coefs || !centers ||
+                                  !atoms || !o || !res))) {
+        g_rhf_error = "m2_rhf_batch: null argument";
+        return 1;
+    }
+    if (o->guess != 0) {
+        g_rhf_error = "m2_rhf_batch: only the core-Hamiltonian guess (guess = 0) is available in batch mode";
+        return 1;
+    }
+    if (nmol == 0) return 0;
+    // offsets of every molecule into the concatenated arrays
+    std::vector<long> a_off(nmol + 1, 0), s_off(nmol + 1, 0), p_off(nmol + 1, 0);
+    std::vector<long> nbf(nmol, 0);
+    for (int m = 0; m < nmol; ++m) {
+        if (natom[m] <= 0 || nsh[m] <= 0) {
+            g_rhf_error = "m2_rhf_batch: molecule " + std::to_string(m) + " has no atoms or no shells";
+            return 1;
+        }
+        a_off[m + 1] = a_off[m] + natom[m];
+        s_off[m + 1] = s_off[m] + nsh[m];
+        long np = 0;
+        for (long s = s_off[m]; s < s_off[m + 1]; ++s) {
+            np += nprim[s];
+            nbf[m] += pure ? 2 * l[s] + 1 : (l[s] + 1) * (l[s] + 2) / 2;
+        }
+        p_off[m + 1] = p_off[m] + np;
+    }
+    std::vector<int> order(nmol);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nbf[a] > nbf[b]; });
+    if (nworkers <= 0) nworkers = 8;
+    nworkers = std::min(nworkers, nmol);
+    std::atomic<int> next{0};
+    std::mutex mu;
+    std::string first_error;
+    auto worker = [&]() {
+        m2_ctx* ctx = nullptr;
+        if (m2_ctx_create(&ctx, device) != 0 || !ctx) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first_error.empty())
+                first_error = std::string("m2_rhf_batch: context creation failed: ") +
+                              (ctx ? m2_last_error(ctx) : "no context");
+            if (ctx) m2_ctx_destroy(ctx);
+            return;
+        }
+        for (int k; (k = next.fetch_add(1)) < nmol;) {
+            const int m = order[k];
+            const long a0 = a_off[m], s0 = s_off[m], p0 = p_off[m];
+            const int st = m2_rhf(ctx, natom[m], Z + a0, xyz + 3 * a0, charge ? charge[m] : 0, 1, pure, nsh[m], l + s0,
+                                  nprim + s0, exps + p0, coefs + p0, centers + 3 * s0, atoms + s0, 0, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                  nullptr, o, &res[m], nullptr, nullptr, nullptr, nullptr, nullptr);
+            if (st != 0) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_error.empty()) first_error = "m2_rhf_batch: molecule " + std::to_string(m) + ": " + g_rhf_error;
+                res[m].converged = 0;
+                res[m].iterations = -1;
+            }
+        }
+        m2_ctx_destroy(ctx);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nworkers; ++w) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    if (!first_error.empty()) {
+        g_rhf_error = first_error;
+        return 1;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -97,6 +97,9 @@ def lib():
                                    C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                                    C.c_int, _ip, _ip, _dp, _dp, _dp, _ip, _dp]
         L.m2_soad_fock.restype = C.c_int
+        L.m2_rhf_batch.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp,
+                                   _dp, _ip, C.POINTER(SCFOptions), C.POINTER(SCFResult)]
+        L.m2_rhf_batch.restype = C.c_int
         L.m2_ctx_device.argtypes = [vp]
         L.m2_ctx_device.restype = C.c_int
         L.m2_grad.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
@@ -314,6 +317,40 @@ class Engine:
                 "total_seconds": res.total_seconds, "D": D, "C": Cm.T.copy(), "eps": eps,
                 "log": log[:res.iterations], "grad": g.reshape(mol.natom, 3) if gradient else None,
                 "grad_seconds": res.grad_seconds}
+
+    def rhf_batch(self, mols, basis="cc-pvdz", pure=True, nworkers=8, **kw):
+        """Closed-shell SCF::rhf of every molecule in `mols` through
+        m2_rhf_batch: nworkers host threads, each with its own engine context
+        and HIP stream on this engine's device, take molecules largest basis
+        first.  Core-Hamiltonian guess.  Returns one result dict per molecule
+        (energy, iterations, converged, n_full, n_incremental, n_fp32,
+        jk_seconds, total_seconds) and the wall seconds of the whole batch."""
+        import time
+        o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
+                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=0, diis_start=2)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        nmol = len(mols)
+        natom = np.array([m.natom for m in mols], np.int32)
+        charge = np.array([m.charge for m in mols], np.int32)
+        Z = np.concatenate([np.asarray(m.Z, float) for m in mols])
+        X = np.concatenate([np.ascontiguousarray(m.xyz, float).reshape(-1) for m in mols])
+        arrs = [shell_arrays(m.shells(basis)) for m in mols]
+        nsh = np.array([len(a[0]) for a in arrs], np.int32)
+        l, npr, ex, cf, cen, at = (np.ascontiguousarray(np.concatenate([a[k] for a in arrs])) for k in range(6))
+        res = (SCFResult * nmol)()
+        ip = lambda a: a.ctypes.data_as(_ip)
+        t = time.perf_counter()
+        st = self.L.m2_rhf_batch(self.L.m2_ctx_device(self.h), int(nworkers), nmol, ip(natom), ip(charge), _p(Z),
+                                 _p(X), int(pure), ip(nsh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
+                                 C.byref(o), res)
+        wall = time.perf_counter() - t
+        if st != 0:
+            raise M2Error(st, self.L.m2_rhf_last_error().decode())
+        out = [{"energy": r.energy, "iterations": r.iterations, "converged": bool(r.converged),
+                "n_full": r.n_full, "n_incremental": r.n_incremental, "n_fp32": r.n_fp32,
+                "jk_seconds": r.jk_seconds, "total_seconds": r.total_seconds} for r in res]
+        return out, wall
 
     def jk_uhf(self, Da, Db, multiplier=1.0, precision=1e-12, mixed=False):
         """UnrestrictedFock::formJKdirect: (Ja, Ka, Jb, Kb), screened with
