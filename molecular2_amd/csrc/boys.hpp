@@ -1,10 +1,12 @@
 // Device Boys function F_m(T), m = 0..M, for the ERI kernels.
 //
-// FP64: 8-term Taylor expansion of F_M about the nearest grid point
-// (|dT| <= 0.05, truncation < 1e-15 relative), then downward recursion
-// F_m = (2T F_{m+1} + e^-T) / (2m+1), which is stable.  For T >= 40 the
-// asymptotic F_0 = sqrt(pi/T)/2 (erfc(sqrt(40)) < 1e-18) and upward
-// recursion F_{m+1} = ((2m+1) F_m - e^-T) / (2T), stable for T > m.
+// FP64: 8-term Taylor expansion about the nearest grid point (|dT| <= 0.05,
+// truncation < 1e-15 relative), using F_m^(k) = (-1)^k F_{m+k}.  For
+// M <= BOYS_DIRECT_MAX every F_m comes from its own expansion over the same
+// table row (no exponential); above it F_M is expanded and the rest follow by
+// the stable downward recursion F_m = (2T F_{m+1} + e^-T) / (2m+1).  For
+// T >= BOYS_TMAX (60) the asymptotic F_0 = sqrt(pi/T)/2 (erfc(sqrt(60)) <
+// 1e-27) and F_{m+1} = (2m+1) F_m / (2T) (the e^-T term is below 1e-26).
 // FP32: 5-term Taylor from a float table (truncation < 3e-9 relative) with
 // the same downward recursion; the large-T branch is evaluated in FP64.
 #pragma once
@@ -27,12 +29,31 @@ __device__ __forceinline__ float sqrt_inv<float>(float x) { return rsqrtf(x); }
 template <typename R, int M>
 struct BoysEval;
 
+// highest M for which every F_m is expanded directly (FP64)
+constexpr int BOYS_DIRECT_MAX = 3;
+
 template <int M>
 struct BoysEval<double, M> {
     static __device__ __forceinline__ void run(double T, double* F, const BoysTables& tab) {
         if (T < BOYS_TMAX) {
             const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
             const double d = k * BOYS_STEP - T;
+            if constexpr (M <= BOYS_DIRECT_MAX) {
+                const double* r = tab.d + k * BOYS_MTAB;
+                const double d2 = d * 0.5, d3 = d * (1.0 / 3.0), d4 = d * 0.25, d5 = d * 0.2,
+                             d6 = d * (1.0 / 6.0), d7 = d * (1.0 / 7.0);
+#pragma unroll
+                for (int m = 0; m <= M; ++m) {
+                    double s = r[m + 6] + d7 * r[m + 7];
+                    s = r[m + 5] + d6 * s;
+                    s = r[m + 4] + d5 * s;
+                    s = r[m + 3] + d4 * s;
+                    s = r[m + 2] + d3 * s;
+                    s = r[m + 1] + d2 * s;
+                    F[m] = r[m] + d * s;
+                }
+                return;
+            }
             const double* r = tab.d + k * BOYS_MTAB + M;
             double s = r[7];
             s = r[6] + d * (1.0 / 7.0) * s;

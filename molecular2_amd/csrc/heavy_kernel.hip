@@ -143,12 +143,28 @@ __device__ __forceinline__ R outv(const HeavyProg& P, const R* X, int a_, int b_
     return X[P.out[((a_ * P.nb + b_) * P.nc + c_) * P.nd + d_]];
 }
 
+// LDS layout of k_jk_heavy: X[nslots] and S[NSCALAR] (type R), then the
+// quartet's outputs Y[NA NB NC ND] in (a,b,c,d) order and the six density
+// blocks it is digested with (FP64), 8-byte aligned.
 template <typename R>
+__host__ __device__ constexpr size_t heavy_y_offset(int nslots) {
+    return ((sizeof(R) * (nslots + NSCALAR) + 7) / 8) * 8;
+}
+
+template <typename R, int NA, int NB, int NC, int ND>
 __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
                                                  double lo, double hi, HeavyProg P) {
     extern __shared__ __align__(16) unsigned char smem[];
     R* X = reinterpret_cast<R*>(smem);
     R* S = X + P.nslots;
+    constexpr int NOUT = NA * NB * NC * ND;
+    double* Y = reinterpret_cast<double*>(smem + heavy_y_offset<R>(P.nslots));
+    double* Dcd = Y + NOUT;           // D(c,d)
+    double* Dab = Dcd + NC * ND;      // D(a,b)
+    double* Dbd = Dab + NA * NB;      // D(b,d)
+    double* Dac = Dbd + NB * ND;      // D(a,c)
+    double* Dbc = Dac + NA * NC;      // D(b,c)
+    double* Dad = Dbc + NB * NC;      // D(a,d)
     const int w = blockIdx.x;
     if (w >= ntasks) return;
     const int lane = threadIdx.x;
@@ -159,7 +175,8 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
     const double* O = a.sh_O;
     const double ABx = O[3 * sa] - O[3 * sb], ABy = O[3 * sa + 1] - O[3 * sb + 1], ABz = O[3 * sa + 2] - O[3 * sb + 2];
     const int fa = a.sh_bf[sa], fb = a.sh_bf[sb];
-    const int NA = P.na, NB = P.nb, NC = P.nc, ND = P.nd;
+    const double* D = a.D;
+    for (int i = lane; i < NA * NB; i += 64) Dab[i] = D[(fa + i / NB) * n + fb + i % NB];
     for (int kk = 0; kk < tk.nket; ++kk) {
         const int ket = tk.ket0 + kk;
         const int2 scd = a.pair_sh[ket];
@@ -174,54 +191,69 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
             atomicAdd(&a.counters[2 * a.combo + 1], (unsigned long long)a.pair_np[tk.bra] * a.pair_np[ket]);
         }
         const double CDx = O[3 * sc] - O[3 * sd], CDy = O[3 * sc + 1] - O[3 * sd + 1], CDz = O[3 * sc + 2] - O[3 * sd + 2];
+        const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+        // density blocks of this quartet (read by the digestion below, after
+        // the barriers inside heavy_quartet)
+        for (int i = lane; i < NC * ND; i += 64) Dcd[i] = D[(fc + i / ND) * n + fd + i % ND];
+        for (int i = lane; i < NB * ND; i += 64) Dbd[i] = D[(fb + i / ND) * n + fd + i % ND];
+        for (int i = lane; i < NA * NC; i += 64) Dac[i] = D[(fa + i / NC) * n + fc + i % NC];
+        for (int i = lane; i < NB * NC; i += 64) Dbc[i] = D[(fb + i / NC) * n + fc + i % NC];
+        for (int i = lane; i < NA * ND; i += 64) Dad[i] = D[(fa + i / ND) * n + fd + i % ND];
         heavy_quartet<R>(P, a, tab, tk.bra, ket, ABx, ABy, ABz, CDx, CDy, CDz, X, S);
+        for (int o = lane; o < NOUT; o += 64) Y[o] = double(X[P.out[o]]);
+        __syncthreads();
         const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);
         const double f = deg * a.scale, h = 0.5 * f;
-        const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
-        const double* D = a.D;
-        for (int i = lane; i < NA * NB; i += 64) {   // J(ab)
-            const int x = i / NB, y = i % NB;
+        for (int i = lane; i < NA * NB; i += 64) {   // J(ab) += D(cd) (ab|cd)
+            const double* y = Y + i * (NC * ND);
             double s = 0.0;
-            for (int c = 0; c < NC; ++c)
-                for (int d = 0; d < ND; ++d) s += D[(fc + c) * n + fd + d] * double(outv(P, X, x, y, c, d));
-            atomicAdd(&a.J[(fa + x) * n + fb + y], f * s);
+#pragma unroll
+            for (int t = 0; t < NC * ND; ++t) s += Dcd[t] * y[t];
+            atomicAdd(&a.J[(fa + i / NB) * n + fb + i % NB], f * s);
         }
-        for (int i = lane; i < NC * ND; i += 64) {   // J(cd)
-            const int c = i / ND, d = i % ND;
+        for (int i = lane; i < NC * ND; i += 64) {   // J(cd) += D(ab) (ab|cd)
             double s = 0.0;
-            for (int x = 0; x < NA; ++x)
-                for (int y = 0; y < NB; ++y) s += D[(fa + x) * n + fb + y] * double(outv(P, X, x, y, c, d));
-            atomicAdd(&a.J[(fc + c) * n + fd + d], f * s);
+#pragma unroll
+            for (int t = 0; t < NA * NB; ++t) s += Dab[t] * Y[t * (NC * ND) + i];
+            atomicAdd(&a.J[(fc + i / ND) * n + fd + i % ND], f * s);
         }
-        for (int i = lane; i < NA * NC; i += 64) {   // K(ac) += D(bd)
+        for (int i = lane; i < NA * NC; i += 64) {   // K(ac) += D(bd) (ab|cd)
             const int x = i / NC, c = i % NC;
             double s = 0.0;
+#pragma unroll
             for (int y = 0; y < NB; ++y)
-                for (int d = 0; d < ND; ++d) s += D[(fb + y) * n + fd + d] * double(outv(P, X, x, y, c, d));
+#pragma unroll
+                for (int d = 0; d < ND; ++d) s += Dbd[y * ND + d] * Y[((x * NB + y) * NC + c) * ND + d];
             atomicAdd(&a.K[(fa + x) * n + fc + c], h * s);
         }
-        for (int i = lane; i < NB * ND; i += 64) {   // K(bd) += D(ac)
+        for (int i = lane; i < NB * ND; i += 64) {   // K(bd) += D(ac) (ab|cd)
             const int y = i / ND, d = i % ND;
             double s = 0.0;
+#pragma unroll
             for (int x = 0; x < NA; ++x)
-                for (int c = 0; c < NC; ++c) s += D[(fa + x) * n + fc + c] * double(outv(P, X, x, y, c, d));
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s += Dac[x * NC + c] * Y[((x * NB + y) * NC + c) * ND + d];
             atomicAdd(&a.K[(fb + y) * n + fd + d], h * s);
         }
-        for (int i = lane; i < NA * ND; i += 64) {   // K(ad) += D(bc)
+        for (int i = lane; i < NA * ND; i += 64) {   // K(ad) += D(bc) (ab|cd)
             const int x = i / ND, d = i % ND;
             double s = 0.0;
+#pragma unroll
             for (int y = 0; y < NB; ++y)
-                for (int c = 0; c < NC; ++c) s += D[(fb + y) * n + fc + c] * double(outv(P, X, x, y, c, d));
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s += Dbc[y * NC + c] * Y[((x * NB + y) * NC + c) * ND + d];
             atomicAdd(&a.K[(fa + x) * n + fd + d], h * s);
         }
-        for (int i = lane; i < NB * NC; i += 64) {   // K(bc) += D(ad)
+        for (int i = lane; i < NB * NC; i += 64) {   // K(bc) += D(ad) (ab|cd)
             const int y = i / NC, c = i % NC;
             double s = 0.0;
+#pragma unroll
             for (int x = 0; x < NA; ++x)
-                for (int d = 0; d < ND; ++d) s += D[(fa + x) * n + fd + d] * double(outv(P, X, x, y, c, d));
+#pragma unroll
+                for (int d = 0; d < ND; ++d) s += Dad[x * ND + d] * Y[((x * NB + y) * NC + c) * ND + d];
             atomicAdd(&a.K[(fb + y) * n + fc + c], h * s);
         }
-        __syncthreads();   // outputs consumed before the next quartet overwrites X
+        __syncthreads();   // Y and the density blocks consumed before the next quartet
     }
 }
 
@@ -316,20 +348,45 @@ bool heavy_has(int la, int lb, int lc, int ld) {
     return false;
 }
 
+template <int NA, int NB, int NC, int ND>
+static void launch_heavy_dims(const HeavyProg& P, const JKArgs& a, const BoysTables& tab, const Task* tasks,
+                              int ntasks, int mode, hipStream_t st) {
+    const double inf = __builtin_huge_val();
+    const size_t extra = sizeof(double) * (NA * NB * NC * ND + NC * ND + NA * NB + NB * ND + NA * NC + NB * NC + NA * ND);
+    if (mode == 2) {
+        const size_t sm = heavy_y_offset<float>(P.nslots) + extra;
+        hipLaunchKernelGGL((k_jk_heavy<float, NA, NB, NC, ND>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks,
+                           a.precision, inf, P);
+    } else {
+        const size_t sm = heavy_y_offset<double>(P.nslots) + extra;
+        hipLaunchKernelGGL((k_jk_heavy<double, NA, NB, NC, ND>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks,
+                           a.precision, inf, P);
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+static void launch_heavy_class(int pure, const HeavyProg& P, const JKArgs& a, const BoysTables& tab,
+                               const Task* tasks, int ntasks, int mode, hipStream_t st) {
+    constexpr int PA = 2 * LA + 1, PB = 2 * LB + 1, PC = 2 * LC + 1, PD = 2 * LD + 1;
+    constexpr int CA = (LA + 1) * (LA + 2) / 2, CB = (LB + 1) * (LB + 2) / 2, CC = (LC + 1) * (LC + 2) / 2,
+                  CD = (LD + 1) * (LD + 2) / 2;
+    if (pure) launch_heavy_dims<PA, PB, PC, PD>(P, a, tab, tasks, ntasks, mode, st);
+    else launch_heavy_dims<CA, CB, CC, CD>(P, a, tab, tasks, ntasks, mode, st);
+}
+
 hipError_t launch_jk_heavy(int device, int la, int lb, int lc, int ld, int pure, const JKArgs& a,
                            const BoysTables& tab, const Task* tasks, int ntasks, int mode, hipStream_t st) {
     if (ntasks <= 0) return hipSuccess;
     HeavyProg P;
     if (!find_prog(device, la, lb, lc, ld, pure, P)) return hipErrorInvalidValue;
-    const double inf = __builtin_huge_val();
-    if (mode == 2) {
-        const size_t sm = sizeof(float) * (P.nslots + NSCALAR);
-        hipLaunchKernelGGL(k_jk_heavy<float>, dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks, a.precision,
-                           inf, P);
-    } else {
-        const size_t sm = sizeof(double) * (P.nslots + NSCALAR);
-        hipLaunchKernelGGL(k_jk_heavy<double>, dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks, a.precision,
-                           inf, P);
+    const int key = ((la * 3 + lb) * 3 + lc) * 3 + ld;
+    switch (key) {
+        case ((2 * 3 + 1) * 3 + 2) * 3 + 1: launch_heavy_class<2, 1, 2, 1>(pure, P, a, tab, tasks, ntasks, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 1) * 3 + 1: launch_heavy_class<2, 2, 1, 1>(pure, P, a, tab, tasks, ntasks, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 2) * 3 + 0: launch_heavy_class<2, 2, 2, 0>(pure, P, a, tab, tasks, ntasks, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 2) * 3 + 1: launch_heavy_class<2, 2, 2, 1>(pure, P, a, tab, tasks, ntasks, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 2) * 3 + 2: launch_heavy_class<2, 2, 2, 2>(pure, P, a, tab, tasks, ntasks, mode, st); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
