@@ -101,6 +101,15 @@ m2_status m2_jk_direct_uhf(m2_ctx* ctx, const double* Da, const double* Db, doub
 /* quartets with Schwarz bound x block norm below this run in FP32 under
  * M2_MIXED (default 1e-4) */
 m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below);
+/* s-s ket families for the direct J/K build, applied at the next
+ * m2_basis_set.  No reference counterpart (libint computes every shell
+ * quartet on its own): s shells of one centre with identical exponents share
+ * their primitive pair integrals across the quartets they enter, with the
+ * reference's screening and unique-quartet rule kept per contracted quartet.
+ * mode 0: off; 1 (default): on when they save work, for (ss|ss) and (pp|ss)
+ * always and for (ps|ss) and (ds|ss) when most family members are
+ * significant together; 2: on for all four. */
+m2_status m2_set_jk_families(m2_ctx* ctx, int mode);
 /* statistics of the last J/K build: computed quartets (FP64, FP32), tasks */
 m2_status m2_last_stats(m2_ctx* ctx, long long* q64, long long* q32, long long* ntasks);
 

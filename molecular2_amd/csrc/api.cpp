@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -125,7 +126,210 @@ JKArgs jk_args(m2_ctx* ctx) {
     a.Dn = ctx->d_Dn.p;
     a.n = ctx->obs.nbf;
     a.nsh = ctx->obs.nsh;
+    if (ctx->fam_on) {
+        a.fam_prims = ctx->d_fam_prims.p;
+        a.fam_prims_f = ctx->d_fam_prims_f.p;
+        a.fam_km = ctx->d_fam_km.p;
+        a.fam_km_f = ctx->d_fam_km_f.p;
+        a.fam_off = ctx->d_fam_off.p;
+        a.fam_np = ctx->d_fam_np.p;
+        a.fam_mem = ctx->d_fam_mem.p;
+        a.pair_fam = ctx->d_pair_fam.p;
+        a.pair_mem = ctx->d_pair_mem.p;
+    }
     return a;
+}
+
+// s-s ket families: s shells with the same centre and identical exponents
+// (a general contraction written as segmented shells, e.g. the two contracted
+// s shells of O in cc-pVDZ) form a family of at most two shells; the class-0
+// shell pairs whose shells come from the same two families form a family
+// pair (<= FAM_MAXMEM members) whose primitive pairs are computed once; the
+// record's K carries no contraction coefficients, and beside it every member
+// keeps its full factor K c_c[i] c_d[j] (what K would be in that member's own
+// shell-pair record), which the kernel applies per member.  Families are used when they cut the class-0 ket
+// count by at least 15%.  Family pairs are ordered like pairs (np bucket, then
+// the largest member Q descending); pair_fam / pair_mem give every class-0
+// pair its family pair and slot.  Which class combinations use them is
+// fam_combo (m2_common.hpp).
+static m2_status build_families(m2_ctx* ctx) {
+    BasisData& b = ctx->obs;
+    ctx->fam_on = false;
+    ctx->fam_wide = false;
+    ctx->fam_seg = PairClassSeg{};
+    const PairClassSeg& c0 = ctx->cls[0];
+    if (c0.end <= c0.begin) return M2_OK;
+    std::vector<int> fid(b.nsh, -1), fsize;
+    {
+        std::map<std::vector<double>, int> last;   // centre + exponents -> newest family of that key
+        for (int s = 0; s < b.nsh; ++s) {
+            const HostShell& h = b.sh[s];
+            if (h.l != 0) continue;
+            std::vector<double> key = {h.O[0], h.O[1], h.O[2]};
+            key.insert(key.end(), h.a.begin(), h.a.end());
+            auto it = last.find(key);
+            if (it != last.end() && fsize[it->second] < 2) {
+                fid[s] = it->second;
+                ++fsize[fid[s]];
+            } else {
+                fid[s] = (int)fsize.size();
+                fsize.push_back(1);
+                last[key] = fid[s];
+            }
+        }
+    }
+    struct FP {
+        int f1, f2, rep1, rep2;   // families (f1 >= f2) and a shell of each
+        std::vector<int> mem;     // member pair indices
+        double Q = 0.0;
+        std::vector<PrimPair> pp;
+        std::vector<double> km;
+    };
+    std::map<std::pair<int, int>, int> index;
+    std::vector<FP> fps;
+    for (int i = c0.begin; i < c0.end; ++i) {
+        const int sa = ctx->pair_sh[i].x, sb = ctx->pair_sh[i].y;
+        const int f1 = std::max(fid[sa], fid[sb]), f2 = std::min(fid[sa], fid[sb]);
+        auto it = index.find({f1, f2});
+        int k;
+        if (it == index.end()) {
+            k = (int)fps.size();
+            index[{f1, f2}] = k;
+            FP fp;
+            fp.f1 = f1;
+            fp.f2 = f2;
+            fp.rep1 = fid[sa] == f1 ? sa : sb;
+            fp.rep2 = fid[sa] == f1 ? sb : sa;
+            fps.push_back(fp);
+        } else {
+            k = it->second;
+        }
+        ARGCHK((int)fps[k].mem.size() < FAM_MAXMEM, "m2_basis_set: s-shell family pair with too many members");
+        fps[k].mem.push_back(i);
+        fps[k].Q = std::max(fps[k].Q, ctx->pair_Q[i]);
+    }
+    const int nfp = (int)fps.size();
+    if (ctx->fam_mode == 0) return M2_OK;
+    if (ctx->fam_mode == 1 && nfp * 1.15 > c0.end - c0.begin) return M2_OK;   // too little sharing
+    // member activity: over the family pairs any quartet can reach
+    // (Q_F max Q >= 1e-10), the mean number of members within 1e-4 of the
+    // family's largest Q
+    {
+        double qmax = 0.0, nact = 0.0, nused = 0.0;
+        for (double q : ctx->pair_Q) qmax = std::max(qmax, q);
+        for (const FP& fp : fps) {
+            if (fp.Q * qmax < 1e-10) continue;
+            nused += 1.0;
+            for (int j : fp.mem) nact += ctx->pair_Q[j] >= 1e-4 * fp.Q ? 1.0 : 0.0;
+        }
+        ctx->fam_wide = ctx->fam_mode == 2 || (nused > 0.0 && nact >= 1.6 * nused);
+    }
+    const double sqrt2pi52 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
+    for (FP& fp : fps) {
+        const HostShell &A = b.sh[fp.rep1], &B = b.sh[fp.rep2];
+        double AB2 = 0;
+        for (int k = 0; k < 3; ++k) AB2 += (A.O[k] - B.O[k]) * (A.O[k] - B.O[k]);
+        for (size_t i = 0; i < A.a.size(); ++i)
+            for (size_t j = 0; j < B.a.size(); ++j) {
+                const double a = A.a[i], bb = B.a[j], p = a + bb;
+                const double ex = std::exp(-a * bb / p * AB2);
+                if (ex < 1e-20) continue;   // the same primitive-pair screen as the shell pairs
+                PrimPair q{};
+                q.p = p;
+                q.inv2p = 0.5 / p;
+                q.K = sqrt2pi52 * ex / p;
+                for (int k = 0; k < 3; ++k) {
+                    q.P[k] = (a * A.O[k] + bb * B.O[k]) / p;
+                    q.PA[k] = q.P[k] - A.O[k];
+                }
+                q.a = a;
+                fp.pp.push_back(q);
+                for (int m = 0; m < FAM_MAXMEM; ++m) {
+                    double c = 0.0;
+                    if (m < (int)fp.mem.size()) {
+                        const int sc = ctx->pair_sh[fp.mem[m]].x, sd = ctx->pair_sh[fp.mem[m]].y;
+                        // (sc sd| = sum_ij c_sc[i] c_sd[j] G_ij with i over the f1 family's exponents
+                        c = fid[sc] == fp.f1 ? b.sh[sc].c[i] * b.sh[sd].c[j] : b.sh[sd].c[i] * b.sh[sc].c[j];
+                    }
+                    c *= q.K;   // the member's full ket factor K_cd of this primitive pair
+                    fp.km.push_back(c);
+                }
+            }
+    }
+    std::vector<int> order(nfp);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int bx = np_bucket((int)fps[x].pp.size()), by = np_bucket((int)fps[y].pp.size());
+        if (bx != by) return bx > by;
+        return fps[x].Q > fps[y].Q;
+    });
+    const int npair = (int)ctx->pair_sh.size();
+    std::vector<PrimPair> prims;
+    std::vector<BraPair<float>> prims_f;
+    std::vector<double> km, fQ(nfp);
+    std::vector<float> km_f;
+    std::vector<int> off(nfp), np(nfp), pfam(npair, -1), pmem(npair, -1);
+    std::vector<int4> mem(nfp);
+    for (int r = 0; r < nfp; ++r) {
+        const FP& fp = fps[order[r]];
+        off[r] = (int)prims.size();
+        np[r] = (int)fp.pp.size();
+        fQ[r] = fp.Q;
+        int m4[FAM_MAXMEM] = {-1, -1, -1, -1};
+        for (size_t m = 0; m < fp.mem.size(); ++m) {
+            m4[m] = fp.mem[m];
+            pfam[fp.mem[m]] = r;
+            pmem[fp.mem[m]] = (int)m;
+        }
+        mem[r] = make_int4(m4[0], m4[1], m4[2], m4[3]);
+        for (const PrimPair& q : fp.pp) {
+            prims.push_back(q);
+            BraPair<float> f{};
+            f.p = (float)q.p;
+            f.inv2p = (float)q.inv2p;
+            f.K = (float)q.K;
+            for (int k = 0; k < 3; ++k) {
+                f.P[k] = (float)q.P[k];
+                f.PA[k] = (float)q.PA[k];
+            }
+            prims_f.push_back(f);
+        }
+        km.insert(km.end(), fp.km.begin(), fp.km.end());
+    }
+    km_f.assign(km.begin(), km.end());
+    HIPCHK(ctx->d_fam_prims.reserve(std::max<size_t>(prims.size(), 1)));
+    HIPCHK(ctx->d_fam_prims_f.reserve(std::max<size_t>(prims.size(), 1)));
+    HIPCHK(ctx->d_fam_km.reserve(std::max<size_t>(km.size(), 1)));
+    HIPCHK(ctx->d_fam_km_f.reserve(std::max<size_t>(km.size(), 1)));
+    HIPCHK(ctx->d_fam_Q.reserve(nfp));
+    HIPCHK(ctx->d_fam_off.reserve(nfp));
+    HIPCHK(ctx->d_fam_np.reserve(nfp));
+    HIPCHK(ctx->d_fam_mem.reserve(nfp));
+    HIPCHK(ctx->d_pair_fam.reserve(npair));
+    HIPCHK(ctx->d_pair_mem.reserve(npair));
+    if (!prims.empty()) {
+        HIPCHK(hipMemcpy(ctx->d_fam_prims.p, prims.data(), sizeof(PrimPair) * prims.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_fam_prims_f.p, prims_f.data(), sizeof(BraPair<float>) * prims.size(),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_fam_km.p, km.data(), sizeof(double) * km.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_fam_km_f.p, km_f.data(), sizeof(float) * km.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(ctx->d_fam_Q.p, fQ.data(), sizeof(double) * nfp, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_fam_off.p, off.data(), sizeof(int) * nfp, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_fam_np.p, np.data(), sizeof(int) * nfp, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_fam_mem.p, mem.data(), sizeof(int4) * nfp, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_pair_fam.p, pfam.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_pair_mem.p, pmem.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
+    PairClassSeg& seg = ctx->fam_seg;
+    seg.begin = 0;
+    seg.end = nfp;
+    for (int r = 0; r < nfp;) {
+        const int bk = np_bucket(np[r]), g0 = r;
+        while (r < nfp && np_bucket(np[r]) == bk) ++r;
+        seg.groups.push_back(PairGroup{g0, r});
+    }
+    ctx->fam_on = true;
+    return M2_OK;
 }
 
 // Build canonical shell pairs (la >= lb; equal l: a >= b), their primitive
@@ -277,18 +481,23 @@ m2_status build_pairs(m2_ctx* ctx) {
         seg.end = i1;
         i0 = i1;
     }
+    st = build_families(ctx);
+    if (st) return st;
     // device work-list buffers of the register-class combinations (sizes
     // depend on the basis only)
     for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
         for (int Y = 0; Y <= X; ++Y, ++ci) {
             JKList& L = ctx->lists[ci];
             L = JKList{};
-            const PairClassSeg &sx = ctx->cls[X], &sy = ctx->cls[Y];
+            const bool fam = ctx->fam_on && fam_combo(X, Y, ctx->fam_wide);
+            const PairClassSeg &sx = ctx->cls[X], &sy = fam ? ctx->fam_seg : ctx->cls[Y];
             if (!JK_COMBO[X][Y] || sx.end <= sx.begin || sy.end <= sy.begin) continue;
             ARGCHK((int)sy.groups.size() <= JK_MAXG, "m2_basis_set: too many primitive-count groups");
             L.bra0 = sx.begin;
             L.nbra = sx.end - sx.begin;
             L.same = X == Y;
+            L.Qk = fam ? ctx->d_fam_Q.p : ctx->d_pair_Q.p;
+            L.bra_fam = fam && X == Y ? ctx->d_pair_fam.p : nullptr;
             L.ngrp = (int)sy.groups.size();
             int maxch = 0;
             for (int g = 0; g < L.ngrp; ++g) {
@@ -468,6 +677,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
         const Launch& pl = plan[pi];
         a.combo = 6 * pl.X + pl.Y;
         if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi], ctx->stream));
+        a.fam_kets = ctx->fam_on && fam_combo(pl.X, pl.Y, ctx->fam_wide) ? 1 : 0;
         if (pl.ci >= 0) {
             const JKList& L = ctx->lists[pl.ci];
             a.cut = L.cut;
@@ -831,6 +1041,13 @@ m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below) {
     if (!ctx) return M2_ERR_ARG;
     ARGCHK(fp32_below >= 0.0, "m2_set_mixed_threshold: threshold must be >= 0");
     ctx->fp32_below = fp32_below;
+    return M2_OK;
+}
+
+m2_status m2_set_jk_families(m2_ctx* ctx, int mode) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(mode >= 0 && mode <= 2, "m2_set_jk_families: mode must be 0, 1 or 2");
+    ctx->fam_mode = mode;
     return M2_OK;
 }
 

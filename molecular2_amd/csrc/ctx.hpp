@@ -80,6 +80,18 @@ struct m2_ctx {
     m2::DevBuf<int> d_pair_off, d_pair_np;
     m2::DevBuf<int2> d_pair_sh;
     m2::DevBuf<double> d_pair_Q;
+    // s-s ket family pairs (shells of one atom with identical exponents share
+    // their primitive pairs), ordered like pairs: np bucket, Q descending
+    int fam_mode = 1;                  // m2_set_jk_families
+    bool fam_on = false;
+    bool fam_wide = false;             // also for the (ps|ss) and (ds|ss) combinations
+    m2::PairClassSeg fam_seg;          // groups over family-pair indices
+    m2::DevBuf<m2::PrimPair> d_fam_prims;
+    m2::DevBuf<m2::BraPair<float>> d_fam_prims_f;
+    m2::DevBuf<double> d_fam_km, d_fam_Q;
+    m2::DevBuf<float> d_fam_km_f;
+    m2::DevBuf<int> d_fam_off, d_fam_np, d_pair_fam, d_pair_mem;
+    m2::DevBuf<int4> d_fam_mem;
     // Boys tables
     m2::DevBuf<double> d_boys;
     m2::DevBuf<float> d_boys_f;

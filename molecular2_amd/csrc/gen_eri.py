@@ -391,6 +391,73 @@ def gen_class(la, lb, lc, ld, deriv=False):
     return "\n".join(em.lines)
 
 
+# Bra classes whose (ab|ss) quartets also get run_fam: the ket is a family
+# pair of s shells (shells of one atom with identical exponents, e.g. the two
+# contracted s shells of O in cc-pVDZ), whose primitive pairs are shared by up
+# to FAM_MAXMEM contracted member pairs.
+FAM_BRA = [(0, 0), (1, 0), (1, 1), (2, 0)]
+FAM_MAXMEM = 4
+
+
+def gen_class_fam(la, lb):
+    """run_fam for (la lb|ss): per ket primitive pair the bra-contracted
+    [e0|s] are formed once without any ket factor, then folded into the
+    accumulators of every member with that member's ket factor
+    km[FAM_MAXMEM * ik + m] (K_cd of the member's own primitive pair); the
+    HRR / solid-harmonic post-processing then runs once per member, writing
+    out + m * NCART."""
+    base = gen_class(la, lb, 0, 0)
+    lines = base.splitlines()
+    E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
+    accs = ["a%s_%s" % (cname(e), cname((0, 0, 0))) for e in E]
+    i_loop = next(i for i, t in enumerate(lines) if t.strip().startswith("for (int ik = 0; ik < nket; ++ik)"))
+    i_bra = next(i for i in range(i_loop, len(lines)) if lines[i].strip().startswith("for (int ib = 0"))
+    i_acc = next(i for i in range(i_bra, len(lines)) if lines[i].strip().startswith(accs[0] + " +="))
+    i_end = i_acc + len(accs)
+    assert lines[i_end].strip() == "}" and lines[i_end + 1].strip() == "}"
+    post = lines[i_end + 2:]
+    # drop the closing of run() and of the struct from the post block
+    assert post[-1] == "};" and post[-2].strip() == "}"
+    post = post[:-2]
+    ket_head = [t.replace("const R q = R(kp.p), Kcd = R(kp.K);", "const R q = R(kp.p);")
+                for t in lines[i_loop + 1:i_bra]]
+    bra_body = [t.replace("Kab * Kcd * rs_pq", "Kab * rs_pq") for t in lines[i_bra:i_acc]]
+    em = []
+    em.append("  static constexpr int NACC_FAM = %d;   // accumulators of run_fam" % (len(accs) * (FAM_MAXMEM + 1)))
+    em.append("  template <typename R, bool PURE, typename BPtr, typename KPtr, typename CPtr>")
+    em.append("  static __device__ __forceinline__ void run_fam(BPtr bra, int nbra,")
+    em.append("      KPtr ket, CPtr km, int nket, const R ABx, const R ABy, const R ABz,")
+    em.append("      const BoysTables& tab, R* __restrict__ out) {")
+    for m in range(FAM_MAXMEM):
+        for a in accs:
+            em.append("    R %s_m%d = R(0);" % (a, m))
+    em.append("    for (int ik = 0; ik < nket; ++ik) {")
+    em.extend(ket_head)
+    for a in accs:
+        em.append("      R t%s = R(0);" % a[1:])
+    em.extend(bra_body)
+    for e in E:
+        em.append("        t%s_%s += %s;" % (cname(e), cname((0, 0, 0)), vrr_name(e, (0, 0, 0), 0)))
+    em.append("      }")
+    for m in range(FAM_MAXMEM):
+        em.append("      const R c%d = R(km[%d * ik + %d]);" % (m, FAM_MAXMEM, m))
+    for m in range(FAM_MAXMEM):
+        for a in accs:
+            em.append("      %s_m%d += c%d * t%s;" % (a, m, m, a[1:]))
+    em.append("    }")
+    for m in range(FAM_MAXMEM):
+        em.append("    {")
+        for a in accs:
+            em.append("      const R %s = %s_m%d;" % (a, a, m))
+        em.append("      R* __restrict__ outm = out + %d * NCART;" % m)
+        for t in post:
+            em.append("  " + t.replace("out[", "outm["))
+        em.append("    }")
+    em.append("  }")
+    assert lines[-1] == "};"
+    return "\n".join(lines[:-1] + em + ["};"])
+
+
 def class_list(lmax_bra, lmax_ket):
     out = []
     for la in range(lmax_bra + 1):
@@ -414,7 +481,8 @@ def main(outdir):
             classes.add((lp, 0, lq, 0))
     written = []
     for c in sorted(classes):
-        text = "// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.\n#pragma once\n" + gen_class(*c) + "\n"
+        body = gen_class_fam(c[0], c[1]) if (c[2], c[3]) == (0, 0) and (c[0], c[1]) in FAM_BRA else gen_class(*c)
+        text = "// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.\n#pragma once\n" + body + "\n"
         path = os.path.join(outdir, "eri_%d%d%d%d.inc" % c)
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as fh:
@@ -589,6 +657,12 @@ def write_tus(outdir):
             lines = list(head)
             lines.append('#include "eri_%d%d%d%d.inc"' % (la, lb, lc, ld))
             lines.append("void launch_jk_%d_%d(int pure, const JKArgs& a, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {" % (X, Y))
+            if (lc, ld) == (0, 0) and (la, lb) in FAM_BRA:
+                lines.append("  if (a.fam_kets) {")
+                lines.append("    if (pure) launch_jk_fam<%d, %d, true>(a, tab, nblocks, mode, st);" % (la, lb))
+                lines.append("    else launch_jk_fam<%d, %d, false>(a, tab, nblocks, mode, st);" % (la, lb))
+                lines.append("    return;")
+                lines.append("  }")
             lines.append("  if (pure) launch_jk<%d, %d, %d, %d, true>(a, tab, nblocks, mode, st);" % (la, lb, lc, ld))
             lines.append("  else launch_jk<%d, %d, %d, %d, false>(a, tab, nblocks, mode, st);" % (la, lb, lc, ld))
             lines.append("}")

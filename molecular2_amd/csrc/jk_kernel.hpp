@@ -266,6 +266,156 @@ __global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk
     }
 }
 
+// (ab|ss) with s-s family kets: one lane per ket family pair.  Its primitive
+// pairs are shared by up to FAM_MAXMEM contracted member pairs (cd); each
+// member keeps the exact screening of src/fockbuild.cpp:98-115 and, for the
+// same class (X == Y == 0), the unique-pair rule: member rank (family,
+// slot) <= the bra's rank.  Digestion per member as in k_jk.
+template <typename R>
+struct FamPrimOf;
+template <>
+struct FamPrimOf<double> {
+    static __device__ __forceinline__ const PrimPair* prims(const JKArgs& a) { return a.fam_prims; }
+    static __device__ __forceinline__ const double* km(const JKArgs& a) { return a.fam_km; }
+};
+template <>
+struct FamPrimOf<float> {
+    static __device__ __forceinline__ const BraPair<float>* prims(const JKArgs& a) { return a.fam_prims_f; }
+    static __device__ __forceinline__ const float* km(const JKArgs& a) { return a.fam_km_f; }
+};
+
+template <int LA, int LB, typename R>
+struct JKWavesFam {
+    static constexpr int value = EriClass<LA, LB, 0, 0>::NACC_FAM * (int)sizeof(R) <= 280 ? 8 : 4;
+};
+
+template <int LA, int LB, typename R, bool PURE>
+__global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64)) void k_jk_fam(JKArgs a, BoysTables tab, double lo,
+                                                                               double hi) {
+    constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
+    constexpr int NCART = EriClass<LA, LB, 0, 0>::NCART;
+    constexpr int W = JKWavesFam<LA, LB, R>::value;
+    typedef const __attribute__((address_space(4))) int cint;
+    const int bl = ((cint*)a.blk_bra)[blockIdx.x];
+    const int lb = blockIdx.x - ((cint*)a.blk_off)[bl];
+    const int bra = a.bra0 + bl;
+    cint* cut = (cint*)a.cut + 2 * bl * JK_MAXG;
+    int nch = 0;
+    for (int g = 0; g < a.ngrp; ++g) {
+        const int mid = cut[2 * g], end = cut[2 * g + 1];
+        nch += ((a.tail ? end - mid : mid - a.gbeg[g]) + 63) >> 6;
+    }
+    const int c0 = lb * JK_BLOCK_TASKS, c1 = min(c0 + JK_BLOCK_TASKS, nch);
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int2 sab = a.pair_sh[bra];
+    const int sa = __builtin_amdgcn_readfirstlane(sab.x), sb = __builtin_amdgcn_readfirstlane(sab.y);
+    const int n = a.n, nsh = a.nsh;
+    const int fa = a.sh_bf[sa], fb = a.sh_bf[sb];
+    const bool lds = a.k_lds != 0;
+    using PP = typename PrimPairOf<R>::type;
+    typedef const __attribute__((address_space(4))) PP cPP;
+    const int nbp = a.pair_np[bra];
+    cPP* bprims = (cPP*)(PrimPairOf<R>::get(a) + a.pair_off[bra]);
+    const auto* fprims = FamPrimOf<R>::prims(a);
+    const auto* fkm = FamPrimOf<R>::km(a);
+    // unique pairs of pairs inside class 0: the bra's own family and slot
+    const bool same = a.pair_fam != nullptr && a.combo == 0;
+    const int bfam = same ? a.pair_fam[bra] : -1, bmem = same ? a.pair_mem[bra] : 0;
+    extern __shared__ double sK[];
+    if (lds)
+        for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
+    __syncthreads();
+    const double* D = a.D;
+    double Dab[NA * NB], jab[NA * NB];
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            Dab[x * NB + y] = D[(fa + x) * n + fb + y];
+            jab[x * NB + y] = 0.0;
+        }
+    const double* O = a.sh_O;
+    const R ABx = R(O[3 * sa] - O[3 * sb]), ABy = R(O[3 * sa + 1] - O[3 * sb + 1]), ABz = R(O[3 * sa + 2] - O[3 * sb + 2]);
+    const double* Dn = a.Dn;
+    const double dnab = Dn[sa * nsh + sb];
+    const double qab = a.pair_Q[bra];
+    const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
+    unsigned long long cnt_q = 0, cnt_pq = 0;
+    for (int ch = c0 + wave; ch < c1; ch += W) {
+        int ket0 = 0, nket = 0, rem = ch;
+        for (int g = 0; g < a.ngrp; ++g) {
+            const int mid = cut[2 * g], end = cut[2 * g + 1];
+            const int s0 = a.tail ? mid : a.gbeg[g], s1 = a.tail ? end : mid;
+            const int nc = (s1 - s0 + 63) >> 6;
+            if (rem < nc) {
+                ket0 = s0 + 64 * rem;
+                nket = min(64, s1 - ket0);
+                break;
+            }
+            rem -= nc;
+        }
+        const int ket = ket0 + (lane < nket ? lane : 0);
+        const int4 mem4 = a.fam_mem[ket];
+        const int mem[FAM_MAXMEM] = {mem4.x, mem4.y, mem4.z, mem4.w};
+        bool act[FAM_MAXMEM];
+        bool active = false;
+        int nact = 0;
+#pragma unroll
+        for (int m = 0; m < FAM_MAXMEM; ++m) {
+            act[m] = false;
+            const int j = mem[m];
+            if (lane < nket && j >= 0 && !(same && ket == bfam && m > bmem)) {
+                const int2 scd = a.pair_sh[j];
+                const int sc = scd.x, sd = scd.y;
+                const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]), fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]),
+                                                                           fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+                const double bound = dn * qab * a.pair_Q[j];
+                act[m] = bound >= lo && bound < hi;
+            }
+            active = active || act[m];
+            nact += act[m] ? 1 : 0;
+        }
+        if (!__any(active)) continue;
+        if (a.counters) {
+            cnt_q += (unsigned long long)(wave_sum(double(nact)) + 0.5);
+            cnt_pq += (unsigned long long)(wave_sum(active ? double(a.fam_np[ket]) : 0.0) * nbp + 0.5);
+        }
+        R v[FAM_MAXMEM * NCART];
+        const int nkp = active ? a.fam_np[ket] : 0;
+        const int off = a.fam_off[ket];
+        EriClass<LA, LB, 0, 0>::template run_fam<R, PURE>(bprims, nbp, fprims + off, fkm + FAM_MAXMEM * off, nkp,
+                                                          ABx, ABy, ABz, tab, v);
+#pragma unroll
+        for (int m = 0; m < FAM_MAXMEM; ++m)
+            if (act[m]) {
+                const int2 scd = a.pair_sh[mem[m]];
+                const int sc = scd.x, sd = scd.y;
+                const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == mem[m] ? 1.0 : 2.0);
+                digest<NA, NB, 1, 1, R>(v + m * NCART, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K,
+                                        lds, sK);
+            }
+    }
+    if (a.counters && lane == 0 && cnt_q) {
+        atomicAdd(&a.counters[2 * a.combo], cnt_q);
+        atomicAdd(&a.counters[2 * a.combo + 1], cnt_pq);
+    }
+#pragma unroll
+    for (int x = 0; x < NA * NB; ++x) {
+        const double s = wave_sum(jab[x]);
+        if (lane == 0 && s != 0.0) atomic_add_f64(&a.J[(fa + x / NB) * n + fb + x % NB], s);
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) {
+        const double s = sK[i];
+        if (s != 0.0) {
+            const int row = i / n, col = i - row * n;
+            atomic_add_f64(&a.K[(row < NA ? fa + row : fb + row - NA) * n + col], s);
+        }
+    }
+}
+
 // Schwarz: Q(pair) = sqrt(max_ab |(ab|ab)|), one lane per pair
 // (IntegralEngine::compute_schwarz_ints, src/integrals.cpp:471-513).
 template <int LA, int LB, bool PURE>
@@ -306,6 +456,23 @@ void launch_jk(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, h
                            lds, st, a, tab, a.precision, inf);
     else
         hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, dim3(64 * JKWaves<LA, LB, LC, LD, double>::value),
+                           lds, st, a, tab, a.precision, inf);
+}
+
+template <int LA, int LB, bool PURE>
+void launch_jk_fam(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
+    if (nblocks <= 0) return;
+    constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
+    JKArgs a = a0;
+    const size_t kbytes = sizeof(double) * (size_t)(NA + NB) * a.n;
+    a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
+    const double inf = __builtin_huge_val();
+    const size_t lds = a.k_lds ? kbytes : 0;
+    if (mode == 2)
+        hipLaunchKernelGGL((k_jk_fam<LA, LB, float, PURE>), dim3(nblocks), dim3(64 * JKWavesFam<LA, LB, float>::value),
+                           lds, st, a, tab, a.precision, inf);
+    else
+        hipLaunchKernelGGL((k_jk_fam<LA, LB, double, PURE>), dim3(nblocks), dim3(64 * JKWavesFam<LA, LB, double>::value),
                            lds, st, a, tab, a.precision, inf);
 }
 

@@ -13,6 +13,8 @@ struct JKList {
     int* off;                 // [2][nbra + 1] workgroups per bra -> exclusive offsets (FP64 head, FP32 tail)
     int* bra_of;              // [2][cap] workgroup -> bra (local index)
     int cap;                  // capacity of each half of bra_of
+    const double* Qk;         // Schwarz bounds the ket ranges are sorted by (pairs or family pairs)
+    const int* bra_fam;       // same class with family kets: bra pair -> its family pair (else null)
 };
 
 void launch_dmax(const double* Dn, int m, double* out, hipStream_t st);

@@ -73,7 +73,29 @@ struct JKArgs {
     int ngrp;                // ket groups
     int tail;                // 0: FP64 head [gbeg, mid) of each group; 1: FP32 tail [mid, end)
     int gbeg[JK_MAXG];
+    // s-s ket families (k_jk_fam): family pairs of s shells sharing their
+    // primitive pairs among up to FAM_MAXMEM contracted member pairs
+    int fam_kets;            // set by the host: ket indices of this launch are family pairs
+    const PrimPair* fam_prims;        // primitive pairs, K without contraction coefficients
+    const BraPair<float>* fam_prims_f;
+    const double* fam_km;    // [prim][FAM_MAXMEM] member factors K c_c c_d (0 for absent members)
+    const float* fam_km_f;
+    const int* fam_off;      // first primitive pair of each family pair
+    const int* fam_np;
+    const int4* fam_mem;     // member shell-pair indices (-1 pads)
+    const int* pair_fam;     // class-0 shell pair -> its family pair
+    const int* pair_mem;     // class-0 shell pair -> its member slot
 };
+
+constexpr int FAM_MAXMEM = 4;
+// class combinations (bra X | ket Y) run with s-s family kets: Y = (ss) and
+// X = (ss) or (pp) whenever the basis has families; X = (ps) or (ds) only when
+// the family members are mostly significant together (`wide`), since those
+// bra classes lose to the per-member bookkeeping when screening leaves about
+// one member per family.  All four have run_fam (csrc/gen_eri.py FAM_BRA).
+constexpr bool fam_combo(int X, int Y, bool wide) {
+    return Y == 0 && (X == 0 || X == 2 || (wide && (X == 1 || X == 3)));
+}
 
 
 

@@ -8,7 +8,9 @@
 // prefix of each group: Q_j >= precision / (Q_i max|D|).  Under mixed
 // precision the prefix splits again at max(precision, fp32_below) / (Q_i
 // max|D|) into an FP64 head and an FP32 tail.  With X == Y only kets j <= i
-// are unique quartets (src/fockbuild.cpp:88-109).
+// are unique quartets (src/fockbuild.cpp:88-109).  With s-s family kets the
+// groups hold family pairs, sorted by the largest member Q; the triangle is
+// cut at the bra's own family and finished per member by the kernel.
 //
 //   k_task_count: per bra, the (mid, end) cut of every group (two binary
 //                 searches) and the number of workgroups per precision;
@@ -59,11 +61,12 @@ __global__ __launch_bounds__(256) void k_task_count(JKList c, const double* __re
         const int gb = c.gbeg[g], ge = c.gend[g];
         int end = gb, mid = gb;
         if (any) {
-            end = gb + count_ge_dev(Q, gb, ge, thr);
-            mid = gb + count_ge_dev(Q, gb, ge, thr64);
-            if (c.same) {
-                end = min(end, bra + 1);
-                mid = min(mid, bra + 1);
+            end = gb + count_ge_dev(c.Qk, gb, ge, thr);
+            mid = gb + count_ge_dev(c.Qk, gb, ge, thr64);
+            if (c.same) {   // unique pairs of pairs: kets up to the bra's own position
+                const int lim = (c.bra_fam ? c.bra_fam[bra] : bra) + 1;
+                end = min(end, lim);
+                mid = min(mid, lim);
             }
             end = max(end, gb);
             mid = max(min(mid, end), gb);

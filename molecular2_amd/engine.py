@@ -67,6 +67,8 @@ def lib():
         L.m2_jk_direct.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_uint, _dp, _dp]
         L.m2_jk_direct_device.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint, vp, vp]
         L.m2_set_mixed_threshold.argtypes = [vp, C.c_double]
+        L.m2_set_jk_families.argtypes = [vp, C.c_int]
+        L.m2_set_jk_families.restype = C.c_int
         L.m2_last_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.m2_set_profiling.argtypes = [vp, C.c_int]
         _ll = C.POINTER(C.c_longlong)
@@ -209,6 +211,11 @@ class Engine:
 
     def set_mixed_threshold(self, t):
         self._chk(self.L.m2_set_mixed_threshold(self.h, t))
+
+    def set_jk_families(self, mode):
+        """s-s ket families of the direct J/K build (0 off, 1 automatic, 2 all
+        four class combinations); applies to the next set_basis."""
+        self._chk(self.L.m2_set_jk_families(self.h, int(mode)))
 
     def set_profiling(self, on=True, counters=False):
         """HIP events around every class launch; counters=True also counts

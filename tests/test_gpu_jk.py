@@ -75,3 +75,27 @@ def test_water_scf_energy_gpu_jk(oracle, engine):
     E = scf.run()
     assert scf.converged
     assert abs(E - (-75.990178761701)) < 1e-8
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_jk_family_modes_match_oracle(oracle, engine, mode):
+    """s-s ket families (shared primitive pairs of same-centre s shells with
+    identical exponents) off / automatic / on for every eligible class
+    combination: FP64 J and K equal the oracle's formJKdirect to 1e-10, and
+    the mixed-precision build stays within its usual error."""
+    mol = M.water_cluster(4, seed=4)
+    sh = mol.shells("cc-pvdz")
+    print("case: (H2O)4 jk families mode=%d" % mode, flush=True)
+    ob = oracle.Basis(sh, pure=True)
+    D = _rand_density(ob.nbf, 3)
+    Jo, Ko = ob.jk_direct(D, ob.schwarz(), 1e-12, 1.0)
+    try:
+        engine.set_jk_families(mode)
+        engine.set_basis(sh, pure=True)
+        J, K = engine.jk(D, 1.0, 1e-12)
+        assert np.max(np.abs(J - Jo)) < 1e-10, np.max(np.abs(J - Jo))
+        assert np.max(np.abs(K - Ko)) < 1e-10, np.max(np.abs(K - Ko))
+        Jm, Km = engine.jk(D, 1.0, 1e-12, mixed=True)
+        assert np.max(np.abs(Jm - Jo)) < 1e-7 and np.max(np.abs(Km - Ko)) < 1e-7
+    finally:
+        engine.set_jk_families(1)
