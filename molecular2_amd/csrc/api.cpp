@@ -643,7 +643,9 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
         for (int Y = 0; Y <= X; ++Y, ++ci) {
             if (JK_COMBO[X][Y]) {
-                const int t64 = ctx->h_lists[2 * ci], t32 = ctx->h_lists[2 * ci + 1];
+                // combinations without bra pairs launch no list kernels: their totals are not written
+                const bool any = ctx->lists[ci].nbra > 0;
+                const int t64 = any ? ctx->h_lists[2 * ci] : 0, t32 = any ? ctx->h_lists[2 * ci + 1] : 0;
                 ARGCHK(t64 <= ctx->lists[ci].cap && t32 <= ctx->lists[ci].cap, "m2_jk_direct: work list overflow");
                 units += t64 + t32;
                 if (t64 + t32 > 0) plan.push_back(Launch{X, Y, ci, -1});

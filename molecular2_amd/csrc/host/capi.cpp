@@ -7,10 +7,13 @@
 #include <exception>
 #include <mutex>
 #include <numeric>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include "inputs.hpp"
+#include "m2_input.h"
 #include "m2_scf.h"
 #include "scf_host.hpp"
 
@@ -285,6 +288,183 @@ int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int
         return 1;
     }
     return 0;
+}
+
+static void copy_shells(const std::vector<molecular::RawShell>& sh, m2_shells_out* o) {
+    if (!o || !o->l) return;
+    long p = 0;
+    for (size_t s = 0; s < sh.size(); ++s) {
+        o->l[s] = sh[s].l;
+        if (o->nprim) o->nprim[s] = (int)sh[s].exps.size();
+        if (o->atoms) o->atoms[s] = sh[s].atom;
+        for (int k = 0; k < 3; ++k)
+            if (o->centers) o->centers[3 * s + k] = sh[s].O[k];
+        for (size_t i = 0; i < sh[s].exps.size(); ++i, ++p) {
+            if (o->exps) o->exps[p] = sh[s].exps[i];
+            if (o->coefs) o->coefs[p] = sh[s].coefs[i];
+        }
+    }
+}
+
+static int nprim_of(const std::vector<molecular::RawShell>& sh) {
+    int n = 0;
+    for (const auto& s : sh) n += (int)s.exps.size();
+    return n;
+}
+
+// The molecule construct number `imol` of a deck, with its orbital, jkbasis
+// and minimal (STO-3G, for the SOAD guess) shells read from the library.
+struct DeckData {
+    molecular::DeckMolecule mol;
+    std::vector<molecular::RawShell> orb, aux, min;
+};
+
+static DeckData load_deck_molecule(const std::string& text, int imol, const std::string& lib) {
+    using namespace molecular;
+    const Deck deck = parse_deck(text);
+    if (imol < 0 || imol >= (int)deck.constructs.size())
+        throw std::runtime_error("INPUT: the deck has no molecule construct " + std::to_string(imol));
+    DeckData d;
+    d.mol = deck_molecule(deck.constructs[imol]);
+    d.orb = library_shells(lib, "basis", d.mol.atoms, d.mol.basis);
+    if (!d.mol.jkbasis.empty()) d.aux = library_shells(lib, "jkfit", d.mol.atoms, d.mol.jkbasis);
+    d.min = library_shells(lib, "basis", d.mol.atoms, std::map<int, std::string>{{0, "sto-3g"}});
+    return d;
+}
+
+int m2_deck_molecule(const char* deck_text, int imol, const char* basislib, m2_deck_info* info, double* Z,
+                     double* xyz, m2_shells_out* orb, m2_shells_out* aux, m2_shells_out* minimal) {
+    g_rhf_error.clear();
+    try {
+        if (!deck_text || !basislib || !info) throw std::runtime_error("m2_deck_molecule: null argument");
+        const DeckData d = load_deck_molecule(deck_text, imol, basislib);
+        info->natom = (int)d.mol.atoms.size();
+        info->charge = d.mol.charge;
+        info->multiplicity = d.mol.multiplicity;
+        info->nsh = (int)d.orb.size();
+        info->nprim = nprim_of(d.orb);
+        info->nsh_aux = (int)d.aux.size();
+        info->nprim_aux = nprim_of(d.aux);
+        info->nsh_min = (int)d.min.size();
+        info->nprim_min = nprim_of(d.min);
+        for (size_t a = 0; a < d.mol.atoms.size(); ++a) {
+            if (Z) Z[a] = d.mol.atoms[a].Z;
+            if (xyz) {
+                xyz[3 * a] = d.mol.atoms[a].x;
+                xyz[3 * a + 1] = d.mol.atoms[a].y;
+                xyz[3 * a + 2] = d.mol.atoms[a].z;
+            }
+        }
+        copy_shells(d.orb, orb);
+        copy_shells(d.aux, aux);
+        copy_shells(d.min, minimal);
+        return 0;
+    } catch (const std::exception& e) {
+        g_rhf_error = e.what();
+        return 1;
+    }
+}
+
+int m2_deck_run(const char* deck_text, const char* basislib, int device, int pure, m2_deck_result* res, int maxres,
+                int* nres) {
+    using namespace molecular;
+    g_rhf_error.clear();
+    m2_ctx* ctx = nullptr;
+    try {
+        if (!deck_text || !basislib || !nres) throw std::runtime_error("m2_deck_run: null argument");
+        *nres = 0;
+        const Deck deck = parse_deck(deck_text);
+        if (m2_ctx_create(&ctx, device) != 0 || !ctx)
+            throw std::runtime_error(std::string("m2_deck_run: ") + (ctx ? m2_last_error(ctx) : "no context"));
+        for (size_t ic = 0; ic < deck.commands.size(); ++ic) {
+            const DeckCommand& cmd = deck.commands[ic];
+            m2_deck_result r{};
+            r.command = (int)ic;
+            r.molecule = cmd.molecule;
+            r.method = -1;
+            if ((cmd.name == "hf" || cmd.name == "rhf" || cmd.name == "uhf") && cmd.molecule >= 0) {
+                const DeckData d = load_deck_molecule(deck_text, cmd.molecule, basislib);
+                int nel = -d.mol.charge;
+                for (const Atom& a : d.mol.atoms) nel += a.Z;
+                // ProgramController::call_hf (src/ProgramController.cpp:288-293)
+                const bool uhf = cmd.name == "uhf" || (cmd.name == "hf" && (d.mol.multiplicity > 1 || nel % 2 != 0));
+                // call_rhf / call_uhf defaults (src/ProgramController.cpp:295-326)
+                m2_scf_options_c o{};
+                o.diis = 1;
+                o.maxdiis = 8;
+                o.maxiter = 40;
+                o.enconverge = 1e-7;
+                o.densconverge = 1e-4;
+                o.precision = 1e-12;
+                o.df = 0;
+                o.mixed = 0;
+                o.mixed_threshold = 1e-4;
+                o.fp64_final = 2;
+                o.guess = 1;   // soad
+                o.diis_start = 2;
+                if (const std::string* v = cmd.find("diis")) o.diis = std::stoi(*v) != 0;
+                if (const std::string* v = cmd.find("maxdiis")) o.maxdiis = std::stoi(*v);
+                if (const std::string* v = cmd.find("maxiter")) o.maxiter = std::stoi(*v);
+                if (const std::string* v = cmd.find("enconverge")) o.enconverge = std::stod(*v);
+                if (const std::string* v = cmd.find("densconverge")) o.densconverge = std::stod(*v);
+                if (const std::string* v = cmd.find("precision")) o.precision = std::stod(*v);
+                if (const std::string* v = cmd.find("df")) o.df = std::stoi(*v) != 0;
+                if (const std::string* v = cmd.find("guess")) o.guess = *v == "soad" ? 1 : 0;
+                if (o.df && d.aux.empty()) throw std::runtime_error("m2_deck_run: df needs a jkbasis construct");
+                std::vector<double> Z, xyz;
+                for (const Atom& a : d.mol.atoms) {
+                    Z.push_back(a.Z);
+                    xyz.insert(xyz.end(), {a.x, a.y, a.z});
+                }
+                struct Flat {
+                    std::vector<int> l, np, at;
+                    std::vector<double> ex, cf, cen;
+                };
+                auto flat = [](const std::vector<RawShell>& sh) {
+                    Flat f;
+                    for (const RawShell& s : sh) {
+                        f.l.push_back(s.l);
+                        f.np.push_back((int)s.exps.size());
+                        f.at.push_back(s.atom);
+                        f.ex.insert(f.ex.end(), s.exps.begin(), s.exps.end());
+                        f.cf.insert(f.cf.end(), s.coefs.begin(), s.coefs.end());
+                        f.cen.insert(f.cen.end(), {s.O[0], s.O[1], s.O[2]});
+                    }
+                    return f;
+                };
+                const Flat fo = flat(d.orb), fa = flat(d.aux), fm = flat(d.min);
+                m2_scf_result_c sr{};
+                const int na = o.df ? (int)d.aux.size() : 0, nm = o.guess == 1 ? (int)d.min.size() : 0;
+                const int st =
+                    uhf ? m2_uhf(ctx, (int)Z.size(), Z.data(), xyz.data(), d.mol.charge, d.mol.multiplicity, pure,
+                                 (int)d.orb.size(), fo.l.data(), fo.np.data(), fo.ex.data(), fo.cf.data(),
+                                 fo.cen.data(), fo.at.data(), na, fa.l.data(), fa.np.data(), fa.ex.data(),
+                                 fa.cf.data(), fa.cen.data(), fa.at.data(), nm, fm.l.data(), fm.np.data(),
+                                 fm.ex.data(), fm.cf.data(), fm.cen.data(), fm.at.data(), &o, &sr, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr)
+                        : m2_rhf(ctx, (int)Z.size(), Z.data(), xyz.data(), d.mol.charge, d.mol.multiplicity, pure,
+                                 (int)d.orb.size(), fo.l.data(), fo.np.data(), fo.ex.data(), fo.cf.data(),
+                                 fo.cen.data(), fo.at.data(), na, fa.l.data(), fa.np.data(), fa.ex.data(),
+                                 fa.cf.data(), fa.cen.data(), fa.at.data(), nm, fm.l.data(), fm.np.data(),
+                                 fm.ex.data(), fm.cf.data(), fm.cen.data(), fm.at.data(), &o, &sr, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr);
+                if (st != 0) throw std::runtime_error("m2_deck_run: command " + cmd.name + ": " + g_rhf_error);
+                r.method = uhf ? 1 : 0;
+                r.energy = sr.energy;
+                r.iterations = sr.iterations;
+                r.converged = sr.converged;
+                r.seconds = sr.total_seconds;
+            }
+            if (res && *nres < maxres) res[*nres] = r;
+            ++*nres;
+        }
+        m2_ctx_destroy(ctx);
+        return 0;
+    } catch (const std::exception& e) {
+        if (ctx) m2_ctx_destroy(ctx);
+        g_rhf_error = e.what();
+        return 1;
+    }
 }
 
 }  // extern "C"

@@ -36,6 +36,23 @@ class SCFResult(C.Structure):
                 ("jk_seconds", C.c_double), ("total_seconds", C.c_double), ("grad_seconds", C.c_double)]
 
 
+class DeckInfo(C.Structure):
+    _fields_ = [("natom", C.c_int), ("charge", C.c_int), ("multiplicity", C.c_int), ("nsh", C.c_int),
+                ("nprim", C.c_int), ("nsh_aux", C.c_int), ("nprim_aux", C.c_int), ("nsh_min", C.c_int),
+                ("nprim_min", C.c_int)]
+
+
+class ShellsOut(C.Structure):
+    _fields_ = [("l", C.POINTER(C.c_int)), ("nprim", C.POINTER(C.c_int)), ("exps", C.POINTER(C.c_double)),
+                ("coefs", C.POINTER(C.c_double)), ("centers", C.POINTER(C.c_double)),
+                ("atoms", C.POINTER(C.c_int))]
+
+
+class DeckResult(C.Structure):
+    _fields_ = [("command", C.c_int), ("molecule", C.c_int), ("method", C.c_int), ("energy", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int), ("seconds", C.c_double)]
+
+
 class M2Error(RuntimeError):
     """Mirrors the reference's Error(code, msg) (inc/error.hpp:18-35)."""
 
@@ -102,6 +119,12 @@ def lib():
         L.m2_rhf_batch.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp,
                                    _dp, _ip, C.POINTER(SCFOptions), C.POINTER(SCFResult)]
         L.m2_rhf_batch.restype = C.c_int
+        L.m2_deck_molecule.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.POINTER(DeckInfo), _dp, _dp,
+                                       C.POINTER(ShellsOut), C.POINTER(ShellsOut), C.POINTER(ShellsOut)]
+        L.m2_deck_molecule.restype = C.c_int
+        L.m2_deck_run.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(DeckResult), C.c_int,
+                                  C.POINTER(C.c_int)]
+        L.m2_deck_run.restype = C.c_int
         L.m2_ctx_device.argtypes = [vp]
         L.m2_ctx_device.restype = C.c_int
         L.m2_grad.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
@@ -127,6 +150,47 @@ def shell_arrays(shells):
     cen = np.ascontiguousarray(np.array([s.center for s in shells], float).reshape(-1))
     at = np.array([s.atom for s in shells], np.int32)
     return l, npr, exps, coefs, cen, at
+
+
+def _shells_buffers(nsh, nprim):
+    b = {"l": np.zeros(nsh, np.int32), "nprim": np.zeros(nsh, np.int32), "exps": np.zeros(max(nprim, 1)),
+         "coefs": np.zeros(max(nprim, 1)), "centers": np.zeros(max(3 * nsh, 1)), "atoms": np.zeros(nsh, np.int32)}
+    o = ShellsOut(b["l"].ctypes.data_as(_ip), b["nprim"].ctypes.data_as(_ip), _p(b["exps"]), _p(b["coefs"]),
+                  _p(b["centers"]), b["atoms"].ctypes.data_as(_ip))
+    return b, o
+
+
+def _shells_from(b):
+    from .molecules import Shell
+    out, p = [], 0
+    for s in range(len(b["l"])):
+        k = int(b["nprim"][s])
+        out.append(Shell(int(b["l"][s]), list(b["exps"][p:p + k]), list(b["coefs"][p:p + k]),
+                         tuple(b["centers"][3 * s:3 * s + 3]), int(b["atoms"][s])))
+        p += k
+    return out
+
+
+def deck_molecule(deck_text: str, imol: int = 0, basislib: str = ""):
+    """Molecule construct `imol` of a reference input deck (text of a .mol
+    file) with its shells from the XML basis library directory `basislib`
+    (m2_deck_molecule, include/m2_input.h; host code, no GPU).  Returns
+    (Molecule, orbital shells, jkbasis shells, STO-3G shells)."""
+    from .molecules import Molecule
+    L = lib()
+    info = DeckInfo()
+    t, bl = deck_text.encode(), basislib.encode()
+    if L.m2_deck_molecule(t, imol, bl, C.byref(info), None, None, None, None, None) != 0:
+        raise M2Error(1, L.m2_rhf_last_error().decode())
+    Z = np.zeros(info.natom)
+    X = np.zeros(3 * info.natom)
+    bo, oo = _shells_buffers(info.nsh, info.nprim)
+    ba, oa = _shells_buffers(info.nsh_aux, info.nprim_aux)
+    bm, om = _shells_buffers(info.nsh_min, info.nprim_min)
+    if L.m2_deck_molecule(t, imol, bl, C.byref(info), _p(Z), _p(X), C.byref(oo), C.byref(oa), C.byref(om)) != 0:
+        raise M2Error(1, L.m2_rhf_last_error().decode())
+    mol = Molecule([int(z) for z in Z], X.reshape(-1, 3), charge=info.charge, multiplicity=info.multiplicity)
+    return mol, _shells_from(bo), _shells_from(ba), _shells_from(bm)
 
 
 class Engine:
@@ -358,6 +422,23 @@ class Engine:
                 "n_full": r.n_full, "n_incremental": r.n_incremental, "n_fp32": r.n_fp32,
                 "jk_seconds": r.jk_seconds, "total_seconds": r.total_seconds} for r in res]
         return out, wall
+
+    def run_deck(self, deck_text, basislib, pure=False, maxres=64):
+        """Run the hf / rhf / uhf commands of a reference input deck on this
+        engine's device (m2_deck_run).  pure=False: Cartesian d functions, as
+        the reference's recorded outputs.  One dict per deck command: command,
+        molecule, method ("rhf", "uhf" or None when not an SCF command),
+        energy, iterations, converged, seconds."""
+        res = (DeckResult * maxres)()
+        n = C.c_int(0)
+        st = self.L.m2_deck_run(deck_text.encode(), basislib.encode(), self.L.m2_ctx_device(self.h), int(pure), res,
+                                maxres, C.byref(n))
+        if st != 0:
+            raise M2Error(st, self.L.m2_rhf_last_error().decode())
+        names = {0: "rhf", 1: "uhf", -1: None}
+        return [{"command": r.command, "molecule": r.molecule, "method": names[r.method], "energy": r.energy,
+                 "iterations": r.iterations, "converged": bool(r.converged), "seconds": r.seconds}
+                for r in res[:min(n.value, maxres)]]
 
     def jk_uhf(self, Da, Db, multiplier=1.0, precision=1e-12, mixed=False):
         """UnrestrictedFock::formJKdirect: (Ja, Ka, Jb, Kb), screened with
