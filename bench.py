@@ -128,6 +128,27 @@ def df_bench(eng, mol, Cocc, n, steps, torch, dev):
     return res
 
 
+def batch_bench(eng, nmol=8):
+    """Batched SCF (m2_rhf_batch) on the first `nmol` molecules of the seeded
+    256-molecule H/C/N/O batch (3-30 atoms, cc-pVDZ), FP64 and mixed;
+    molecules/s counts every molecule of the sample,
+    converged or not.  maxiter is raised to 100 (the random clusters need
+    50-100 iterations); the other options are the reference's defaults."""
+    from molecular2_amd import molecules as M
+    mols = M.random_batch(256, seed=256)[:nmol]
+    r64, w64 = eng.rhf_batch(mols, nworkers=8, maxiter=100)
+    rmx, wmx = eng.rhf_batch(mols, nworkers=8, maxiter=100, mixed=1)
+    both = [(a["energy"], b["energy"]) for a, b in zip(r64, rmx) if a["converged"] and b["converged"]]
+    dE = max((abs(x - y) for x, y in both), default=None)
+    return {"sample": "first %d of random_batch(256, seed=256): %s atoms" % (nmol, [m.natom for m in mols]),
+            "workers": 8,
+            "fp64": {"molecules_per_s": round(nmol / w64, 3), "wall_s": round(w64, 2),
+                     "converged": sum(r["converged"] for r in r64), "iterations": sum(r["iterations"] for r in r64)},
+            "mixed": {"molecules_per_s": round(nmol / wmx, 3), "wall_s": round(wmx, 2),
+                      "converged": sum(r["converged"] for r in rmx), "iterations": sum(r["iterations"] for r in rmx)},
+            "max_abs_dE_Eh_converged": dE}
+
+
 def traffic_for(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary."""
     try:
@@ -149,6 +170,7 @@ def main():
     ap.add_argument("--no-scf", action="store_true",
                     help="skip the SCF/gradient runs (J/K timing at the core-guess density)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-SCF sample")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (HBM byte-counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -302,6 +324,8 @@ def main():
     if out_scf is not None:
         out["scf"] = out_scf
         out["df"] = df_bench(eng, mol, r64["C"][:, :mol.nel // 2], n, a.steps, torch, dev)
+    if not a.no_batch and not a.no_scf:
+        out["batch"] = batch_bench(eng)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mol, D, tot_primq, a.cpu_budget)
     print(json.dumps(out), flush=True)

@@ -377,9 +377,9 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64)) void k_jk_fam(
             nact += act[m] ? 1 : 0;
         }
         if (!__any(active)) continue;
-        if (a.counters) {
+        if (a.counters) {   // algorithmic counts: every member quartet with its own primitive quartets
             cnt_q += (unsigned long long)(wave_sum(double(nact)) + 0.5);
-            cnt_pq += (unsigned long long)(wave_sum(active ? double(a.fam_np[ket]) : 0.0) * nbp + 0.5);
+            cnt_pq += (unsigned long long)(wave_sum(double(nact) * a.fam_np[ket]) * nbp + 0.5);
         }
         R v[FAM_MAXMEM * NCART];
         const int nkp = active ? a.fam_np[ket] : 0;
