@@ -928,6 +928,49 @@ def write_programs(outdir, classes=None):
             lines.append("  0")
         lines.append("};")
 
+    # VRR instruction words of the heavy J/K kernel in pass order: for every
+    # VRR level ceil(n / 64) passes; in a pass lane j executes instruction
+    # 64 q + j of the level.  Per pass and lane 7 ints, stored [pass][word]
+    # [lane] so one pass's words load coalesced: header dst | nterm << 16 (-1:
+    # idle lane), then per term src | sid << 13 | (coef & 63) << 18 (coef a
+    # small signed integer).  The VRR part of a class's program does not
+    # depend on pure / Cartesian output.
+    vdesc, vwords, vend = [], [], []
+    for (la, lb, lc, ld) in classes:
+        P0 = gen_program(la, lb, lc, ld, 0)
+        P1 = gen_program(la, lb, lc, ld, 1)
+        assert P0["vrr_levels"] == P1["vrr_levels"] and P0["base"] == P1["base"]
+        passes, ends = [], []
+        for lvl in P0["vrr_levels"]:
+            npass = (len(lvl) + 63) // 64
+            for q in range(npass):
+                w = [[-1] * 64] + [[0] * 64 for _ in range(MAXTERMS)]
+                for j in range(64):
+                    k = 64 * q + j
+                    if k >= len(lvl):
+                        continue
+                    dst, t = lvl[k]
+                    assert dst < 65536 and len(t) <= MAXTERMS
+                    w[0][j] = dst | (len(t) << 16)
+                    for ti, (src, sid, c) in enumerate(t):
+                        ci = int(round(c))
+                        assert abs(ci - c) < 1e-12 and -32 <= ci < 32 and src < 8192 and sid < 32
+                        w[1 + ti][j] = src | (sid << 13) | ((ci & 63) << 18)
+                passes.append(w)
+                ends.append(q == npass - 1)
+        assert len(passes) <= 64
+        vdesc.append((la, lb, lc, ld, len(passes), len(vwords)))
+        for w in passes:
+            for row in w:
+                vwords += row
+        vend.append(sum(1 << i for i, e in enumerate(ends) if e))
+    lines.append("constexpr int VRRW_NCLASS = %d;" % len(vdesc))
+    lines.append("// per class: la lb lc ld npass word_offset")
+    lines.append("constexpr int VRRW_DESC[%d] = {%s};" % (6 * len(vdesc), ", ".join(str(x) for d in vdesc for x in d)))
+    lines.append("// per class: bit p set when VRR pass p ends a level")
+    lines.append("constexpr unsigned long long VRRW_LEVEND[%d] = {%s};" % (
+        len(vend), ", ".join("%dull" % e for e in vend)))
+    arr("VRRW_WORDS", "int", vwords)
     arr("PROG_INS", "int", ins_i)
     arr("PROG_COEF", "double", [float(x) for x in ins_c], 8)
     arr("PROG_LEV", "int", lev)

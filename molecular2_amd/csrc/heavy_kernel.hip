@@ -32,7 +32,46 @@ struct HeavyProg {
     int na, nb, nc, nd;
     const int* ins;      // 14 ints per instruction
     const double* coef;  // PROG_MAXTERMS per instruction
+    const int* vrrw;     // VRR instruction words in pass order (VRRW_WORDS of the class)
+    int nvpass;          // VRR passes
+    unsigned long long levend;   // bit p: pass p ends a VRR level
 };
+
+// One primitive quartet's VRR from the pass-ordered instruction words: in
+// pass p lane j computes one [e0|f0]^(m) slot from <= 6 terms coef * S[sid]
+// * X[src].  The next pass's words load while this pass computes, so the
+// table fetch overlaps the LDS work; a barrier ends every level.
+template <typename R>
+__device__ __forceinline__ void vrr_passes(const HeavyProg& P, R* __restrict__ X, const R* __restrict__ S) {
+    constexpr int NW = 1 + PROG_MAXTERMS;
+    const int lane = threadIdx.x;
+    const int* wp = P.vrrw + lane;
+    int cur[NW], nxt[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) cur[k] = wp[k * 64];
+    for (int p = 0; p < P.nvpass; ++p) {
+        if (p + 1 < P.nvpass) {
+#pragma unroll
+            for (int k = 0; k < NW; ++k) nxt[k] = wp[((p + 1) * NW + k) * 64];
+        }
+        const int h = cur[0];
+        if (h >= 0) {
+            const int nt = h >> 16;
+            R v = R(0);
+#pragma unroll
+            for (int t = 0; t < PROG_MAXTERMS; ++t)
+                if (t < nt) {
+                    const int tw = cur[1 + t];
+                    const int c6 = (tw >> 18) & 63;
+                    v += R(c6 >= 32 ? c6 - 64 : c6) * S[(tw >> 13) & 31] * X[tw & 8191];
+                }
+            X[h & 65535] = v;
+        }
+        if ((P.levend >> p) & 1ull) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NW; ++k) cur[k] = nxt[k];
+    }
+}
 
 template <typename R>
 __device__ __forceinline__ R boys_one(double T, int m, const BoysTables& tab);
@@ -124,9 +163,13 @@ __device__ void heavy_quartet(const HeavyProg& P, const JKArgs& a, const BoysTab
                 S[21] = R(CDx); S[22] = R(CDy); S[23] = R(CDz);
             }
             __syncthreads();
-            for (int l = 0; l < P.nvlev; ++l) {
-                run_level<R>(P, P.vlev[l], P.vlev[l + 1], X, S);
-                __syncthreads();
+            if (P.vrrw) {
+                vrr_passes<R>(P, X, S);
+            } else {
+                for (int l = 0; l < P.nvlev; ++l) {
+                    run_level<R>(P, P.vlev[l], P.vlev[l + 1], X, S);
+                    __syncthreads();
+                }
             }
             for (int t = lane; t < P.nacc; t += 64) X[P.acc[2 * t]] += X[P.acc[2 * t + 1]];
         }
@@ -283,6 +326,7 @@ __global__ __launch_bounds__(64) void k_schwarz_heavy(JKArgs a, BoysTables tab, 
 // ---------------------------------------------------------------------------
 
 struct HeavyTables {
+    int* vrrw = nullptr;
     int* ins = nullptr;
     double* coef = nullptr;
     int* lev = nullptr;
@@ -304,6 +348,8 @@ hipError_t heavy_upload(int device) {
     if ((e = hipMalloc((void**)&t.lev, sizeof(PROG_LEV)))) return e;
     if ((e = hipMalloc((void**)&t.acc, sizeof(PROG_ACC)))) return e;
     if ((e = hipMalloc((void**)&t.out, sizeof(PROG_OUT)))) return e;
+    if ((e = hipMalloc((void**)&t.vrrw, sizeof(VRRW_WORDS)))) return e;
+    if ((e = hipMemcpy(t.vrrw, VRRW_WORDS, sizeof(VRRW_WORDS), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.ins, PROG_INS, sizeof(PROG_INS), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.coef, PROG_COEF, sizeof(PROG_COEF), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.lev, PROG_LEV, sizeof(PROG_LEV), hipMemcpyHostToDevice))) return e;
@@ -334,6 +380,17 @@ static bool find_prog(int device, int la, int lb, int lc, int ld, int pure, Heav
             P.nd = d[19];
             P.ins = t.ins;
             P.coef = t.coef;
+            P.vrrw = nullptr;
+            P.nvpass = 0;
+            P.levend = 0;
+            for (int v = 0; v < VRRW_NCLASS; ++v) {
+                const int* q = VRRW_DESC + 6 * v;
+                if (q[0] == la && q[1] == lb && q[2] == lc && q[3] == ld) {
+                    P.vrrw = t.vrrw + q[5];
+                    P.nvpass = q[4];
+                    P.levend = VRRW_LEVEND[v];
+                }
+            }
             return true;
         }
     }
