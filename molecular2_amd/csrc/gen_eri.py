@@ -964,6 +964,51 @@ def write_programs(outdir, classes=None):
             for row in w:
                 vwords += row
         vend.append(sum(1 << i for i, e in enumerate(ends) if e))
+    # Post (HRR, solid-harmonic) instruction words in pass order, per class
+    # and output kind, laid out as the VRR words.  A post term is either
+    # coef * X[src] (solid-harmonic transform, scalar "one") or S[AB/CD] *
+    # X[src] with coefficient 1 (HRR), so a term word is src | sid << 13 with
+    # the program's non-unit coefficients as extra scalars S[24 + i]
+    # (POSTW_COEF, <= 8 per program); the passes that end a level are listed
+    # in POSTW_END.
+    pdesc, pwords, pcoef, pend = [], [], [], []
+    for (la, lb, lc, ld) in classes:
+        for pure in (0, 1):
+            P = gen_program(la, lb, lc, ld, pure)
+            coefs = []
+            word_off, coef_off, end_off = len(pwords), len(pcoef), len(pend)
+            npass = 0
+            for lvl in P["post_levels"]:
+                nq = (len(lvl) + 63) // 64
+                for q in range(nq):
+                    w = [[-1] * 64] + [[0] * 64 for _ in range(MAXTERMS)]
+                    for j in range(64):
+                        k = 64 * q + j
+                        if k >= len(lvl):
+                            continue
+                        dst, t = lvl[k]
+                        w[0][j] = dst | (len(t) << 16)
+                        for ti, (src, sid, c) in enumerate(t):
+                            if c != 1.0:
+                                assert sid == SID["one"]
+                                if c not in coefs:
+                                    coefs.append(c)
+                                sid = NSCALAR + coefs.index(c)
+                            assert src < 8192 and sid < NSCALAR + 8
+                            w[1 + ti][j] = src | (sid << 13)
+                    for row in w:
+                        pwords += row
+                    pend.append(1 if q == nq - 1 else 0)
+                    npass += 1
+            assert len(coefs) <= 8
+            pcoef += coefs
+            pdesc.append((la, lb, lc, ld, pure, npass, word_off, coef_off, len(coefs), end_off))
+    lines.append("constexpr int POSTW_NPROG = %d;" % len(pdesc))
+    lines.append("// per program: la lb lc ld pure npass word_offset coef_offset ncoef end_offset")
+    lines.append("constexpr int POSTW_DESC[%d] = {%s};" % (10 * len(pdesc), ", ".join(str(x) for d in pdesc for x in d)))
+    arr("POSTW_WORDS", "int", pwords)
+    arr("POSTW_COEF", "double", [float(c) for c in pcoef], 8)
+    arr("POSTW_END", "int", pend)
     lines.append("constexpr int VRRW_NCLASS = %d;" % len(vdesc))
     lines.append("// per class: la lb lc ld npass word_offset")
     lines.append("constexpr int VRRW_DESC[%d] = {%s};" % (6 * len(vdesc), ", ".join(str(x) for d in vdesc for x in d)))

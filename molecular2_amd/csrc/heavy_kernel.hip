@@ -20,6 +20,9 @@
 namespace m2 {
 
 constexpr int NSCALAR = 24;
+// LDS scalar slots: the NSCALAR per-primitive scalars, then up to 8 constant
+// coefficients of the pass-ordered post program (POSTW_COEF)
+constexpr int NSCALAR_LDS = NSCALAR + 8;
 
 struct HeavyProg {
     int L, nslots, base, nacc;
@@ -35,7 +38,49 @@ struct HeavyProg {
     const int* vrrw;     // VRR instruction words in pass order (VRRW_WORDS of the class)
     int nvpass;          // VRR passes
     unsigned long long levend;   // bit p: pass p ends a VRR level
+    const int* postw;    // post (HRR, transform) words in pass order (POSTW_WORDS)
+    const int* pend;     // per post pass: 1 when it ends a level
+    int npp;             // post passes
+    const double* pcoef; // the program's non-unit coefficients (S[NSCALAR + i])
+    int ncoef;
 };
+
+// The HRR / solid-harmonic levels from their pass-ordered words, next pass
+// prefetched as in vrr_passes; a term is S[sid] * X[src] (coefficients live
+// in S[NSCALAR + i]).
+template <typename R>
+__device__ __forceinline__ void post_passes(const HeavyProg& P, R* __restrict__ X, const R* __restrict__ S) {
+    constexpr int NW = 1 + PROG_MAXTERMS;
+    const int lane = threadIdx.x;
+    const int* wp = P.postw + lane;
+    int cur[NW], nxt[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) cur[k] = wp[k * 64];
+    for (int p = 0; p < P.npp; ++p) {
+        if (p + 1 < P.npp) {
+#pragma unroll
+            for (int k = 0; k < NW; ++k) nxt[k] = wp[((p + 1) * NW + k) * 64];
+        }
+        const int h = cur[0];
+        if (h >= 0) {
+            const int nt = h >> 16;
+            R v = R(0);
+#pragma unroll
+            for (int t = 0; t < PROG_MAXTERMS; ++t)
+                if (t < nt) v += S[cur[1 + t] >> 13] * X[cur[1 + t] & 8191];
+            X[h & 65535] = v;
+        }
+        if (P.pend[p]) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NW; ++k) cur[k] = nxt[k];
+    }
+}
+
+// the post program's coefficients into S[NSCALAR ..] (once per workgroup)
+template <typename R>
+__device__ __forceinline__ void post_coefs(const HeavyProg& P, R* __restrict__ S) {
+    if ((int)threadIdx.x < P.ncoef) S[NSCALAR + threadIdx.x] = R(P.pcoef[threadIdx.x]);
+}
 
 // One primitive quartet's VRR from the pass-ordered instruction words: in
 // pass p lane j computes one [e0|f0]^(m) slot from <= 6 terms coef * S[sid]
@@ -175,9 +220,13 @@ __device__ void heavy_quartet(const HeavyProg& P, const JKArgs& a, const BoysTab
         }
     }
     __syncthreads();
-    for (int l = 0; l < P.nplev; ++l) {
-        run_level<R>(P, P.plev[l], P.plev[l + 1], X, S);
-        __syncthreads();
+    if (P.postw) {
+        post_passes<R>(P, X, S);
+    } else {
+        for (int l = 0; l < P.nplev; ++l) {
+            run_level<R>(P, P.plev[l], P.plev[l + 1], X, S);
+            __syncthreads();
+        }
     }
 }
 
@@ -186,12 +235,12 @@ __device__ __forceinline__ R outv(const HeavyProg& P, const R* X, int a_, int b_
     return X[P.out[((a_ * P.nb + b_) * P.nc + c_) * P.nd + d_]];
 }
 
-// LDS layout of k_jk_heavy: X[nslots] and S[NSCALAR] (type R), then the
+// LDS layout of k_jk_heavy: X[nslots] and S[NSCALAR_LDS] (type R), then the
 // quartet's outputs Y[NA NB NC ND] in (a,b,c,d) order and the six density
 // blocks it is digested with (FP64), 8-byte aligned.
 template <typename R>
 __host__ __device__ constexpr size_t heavy_y_offset(int nslots) {
-    return ((sizeof(R) * (nslots + NSCALAR) + 7) / 8) * 8;
+    return ((sizeof(R) * (nslots + NSCALAR_LDS) + 7) / 8) * 8;
 }
 
 template <typename R, int NA, int NB, int NC, int ND>
@@ -220,6 +269,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
     const int fa = a.sh_bf[sa], fb = a.sh_bf[sb];
     const double* D = a.D;
     for (int i = lane; i < NA * NB; i += 64) Dab[i] = D[(fa + i / NB) * n + fb + i % NB];
+    post_coefs<R>(P, S);
     for (int kk = 0; kk < tk.nket; ++kk) {
         const int ket = tk.ket0 + kk;
         const int2 scd = a.pair_sh[ket];
@@ -312,6 +362,7 @@ __global__ __launch_bounds__(64) void k_schwarz_heavy(JKArgs a, BoysTables tab, 
     const int2 s = a.pair_sh[pr];
     const double* O = a.sh_O;
     const double ABx = O[3 * s.x] - O[3 * s.y], ABy = O[3 * s.x + 1] - O[3 * s.y + 1], ABz = O[3 * s.x + 2] - O[3 * s.y + 2];
+    post_coefs<double>(P, S);
     heavy_quartet<double>(P, a, tab, pr, pr, ABx, ABy, ABz, ABx, ABy, ABz, X, S);
     if (threadIdx.x == 0) {
         double mx = 0.0;
@@ -327,6 +378,9 @@ __global__ __launch_bounds__(64) void k_schwarz_heavy(JKArgs a, BoysTables tab, 
 
 struct HeavyTables {
     int* vrrw = nullptr;
+    int* postw = nullptr;
+    int* pend = nullptr;
+    double* pcoef = nullptr;
     int* ins = nullptr;
     double* coef = nullptr;
     int* lev = nullptr;
@@ -349,6 +403,12 @@ hipError_t heavy_upload(int device) {
     if ((e = hipMalloc((void**)&t.acc, sizeof(PROG_ACC)))) return e;
     if ((e = hipMalloc((void**)&t.out, sizeof(PROG_OUT)))) return e;
     if ((e = hipMalloc((void**)&t.vrrw, sizeof(VRRW_WORDS)))) return e;
+    if ((e = hipMalloc((void**)&t.postw, sizeof(POSTW_WORDS)))) return e;
+    if ((e = hipMalloc((void**)&t.pend, sizeof(POSTW_END)))) return e;
+    if ((e = hipMalloc((void**)&t.pcoef, sizeof(POSTW_COEF)))) return e;
+    if ((e = hipMemcpy(t.postw, POSTW_WORDS, sizeof(POSTW_WORDS), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.pend, POSTW_END, sizeof(POSTW_END), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.pcoef, POSTW_COEF, sizeof(POSTW_COEF), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.vrrw, VRRW_WORDS, sizeof(VRRW_WORDS), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.ins, PROG_INS, sizeof(PROG_INS), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.coef, PROG_COEF, sizeof(PROG_COEF), hipMemcpyHostToDevice))) return e;
@@ -383,6 +443,21 @@ static bool find_prog(int device, int la, int lb, int lc, int ld, int pure, Heav
             P.vrrw = nullptr;
             P.nvpass = 0;
             P.levend = 0;
+            P.postw = nullptr;
+            P.pend = nullptr;
+            P.npp = 0;
+            P.pcoef = nullptr;
+            P.ncoef = 0;
+            for (int v = 0; v < POSTW_NPROG; ++v) {
+                const int* q = POSTW_DESC + 10 * v;
+                if (q[0] == la && q[1] == lb && q[2] == lc && q[3] == ld && q[4] == pure) {
+                    P.npp = q[5];
+                    P.postw = t.postw + q[6];
+                    P.pcoef = t.pcoef + q[7];
+                    P.ncoef = q[8];
+                    P.pend = t.pend + q[9];
+                }
+            }
             for (int v = 0; v < VRRW_NCLASS; ++v) {
                 const int* q = VRRW_DESC + 6 * v;
                 if (q[0] == la && q[1] == lb && q[2] == lc && q[3] == ld) {
@@ -453,7 +528,7 @@ hipError_t launch_schwarz_heavy(int device, int la, int lb, int pure, const JKAr
     if (np <= 0) return hipSuccess;
     HeavyProg P;
     if (!find_prog(device, la, lb, la, lb, pure, P)) return hipErrorInvalidValue;
-    const size_t sm = sizeof(double) * (P.nslots + NSCALAR);
+    const size_t sm = sizeof(double) * (P.nslots + NSCALAR_LDS);
     hipLaunchKernelGGL(k_schwarz_heavy, dim3(np), dim3(64), sm, st, a, tab, p0, np, P, Q);
     return hipGetLastError();
 }
