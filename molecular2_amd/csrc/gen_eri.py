@@ -1336,6 +1336,51 @@ def write_deriv_programs(outdir):
             lines.append("  0")
         lines.append("};")
 
+    # The VRR phase (run once per primitive quartet) also in pass order with
+    # prefetch, as the J/K heavy VRR: per pass and lane 7 ints.  Instruction
+    # passes: header dst | nterm << 16 (-1: idle), terms src | sid << 14 |
+    # (coef & 63) << 19.  Accumulation passes (X[acc] += S[sid] X[vrr]):
+    # word 0 = acc | sid << 16 (-1: idle), word 1 = vrr slot.  DVRR_KIND per
+    # pass: 0 instructions, 1 accumulation, + 4 when a barrier follows.
+    dwords, dkind, ddesc = [], [], []
+    for X, (la, lb) in enumerate(PAIR_CLASSES):
+        for Y in range(X + 1):
+            lc, ld = PAIR_CLASSES[Y]
+            if (la, lb, lc, ld) in DERIV_REGISTER:
+                continue
+            P = gen_deriv_program(la, lb, lc, ld)
+            first = len(dkind)
+            for lv, al in zip(P["vrr_levels"], P["acc_levels"]):
+                for kind, items in ((0, lv), (1, al)):
+                    nq = (len(items) + 63) // 64
+                    for q in range(nq):
+                        w = [[-1] * 64] + [[0] * 64 for _ in range(MAXTERMS)]
+                        for j in range(64):
+                            k = 64 * q + j
+                            if k >= len(items):
+                                continue
+                            if kind == 0:
+                                dst, t = items[k]
+                                assert dst < 65536 and len(t) <= MAXTERMS
+                                w[0][j] = dst | (len(t) << 16)
+                                for ti, (src, sid, c) in enumerate(t):
+                                    ci = int(round(c))
+                                    assert abs(ci - c) < 1e-12 and -32 <= ci < 32 and src < 16384 and sid < 32
+                                    w[1 + ti][j] = src | (sid << 14) | ((ci & 63) << 19)
+                            else:
+                                acc_, vrr_, sid = items[k]
+                                assert acc_ < 65536 and sid < 32
+                                w[0][j] = acc_ | (sid << 16)
+                                w[1][j] = vrr_
+                        for row in w:
+                            dwords += row
+                        dkind.append(kind + (4 if q == nq - 1 else 0))
+            ddesc.append((la, lb, lc, ld, first, len(dkind) - first))
+    lines.append("constexpr int DVRR_NCLASS = %d;" % len(ddesc))
+    lines.append("// per class: la lb lc ld first_pass npass")
+    lines.append("constexpr int DVRR_DESC[%d] = {%s};" % (6 * len(ddesc), ", ".join(str(x) for d in ddesc for x in d)))
+    arr("DVRR_KIND", "int", dkind)
+    arr("DVRR_WORDS", "int", dwords)
     arr("DPROG_INS", "int", ins_i)
     arr("DPROG_COEF", "double", [float(x) for x in ins_c], 8)
     arr("DPROG_LEV", "int", lev)

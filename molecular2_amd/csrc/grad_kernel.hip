@@ -36,7 +36,49 @@ struct DerivProg {
     const int* contr;      // (slot, gamma index, g index)
     const double* ccoef;
     int la, lb, lc, ld;
+    const int* vw;         // VRR phase in pass order (DVRR_WORDS of the class)
+    const int* vkind;      // per pass: 0 instructions, 1 accumulation, +4 barrier after
+    int nvpass;
 };
+
+// The VRR phase of one primitive quartet from its pass-ordered words, the
+// next pass's words loaded while the current one computes.
+__device__ __forceinline__ void vrr_phase_passes(const DerivProg& P, double* __restrict__ X,
+                                                 const double* __restrict__ S) {
+    constexpr int NW = 7;
+    const int lane = threadIdx.x;
+    const int* wp = P.vw + lane;
+    int cur[NW], nxt[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) cur[k] = wp[k * 64];
+    for (int p = 0; p < P.nvpass; ++p) {
+        if (p + 1 < P.nvpass) {
+#pragma unroll
+            for (int k = 0; k < NW; ++k) nxt[k] = wp[((p + 1) * NW + k) * 64];
+        }
+        const int kind = P.vkind[p];
+        const int h = cur[0];
+        if (h >= 0) {
+            if ((kind & 3) == 0) {
+                const int nt = h >> 16;
+                double v = 0.0;
+#pragma unroll
+                for (int t = 0; t < NW - 1; ++t)
+                    if (t < nt) {
+                        const int tw = cur[1 + t];
+                        const int c6 = (tw >> 19) & 63;
+                        v += double(c6 >= 32 ? c6 - 64 : c6) * S[(tw >> 14) & 31] * X[tw & 16383];
+                    }
+                X[h & 65535] = v;
+            } else {
+                X[h & 65535] += S[h >> 16] * X[cur[1]];
+            }
+        }
+        if (kind & 4) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NW; ++k) cur[k] = nxt[k];
+    }
+}
 
 __device__ __forceinline__ void prog_level(const DerivProg& P, int i0, int i1, double* X, const double* S) {
     for (int j = i0 + (int)threadIdx.x; j < i1; j += 64) {
@@ -157,15 +199,19 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
                     S[24] = 2.0 * B.a; S[25] = 2.0 * (B.p - B.a); S[26] = 2.0 * K.a;
                 }
                 __syncthreads();
-                for (int l = 0; l < P.nvlev; ++l) {
-                    const int* L4 = P.lev + 4 * l;
-                    prog_level(P, L4[0], L4[1], X, S);
-                    __syncthreads();
-                    for (int j = L4[2] + lane; j < L4[3]; j += 64) {
-                        const int* ac = P.acc + 3 * j;
-                        X[ac[0]] += S[ac[2]] * X[ac[1]];
+                if (P.vw) {
+                    vrr_phase_passes(P, X, S);
+                } else {
+                    for (int l = 0; l < P.nvlev; ++l) {
+                        const int* L4 = P.lev + 4 * l;
+                        prog_level(P, L4[0], L4[1], X, S);
+                        __syncthreads();
+                        for (int j = L4[2] + lane; j < L4[3]; j += 64) {
+                            const int* ac = P.acc + 3 * j;
+                            X[ac[0]] += S[ac[2]] * X[ac[1]];
+                        }
+                        __syncthreads();
                     }
-                    __syncthreads();
                 }
             }
         }
@@ -473,6 +519,8 @@ struct DerivTables {
     int* acc = nullptr;
     int* contr = nullptr;
     double* ccoef = nullptr;
+    int* vw = nullptr;
+    int* vkind = nullptr;
 };
 static DerivTables g_dt[64];
 
@@ -489,6 +537,10 @@ hipError_t deriv_upload(int device) {
     if ((e = hipMalloc((void**)&t.acc, sizeof(DPROG_ACC)))) return e;
     if ((e = hipMalloc((void**)&t.contr, sizeof(DPROG_CONTR)))) return e;
     if ((e = hipMalloc((void**)&t.ccoef, sizeof(DPROG_CCOEF)))) return e;
+    if ((e = hipMalloc((void**)&t.vw, sizeof(DVRR_WORDS)))) return e;
+    if ((e = hipMalloc((void**)&t.vkind, sizeof(DVRR_KIND)))) return e;
+    if ((e = hipMemcpy(t.vw, DVRR_WORDS, sizeof(DVRR_WORDS), hipMemcpyHostToDevice))) return e;
+    if ((e = hipMemcpy(t.vkind, DVRR_KIND, sizeof(DVRR_KIND), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.ins, DPROG_INS, sizeof(DPROG_INS), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.coef, DPROG_COEF, sizeof(DPROG_COEF), hipMemcpyHostToDevice))) return e;
     if ((e = hipMemcpy(t.lev, DPROG_LEV, sizeof(DPROG_LEV), hipMemcpyHostToDevice))) return e;
@@ -532,6 +584,17 @@ hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const Grad
         P.acc = T.acc;
         P.contr = T.contr;
         P.ccoef = T.ccoef;
+        P.vw = nullptr;
+        P.vkind = nullptr;
+        P.nvpass = 0;
+        for (int v = 0; v < DVRR_NCLASS; ++v) {
+            const int* q = DVRR_DESC + 6 * v;
+            if (q[0] == la && q[1] == lb && q[2] == lc && q[3] == ld) {
+                P.vw = T.vw + (size_t)q[4] * 7 * 64;
+                P.vkind = T.vkind + q[4];
+                P.nvpass = q[5];
+            }
+        }
         const int ngam = d[14] * d[15] * d[16] * d[17];
         const size_t sm = sizeof(double) * (P.nslots + NSCALAR_D + ngam + 6 * 36);
         hipLaunchKernelGGL(k_grad_prog, dim3(nt), dim3(64), sm, st, a, tab, t, nt, P);
