@@ -136,3 +136,69 @@ def test_missing_basis_raises_basislib():
     mol = M.Molecule([8], np.zeros((1, 3)))
     with pytest.raises(KeyError, match="BASISLIB"):
         mol.shells("nonexistent-basis")
+
+
+def _decode_words(words, npass, nw=7):
+    """[pass][word][lane] int table -> per pass, per lane, the word list."""
+    out = []
+    for p in range(npass):
+        out.append([[words[(p * nw + k) * 64 + j] for k in range(nw)] for j in range(64)])
+    return out
+
+
+def test_heavy_pass_words_encode_the_programs():
+    """The pass-ordered VRR / post words of the heavy J/K classes decode back
+    to exactly the levelled instructions of gen_program (destination, terms,
+    scalar ids, coefficients; one level per run of passes)."""
+    import tempfile
+    import gen_eri as G
+    with tempfile.TemporaryDirectory() as d:
+        G.write_programs(d)
+        text = open(os.path.join(d, "programs.inc")).read()
+
+    def arr(name):
+        m = re.search(r"constexpr (?:int|double|unsigned long long) %s\[\d+\] = \{(.*?)\};" % name, text, re.S)
+        return [float(x.replace("ull", "")) if "." in x or "e" in x else int(x.replace("ull", ""))
+                for x in m.group(1).replace("\n", " ").split(",") if x.strip()]
+    vdesc, vwords, vend = arr("VRRW_DESC"), arr("VRRW_WORDS"), arr("VRRW_LEVEND")
+    pdesc, pwords, pcoef, pend = arr("POSTW_DESC"), arr("POSTW_WORDS"), arr("POSTW_COEF"), arr("POSTW_END")
+    for k in range(len(vdesc) // 6):
+        la, lb, lc, ld, npass, off = vdesc[6 * k:6 * k + 6]
+        P = G.gen_program(la, lb, lc, ld, 1)
+        passes = _decode_words(vwords[off:], npass)
+        got, cur = [], []
+        for p, lanes in enumerate(passes):
+            for w in lanes:
+                if w[0] < 0:
+                    continue
+                dst, nt = w[0] & 0xFFFF, w[0] >> 16
+                terms = []
+                for t in w[1:1 + nt]:
+                    c = (t >> 18) & 63
+                    terms.append((t & 8191, (t >> 13) & 31, float(c - 64 if c >= 32 else c)))
+                cur.append((dst, terms))
+            if (vend[k] >> p) & 1:
+                got.append(cur)
+                cur = []
+        assert got == [[(dst, [tuple(x) for x in t]) for dst, t in lvl] for lvl in P["vrr_levels"]]
+    for k in range(len(pdesc) // 10):
+        la, lb, lc, ld, pure, npass, off, coff, ncoef, eoff = pdesc[10 * k:10 * k + 10]
+        P = G.gen_program(la, lb, lc, ld, pure)
+        passes = _decode_words(pwords[off:], npass)
+        got, cur = [], []
+        for p, lanes in enumerate(passes):
+            for w in lanes:
+                if w[0] < 0:
+                    continue
+                terms = []
+                for t in w[1:1 + (w[0] >> 16)]:
+                    sid = t >> 13
+                    if sid >= G.NSCALAR:
+                        terms.append((t & 8191, G.SID["one"], pcoef[coff + sid - G.NSCALAR]))
+                    else:
+                        terms.append((t & 8191, sid, 1.0))
+                cur.append((w[0] & 0xFFFF, terms))
+            if pend[eoff + p]:
+                got.append(cur)
+                cur = []
+        assert got == [[(dst, [tuple(x) for x in t]) for dst, t in lvl] for lvl in P["post_levels"]]
