@@ -173,12 +173,20 @@ __device__ __forceinline__ void run_level(const HeavyProg& P, int i0, int i1, R*
 
 // computes the quartet (bra pair | ket pair) into X; returns with all lanes
 // synchronised and the outputs at X[P.out[o]]
-template <typename R>
+// NAP > 0: the lane's (accumulator, VRR slot) pairs come in accw (-1: none),
+// loaded once per workgroup; NAP == 0: read from P.acc.
+template <typename R, int NAP>
 __device__ void heavy_quartet(const HeavyProg& P, const JKArgs& a, const BoysTables& tab, int bra, int ket,
                               double ABx, double ABy, double ABz, double CDx, double CDy, double CDz,
-                              R* __restrict__ X, R* __restrict__ S) {
+                              R* __restrict__ X, R* __restrict__ S, const int (&accw)[NAP > 0 ? NAP : 1][2]) {
     const int lane = threadIdx.x;
-    for (int t = lane; t < P.nacc; t += 64) X[P.acc[2 * t]] = R(0);
+    if constexpr (NAP > 0) {
+#pragma unroll
+        for (int q = 0; q < NAP; ++q)
+            if (accw[q][0] >= 0) X[accw[q][0]] = R(0);
+    } else {
+        for (int t = lane; t < P.nacc; t += 64) X[P.acc[2 * t]] = R(0);
+    }
     const PrimPair* bp = a.prims + a.pair_off[bra];
     const PrimPair* kp = a.prims + a.pair_off[ket];
     const int nbp = a.pair_np[bra], nkp = a.pair_np[ket];
@@ -216,7 +224,13 @@ __device__ void heavy_quartet(const HeavyProg& P, const JKArgs& a, const BoysTab
                     __syncthreads();
                 }
             }
-            for (int t = lane; t < P.nacc; t += 64) X[P.acc[2 * t]] += X[P.acc[2 * t + 1]];
+            if constexpr (NAP > 0) {
+#pragma unroll
+                for (int q = 0; q < NAP; ++q)
+                    if (accw[q][0] >= 0) X[accw[q][0]] += X[accw[q][1]];
+            } else {
+                for (int t = lane; t < P.nacc; t += 64) X[P.acc[2 * t]] += X[P.acc[2 * t + 1]];
+            }
         }
     }
     __syncthreads();
@@ -243,7 +257,7 @@ __host__ __device__ constexpr size_t heavy_y_offset(int nslots) {
     return ((sizeof(R) * (nslots + NSCALAR_LDS) + 7) / 8) * 8;
 }
 
-template <typename R, int NA, int NB, int NC, int ND>
+template <typename R, int NA, int NB, int NC, int ND, int NAP>
 __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
                                                  double lo, double hi, HeavyProg P) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -270,6 +284,13 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
     const double* D = a.D;
     for (int i = lane; i < NA * NB; i += 64) Dab[i] = D[(fa + i / NB) * n + fb + i % NB];
     post_coefs<R>(P, S);
+    int accw[NAP][2];   // this lane's accumulation pairs, the same for every primitive quartet
+#pragma unroll
+    for (int q = 0; q < NAP; ++q) {
+        const int t = 64 * q + lane;
+        accw[q][0] = t < P.nacc ? P.acc[2 * t] : -1;
+        accw[q][1] = t < P.nacc ? P.acc[2 * t + 1] : 0;
+    }
     for (int kk = 0; kk < tk.nket; ++kk) {
         const int ket = tk.ket0 + kk;
         const int2 scd = a.pair_sh[ket];
@@ -292,7 +313,7 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
         for (int i = lane; i < NA * NC; i += 64) Dac[i] = D[(fa + i / NC) * n + fc + i % NC];
         for (int i = lane; i < NB * NC; i += 64) Dbc[i] = D[(fb + i / NC) * n + fc + i % NC];
         for (int i = lane; i < NA * ND; i += 64) Dad[i] = D[(fa + i / ND) * n + fd + i % ND];
-        heavy_quartet<R>(P, a, tab, tk.bra, ket, ABx, ABy, ABz, CDx, CDy, CDz, X, S);
+        heavy_quartet<R, NAP>(P, a, tab, tk.bra, ket, ABx, ABy, ABz, CDx, CDy, CDz, X, S, accw);
         for (int o = lane; o < NOUT; o += 64) Y[o] = double(X[P.out[o]]);
         __syncthreads();
         const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);
@@ -363,7 +384,8 @@ __global__ __launch_bounds__(64) void k_schwarz_heavy(JKArgs a, BoysTables tab, 
     const double* O = a.sh_O;
     const double ABx = O[3 * s.x] - O[3 * s.y], ABy = O[3 * s.x + 1] - O[3 * s.y + 1], ABz = O[3 * s.x + 2] - O[3 * s.y + 2];
     post_coefs<double>(P, S);
-    heavy_quartet<double>(P, a, tab, pr, pr, ABx, ABy, ABz, ABx, ABy, ABz, X, S);
+    const int acc0[1][2] = {{-1, 0}};
+    heavy_quartet<double, 0>(P, a, tab, pr, pr, ABx, ABy, ABz, ABx, ABy, ABz, X, S, acc0);
     if (threadIdx.x == 0) {
         double mx = 0.0;
         for (int x = 0; x < P.na; ++x)
@@ -480,19 +502,19 @@ bool heavy_has(int la, int lb, int lc, int ld) {
     return false;
 }
 
-template <int NA, int NB, int NC, int ND>
+template <int NA, int NB, int NC, int ND, int NAP>
 static void launch_heavy_dims(const HeavyProg& P, const JKArgs& a, const BoysTables& tab, const Task* tasks,
                               int ntasks, int mode, hipStream_t st) {
     const double inf = __builtin_huge_val();
     const size_t extra = sizeof(double) * (NA * NB * NC * ND + NC * ND + NA * NB + NB * ND + NA * NC + NB * NC + NA * ND);
     if (mode == 2) {
         const size_t sm = heavy_y_offset<float>(P.nslots) + extra;
-        hipLaunchKernelGGL((k_jk_heavy<float, NA, NB, NC, ND>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks,
-                           a.precision, inf, P);
+        hipLaunchKernelGGL((k_jk_heavy<float, NA, NB, NC, ND, NAP>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks,
+                           ntasks, a.precision, inf, P);
     } else {
         const size_t sm = heavy_y_offset<double>(P.nslots) + extra;
-        hipLaunchKernelGGL((k_jk_heavy<double, NA, NB, NC, ND>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks, ntasks,
-                           a.precision, inf, P);
+        hipLaunchKernelGGL((k_jk_heavy<double, NA, NB, NC, ND, NAP>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks,
+                           ntasks, a.precision, inf, P);
     }
 }
 
@@ -502,8 +524,15 @@ static void launch_heavy_class(int pure, const HeavyProg& P, const JKArgs& a, co
     constexpr int PA = 2 * LA + 1, PB = 2 * LB + 1, PC = 2 * LC + 1, PD = 2 * LD + 1;
     constexpr int CA = (LA + 1) * (LA + 2) / 2, CB = (LB + 1) * (LB + 2) / 2, CC = (LC + 1) * (LC + 2) / 2,
                   CD = (LD + 1) * (LD + 2) / 2;
-    if (pure) launch_heavy_dims<PA, PB, PC, PD>(P, a, tab, tasks, ntasks, mode, st);
-    else launch_heavy_dims<CA, CB, CC, CD>(P, a, tab, tasks, ntasks, mode, st);
+    // accumulators: [e0|f0] with e over l = LA..LA+LB and f over l = LC..LC+LD, 64 per pass
+    constexpr auto ncart_sum = [](int l0, int l1) {
+        int n = 0;
+        for (int l = l0; l <= l1; ++l) n += (l + 1) * (l + 2) / 2;
+        return n;
+    };
+    constexpr int NAP = (ncart_sum(LA, LA + LB) * ncart_sum(LC, LC + LD) + 63) / 64;
+    if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP>(P, a, tab, tasks, ntasks, mode, st);
+    else launch_heavy_dims<CA, CB, CC, CD, NAP>(P, a, tab, tasks, ntasks, mode, st);
 }
 
 hipError_t launch_jk_heavy(int device, int la, int lb, int lc, int ld, int pure, const JKArgs& a,
