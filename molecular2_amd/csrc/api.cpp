@@ -600,16 +600,21 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
 
 // dScreen (optional): the density whose shell-block norms screen the quartets
 // (UnrestrictedFock::formJKdirect screens both spin builds with Da + Db).
+// dD2 (optional): a second density digested from the same integrals into
+// dJ2, dK2 (the unrestricted build); dScreen: the density whose block norms
+// screen (default dD).
 m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, double precision, unsigned flags,
-                         double* dJ, double* dK, const double* dScreen = nullptr) {
+                         double* dJ, double* dK, const double* dScreen = nullptr, const double* dD2 = nullptr,
+                         double* dJ2 = nullptr, double* dK2 = nullptr) {
     BasisData& b = ctx->obs;
     const int n = b.nbf, nsh = b.nsh;
     ARGCHK(n > 0, "m2_jk_direct: no orbital basis set");
     ARGCHK(flags <= M2_MIXED, "m2_jk_direct: unknown flags");
     const bool mixed = flags == M2_MIXED;
     HIPCHK(ctx->d_Dn.reserve((size_t)nsh * nsh));
-    HIPCHK(ctx->d_J.reserve((size_t)n * n));
-    HIPCHK(ctx->d_K.reserve((size_t)n * n));
+    const int nden = dD2 ? 2 : 1;
+    HIPCHK(ctx->d_J.reserve((size_t)nden * n * n));
+    HIPCHK(ctx->d_K.reserve((size_t)nden * n * n));
     HIPCHK(ctx->d_lists.reserve(2 * NCOMBO + 2));
     // shell-block norms, max|D| and the register-class work lists, on the device
     launch_blocknorm(dScreen ? dScreen : dD, n, nsh, b.d_bf.p, b.d_nbf.p, ctx->d_Dn.p, ctx->stream);
@@ -655,8 +660,8 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
             }
         }
     ctx->last_tasks = units;
-    launch_zero(ctx->d_J.p, (size_t)n * n, ctx->stream);
-    launch_zero(ctx->d_K.p, (size_t)n * n, ctx->stream);
+    launch_zero(ctx->d_J.p, (size_t)nden * n * n, ctx->stream);
+    launch_zero(ctx->d_K.p, (size_t)nden * n * n, ctx->stream);
     JKArgs a = jk_args(ctx);
     a.counters = nullptr;
     if (ctx->prof) {
@@ -672,6 +677,10 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     a.D = dD;
     a.J = ctx->d_J.p;
     a.K = ctx->d_K.p;
+    a.nden = nden;
+    a.D2 = dD2;
+    a.J2 = nden == 2 ? ctx->d_J.p + (size_t)n * n : nullptr;
+    a.K2 = nden == 2 ? ctx->d_K.p + (size_t)n * n : nullptr;
     a.precision = precision;
     a.fp32_below = ctx->fp32_below;
     a.scale = multiplier;
@@ -708,6 +717,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
         if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi + 1], ctx->stream));
     }
     launch_symmetrize(ctx->d_J.p, ctx->d_K.p, dJ, dK, n, ctx->stream);
+    if (nden == 2) launch_symmetrize(ctx->d_J.p + (size_t)n * n, ctx->d_K.p + (size_t)n * n, dJ2, dK2, n, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->prof) {
@@ -1027,9 +1037,9 @@ m2_status m2_jk_direct_uhf(m2_ctx* ctx, const double* Da, const double* Db, doub
     HIPCHK(hipMemcpyAsync(dDa, Da, sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dDb, Db, sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(dDt, Dt.data(), sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
-    m2_status st = jk_device_core(ctx, dDa, multiplier, precision, flags, ctx->d_Jo.p, ctx->d_Ko.p, dDt);
-    if (st) return st;
-    st = jk_device_core(ctx, dDb, multiplier, precision, flags, ctx->d_Jo.p + n2, ctx->d_Ko.p + n2, dDt);
+    // one pass over the quartets, both spin densities digested from the same integrals
+    m2_status st = jk_device_core(ctx, dDa, multiplier, precision, flags, ctx->d_Jo.p, ctx->d_Ko.p, dDt, dDb,
+                                  ctx->d_Jo.p + n2, ctx->d_Ko.p + n2);
     if (st) return st;
     HIPCHK(hipMemcpyAsync(Ja, ctx->d_Jo.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(Ka, ctx->d_Ko.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));

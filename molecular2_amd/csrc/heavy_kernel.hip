@@ -257,20 +257,96 @@ __host__ __device__ constexpr size_t heavy_y_offset(int nslots) {
     return ((sizeof(R) * (nslots + NSCALAR_LDS) + 7) / 8) * 8;
 }
 
-template <typename R, int NA, int NB, int NC, int ND, int NAP>
+// The six density blocks a quartet is digested with, in LDS: D(c,d), D(a,b),
+// D(b,d), D(a,c), D(b,c), D(a,d); one set per density.
+template <int NA, int NB, int NC, int ND>
+struct HeavyBlocks {
+    static constexpr int CD = 0, AB = NC * ND, BD = AB + NA * NB, AC = BD + NB * ND, BC = AC + NA * NC,
+                         AD = BC + NB * NC, SIZE = AD + NA * ND;
+};
+
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void heavy_stage_ket(const double* __restrict__ D, double* __restrict__ blk, int fa, int fb,
+                                                int fc, int fd, int n, int lane) {
+    using B = HeavyBlocks<NA, NB, NC, ND>;
+    for (int i = lane; i < NC * ND; i += 64) blk[B::CD + i] = D[(fc + i / ND) * n + fd + i % ND];
+    for (int i = lane; i < NB * ND; i += 64) blk[B::BD + i] = D[(fb + i / ND) * n + fd + i % ND];
+    for (int i = lane; i < NA * NC; i += 64) blk[B::AC + i] = D[(fa + i / NC) * n + fc + i % NC];
+    for (int i = lane; i < NB * NC; i += 64) blk[B::BC + i] = D[(fb + i / NC) * n + fc + i % NC];
+    for (int i = lane; i < NA * ND; i += 64) blk[B::AD + i] = D[(fa + i / ND) * n + fd + i % ND];
+}
+
+// J and K contributions of one quartet (outputs Y in (a,b,c,d) order) with
+// one density's blocks, degeneracy factor f (src/fockbuild.cpp:121-149)
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void heavy_digest(const double* __restrict__ Y, const double* __restrict__ blk,
+                                             double* __restrict__ J, double* __restrict__ K, double f, int fa, int fb,
+                                             int fc, int fd, int n, int lane) {
+    using B = HeavyBlocks<NA, NB, NC, ND>;
+    const double h = 0.5 * f;
+    for (int i = lane; i < NA * NB; i += 64) {   // J(ab) += D(cd) (ab|cd)
+        const double* y = Y + i * (NC * ND);
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < NC * ND; ++t) s += blk[B::CD + t] * y[t];
+        atomicAdd(&J[(fa + i / NB) * n + fb + i % NB], f * s);
+    }
+    for (int i = lane; i < NC * ND; i += 64) {   // J(cd) += D(ab) (ab|cd)
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < NA * NB; ++t) s += blk[B::AB + t] * Y[t * (NC * ND) + i];
+        atomicAdd(&J[(fc + i / ND) * n + fd + i % ND], f * s);
+    }
+    for (int i = lane; i < NA * NC; i += 64) {   // K(ac) += D(bd) (ab|cd)
+        const int x = i / NC, c = i % NC;
+        double s = 0.0;
+#pragma unroll
+        for (int y = 0; y < NB; ++y)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) s += blk[B::BD + y * ND + d] * Y[((x * NB + y) * NC + c) * ND + d];
+        atomicAdd(&K[(fa + x) * n + fc + c], h * s);
+    }
+    for (int i = lane; i < NB * ND; i += 64) {   // K(bd) += D(ac) (ab|cd)
+        const int y = i / ND, d = i % ND;
+        double s = 0.0;
+#pragma unroll
+        for (int x = 0; x < NA; ++x)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) s += blk[B::AC + x * NC + c] * Y[((x * NB + y) * NC + c) * ND + d];
+        atomicAdd(&K[(fb + y) * n + fd + d], h * s);
+    }
+    for (int i = lane; i < NA * ND; i += 64) {   // K(ad) += D(bc) (ab|cd)
+        const int x = i / ND, d = i % ND;
+        double s = 0.0;
+#pragma unroll
+        for (int y = 0; y < NB; ++y)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) s += blk[B::BC + y * NC + c] * Y[((x * NB + y) * NC + c) * ND + d];
+        atomicAdd(&K[(fa + x) * n + fd + d], h * s);
+    }
+    for (int i = lane; i < NB * NC; i += 64) {   // K(bc) += D(ad) (ab|cd)
+        const int y = i / NC, c = i % NC;
+        double s = 0.0;
+#pragma unroll
+        for (int x = 0; x < NA; ++x)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) s += blk[B::AD + x * ND + d] * Y[((x * NB + y) * NC + c) * ND + d];
+        atomicAdd(&K[(fb + y) * n + fc + c], h * s);
+    }
+}
+
+// NDEN = 2 (unrestricted build): each quartet is digested with D into (J, K)
+// and with D2 into (J2, K2).
+template <typename R, int NA, int NB, int NC, int ND, int NAP, int NDEN>
 __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
                                                  double lo, double hi, HeavyProg P) {
+    using B = HeavyBlocks<NA, NB, NC, ND>;
     extern __shared__ __align__(16) unsigned char smem[];
     R* X = reinterpret_cast<R*>(smem);
     R* S = X + P.nslots;
     constexpr int NOUT = NA * NB * NC * ND;
     double* Y = reinterpret_cast<double*>(smem + heavy_y_offset<R>(P.nslots));
-    double* Dcd = Y + NOUT;           // D(c,d)
-    double* Dab = Dcd + NC * ND;      // D(a,b)
-    double* Dbd = Dab + NA * NB;      // D(b,d)
-    double* Dac = Dbd + NB * ND;      // D(a,c)
-    double* Dbc = Dac + NA * NC;      // D(b,c)
-    double* Dad = Dbc + NB * NC;      // D(a,d)
+    double* blk = Y + NOUT;   // NDEN sets of HeavyBlocks
     const int w = blockIdx.x;
     if (w >= ntasks) return;
     const int lane = threadIdx.x;
@@ -281,8 +357,11 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
     const double* O = a.sh_O;
     const double ABx = O[3 * sa] - O[3 * sb], ABy = O[3 * sa + 1] - O[3 * sb + 1], ABz = O[3 * sa + 2] - O[3 * sb + 2];
     const int fa = a.sh_bf[sa], fb = a.sh_bf[sb];
-    const double* D = a.D;
-    for (int i = lane; i < NA * NB; i += 64) Dab[i] = D[(fa + i / NB) * n + fb + i % NB];
+#pragma unroll
+    for (int d = 0; d < NDEN; ++d) {
+        const double* D = d ? a.D2 : a.D;
+        for (int i = lane; i < NA * NB; i += 64) blk[d * B::SIZE + B::AB + i] = D[(fa + i / NB) * n + fb + i % NB];
+    }
     post_coefs<R>(P, S);
     int accw[NAP][2];   // this lane's accumulation pairs, the same for every primitive quartet
 #pragma unroll
@@ -308,65 +387,18 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
         const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
         // density blocks of this quartet (read by the digestion below, after
         // the barriers inside heavy_quartet)
-        for (int i = lane; i < NC * ND; i += 64) Dcd[i] = D[(fc + i / ND) * n + fd + i % ND];
-        for (int i = lane; i < NB * ND; i += 64) Dbd[i] = D[(fb + i / ND) * n + fd + i % ND];
-        for (int i = lane; i < NA * NC; i += 64) Dac[i] = D[(fa + i / NC) * n + fc + i % NC];
-        for (int i = lane; i < NB * NC; i += 64) Dbc[i] = D[(fb + i / NC) * n + fc + i % NC];
-        for (int i = lane; i < NA * ND; i += 64) Dad[i] = D[(fa + i / ND) * n + fd + i % ND];
+#pragma unroll
+        for (int d = 0; d < NDEN; ++d)
+            heavy_stage_ket<NA, NB, NC, ND>(d ? a.D2 : a.D, blk + d * B::SIZE, fa, fb, fc, fd, n, lane);
         heavy_quartet<R, NAP>(P, a, tab, tk.bra, ket, ABx, ABy, ABz, CDx, CDy, CDz, X, S, accw);
         for (int o = lane; o < NOUT; o += 64) Y[o] = double(X[P.out[o]]);
         __syncthreads();
         const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);
-        const double f = deg * a.scale, h = 0.5 * f;
-        for (int i = lane; i < NA * NB; i += 64) {   // J(ab) += D(cd) (ab|cd)
-            const double* y = Y + i * (NC * ND);
-            double s = 0.0;
+        const double f = deg * a.scale;
 #pragma unroll
-            for (int t = 0; t < NC * ND; ++t) s += Dcd[t] * y[t];
-            atomicAdd(&a.J[(fa + i / NB) * n + fb + i % NB], f * s);
-        }
-        for (int i = lane; i < NC * ND; i += 64) {   // J(cd) += D(ab) (ab|cd)
-            double s = 0.0;
-#pragma unroll
-            for (int t = 0; t < NA * NB; ++t) s += Dab[t] * Y[t * (NC * ND) + i];
-            atomicAdd(&a.J[(fc + i / ND) * n + fd + i % ND], f * s);
-        }
-        for (int i = lane; i < NA * NC; i += 64) {   // K(ac) += D(bd) (ab|cd)
-            const int x = i / NC, c = i % NC;
-            double s = 0.0;
-#pragma unroll
-            for (int y = 0; y < NB; ++y)
-#pragma unroll
-                for (int d = 0; d < ND; ++d) s += Dbd[y * ND + d] * Y[((x * NB + y) * NC + c) * ND + d];
-            atomicAdd(&a.K[(fa + x) * n + fc + c], h * s);
-        }
-        for (int i = lane; i < NB * ND; i += 64) {   // K(bd) += D(ac) (ab|cd)
-            const int y = i / ND, d = i % ND;
-            double s = 0.0;
-#pragma unroll
-            for (int x = 0; x < NA; ++x)
-#pragma unroll
-                for (int c = 0; c < NC; ++c) s += Dac[x * NC + c] * Y[((x * NB + y) * NC + c) * ND + d];
-            atomicAdd(&a.K[(fb + y) * n + fd + d], h * s);
-        }
-        for (int i = lane; i < NA * ND; i += 64) {   // K(ad) += D(bc) (ab|cd)
-            const int x = i / ND, d = i % ND;
-            double s = 0.0;
-#pragma unroll
-            for (int y = 0; y < NB; ++y)
-#pragma unroll
-                for (int c = 0; c < NC; ++c) s += Dbc[y * NC + c] * Y[((x * NB + y) * NC + c) * ND + d];
-            atomicAdd(&a.K[(fa + x) * n + fd + d], h * s);
-        }
-        for (int i = lane; i < NB * NC; i += 64) {   // K(bc) += D(ad) (ab|cd)
-            const int y = i / NC, c = i % NC;
-            double s = 0.0;
-#pragma unroll
-            for (int x = 0; x < NA; ++x)
-#pragma unroll
-                for (int d = 0; d < ND; ++d) s += Dad[x * ND + d] * Y[((x * NB + y) * NC + c) * ND + d];
-            atomicAdd(&a.K[(fb + y) * n + fc + c], h * s);
-        }
+        for (int d = 0; d < NDEN; ++d)
+            heavy_digest<NA, NB, NC, ND>(Y, blk + d * B::SIZE, d ? a.J2 : a.J, d ? a.K2 : a.K, f, fa, fb, fc, fd, n,
+                                         lane);
         __syncthreads();   // Y and the density blocks consumed before the next quartet
     }
 }
@@ -502,19 +534,19 @@ bool heavy_has(int la, int lb, int lc, int ld) {
     return false;
 }
 
-template <int NA, int NB, int NC, int ND, int NAP>
+template <int NA, int NB, int NC, int ND, int NAP, int NDEN>
 static void launch_heavy_dims(const HeavyProg& P, const JKArgs& a, const BoysTables& tab, const Task* tasks,
                               int ntasks, int mode, hipStream_t st) {
     const double inf = __builtin_huge_val();
-    const size_t extra = sizeof(double) * (NA * NB * NC * ND + NC * ND + NA * NB + NB * ND + NA * NC + NB * NC + NA * ND);
+    const size_t extra = sizeof(double) * (NA * NB * NC * ND + NDEN * HeavyBlocks<NA, NB, NC, ND>::SIZE);
     if (mode == 2) {
         const size_t sm = heavy_y_offset<float>(P.nslots) + extra;
-        hipLaunchKernelGGL((k_jk_heavy<float, NA, NB, NC, ND, NAP>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks,
-                           ntasks, a.precision, inf, P);
+        hipLaunchKernelGGL((k_jk_heavy<float, NA, NB, NC, ND, NAP, NDEN>), dim3(ntasks), dim3(64), sm, st, a, tab,
+                           tasks, ntasks, a.precision, inf, P);
     } else {
         const size_t sm = heavy_y_offset<double>(P.nslots) + extra;
-        hipLaunchKernelGGL((k_jk_heavy<double, NA, NB, NC, ND, NAP>), dim3(ntasks), dim3(64), sm, st, a, tab, tasks,
-                           ntasks, a.precision, inf, P);
+        hipLaunchKernelGGL((k_jk_heavy<double, NA, NB, NC, ND, NAP, NDEN>), dim3(ntasks), dim3(64), sm, st, a, tab,
+                           tasks, ntasks, a.precision, inf, P);
     }
 }
 
@@ -531,8 +563,13 @@ static void launch_heavy_class(int pure, const HeavyProg& P, const JKArgs& a, co
         return n;
     };
     constexpr int NAP = (ncart_sum(LA, LA + LB) * ncart_sum(LC, LC + LD) + 63) / 64;
-    if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP>(P, a, tab, tasks, ntasks, mode, st);
-    else launch_heavy_dims<CA, CB, CC, CD, NAP>(P, a, tab, tasks, ntasks, mode, st);
+    if (a.nden > 1) {
+        if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP, 2>(P, a, tab, tasks, ntasks, mode, st);
+        else launch_heavy_dims<CA, CB, CC, CD, NAP, 2>(P, a, tab, tasks, ntasks, mode, st);
+    } else {
+        if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP, 1>(P, a, tab, tasks, ntasks, mode, st);
+        else launch_heavy_dims<CA, CB, CC, CD, NAP, 1>(P, a, tab, tasks, ntasks, mode, st);
+    }
 }
 
 hipError_t launch_jk_heavy(int device, int la, int lb, int lc, int ld, int pure, const JKArgs& a,

@@ -155,7 +155,9 @@ struct JKWaves {
 // Quartets are computed when lo <= bound < hi (the launchers pass
 // [precision, inf); precision is chosen per task list).
 // Dynamic LDS (when a.k_lds): the (NA + NB) x n FP64 K rows of shells a and b.
-template <int LA, int LB, int LC, int LD, typename R, bool PURE>
+// NDEN = 2 (unrestricted build): every quartet is digested with D and with D2
+// into (J, K) and (J2, K2); the LDS then holds the K rows of both.
+template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN>
 __global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk(JKArgs a, BoysTables tab, double lo,
                                                                                  double hi) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
@@ -190,16 +192,22 @@ __global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk
     cPP* bprims = (cPP*)(gp + a.pair_off[bra]);
     extern __shared__ double sK[];
     if (lds)
-        for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
+        for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
     __syncthreads();
     const double* D = a.D;
+    const double* D2 = a.D2;
     double Dab[NA * NB], jab[NA * NB];
+    double Dab2[NDEN > 1 ? NA * NB : 1], jab2[NDEN > 1 ? NA * NB : 1];
 #pragma unroll
     for (int x = 0; x < NA; ++x)
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
             Dab[x * NB + y] = D[(fa + x) * n + fb + y];
             jab[x * NB + y] = 0.0;
+            if constexpr (NDEN > 1) {
+                Dab2[x * NB + y] = D2[(fa + x) * n + fb + y];
+                jab2[x * NB + y] = 0.0;
+            }
         }
     const double* O = a.sh_O;
     const R ABx = R(O[3 * sa] - O[3 * sb]), ABy = R(O[3 * sa + 1] - O[3 * sb + 1]), ABz = R(O[3 * sa + 2] - O[3 * sb + 2]);
@@ -243,6 +251,9 @@ __global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk
         if (active) {
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
             digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK);
+            if constexpr (NDEN > 1)
+                digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab2, jab2, D2, n, a.J2, a.K2, lds,
+                                          sK + (NA + NB) * n);
         }
     }
     if (a.counters && lane == 0 && cnt_q) {
@@ -254,14 +265,19 @@ __global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk
     for (int x = 0; x < NA * NB; ++x) {
         const double s = wave_sum(jab[x]);
         if (lane == 0 && s != 0.0) atomic_add_f64(&a.J[(fa + x / NB) * n + fb + x % NB], s);
+        if constexpr (NDEN > 1) {
+            const double s2 = wave_sum(jab2[x]);
+            if (lane == 0 && s2 != 0.0) atomic_add_f64(&a.J2[(fa + x / NB) * n + fb + x % NB], s2);
+        }
     }
     if (!lds) return;
     __syncthreads();
-    for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) {
+    for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) {
         const double s = sK[i];
         if (s != 0.0) {
-            const int row = i / n, col = i - row * n;
-            atomic_add_f64(&a.K[(row < NA ? fa + row : fb + row - NA) * n + col], s);
+            const int set = i / ((NA + NB) * n), r = i - set * (NA + NB) * n;
+            const int row = r / n, col = r - row * n;
+            atomic_add_f64(&(set ? a.K2 : a.K)[(row < NA ? fa + row : fb + row - NA) * n + col], s);
         }
     }
 }
@@ -289,7 +305,7 @@ struct JKWavesFam {
     static constexpr int value = EriClass<LA, LB, 0, 0>::NACC_FAM * (int)sizeof(R) <= 280 ? 8 : 4;
 };
 
-template <int LA, int LB, typename R, bool PURE>
+template <int LA, int LB, typename R, bool PURE, int NDEN>
 __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64)) void k_jk_fam(JKArgs a, BoysTables tab, double lo,
                                                                                double hi) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
@@ -324,16 +340,22 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64)) void k_jk_fam(
     const int bfam = same ? a.pair_fam[bra] : -1, bmem = same ? a.pair_mem[bra] : 0;
     extern __shared__ double sK[];
     if (lds)
-        for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
+        for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
     __syncthreads();
     const double* D = a.D;
+    const double* D2 = a.D2;
     double Dab[NA * NB], jab[NA * NB];
+    double Dab2[NDEN > 1 ? NA * NB : 1], jab2[NDEN > 1 ? NA * NB : 1];
 #pragma unroll
     for (int x = 0; x < NA; ++x)
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
             Dab[x * NB + y] = D[(fa + x) * n + fb + y];
             jab[x * NB + y] = 0.0;
+            if constexpr (NDEN > 1) {
+                Dab2[x * NB + y] = D2[(fa + x) * n + fb + y];
+                jab2[x * NB + y] = 0.0;
+            }
         }
     const double* O = a.sh_O;
     const R ABx = R(O[3 * sa] - O[3 * sb]), ABy = R(O[3 * sa + 1] - O[3 * sb + 1]), ABz = R(O[3 * sa + 2] - O[3 * sb + 2]);
@@ -394,6 +416,9 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64)) void k_jk_fam(
                 const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == mem[m] ? 1.0 : 2.0);
                 digest<NA, NB, 1, 1, R>(v + m * NCART, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K,
                                         lds, sK);
+                if constexpr (NDEN > 1)
+                    digest<NA, NB, 1, 1, R>(v + m * NCART, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab2, jab2, D2, n,
+                                            a.J2, a.K2, lds, sK + (NA + NB) * n);
             }
     }
     if (a.counters && lane == 0 && cnt_q) {
@@ -404,14 +429,19 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64)) void k_jk_fam(
     for (int x = 0; x < NA * NB; ++x) {
         const double s = wave_sum(jab[x]);
         if (lane == 0 && s != 0.0) atomic_add_f64(&a.J[(fa + x / NB) * n + fb + x % NB], s);
+        if constexpr (NDEN > 1) {
+            const double s2 = wave_sum(jab2[x]);
+            if (lane == 0 && s2 != 0.0) atomic_add_f64(&a.J2[(fa + x / NB) * n + fb + x % NB], s2);
+        }
     }
     if (!lds) return;
     __syncthreads();
-    for (int i = threadIdx.x; i < (NA + NB) * n; i += blockDim.x) {
+    for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) {
         const double s = sK[i];
         if (s != 0.0) {
-            const int row = i / n, col = i - row * n;
-            atomic_add_f64(&a.K[(row < NA ? fa + row : fb + row - NA) * n + col], s);
+            const int set = i / ((NA + NB) * n), r = i - set * (NA + NB) * n;
+            const int row = r / n, col = r - row * n;
+            atomic_add_f64(&(set ? a.K2 : a.K)[(row < NA ? fa + row : fb + row - NA) * n + col], s);
         }
     }
 }
@@ -441,39 +471,51 @@ __global__ __launch_bounds__(256) void k_schwarz(JKArgs a, BoysTables tab, int p
 // mode 0: FP64 kernel over the FP64 heads of the ket groups; mode 2: FP32
 // kernel over the FP32 tails (a.tail set by the caller).  `nblocks`
 // workgroups of the work list in a.blk_bra.
-template <int LA, int LB, int LC, int LD, bool PURE>
-void launch_jk(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
-    if (nblocks <= 0) return;
+template <int LA, int LB, int LC, int LD, bool PURE, int NDEN>
+void launch_jk_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     JKArgs a = a0;
-    const size_t kbytes = sizeof(double) * (size_t)(NA + NB) * a.n;
+    const size_t kbytes = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
     a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
     const double inf = __builtin_huge_val();
     const dim3 grid(nblocks);
     const size_t lds = a.k_lds ? kbytes : 0;
     if (mode == 2)
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE>), grid, dim3(64 * JKWaves<LA, LB, LC, LD, float>::value),
-                           lds, st, a, tab, a.precision, inf);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE, NDEN>), grid,
+                           dim3(64 * JKWaves<LA, LB, LC, LD, float>::value), lds, st, a, tab, a.precision, inf);
     else
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE>), grid, dim3(64 * JKWaves<LA, LB, LC, LD, double>::value),
-                           lds, st, a, tab, a.precision, inf);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE, NDEN>), grid,
+                           dim3(64 * JKWaves<LA, LB, LC, LD, double>::value), lds, st, a, tab, a.precision, inf);
+}
+
+template <int LA, int LB, int LC, int LD, bool PURE>
+void launch_jk(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
+    if (nblocks <= 0) return;
+    if (a0.nden > 1) launch_jk_n<LA, LB, LC, LD, PURE, 2>(a0, tab, nblocks, mode, st);
+    else launch_jk_n<LA, LB, LC, LD, PURE, 1>(a0, tab, nblocks, mode, st);
+}
+
+template <int LA, int LB, bool PURE, int NDEN>
+void launch_jk_fam_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
+    constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
+    JKArgs a = a0;
+    const size_t kbytes = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
+    a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
+    const double inf = __builtin_huge_val();
+    const size_t lds = a.k_lds ? kbytes : 0;
+    if (mode == 2)
+        hipLaunchKernelGGL((k_jk_fam<LA, LB, float, PURE, NDEN>), dim3(nblocks),
+                           dim3(64 * JKWavesFam<LA, LB, float>::value), lds, st, a, tab, a.precision, inf);
+    else
+        hipLaunchKernelGGL((k_jk_fam<LA, LB, double, PURE, NDEN>), dim3(nblocks),
+                           dim3(64 * JKWavesFam<LA, LB, double>::value), lds, st, a, tab, a.precision, inf);
 }
 
 template <int LA, int LB, bool PURE>
 void launch_jk_fam(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
     if (nblocks <= 0) return;
-    constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
-    JKArgs a = a0;
-    const size_t kbytes = sizeof(double) * (size_t)(NA + NB) * a.n;
-    a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
-    const double inf = __builtin_huge_val();
-    const size_t lds = a.k_lds ? kbytes : 0;
-    if (mode == 2)
-        hipLaunchKernelGGL((k_jk_fam<LA, LB, float, PURE>), dim3(nblocks), dim3(64 * JKWavesFam<LA, LB, float>::value),
-                           lds, st, a, tab, a.precision, inf);
-    else
-        hipLaunchKernelGGL((k_jk_fam<LA, LB, double, PURE>), dim3(nblocks), dim3(64 * JKWavesFam<LA, LB, double>::value),
-                           lds, st, a, tab, a.precision, inf);
+    if (a0.nden > 1) launch_jk_fam_n<LA, LB, PURE, 2>(a0, tab, nblocks, mode, st);
+    else launch_jk_fam_n<LA, LB, PURE, 1>(a0, tab, nblocks, mode, st);
 }
 
 template <int LA, int LB, bool PURE>

@@ -73,6 +73,11 @@ struct JKArgs {
     int ngrp;                // ket groups
     int tail;                // 0: FP64 head [gbeg, mid) of each group; 1: FP32 tail [mid, end)
     int gbeg[JK_MAXG];
+    // unrestricted builds: a second density digested from the same integrals
+    int nden;                // 1, or 2 with D2 / J2 / K2
+    const double* D2;
+    double* J2;
+    double* K2;
     // s-s ket families (k_jk_fam): family pairs of s shells sharing their
     // primitive pairs among up to FAM_MAXMEM contracted member pairs
     int fam_kets;            // set by the host: ket indices of this launch are family pairs
