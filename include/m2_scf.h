@@ -92,6 +92,26 @@ int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int
                  const double* coefs, const double* centers, const int* atoms, const m2_scf_options_c* o,
                  m2_scf_result_c* res);
 
+/* Fock::compute_hessian_numeric (src/rhfgrad.cpp:362-416): the RHF nuclear
+ * Hessian (3*natom x 3*natom, Eh/bohr^2) as central differences of the
+ * analytic gradient, H(3i+x, :) = [g(R + h e_ix) - g(R - h e_ix)] / 2h, with
+ * entries of magnitude <= 1e-12 set to zero and the lower triangle copied
+ * from the upper one.  step = h in bohr (<= 0: the reference's 0.01).  Each
+ * of the 6*natom displaced geometries runs a fresh m2_rhf (options o, same
+ * shells with the displaced atom's centres moved) plus its gradient; nworkers
+ * host threads (0: 8) each own an engine context and HIP stream on `device`,
+ * so the displaced SCFs overlap on the GPU.  res (6*natom entries, may be
+ * NULL) receives each displaced run's result, in the order atom, axis, +h
+ * then -h.  Returns 0 when every run finished (converged or not, as the
+ * reference logs and continues); otherwise m2_rhf_last_error() says why. */
+int m2_rhf_hessian(int device, int nworkers, int natom, const double* Z, const double* xyz, int charge,
+                   int multiplicity, int pure, int nsh, const int* l, const int* nprim, const double* exps,
+                   const double* coefs, const double* centers, const int* atoms, int nsh_aux, const int* l_aux,
+                   const int* nprim_aux, const double* exps_aux, const double* coefs_aux, const double* centers_aux,
+                   const int* atoms_aux, int nsh_min, const int* l_min, const int* nprim_min, const double* exps_min,
+                   const double* coefs_min, const double* centers_min, const int* atoms_min,
+                   const m2_scf_options_c* o, double step, double* hess, m2_scf_result_c* res);
+
 /* The two-electron part of the SOAD guess Fock matrix (Fock::compute_soad_guess,
  * src/fock.cpp:174-225, with compute_2body_fock_general, src/fockbuild.cpp:
  * 477-583): G (n x n, orbital basis) = sum_ls D(ls) [2 (uv|ls) - (ul|vs)]

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -287,6 +288,103 @@ int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int
         g_rhf_error = first_error;
         return 1;
     }
+    return 0;
+}
+
+// The shell arrays of one basis (orbital, auxiliary or minimal) as m2_rhf
+// takes them, with the centres of one atom's shells movable.
+struct ShellArgs {
+    int nsh = 0;
+    const int *l = nullptr, *nprim = nullptr, *atoms = nullptr;
+    const double *exps = nullptr, *coefs = nullptr, *centers = nullptr;
+    std::vector<double> moved;   // centres after a displacement
+    const double* centres_for(int atom, int axis, double d) {
+        if (nsh <= 0) return centers;
+        moved.assign(centers, centers + 3 * nsh);
+        for (int s = 0; s < nsh; ++s)
+            if (atoms[s] == atom) moved[3 * s + axis] += d;
+        return moved.data();
+    }
+};
+
+int m2_rhf_hessian(int device, int nworkers, int natom, const double* Z, const double* xyz, int charge,
+                   int multiplicity, int pure, int nsh, const int* l, const int* nprim, const double* exps,
+                   const double* coefs, const double* centers, const int* atoms, int nsh_aux, const int* l_aux,
+                   const int* nprim_aux, const double* exps_aux, const double* coefs_aux, const double* centers_aux,
+                   const int* atoms_aux, int nsh_min, const int* l_min, const int* nprim_min, const double* exps_min,
+                   const double* coefs_min, const double* centers_min, const int* atoms_min,
+                   const m2_scf_options_c* o, double step, double* hess, m2_scf_result_c* res) {
+    g_rhf_error.clear();
+    if (natom <= 0 || !Z || !xyz || nsh <= 0 || !l || !nprim || !exps || !coefs || !centers || !atoms || !o ||
+        !hess || (nsh_aux > 0 && (!l_aux || !nprim_aux || !exps_aux || !coefs_aux || !centers_aux || !atoms_aux)) ||
+        (nsh_min > 0 && (!l_min || !nprim_min || !exps_min || !coefs_min || !centers_min || !atoms_min))) {
+        g_rhf_error = "m2_rhf_hessian: null or empty argument";
+        return 1;
+    }
+    const double h = step > 0 ? step : 0.01;
+    const int ncoord = 3 * natom, njob = 2 * ncoord;
+    // job k: atom k / 6, axis (k / 2) % 3, +h for even k, -h for odd k
+    std::vector<double> g((size_t)njob * ncoord, 0.0);
+    if (nworkers <= 0) nworkers = 8;
+    nworkers = std::min(nworkers, njob);
+    std::atomic<int> next{0};
+    std::mutex mu;
+    std::string first_error;
+    auto worker = [&]() {
+        m2_ctx* ctx = nullptr;
+        if (m2_ctx_create(&ctx, device) != 0 || !ctx) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first_error.empty())
+                first_error = std::string("m2_rhf_hessian: context creation failed: ") +
+                              (ctx ? m2_last_error(ctx) : "no context");
+            if (ctx) m2_ctx_destroy(ctx);
+            return;
+        }
+        ShellArgs orb{nsh, l, nprim, atoms, exps, coefs, centers, {}};
+        ShellArgs aux{nsh_aux, l_aux, nprim_aux, atoms_aux, exps_aux, coefs_aux, centers_aux, {}};
+        ShellArgs mn{nsh_min, l_min, nprim_min, atoms_min, exps_min, coefs_min, centers_min, {}};
+        std::vector<double> X(xyz, xyz + ncoord);
+        for (int k; (k = next.fetch_add(1)) < njob;) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!first_error.empty()) break;
+            }
+            const int atom = k / 6, axis = (k / 2) % 3;
+            const double d = (k & 1) ? -h : h;
+            X.assign(xyz, xyz + ncoord);
+            X[3 * atom + axis] += d;
+            m2_scf_result_c r{};
+            const int st = m2_rhf(ctx, natom, Z, X.data(), charge, multiplicity, pure, orb.nsh, orb.l, orb.nprim,
+                                  orb.exps, orb.coefs, orb.centres_for(atom, axis, d), orb.atoms, aux.nsh, aux.l,
+                                  aux.nprim, aux.exps, aux.coefs, aux.centres_for(atom, axis, d), aux.atoms, mn.nsh,
+                                  mn.l, mn.nprim, mn.exps, mn.coefs, mn.centres_for(atom, axis, d), mn.atoms, o, &r,
+                                  nullptr, nullptr, nullptr, nullptr, g.data() + (size_t)k * ncoord);
+            if (res) res[k] = r;
+            if (st != 0) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_error.empty())
+                    first_error = "m2_rhf_hessian: displacement " + std::to_string(k) + ": " + g_rhf_error;
+            }
+        }
+        m2_ctx_destroy(ctx);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nworkers; ++w) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    if (!first_error.empty()) {
+        g_rhf_error = first_error;
+        return 1;
+    }
+    // row 3i+x: central difference of the whole gradient in the displacement
+    // of atom i along x; entries within 1e-12 of zero are set to zero and the
+    // lower triangle is then taken from the upper one
+    for (int r = 0; r < ncoord; ++r)
+        for (int c = 0; c < ncoord; ++c) {
+            const double v = (g[(size_t)(2 * r) * ncoord + c] - g[(size_t)(2 * r + 1) * ncoord + c]) / (2.0 * h);
+            hess[(size_t)r * ncoord + c] = std::fabs(v) > 1e-12 ? v : 0.0;
+        }
+    for (int r = 0; r < ncoord; ++r)
+        for (int c = 0; c < r; ++c) hess[(size_t)r * ncoord + c] = hess[(size_t)c * ncoord + r];
     return 0;
 }
 

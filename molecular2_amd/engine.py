@@ -119,6 +119,12 @@ def lib():
         L.m2_rhf_batch.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp,
                                    _dp, _ip, C.POINTER(SCFOptions), C.POINTER(SCFResult)]
         L.m2_rhf_batch.restype = C.c_int
+        L.m2_rhf_hessian.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                                     C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                                     C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                                     C.POINTER(SCFOptions), C.c_double, _dp, C.POINTER(SCFResult)]
+        L.m2_rhf_hessian.restype = C.c_int
         L.m2_deck_molecule.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.POINTER(DeckInfo), _dp, _dp,
                                        C.POINTER(ShellsOut), C.POINTER(ShellsOut), C.POINTER(ShellsOut)]
         L.m2_deck_molecule.restype = C.c_int
@@ -150,6 +156,30 @@ def shell_arrays(shells):
     cen = np.ascontiguousarray(np.array([s.center for s in shells], float).reshape(-1))
     at = np.array([s.atom for s in shells], np.int32)
     return l, npr, exps, coefs, cen, at
+
+
+def _scf_shells(mol, basis, aux, guess):
+    """The 21 shell arguments of m2_rhf / m2_uhf / m2_rhf_hessian (orbital,
+    auxiliary, minimal STO-3G for the SOAD guess) for `mol`, and the orbital
+    shells.  The argument list holds the numpy arrays it points into."""
+    ip = lambda a: a.ctypes.data_as(_ip)
+    empty = (np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1), np.zeros(1), np.zeros(1),
+             np.zeros(1, np.int32))
+
+    def group(shells):
+        if not shells:
+            return [0] + [ip(a) if a.dtype == np.int32 else _p(a) for a in empty], empty
+        arrs = shell_arrays(shells)
+        return [len(shells)] + [ip(a) if a.dtype == np.int32 else _p(a) for a in arrs], arrs
+
+    sh = mol.shells(basis)
+    args, keep = [], []
+    for shells in (sh, mol.shells(aux) if aux else None, mol.shells("sto-3g") if guess == "soad" else None):
+        a, k = group(shells)
+        args += a
+        keep.append(k)
+    args.append(keep)        # not passed: only holds the arrays
+    return sh, args
 
 
 def _shells_buffers(nsh, nprim):
@@ -347,21 +377,7 @@ class Engine:
                        diis_start=2)
         for k, v in kw.items():
             setattr(o, k, v)
-        if guess == "soad":
-            lm, npm, exm, cfm, cenm, atm = shell_arrays(mol.shells("sto-3g"))
-        else:
-            lm = npm = atm = np.zeros(1, np.int32)
-            exm = cfm = cenm = np.zeros(1)
-        sh = mol.shells(basis)
-        l, npr, ex, cf, cen, at = shell_arrays(sh)
-        if aux:
-            ash = mol.shells(aux)
-            la, npa, exa, cfa, cena, ata = shell_arrays(ash)
-            nsa = len(ash)
-        else:
-            la = npa = ata = np.zeros(1, np.int32)
-            exa = cfa = cena = np.zeros(1)
-            nsa = 0
+        sh, shell_args = _scf_shells(mol, basis, aux, guess)
         Z = np.asarray(mol.Z, float)
         X = np.ascontiguousarray(mol.xyz, float)
         nbf = sum((2 * s.l + 1) if pure else (s.l + 1) * (s.l + 2) // 2 for s in sh)
@@ -371,12 +387,8 @@ class Engine:
         log = np.zeros((max(1, o.maxiter), 5))
         g = np.zeros(mol.natom * 3)
         res = SCFResult()
-        ip = lambda a: a.ctypes.data_as(_ip)
         st = self.L.m2_rhf(self.h, mol.natom, _p(Z), _p(X), mol.charge, mol.multiplicity, int(pure),
-                           len(sh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
-                           nsa, ip(la), ip(npa), _p(exa), _p(cfa), _p(cena), ip(ata),
-                           len(lm) if guess == "soad" else 0, ip(lm), ip(npm), _p(exm), _p(cfm), _p(cenm), ip(atm),
-                           C.byref(o), C.byref(res), _p(D), _p(Cm), _p(eps), _p(log),
+                           *shell_args[:-1], C.byref(o), C.byref(res), _p(D), _p(Cm), _p(eps), _p(log),
                            _p(g) if gradient else None)
         if st != 0:
             raise M2Error(st, self.L.m2_rhf_last_error().decode())
@@ -422,6 +434,34 @@ class Engine:
                 "n_full": r.n_full, "n_incremental": r.n_incremental, "n_fp32": r.n_fp32,
                 "jk_seconds": r.jk_seconds, "total_seconds": r.total_seconds} for r in res]
         return out, wall
+
+    def rhf_hessian(self, mol, basis="cc-pvdz", pure=True, aux=None, guess="hcore", step=0.01, nworkers=8, **kw):
+        """Fock::compute_hessian_numeric through m2_rhf_hessian: the RHF
+        nuclear Hessian (3N x 3N, Eh/bohr^2) from central differences (step
+        in bohr) of the analytic gradient over the 6N displaced geometries,
+        which run as concurrent SCFs on nworkers engine contexts of this
+        engine's device.  Options as for rhf().  Returns the Hessian and one
+        result dict per displaced run (order: atom, axis, +step, -step)."""
+        if guess not in ("hcore", "soad"):
+            raise ValueError("guess must be 'hcore' or 'soad'")
+        o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
+                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=1 if guess == "soad" else 0,
+                       diis_start=2)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        _, shell_args = _scf_shells(mol, basis, aux, guess)
+        n = 3 * mol.natom
+        H = np.zeros((n, n))
+        res = (SCFResult * (2 * n))()
+        st = self.L.m2_rhf_hessian(self.L.m2_ctx_device(self.h), int(nworkers), mol.natom,
+                                   _p(np.asarray(mol.Z, float)), _p(np.ascontiguousarray(mol.xyz, float)),
+                                   mol.charge, mol.multiplicity, int(pure), *shell_args[:-1], C.byref(o),
+                                   float(step), _p(H), res)
+        if st != 0:
+            raise M2Error(st, self.L.m2_rhf_last_error().decode())
+        runs = [{"energy": r.energy, "iterations": r.iterations, "converged": bool(r.converged),
+                 "total_seconds": r.total_seconds, "grad_seconds": r.grad_seconds} for r in res]
+        return H, runs
 
     def run_deck(self, deck_text, basislib, pure=False, maxres=64):
         """Run the hf / rhf / uhf commands of a reference input deck on this

@@ -662,3 +662,25 @@ class DFJK:
         gam = np.einsum("xiK,xi->K", xiK, Cocc)
         J = Bf @ gam
         return J, K
+
+
+def hessian_numeric(gradient_at, xyz, h=0.01):
+    """Fock::compute_hessian_numeric (src/rhfgrad.cpp:362-416) over any
+    gradient: gradient_at(xyz) -> (natom, 3) dE/dR at positions xyz (natom, 3,
+    bohr).  Row 3i+x is [g(R + h e_ix) - g(R - h e_ix)] / 2h flattened over
+    (atom, axis) (:375-405); entries of magnitude <= 1e-12 become zero (:405)
+    and the lower triangle is then copied from the upper one (:413-415)."""
+    xyz = np.asarray(xyz, float)
+    n = xyz.size
+    H = np.zeros((n, n))
+    for r in range(n):
+        p = xyz.copy().reshape(-1)
+        p[r] += h
+        gp = np.asarray(gradient_at(p.reshape(xyz.shape)), float).reshape(-1)
+        p[r] -= 2 * h
+        gm = np.asarray(gradient_at(p.reshape(xyz.shape)), float).reshape(-1)
+        row = (gp - gm) / (2 * h)
+        H[r] = np.where(np.abs(row) > 1e-12, row, 0.0)
+    low = np.tril_indices(n, -1)
+    H[low] = H.T[low]
+    return H

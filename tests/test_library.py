@@ -46,6 +46,23 @@ def test_autograd_operator_registers():
     assert schema.endswith("-> Tensor")
 
 
+def test_hessian_rejects_empty_molecule():
+    """m2_rhf_hessian checks its arguments before any engine context exists"""
+    from molecular2_amd import engine
+    if not os.path.exists(engine.LIBPATH):
+        pytest.skip("libm2.so not built")
+    L = engine.lib()
+    o = engine.SCFOptions()
+    H = np.zeros(1)
+    nul = ctypes.POINTER(ctypes.c_int)()
+    dnul = ctypes.POINTER(ctypes.c_double)()
+    st = L.m2_rhf_hessian(0, 1, 0, dnul, dnul, 0, 1, 1, 0, nul, nul, dnul, dnul, dnul, nul,
+                          0, nul, nul, dnul, dnul, dnul, nul, 0, nul, nul, dnul, dnul, dnul, nul,
+                          ctypes.byref(o), 0.01, H.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None)
+    assert st != 0
+    assert b"null or empty" in L.m2_rhf_last_error()
+
+
 def test_engine_fails_loudly_without_library(monkeypatch):
     from molecular2_amd import engine
     monkeypatch.setattr(engine, "LIBPATH", "/nonexistent/libm2.so")
