@@ -1471,11 +1471,19 @@ def gen_deriv_class(la, lb, lc, ld, mask=7):
     em.out("        const R rho_p = q * inv_pq, rho_q = p * inv_pq, inv2pq = R(0.5) * inv_pq;")
     em.out("        const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
     em.out("        const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
-    emit_vrr(em, order, nodes, "        ")
+    # each target is accumulated right after its VRR node, so that it does
+    # not stay live to the end of the primitive iteration
+    acc_of = {}
     for nm, (T_, w) in sets.items():
         for e, f in T_:
+            acc_of.setdefault((e, f), []).append((nm, w))
+    for key in order:
+        emit_vrr(em, [key], nodes, "        ")
+        e, f, m = key
+        if m == 0 and (e, f) in acc_of:
             v = vrr_name(e, f, 0)
-            em.out("        %s_%s_%s += %s;" % (nm, cname(e), cname(f), v if w is None else "%s * %s" % (w, v)))
+            for nm, w in acc_of[(e, f)]:
+                em.out("        %s_%s_%s += %s;" % (nm, cname(e), cname(f), v if w is None else "%s * %s" % (w, v)))
     em.out("      }")
     em.out("    }")
     # Gamma is formed only now, so that it does not hold registers in the loop

@@ -175,9 +175,13 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
                         const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
                         const double d = k * BOYS_STEP - T;
                         const double* r = tab.d + k * BOYS_MTAB + lane;
-                        double s = r[7];
-                        for (int j = 6; j >= 0; --j) s = r[j] + d * s / (j + 1);
-                        F = s;
+                        double s = r[6] + d * (1.0 / 7.0) * r[7];
+                        s = r[5] + d * (1.0 / 6.0) * s;
+                        s = r[4] + d * 0.2 * s;
+                        s = r[3] + d * 0.25 * s;
+                        s = r[2] + d * (1.0 / 3.0) * s;
+                        s = r[1] + d * 0.5 * s;
+                        F = r[0] + d * s;
                     } else {
                         F = 0.88622692545275801365 * rsqrt(T);
                         const double h = 0.5 / T;
@@ -408,11 +412,16 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
                         const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
                         const double d = k * BOYS_STEP - T;
                         const double* r = a.tab.d + k * BOYS_MTAB + Lab;
-                        double s = r[7];
-                        for (int j = 6; j >= 0; --j) s = r[j] + d * s / (j + 1);
-                        F[Lab] = s;
+                        double s = r[6] + d * (1.0 / 7.0) * r[7];
+                        s = r[5] + d * (1.0 / 6.0) * s;
+                        s = r[4] + d * 0.2 * s;
+                        s = r[3] + d * 0.25 * s;
+                        s = r[2] + d * (1.0 / 3.0) * s;
+                        s = r[1] + d * 0.5 * s;
+                        F[Lab] = r[0] + d * s;
                         const double e = exp(-T);
-                        for (int m = Lab - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
+                        constexpr double inv_odd[G1_LA + 3] = {1.0, 1.0 / 3, 1.0 / 5, 1.0 / 7, 1.0 / 9, 1.0 / 11};
+                        for (int m = Lab - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) * inv_odd[m];
                     } else {
                         F[0] = 0.88622692545275801365 * rsqrt(T);
                         for (int m = 0; m < Lab; ++m) F[m + 1] = (2 * m + 1) * F[m] * (0.5 / T);
