@@ -239,7 +239,7 @@ def hrr_ket(la, lb, lc, ld):
     return node, defs
 
 
-def gen_class(la, lb, lc, ld, deriv=False):
+def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
     L = la + lb + lc + ld
     em = Emitter()
     E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
@@ -287,10 +287,19 @@ def gen_class(la, lb, lc, ld, deriv=False):
         em.out("        const R rho_p = q * inv_pq, rho_q = p * inv_pq, inv2pq = R(0.5) * inv_pq;")
         em.out("        const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
         em.out("        const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
-    emit_vrr(em, order, nodes, "        ")
-    for e in E:
-        for f in F:
-            em.out("        a%s_%s += %s;" % (cname(e), cname(f), vrr_name(e, f, 0)))
+    if interleave:
+        # each target is accumulated right after its VRR node (shorter live ranges)
+        tset = set(targets)
+        for key in order:
+            emit_vrr(em, [key], nodes, "        ")
+            e, f, m = key
+            if m == 0 and (e, f) in tset:
+                em.out("        a%s_%s += %s;" % (cname(e), cname(f), vrr_name(e, f, 0)))
+    else:
+        emit_vrr(em, order, nodes, "        ")
+        for e in E:
+            for f in F:
+                em.out("        a%s_%s += %s;" % (cname(e), cname(f), vrr_name(e, f, 0)))
     em.out("      }")
     em.out("    }")
     # HRR: ket first for every e in E, then bra
@@ -406,7 +415,7 @@ def gen_class_fam(la, lb):
     km[FAM_MAXMEM * ik + m] (K_cd of the member's own primitive pair); the
     HRR / solid-harmonic post-processing then runs once per member, writing
     out + m * NCART."""
-    base = gen_class(la, lb, 0, 0)
+    base = gen_class(la, lb, 0, 0, interleave=False)
     lines = base.splitlines()
     E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
     accs = ["a%s_%s" % (cname(e), cname((0, 0, 0))) for e in E]
