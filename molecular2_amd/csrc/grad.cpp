@@ -2,29 +2,13 @@
 // src/rhfgrad.cpp:15-89): one-electron and Pulay terms, the two-electron
 // adjoint, and the nuclear-repulsion term.
 #include "ctx.hpp"
+#include "grad_reg.hpp"
 #include "generated/sph_tables.inc"
 
 #include <cmath>
 #include <cstring>
 
 namespace m2 {
-struct GradArgs {
-    const PrimPair* prims;
-    const int* pair_off;
-    const int* pair_np;
-    const int2* pair_sh;
-    const double* pair_Q;
-    const unsigned char* pair_sig;
-    const double* sh_O;
-    const int* sh_bf;
-    const int* sh_atom;
-    const double* Dn;
-    const double* D;
-    const double* sph2;
-    int n, nsh, pure;
-    double precision;
-    double* grad;
-};
 struct Grad1Args {
     const double* sh_O;
     const int* sh_l;
@@ -159,6 +143,46 @@ m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, co
     a.pure = b.pure;
     a.precision = precision;
     a.grad = dG.p;
+    // D in the Cartesian basis (Dc = T D T^T; T maps each pure d shell's five
+    // functions to its six Cartesian components, identity elsewhere), so that
+    // the kernels read each quartet's density blocks directly
+    std::vector<int> cfo(nsh);
+    int nc = 0;
+    for (int s = 0; s < nsh; ++s) {
+        cfo[s] = nc;
+        nc += (b.sh[s].l + 1) * (b.sh[s].l + 2) / 2;
+    }
+    const double* sph2 = SPH_TAB + SPH_OFF[2];
+    auto to_cart = [&](int s, const double* src, size_t stride, double* dst, size_t dstride, int ncol) {
+        // rows of shell s: dst[i] = sum_u T[i][u] src[u] (for every column)
+        const HostShell& sh = b.sh[s];
+        const int ncart = (sh.l + 1) * (sh.l + 2) / 2;
+        if (!(b.pure && sh.l >= 2)) {
+            for (int i = 0; i < ncart; ++i)
+                for (int c = 0; c < ncol; ++c) dst[i * dstride + c] = src[(size_t)(sh.bf + i) * stride + c];
+            return;
+        }
+        for (int i = 0; i < ncart; ++i)
+            for (int c = 0; c < ncol; ++c) {
+                double v = 0.0;
+                for (int u = 0; u < 5; ++u) v += sph2[u * 6 + i] * src[(size_t)(sh.bf + u) * stride + c];
+                dst[i * dstride + c] = v;
+            }
+    };
+    std::vector<double> Y((size_t)nc * n), YT((size_t)n * nc), Dct((size_t)nc * nc);
+    for (int s = 0; s < nsh; ++s) to_cart(s, D, n, Y.data() + (size_t)cfo[s] * n, n, n);   // T D (nc x n)
+    for (int r = 0; r < nc; ++r)
+        for (int c = 0; c < n; ++c) YT[(size_t)c * nc + r] = Y[(size_t)r * n + c];
+    for (int s = 0; s < nsh; ++s) to_cart(s, YT.data(), nc, Dct.data() + (size_t)cfo[s] * nc, nc, nc);   // (T D T^T)^T
+    DevBuf<double> dDc;
+    DevBuf<int> dcfo;
+    HIPCHK(dDc.reserve((size_t)nc * nc));
+    HIPCHK(dcfo.reserve(nsh));
+    HIPCHK(hipMemcpy(dDc.p, Dct.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));   // symmetric
+    HIPCHK(hipMemcpy(dcfo.p, cfo.data(), sizeof(int) * nsh, hipMemcpyHostToDevice));
+    a.Dc = dDc.p;
+    a.sh_cf = dcfo.p;
+    a.nc = nc;
     const BoysTables tab{ctx->d_boys.p, ctx->d_boys_f.p};
     for (const ComboRangeG& r : ranges) {
         HIPCHK(launch_grad_2e(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1], a, tab,

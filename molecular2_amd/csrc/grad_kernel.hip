@@ -124,28 +124,14 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
         const double CDx = O[3 * sc] - O[3 * sd], CDy = O[3 * sc + 1] - O[3 * sd + 1], CDz = O[3 * sc + 2] - O[3 * sd + 2];
         const int sh[4] = {sa, sb, sc, sd};
         const int ls[4] = {P.la, P.lb, P.lc, P.ld};
-        // Cartesian density blocks ab, cd, ac, bd, ad, bc (lanes 0..5)
+        // Cartesian density blocks ab, cd, ac, bd, ad, bc (6 x 36 entries)
         const int bx[6] = {0, 2, 0, 1, 0, 1}, by[6] = {1, 3, 2, 3, 3, 2};
-        if (lane < 6) {
-            const int x = bx[lane], y = by[lane];
-            const int lx = ls[x], ly = ls[y], nx = ncart_d(lx), ny = ncart_d(ly);
-            const bool px = a.pure && lx >= 2, py = a.pure && ly >= 2;
-            const int mx = px ? 2 * lx + 1 : nx, my = py ? 2 * ly + 1 : ny;
-            const int fx = a.sh_bf[sh[x]], fy = a.sh_bf[sh[y]];
-            for (int i = 0; i < nx; ++i)
-                for (int j = 0; j < ny; ++j) {
-                    double s = 0.0;
-                    for (int u = 0; u < mx; ++u) {
-                        const double cu = px ? a.sph2[u * 6 + i] : (u == i ? 1.0 : 0.0);
-                        if (cu == 0.0) continue;
-                        for (int v = 0; v < my; ++v) {
-                            const double cv = py ? a.sph2[v * 6 + j] : (v == j ? 1.0 : 0.0);
-                            if (cv == 0.0) continue;
-                            s += cu * cv * a.D[(size_t)(fx + u) * a.n + fy + v];
-                        }
-                    }
-                    Dt[lane * 36 + i * ny + j] = s;
-                }
+        for (int t = lane; t < 6 * 36; t += 64) {
+            const int blk = t / 36, e = t - blk * 36;
+            const int x = bx[blk], y = by[blk];
+            const int ny = ncart_d(ls[y]);
+            const int i = e / ny, j = e - i * ny;
+            if (i < ncart_d(ls[x])) Dt[t] = a.Dc[(size_t)(a.sh_cf[sh[x]] + i) * a.nc + a.sh_cf[sh[y]] + j];
         }
         __syncthreads();
         const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);

@@ -29,32 +29,22 @@ struct GradArgs {
     int n, nsh, pure;
     double precision;
     double* grad;                    // natom x 3
+    const double* Dc;                // D in the Cartesian basis, nc x nc
+    const int* sh_cf;                // first Cartesian function of each shell
+    int nc;
 };
 
 __device__ __forceinline__ int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
 
-// Cartesian density block Dt (na x nb) of shells (s1 with l1, s2 with l2)
+// Cartesian density block Dt (N1 x N2) of two shells whose first Cartesian
+// functions are c1 and c2
 template <int L1, int L2>
-__device__ __forceinline__ void cart_block(const GradArgs& a, int f1, int f2, double* Dt) {
+__device__ __forceinline__ void cart_block(const GradArgs& a, int c1, int c2, double* Dt) {
     constexpr int N1 = (L1 + 1) * (L1 + 2) / 2, N2 = (L2 + 1) * (L2 + 2) / 2;
-    const bool p1 = a.pure && L1 >= 2, p2 = a.pure && L2 >= 2;
-    const int m1 = p1 ? 2 * L1 + 1 : N1, m2 = p2 ? 2 * L2 + 1 : N2;
 #pragma unroll
     for (int i = 0; i < N1; ++i)
 #pragma unroll
-        for (int j = 0; j < N2; ++j) {
-            double s = 0.0;
-            for (int u = 0; u < m1; ++u) {
-                const double cu = p1 ? a.sph2[u * 6 + i] : (u == i ? 1.0 : 0.0);
-                if (cu == 0.0) continue;
-                for (int v = 0; v < m2; ++v) {
-                    const double cv = p2 ? a.sph2[v * 6 + j] : (v == j ? 1.0 : 0.0);
-                    if (cv == 0.0) continue;
-                    s += cu * cv * a.D[(size_t)(f1 + u) * a.n + f2 + v];
-                }
-            }
-            Dt[i * N2 + j] = s;
-        }
+        for (int j = 0; j < N2; ++j) Dt[i * N2 + j] = a.Dc[(size_t)(c1 + i) * a.nc + c2 + j];
 }
 
 __device__ __forceinline__ bool grad_screen(const GradArgs& a, int bra, int ket, int sa, int sb, int sc, int sd) {
@@ -94,7 +84,7 @@ __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, co
         if (__any(active)) {
             double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             if (active) {
-                const int fa = a.sh_bf[sa], fb = a.sh_bf[sb], fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+                const int fa = a.sh_cf[sa], fb = a.sh_cf[sb], fc = a.sh_cf[sc], fd = a.sh_cf[sd];
                 const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
                 // Gamma (Cartesian) is formed after the primitive loop
                 auto gam_fill = [&](double* Gam) {
