@@ -543,6 +543,8 @@ struct ComboRange {
 // Register classes take up to 64 kets per task (one per lane); table-driven
 // heavy classes take HEAVY_CHUNK (one wavefront walks them in turn).
 constexpr int HEAVY_CHUNK = 4;
+// kets per task of the gradient's program classes (split over workgroups)
+constexpr int GRAD_PROG_CHUNK = GRAD_PROG_SPLIT * GRAD_PROG_KETS;
 
 // gradient classes with generated register code (one quartet per lane)
 static bool deriv_register(int la, int lb, int lc, int ld) {
@@ -554,6 +556,22 @@ static bool deriv_register(int la, int lb, int lc, int ld) {
 
 // heavy_only: the J/K path builds host tasks only for the table-driven heavy
 // classes (the register classes use the device work lists of tasks.hip).
+// h_tasks -> d_tasks on the context's stream through the pinned staging
+// buffer.  Callers synchronise the stream before building the next task list,
+// so the staging buffer is never rewritten while a copy from it is pending.
+hipError_t upload_task_list(m2_ctx* ctx) {
+    const size_t nt = ctx->h_tasks.size();
+    if (nt == 0) return hipSuccess;
+    hipError_t e = ctx->d_tasks.reserve(nt);
+    if (e != hipSuccess) return e;
+    if (nt > ctx->h_tasks_pin.n) {
+        if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+        if ((e = ctx->h_tasks_pin.reserve(nt)) != hipSuccess) return e;
+    }
+    std::memcpy(ctx->h_tasks_pin.p, ctx->h_tasks.data(), sizeof(Task) * nt);
+    return hipMemcpyAsync(ctx->d_tasks.p, ctx->h_tasks_pin.p, sizeof(Task) * nt, hipMemcpyHostToDevice, ctx->stream);
+}
+
 std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed, bool gradient,
                                     bool heavy_only) {
     std::vector<ComboRange> ranges;
@@ -569,7 +587,7 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
             const bool reg = gradient ? deriv_register(PAIR_L[X][0], PAIR_L[X][1], PAIR_L[Y][0], PAIR_L[Y][1])
                                       : JK_COMBO[X][Y] != nullptr;
             if (heavy_only && reg) continue;
-            const int chunk = reg ? 64 : HEAVY_CHUNK;
+            const int chunk = reg ? 64 : gradient ? GRAD_PROG_CHUNK : HEAVY_CHUNK;
             ComboRange r{X, Y, (int)T.size(), 0, 0};
             t32.clear();
             for (int i = sx.begin; i < sx.end; ++i) {
@@ -635,11 +653,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     // host tasks of the heavy classes
     std::vector<ComboRange> heavy = build_tasks(ctx, precision, Dmax, mixed, false, true);
     long long units = (long long)ctx->h_tasks.size();
-    if (!ctx->h_tasks.empty()) {
-        HIPCHK(ctx->d_tasks.reserve(ctx->h_tasks.size()));
-        HIPCHK(hipMemcpyAsync(ctx->d_tasks.p, ctx->h_tasks.data(), sizeof(Task) * ctx->h_tasks.size(),
-                              hipMemcpyHostToDevice, ctx->stream));
-    }
+    HIPCHK(upload_task_list(ctx));
     // launch plan in class order: register combos from their lists, heavy ones from host tasks
     struct Launch {
         int X, Y, ci, hv;   // ci: list index (register) or -1; hv: index into `heavy` or -1
@@ -744,11 +758,73 @@ namespace m2 {
 struct ComboRangeG {
     int X, Y, t0, tmid, t1;
 };
-std::vector<ComboRangeG> grad_tasks(m2_ctx* ctx, double precision, double Dmax) {
-    std::vector<ComboRange> r = build_tasks(ctx, precision, Dmax, false, true, false);
-    std::vector<ComboRangeG> out;
-    for (const auto& x : r) out.push_back(ComboRangeG{x.X, x.Y, x.t0, x.tmid, x.t1});
-    return out;
+// The gradient's task list, built on the device into ctx->d_tasks (same
+// tasks and order as build_tasks with gradient = true): per combination the
+// tasks per bra are counted, one exclusive scan over all combinations gives
+// the offsets, and each bra writes its tasks.  Only the scan (one int per bra
+// and combination) comes back to the host, for the launch ranges.
+hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, std::vector<ComboRangeG>& out) {
+    out.clear();
+    std::vector<GradTaskArgs> args;
+    std::vector<int> start;   // first counter of each combination
+    int n = 0;
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        const PairClassSeg& sx = ctx->cls[X];
+        for (int Y = 0; Y <= X; ++Y) {
+            const PairClassSeg& sy = ctx->cls[Y];
+            if (sx.end <= sx.begin || sy.groups.empty()) continue;
+            if ((int)sy.groups.size() > JK_MAXG) return hipErrorInvalidValue;
+            const bool reg = deriv_register(PAIR_L[X][0], PAIR_L[X][1], PAIR_L[Y][0], PAIR_L[Y][1]);
+            GradTaskArgs g{};
+            g.Q = ctx->d_pair_Q.p;
+            g.x0 = sx.begin;
+            g.nbra = sx.end - sx.begin;
+            g.same = X == Y;
+            g.ngrp = (int)sy.groups.size();
+            for (int k = 0; k < g.ngrp; ++k) {
+                g.gbeg[k] = sy.groups[k].begin;
+                g.gend[k] = sy.groups[k].end;
+            }
+            g.chunk = reg ? 64 : GRAD_PROG_CHUNK;
+            g.precision = precision;
+            g.dmax = Dmax;
+            args.push_back(g);
+            out.push_back(ComboRangeG{X, Y, 0, 0, 0});
+            start.push_back(n);
+            n += g.nbra;
+        }
+    }
+    if (n == 0) return hipSuccess;
+    DevBuf<int> counts, offs;
+    hipError_t e;
+    if ((e = counts.reserve(n)) != hipSuccess || (e = offs.reserve((size_t)n + 1)) != hipSuccess) return e;
+    for (size_t c = 0; c < args.size(); ++c) launch_grad_task_count(args[c], counts.p + start[c], ctx->stream);
+    launch_scan_excl(counts.p, offs.p, n, ctx->stream);
+    std::vector<int> h((size_t)n + 1);
+    if ((e = hipMemcpyAsync(h.data(), offs.p, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, ctx->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return e;
+    const int total = h[n];
+    for (size_t c = 0; c < args.size(); ++c) {
+        out[c].t0 = h[start[c]];
+        out[c].t1 = out[c].tmid = h[start[c] + args[c].nbra];
+    }
+    if (total > 0) {
+        if ((e = ctx->d_tasks.reserve(total)) != hipSuccess) return e;
+        for (size_t c = 0; c < args.size(); ++c)
+            launch_grad_task_fill(args[c], offs.p + start[c], ctx->d_tasks.p, ctx->stream);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    // the buffers are freed once the fill kernels have run
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+    counts.release();
+    offs.release();
+    std::vector<ComboRangeG> kept;
+    for (const auto& r : out)
+        if (r.t1 > r.t0) kept.push_back(r);
+    out.swap(kept);
+    return hipSuccess;
 }
 
 }  // namespace m2
@@ -817,6 +893,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_Ko.release();
     ctx->d_Dn.release();
     ctx->d_tasks.release();
+    ctx->h_tasks_pin.release();
     for (int ci = 0; ci < NCOMBO; ++ci) {
         ctx->list_cut[ci].release();
         ctx->list_off[ci].release();

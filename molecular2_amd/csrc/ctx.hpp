@@ -43,6 +43,28 @@ struct DevBuf {
     }
 };
 
+// Pinned host staging buffer (grown, never shrunk; hipHostMalloc'd)
+template <typename T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        const size_t want = std::max<size_t>(cnt + cnt / 2, 1024);
+        hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 struct PairGroup {
     int begin, end;   // global pair index range, Q descending inside
 };
@@ -99,6 +121,7 @@ struct m2_ctx {
     m2::DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
     m2::DevBuf<m2::Task> d_tasks;
     std::vector<m2::Task> h_tasks;   // heavy-class (and gradient) tasks
+    m2::PinnedBuf<m2::Task> h_tasks_pin;   // their staging copy for the upload
     // device work lists of the register-class combinations (index over X >= Y)
     m2::JKList lists[m2::NCOMBO];
     m2::DevBuf<int2> list_cut[m2::NCOMBO];

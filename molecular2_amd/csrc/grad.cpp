@@ -37,7 +37,7 @@ void launch_zero(double* p, size_t n, hipStream_t);
 struct ComboRangeG {
     int X, Y, t0, tmid, t1;
 };
-std::vector<ComboRangeG> grad_tasks(m2_ctx* ctx, double precision, double Dmax);
+hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, std::vector<ComboRangeG>& out);
 }  // namespace m2
 
 using namespace m2;
@@ -119,12 +119,8 @@ m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, co
     HIPCHK(hipStreamSynchronize(ctx->stream));
     double Dmax = 0.0;
     for (double x : Dn) Dmax = std::max(Dmax, x);
-    std::vector<ComboRangeG> ranges = grad_tasks(ctx, precision, Dmax);
-    if (!ctx->h_tasks.empty()) {
-        HIPCHK(ctx->d_tasks.reserve(ctx->h_tasks.size()));
-        HIPCHK(hipMemcpy(ctx->d_tasks.p, ctx->h_tasks.data(), sizeof(Task) * ctx->h_tasks.size(),
-                         hipMemcpyHostToDevice));
-    }
+    std::vector<ComboRangeG> ranges;
+    HIPCHK(grad_tasks_device(ctx, precision, Dmax, ranges));
     GradArgs a{};
     a.prims = ctx->d_prims.p;
     a.pair_off = ctx->d_pair_off.p;
@@ -247,6 +243,8 @@ m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, co
     for (auto* x : {&dD, &dW, &dG, &dZ, &dX, &dsph, &dex, &dcf}) x->release();
     for (auto* x : {&datom, &dl, &dnp, &dpoff}) x->release();
     dsig.release();
+    dDc.release();
+    dcfo.release();
     return M2_OK;
 }
 

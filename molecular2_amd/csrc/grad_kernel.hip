@@ -108,7 +108,8 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
     double* S = X + P.nslots;
     double* Gam = S + NSCALAR_D;
     double* Dt = Gam + ngam;   // 6 Cartesian density blocks, 36 each
-    const int w = blockIdx.x;
+    // GRAD_PROG_SPLIT workgroups per task, each walking GRAD_PROG_KETS of its kets
+    const int w = blockIdx.x / GRAD_PROG_SPLIT, part = blockIdx.x - w * GRAD_PROG_SPLIT;
     if (w >= ntasks) return;
     const int lane = threadIdx.x;
     const Task tk = tasks[w];
@@ -116,7 +117,8 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
     const int sa = sab.x, sb = sab.y;
     const double* O = a.sh_O;
     const double ABx = O[3 * sa] - O[3 * sb], ABy = O[3 * sa + 1] - O[3 * sb + 1], ABz = O[3 * sa + 2] - O[3 * sb + 2];
-    for (int kk = 0; kk < tk.nket; ++kk) {
+    const int kk1 = min(tk.nket, (part + 1) * GRAD_PROG_KETS);
+    for (int kk = part * GRAD_PROG_KETS; kk < kk1; ++kk) {
         const int ket = tk.ket0 + kk;
         const int2 scd = a.pair_sh[ket];
         const int sc = scd.x, sd = scd.y;
@@ -592,7 +594,7 @@ hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const Grad
         }
         const int ngam = d[14] * d[15] * d[16] * d[17];
         const size_t sm = sizeof(double) * (P.nslots + NSCALAR_D + ngam + 6 * 36);
-        hipLaunchKernelGGL(k_grad_prog, dim3(nt), dim3(64), sm, st, a, tab, t, nt, P);
+        hipLaunchKernelGGL(k_grad_prog, dim3((unsigned)nt * GRAD_PROG_SPLIT), dim3(64), sm, st, a, tab, t, nt, P);
     }
     return hipGetLastError();
 }

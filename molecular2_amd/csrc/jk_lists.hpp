@@ -17,6 +17,20 @@ struct JKList {
     const int* bra_fam;       // same class with family kets: bra pair -> its family pair (else null)
 };
 
+// one class combination of the gradient's task list (tasks.hip)
+struct GradTaskArgs {
+    const double* Q;          // device Schwarz bounds of the pairs
+    int x0, nbra;             // bra pairs [x0, x0 + nbra) of class X
+    int same;                 // X == Y: only kets j <= i
+    int ngrp;
+    int gbeg[JK_MAXG], gend[JK_MAXG];
+    int chunk;                // kets per task
+    double precision, dmax;
+};
+void launch_grad_task_count(const GradTaskArgs& g, int* counts, hipStream_t st);
+void launch_scan_excl(const int* in, int* out, int n, hipStream_t st);
+void launch_grad_task_fill(const GradTaskArgs& g, const int* offs, Task* out, hipStream_t st);
+
 void launch_dmax(const double* Dn, int m, double* out, hipStream_t st);
 void launch_task_lists(const JKList& c, const double* Q, const double* dmax, double precision, double fp32_below,
                        int mixed, int* tot, hipStream_t st);

@@ -40,6 +40,9 @@ constexpr double BOYS_TMAX = 60.0;
 // primitive-count groups per pair class, and the LDS budget of the K-row
 // accumulators
 constexpr int JK_BLOCK_TASKS = 64;
+// gradient program classes: a host task holds up to GRAD_PROG_SPLIT *
+// GRAD_PROG_KETS kets; GRAD_PROG_SPLIT workgroups take GRAD_PROG_KETS each
+constexpr int GRAD_PROG_KETS = 4, GRAD_PROG_SPLIT = 8;
 constexpr int JK_MAXG = 10;
 constexpr size_t JK_KROW_LDS_MAX = 96 * 1024;
 

@@ -131,6 +131,86 @@ __global__ __launch_bounds__(256) void k_task_fill(JKList c) {
     for (int b = off[i]; b < e; ++b) dst[b] = i;
 }
 
+// ---------------------------------------------------------------------------
+// Gradient task lists, built on the device in the order of the host's
+// build_tasks: for every bra pair i of class X (in pair order) and every ket
+// group of class Y, the screened prefix Q_j >= precision / (Q_i max|D|) (cut
+// at j <= i when X == Y) in chunks of `chunk` kets, one Task per chunk.
+//   k_grad_task_count: tasks per bra;  k_scan_excl: exclusive prefix sums;
+//   k_grad_task_fill: the Task entries at each bra's offset.
+__device__ __forceinline__ int grad_group_end(const GradTaskArgs& g, int i, double thr, int k) {
+    int end = g.gbeg[k] + count_ge_dev(g.Q, g.gbeg[k], g.gend[k], thr);
+    if (g.same) end = min(end, i + 1);
+    return end;
+}
+
+__global__ __launch_bounds__(256) void k_grad_task_count(GradTaskArgs g, int* __restrict__ counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nbra) return;
+    const int i = g.x0 + t;
+    const double qi = g.Q[i];
+    int n = 0;
+    if (qi > 0.0 && g.dmax > 0.0) {
+        const double thr = g.precision / (qi * g.dmax);
+        for (int k = 0; k < g.ngrp; ++k) {
+            const int len = grad_group_end(g, i, thr, k) - g.gbeg[k];
+            if (len > 0) n += (len + g.chunk - 1) / g.chunk;
+        }
+    }
+    counts[t] = n;
+}
+
+__global__ __launch_bounds__(256) void k_grad_task_fill(GradTaskArgs g, const int* __restrict__ offs,
+                                                        Task* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nbra) return;
+    const int i = g.x0 + t;
+    const double qi = g.Q[i];
+    if (!(qi > 0.0 && g.dmax > 0.0)) return;
+    const double thr = g.precision / (qi * g.dmax);
+    int w = offs[t];
+    for (int k = 0; k < g.ngrp; ++k) {
+        const int end = grad_group_end(g, i, thr, k);
+        for (int j = g.gbeg[k]; j < end; j += g.chunk) out[w++] = Task{i, j, min(g.chunk, end - j), 0};
+    }
+}
+
+// out[0..n-1]: exclusive prefix sums of in[0..n-1]; out[n]: the total
+__global__ __launch_bounds__(1024) void k_scan_excl(const int* __restrict__ in, int* __restrict__ out, int n) {
+    __shared__ int part[1024];
+    const int per = (n + 1023) / 1024;
+    const int b = min(n, (int)threadIdx.x * per), e = min(n, b + per);
+    int s = 0;
+    for (int k = b; k < e; ++k) s += in[k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < 1024; ++k) {
+            const int v = part[k];
+            part[k] = acc;
+            acc += v;
+        }
+        out[n] = acc;
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int k = b; k < e; ++k) {
+        out[k] = acc;
+        acc += in[k];
+    }
+}
+
+void launch_grad_task_count(const GradTaskArgs& g, int* counts, hipStream_t st) {
+    if (g.nbra > 0) hipLaunchKernelGGL(k_grad_task_count, dim3((g.nbra + 255) / 256), dim3(256), 0, st, g, counts);
+}
+void launch_scan_excl(const int* in, int* out, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, in, out, n);
+}
+void launch_grad_task_fill(const GradTaskArgs& g, const int* offs, Task* out, hipStream_t st) {
+    if (g.nbra > 0) hipLaunchKernelGGL(k_grad_task_fill, dim3((g.nbra + 255) / 256), dim3(256), 0, st, g, offs, out);
+}
+
 void launch_dmax(const double* Dn, int m, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_dmax, dim3(1), dim3(1024), 0, st, Dn, m, out);
 }
