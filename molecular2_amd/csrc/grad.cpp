@@ -30,7 +30,7 @@ struct Grad1Args {
 hipError_t deriv_upload(int device);
 hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,
                           const Task* t, int nt, int natom, hipStream_t st);
-hipError_t launch_grad_1e(const Grad1Args& a, hipStream_t st);
+hipError_t launch_grad_1e(const Grad1Args& a, const int2* pairs, const int* count, hipStream_t st);
 void launch_blocknorm(const double* A, int n, int nsh, const int* sh_bf, const int* sh_nbf, double* out, hipStream_t);
 void launch_zero(double* p, size_t n, hipStream_t);
 // task list of the J/K driver (api.cpp)
@@ -223,7 +223,20 @@ m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, co
     g1.natom = natom;
     g1.tab = tab;
     g1.grad = dG.p;
-    HIPCHK(launch_grad_1e(g1, ctx->stream));
+    // ordered shell pairs grouped by (l1, l2) for the class-templated kernel
+    std::vector<int2> p1;
+    int cnt1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 9; ++c)
+        for (int s1 = 0; s1 < nsh; ++s1)
+            for (int s2 = 0; s2 < nsh; ++s2)
+                if (b.sh[s1].l * 3 + b.sh[s2].l == c) {
+                    p1.push_back(make_int2(s1, s2));
+                    ++cnt1[c];
+                }
+    DevBuf<int2> dp1;
+    HIPCHK(dp1.reserve(p1.size()));
+    HIPCHK(hipMemcpy(dp1.p, p1.data(), sizeof(int2) * p1.size(), hipMemcpyHostToDevice));
+    HIPCHK(launch_grad_1e(g1, dp1.p, cnt1, ctx->stream));
     HIPCHK(hipMemcpyAsync(g, dG.p, sizeof(double) * natom * 3, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     // nuclear repulsion (src/rhfgrad.cpp:60-86)
@@ -245,6 +258,7 @@ m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, co
     dsig.release();
     dDc.release();
     dcfo.release();
+    dp1.release();
     return M2_OK;
 }
 

@@ -294,18 +294,19 @@ __device__ void cart_block_rt(const double* M, int n, const double* sph2, int pu
         }
 }
 
-constexpr int G1_LA = 3;   // l of the bra after the +1 shift
-constexpr int G1_CUMA = (G1_LA + 1) * (G1_LA + 2) * (G1_LA + 3) / 6;   // 20
-constexpr int G1_CUMB = 10;                                            // |b| <= 2
-
-// One wavefront per ordered shell pair (s1, s2); lane c takes the nuclei
-// c, c + 64, ...  Lane 0 also does the kinetic / Pulay part.
-__global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
+// One wavefront per ordered shell pair (s1, s2) of angular momenta (LA, LB)
+// (a list per class, so that every array below has its exact compile-time
+// size); lane c takes the nuclei c, c + 64, ...  Lane 0 also does the kinetic
+// / Pulay part.
+template <int LA, int LB>
+__global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a, const int2* __restrict__ pairs, int npairs) {
+    constexpr int la = LA, lb = LB;
+    constexpr int G1_CUMA = (la + 2) * (la + 3) * (la + 4) / 6;   // components with |a| <= la + 1
+    constexpr int G1_CUMB = (lb + 1) * (lb + 2) * (lb + 3) / 6;   // components with |b| <= lb
     const int t = blockIdx.x;
-    if (t >= a.nsh * a.nsh) return;
-    const int s1 = t / a.nsh, s2 = t % a.nsh;
+    if (t >= npairs) return;
+    const int s1 = pairs[t].x, s2 = pairs[t].y;
     const int lane = threadIdx.x;
-    const int la = a.sh_l[s1], lb = a.sh_l[s2];
     const int na = ncart_d(la), nb = ncart_d(lb);
     double Dt[36], Wt[36];
     cart_block_rt(a.D, a.n, a.sph2, a.pure, la, a.sh_bf[s1], lb, a.sh_bf[s2], Dt);
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
                 PA[k] = P[k] - A[k];
                 PB[k] = P[k] - B[k];
             }
-            double O1[3][G1_LA + 1][5];
+            double O1[3][la + 2][lb + 3];
             for (int k = 0; k < 3; ++k)
                 for (int ii = 0; ii <= la + 1; ++ii)
                     for (int jj = 0; jj <= lb + 2; ++jj) {
@@ -393,7 +394,7 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
                     PC[k] = P[k] - Cc[k];
                     PC2 += PC[k] * PC[k];
                 }
-                double F[G1_LA + 3];
+                double F[la + lb + 2];
                 {
                     const double T = p * PC2;
                     if (T < BOYS_TMAX) {
@@ -408,20 +409,24 @@ __global__ __launch_bounds__(64) void k_grad_1e(Grad1Args a) {
                         s = r[1] + d * 0.5 * s;
                         F[Lab] = r[0] + d * s;
                         const double e = exp(-T);
-                        constexpr double inv_odd[G1_LA + 3] = {1.0, 1.0 / 3, 1.0 / 5, 1.0 / 7, 1.0 / 9, 1.0 / 11};
+                        constexpr double inv_odd[6] = {1.0, 1.0 / 3, 1.0 / 5, 1.0 / 7, 1.0 / 9, 1.0 / 11};
                         for (int m = Lab - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) * inv_odd[m];
                     } else {
                         F[0] = 0.88622692545275801365 * rsqrt(T);
                         for (int m = 0; m < Lab; ++m) F[m + 1] = (2 * m + 1) * F[m] * (0.5 / T);
                     }
                 }
-                double Wv[G1_CUMA][G1_CUMB][G1_LA + 3];
+                double Wv[G1_CUMA][G1_CUMB][la + lb + 2];
                 for (int m = 0; m <= Lab; ++m) Wv[0][0][m] = vpref * F[m];
+#pragma unroll
                 for (int lbb = 0; lbb <= lb; ++lbb)
+#pragma unroll
                     for (int ib = cum_of(lbb); ib < cum_of(lbb + 1); ++ib) {
                         int eb[3];
                         comp_of(lbb, ib - cum_of(lbb), eb);
+#pragma unroll
                         for (int laa = 0; laa <= la + 1; ++laa)
+#pragma unroll
                             for (int ia = cum_of(laa); ia < cum_of(laa + 1); ++ia) {
                                 if (ia == 0 && ib == 0) continue;
                                 int ea[3];
@@ -599,9 +604,16 @@ hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const Grad
     return hipGetLastError();
 }
 
-hipError_t launch_grad_1e(const Grad1Args& a, hipStream_t st) {
-    const int nt = a.nsh * a.nsh;
-    hipLaunchKernelGGL(k_grad_1e, dim3(nt), dim3(64), 0, st, a);
+hipError_t launch_grad_1e(const Grad1Args& a, const int2* pairs, const int* count, hipStream_t st) {
+    // pairs grouped by class la * 3 + lb, count[9] pairs per class
+    int off = 0;
+#define M2_G1(LA, LB)                                                                                   \
+    if (count[LA * 3 + LB] > 0)                                                                       \
+        hipLaunchKernelGGL((k_grad_1e<LA, LB>), dim3(count[LA * 3 + LB]), dim3(64), 0, st, a, pairs + off, \
+                           count[LA * 3 + LB]);                                                       \
+    off += count[LA * 3 + LB];
+    M2_G1(0, 0) M2_G1(0, 1) M2_G1(0, 2) M2_G1(1, 0) M2_G1(1, 1) M2_G1(1, 2) M2_G1(2, 0) M2_G1(2, 1) M2_G1(2, 2)
+#undef M2_G1
     return hipGetLastError();
 }
 
