@@ -763,10 +763,10 @@ struct ComboRangeG {
 // tasks per bra are counted, one exclusive scan over all combinations gives
 // the offsets, and each bra writes its tasks.  Only the scan (one int per bra
 // and combination) comes back to the host, for the launch ranges.
-hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, std::vector<ComboRangeG>& out) {
+hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, bool mixed, std::vector<ComboRangeG>& out) {
     out.clear();
-    std::vector<GradTaskArgs> args;
-    std::vector<int> start;   // first counter of each combination
+    std::vector<GradTaskArgs> args;   // per combination: the FP64 head (part 0), then the FP32 tail (part 1)
+    std::vector<int> start;           // first counter of each entry of args
     int n = 0;
     for (int X = 0; X < NPAIRCLASS; ++X) {
         const PairClassSeg& sx = ctx->cls[X];
@@ -788,10 +788,15 @@ hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, std::ve
             g.chunk = reg ? 64 : GRAD_PROG_CHUNK;
             g.precision = precision;
             g.dmax = Dmax;
-            args.push_back(g);
+            // program classes run in FP64 only
+            g.split = mixed && reg ? std::max(precision, ctx->fp32_below) : precision;
+            for (int part = 0; part < 2; ++part) {
+                g.part = part;
+                args.push_back(g);
+                start.push_back(n);
+                n += g.nbra;
+            }
             out.push_back(ComboRangeG{X, Y, 0, 0, 0});
-            start.push_back(n);
-            n += g.nbra;
         }
     }
     if (n == 0) return hipSuccess;
@@ -806,9 +811,10 @@ hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, std::ve
         (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
         return e;
     const int total = h[n];
-    for (size_t c = 0; c < args.size(); ++c) {
-        out[c].t0 = h[start[c]];
-        out[c].t1 = out[c].tmid = h[start[c] + args[c].nbra];
+    for (size_t c = 0; c < out.size(); ++c) {
+        out[c].t0 = h[start[2 * c]];
+        out[c].tmid = h[start[2 * c + 1]];
+        out[c].t1 = h[start[2 * c + 1] + args[2 * c + 1].nbra];
     }
     if (total > 0) {
         if ((e = ctx->d_tasks.reserve(total)) != hipSuccess) return e;

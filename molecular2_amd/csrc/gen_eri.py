@@ -1616,22 +1616,23 @@ def write_deriv_register(outdir):
         for m in masks:
             body.append(gen_deriv_class(*c, mask=m))
         body.append("void launch_grad_reg_%s(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom,"
-                    " hipStream_t st) {" % tag)
+                    " int fp32, hipStream_t st) {" % tag)
         for m in masks:
-            body.append("  launch_reg<%d, %d, %d, %d, %d>(a, tab, t, nt, natom, st);" % (c + (m,)))
+            body.append("  if (fp32) launch_reg<%d, %d, %d, %d, %d, float>(a, tab, t, nt, natom, st);" % (c + (m,)))
+            body.append("  else launch_reg<%d, %d, %d, %d, %d, double>(a, tab, t, nt, natom, st);" % (c + (m,)))
         body += ["}", "}  // namespace m2", ""]
         put(os.path.join(outdir, "grad_reg_%s.hip" % tag), "\n".join(body))
         names.append((c, tag))
     disp = [head, "#pragma once"]
     for c, tag in names:
-        disp.append("void launch_grad_reg_%s(const GradArgs&, const BoysTables&, const Task*, int, int, hipStream_t);"
+        disp.append("void launch_grad_reg_%s(const GradArgs&, const BoysTables&, const Task*, int, int, int, hipStream_t);"
                     % tag)
-    disp.append("// true when (la, lb, lc, ld) has register code (and was launched)")
+    disp.append("// true when (la, lb, lc, ld) has register code (and was launched, in FP32 arithmetic when fp32)")
     disp.append("inline bool launch_grad_reg(int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,"
-                " const Task* t, int nt, int natom, hipStream_t st) {")
+                " const Task* t, int nt, int natom, int fp32, hipStream_t st) {")
     for c, tag in names:
-        disp.append("  if (la == %d && lb == %d && lc == %d && ld == %d) { launch_grad_reg_%s(a, tab, t, nt, natom, st);"
-                    " return true; }" % (c + (tag,)))
+        disp.append("  if (la == %d && lb == %d && lc == %d && ld == %d) {"
+                    " launch_grad_reg_%s(a, tab, t, nt, natom, fp32, st); return true; }" % (c + (tag,)))
     disp += ["  return false;", "}", ""]
     put(os.path.join(outdir, "deriv_dispatch.inc"), "\n".join(disp))
     cls = [head, "#pragma once",

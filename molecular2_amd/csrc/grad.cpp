@@ -29,7 +29,7 @@ struct Grad1Args {
 };
 hipError_t deriv_upload(int device);
 hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,
-                          const Task* t, int nt, int natom, hipStream_t st);
+                          const Task* t, int nt, int natom, int fp32, hipStream_t st);
 hipError_t launch_grad_1e(const Grad1Args& a, const int2* pairs, const int* count, hipStream_t st);
 void launch_blocknorm(const double* A, int n, int nsh, const int* sh_bf, const int* sh_nbf, double* out, hipStream_t);
 void launch_zero(double* p, size_t n, hipStream_t);
@@ -37,7 +37,7 @@ void launch_zero(double* p, size_t n, hipStream_t);
 struct ComboRangeG {
     int X, Y, t0, tmid, t1;
 };
-hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, std::vector<ComboRangeG>& out);
+hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, bool mixed, std::vector<ComboRangeG>& out);
 }  // namespace m2
 
 using namespace m2;
@@ -46,7 +46,13 @@ extern "C" {
 
 m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, const double* D, const double* W,
                   double precision, double* g) {
+    return m2_grad_ex(ctx, natom, Z, xyz, D, W, precision, 0, g);
+}
+
+m2_status m2_grad_ex(m2_ctx* ctx, int natom, const double* Z, const double* xyz, const double* D, const double* W,
+                     double precision, unsigned flags, double* g) {
     if (!ctx) return M2_ERR_ARG;
+    ARGCHK(flags <= M2_MIXED, "m2_grad_ex: unknown flags");
     BasisData& b = ctx->obs;
     ARGCHK(b.nbf > 0, "m2_grad: no orbital basis set");
     ARGCHK(natom > 0 && Z && xyz && D && W && g, "m2_grad: bad arguments");
@@ -120,7 +126,7 @@ m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, co
     double Dmax = 0.0;
     for (double x : Dn) Dmax = std::max(Dmax, x);
     std::vector<ComboRangeG> ranges;
-    HIPCHK(grad_tasks_device(ctx, precision, Dmax, ranges));
+    HIPCHK(grad_tasks_device(ctx, precision, Dmax, flags == M2_MIXED, ranges));
     GradArgs a{};
     a.prims = ctx->d_prims.p;
     a.pair_off = ctx->d_pair_off.p;
@@ -181,8 +187,11 @@ m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, co
     a.nc = nc;
     const BoysTables tab{ctx->d_boys.p, ctx->d_boys_f.p};
     for (const ComboRangeG& r : ranges) {
+        // FP64 tasks [t0, tmid), FP32 tasks [tmid, t1) (mixed precision only)
         HIPCHK(launch_grad_2e(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1], a, tab,
-                              ctx->d_tasks.p + r.t0, r.t1 - r.t0, natom, ctx->stream));
+                              ctx->d_tasks.p + r.t0, r.tmid - r.t0, natom, 0, ctx->stream));
+        HIPCHK(launch_grad_2e(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1], a, tab,
+                              ctx->d_tasks.p + r.tmid, r.t1 - r.tmid, natom, 1, ctx->stream));
     }
     // one-electron part
     std::vector<int> np(nsh), poff(nsh);

@@ -553,10 +553,12 @@ hipError_t deriv_upload(int device) {
 
 #include "generated/deriv_dispatch.inc"
 
+// fp32: FP32 arithmetic (register classes only; the host gives the program
+// classes FP64 tasks alone)
 hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,
-                          const Task* t, int nt, int natom, hipStream_t st) {
+                          const Task* t, int nt, int natom, int fp32, hipStream_t st) {
     if (nt <= 0) return hipSuccess;
-    if (launch_grad_reg(la, lb, lc, ld, a, tab, t, nt, natom, st)) {
+    if (launch_grad_reg(la, lb, lc, ld, a, tab, t, nt, natom, fp32, st)) {
         // register code (grad_reg_*.hip)
     } else {
         const DerivTables& T = g_dt[device];

@@ -63,7 +63,10 @@ __device__ __forceinline__ double wave_sum_g(double v) {
 }
 
 // register classes: one quartet per lane, per-block LDS bins for the atoms
-template <int LA, int LB, int LC, int LD, int MASK>
+// R: arithmetic of the integrals and their contraction (float for the
+// quartets a mixed-precision gradient leaves below its FP32 threshold); the
+// per-quartet sums are added up in double.
+template <int LA, int LB, int LC, int LD, int MASK, typename R>
 __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, const Task* __restrict__ tasks,
                                                   int ntasks, int natom) {
     constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
@@ -82,12 +85,12 @@ __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, co
         const int sa = sab.x, sb = sab.y, sc = scd.x, sd = scd.y;
         const bool active = lane < nket && grad_screen(a, bra, ket, sa, sb, sc, sd);
         if (__any(active)) {
-            double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            R g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             if (active) {
                 const int fa = a.sh_cf[sa], fb = a.sh_cf[sb], fc = a.sh_cf[sc], fd = a.sh_cf[sd];
                 const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
                 // Gamma (Cartesian) is formed after the primitive loop
-                auto gam_fill = [&](double* Gam) {
+                auto gam_fill = [&](R* Gam) {
                     double Dab[NA * NB], Dcd[NC * ND], Dac[NA * NC], Dbd[NB * ND], Dad[NA * ND], Dbc[NB * NC];
                     cart_block<LA, LB>(a, fa, fb, Dab);
                     cart_block<LC, LD>(a, fc, fd, Dcd);
@@ -103,23 +106,22 @@ __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, co
                             for (int k = 0; k < NC; ++k)
 #pragma unroll
                                 for (int l = 0; l < ND; ++l)
-                                    Gam[((i * NB + j) * NC + k) * ND + l] =
+                                    Gam[((i * NB + j) * NC + k) * ND + l] = R(
                                         deg * (0.5 * Dab[i * NB + j] * Dcd[k * ND + l] -
-                                               0.125 * (Dac[i * NC + k] * Dbd[j * ND + l] + Dad[i * ND + l] * Dbc[j * NC + k]));
+                                               0.125 * (Dac[i * NC + k] * Dbd[j * ND + l] + Dad[i * ND + l] * Dbc[j * NC + k])));
                 };
                 const double* O = a.sh_O;
-                EriDeriv<LA, LB, LC, LD, MASK>::template run<double>(
+                EriDeriv<LA, LB, LC, LD, MASK>::template run<R>(
                     a.prims + a.pair_off[bra], a.pair_np[bra], a.prims + a.pair_off[ket], a.pair_np[ket],
-                    O[3 * sa] - O[3 * sb], O[3 * sa + 1] - O[3 * sb + 1], O[3 * sa + 2] - O[3 * sb + 2],
-                    O[3 * sc] - O[3 * sd], O[3 * sc + 1] - O[3 * sd + 1], O[3 * sc + 2] - O[3 * sd + 2], tab, gam_fill, g);
+                    R(O[3 * sa] - O[3 * sb]), R(O[3 * sa + 1] - O[3 * sb + 1]), R(O[3 * sa + 2] - O[3 * sb + 2]),
+                    R(O[3 * sc] - O[3 * sd]), R(O[3 * sc + 1] - O[3 * sd + 1]), R(O[3 * sc + 2] - O[3 * sd + 2]), tab,
+                    gam_fill, g);
             }
             const int at_a = a.sh_atom[sa], at_b = a.sh_atom[sb];
             // A and B are the same for the whole wavefront
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const double ga = wave_sum_g(g[i]), gb = wave_sum_g(g[3 + i]);
-                const double gc = wave_sum_g(g[6 + i]);
-                (void)gc;
+                const double ga = wave_sum_g(double(g[i])), gb = wave_sum_g(double(g[3 + i]));
                 if (lane == 0) {
                     atomicAdd(&bins[at_a * 3 + i], ga);
                     atomicAdd(&bins[at_b * 3 + i], gb);
@@ -129,8 +131,8 @@ __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, co
                 const int at_c = a.sh_atom[sc], at_d = a.sh_atom[sd];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    atomicAdd(&bins[at_c * 3 + i], g[6 + i]);
-                    atomicAdd(&bins[at_d * 3 + i], -(g[i] + g[3 + i] + g[6 + i]));
+                    atomicAdd(&bins[at_c * 3 + i], double(g[6 + i]));
+                    atomicAdd(&bins[at_d * 3 + i], -(double(g[i]) + double(g[3 + i]) + double(g[6 + i])));
                 }
             }
         }
@@ -141,10 +143,10 @@ __global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, co
 }
 
 
-template <int LA, int LB, int LC, int LD, int MASK>
+template <int LA, int LB, int LC, int LD, int MASK, typename R>
 void launch_reg(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom, hipStream_t st) {
-    hipLaunchKernelGGL((k_grad_reg<LA, LB, LC, LD, MASK>), dim3((nt + 3) / 4), dim3(256), sizeof(double) * natom * 3,
-                       st, a, tab, t, nt, natom);
+    hipLaunchKernelGGL((k_grad_reg<LA, LB, LC, LD, MASK, R>), dim3((nt + 3) / 4), dim3(256),
+                       sizeof(double) * natom * 3, st, a, tab, t, nt, natom);
 }
 
 }  // namespace m2

@@ -559,8 +559,16 @@ void Fock::compute_forces(const std::vector<Atom>& atoms, int nocc_) {
         xyz.push_back(a.z);
     }
     std::vector<double> g(3 * atoms.size());
-    check(ctx, m2_grad(ctx, (int)atoms.size(), Z.data(), xyz.data(), dens.data(), W.data(), precision, g.data()),
-          "m2_grad");
+    // a mixed-precision SCF also takes the quartets below its FP32 threshold
+    // of the gradient in FP32
+    unsigned flags = 0;
+    if (opt.mixed) {
+        check(ctx, m2_set_mixed_threshold(ctx, opt.mixed_threshold), "m2_set_mixed_threshold");
+        flags = M2_MIXED;
+    }
+    check(ctx,
+          m2_grad_ex(ctx, (int)atoms.size(), Z.data(), xyz.data(), dens.data(), W.data(), precision, flags, g.data()),
+          "m2_grad_ex");
     forces = Matrix((int)atoms.size(), 3);
     for (size_t a = 0; a < atoms.size(); ++a)
         for (int k = 0; k < 3; ++k) forces((int)a, k) = g[3 * a + k];

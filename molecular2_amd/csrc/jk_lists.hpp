@@ -26,6 +26,8 @@ struct GradTaskArgs {
     int gbeg[JK_MAXG], gend[JK_MAXG];
     int chunk;                // kets per task
     double precision, dmax;
+    int part;                 // 0: kets with Q_i Q_j max|D| >= split (FP64), 1: the rest (FP32)
+    double split;             // = precision when nothing runs in FP32
 };
 void launch_grad_task_count(const GradTaskArgs& g, int* counts, hipStream_t st);
 void launch_scan_excl(const int* in, int* out, int n, hipStream_t st);

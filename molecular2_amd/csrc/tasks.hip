@@ -132,16 +132,23 @@ __global__ __launch_bounds__(256) void k_task_fill(JKList c) {
 }
 
 // ---------------------------------------------------------------------------
-// Gradient task lists, built on the device in the order of the host's
-// build_tasks: for every bra pair i of class X (in pair order) and every ket
-// group of class Y, the screened prefix Q_j >= precision / (Q_i max|D|) (cut
-// at j <= i when X == Y) in chunks of `chunk` kets, one Task per chunk.
+// Gradient task lists, built on the device: for every bra pair i of class X
+// (in pair order) and every ket group of class Y, the screened prefix Q_j >=
+// precision / (Q_i max|D|) (cut at j <= i when X == Y) in chunks of `chunk`
+// kets, one Task per chunk; under mixed precision the prefix splits at
+// split / (Q_i max|D|) into an FP64 head (part 0) and an FP32 tail (part 1).
 //   k_grad_task_count: tasks per bra;  k_scan_excl: exclusive prefix sums;
 //   k_grad_task_fill: the Task entries at each bra's offset.
-__device__ __forceinline__ int grad_group_end(const GradTaskArgs& g, int i, double thr, int k) {
-    int end = g.gbeg[k] + count_ge_dev(g.Q, g.gbeg[k], g.gend[k], thr);
-    if (g.same) end = min(end, i + 1);
-    return end;
+// the ket range [lo, hi) of group k that bra i takes in g.part; qd = Q_i max|D|
+__device__ __forceinline__ int2 grad_group_range(const GradTaskArgs& g, int i, double qd, int k) {
+    int end = g.gbeg[k] + count_ge_dev(g.Q, g.gbeg[k], g.gend[k], g.precision / qd);
+    int mid = g.gbeg[k] + count_ge_dev(g.Q, g.gbeg[k], g.gend[k], g.split / qd);
+    if (g.same) {
+        end = min(end, i + 1);
+        mid = min(mid, i + 1);
+    }
+    mid = min(mid, end);
+    return g.part ? make_int2(mid, end) : make_int2(g.gbeg[k], mid);
 }
 
 __global__ __launch_bounds__(256) void k_grad_task_count(GradTaskArgs g, int* __restrict__ counts) {
@@ -151,10 +158,9 @@ __global__ __launch_bounds__(256) void k_grad_task_count(GradTaskArgs g, int* __
     const double qi = g.Q[i];
     int n = 0;
     if (qi > 0.0 && g.dmax > 0.0) {
-        const double thr = g.precision / (qi * g.dmax);
         for (int k = 0; k < g.ngrp; ++k) {
-            const int len = grad_group_end(g, i, thr, k) - g.gbeg[k];
-            if (len > 0) n += (len + g.chunk - 1) / g.chunk;
+            const int2 r = grad_group_range(g, i, qi * g.dmax, k);
+            if (r.y > r.x) n += (r.y - r.x + g.chunk - 1) / g.chunk;
         }
     }
     counts[t] = n;
@@ -167,11 +173,10 @@ __global__ __launch_bounds__(256) void k_grad_task_fill(GradTaskArgs g, const in
     const int i = g.x0 + t;
     const double qi = g.Q[i];
     if (!(qi > 0.0 && g.dmax > 0.0)) return;
-    const double thr = g.precision / (qi * g.dmax);
     int w = offs[t];
     for (int k = 0; k < g.ngrp; ++k) {
-        const int end = grad_group_end(g, i, thr, k);
-        for (int j = g.gbeg[k]; j < end; j += g.chunk) out[w++] = Task{i, j, min(g.chunk, end - j), 0};
+        const int2 r = grad_group_range(g, i, qi * g.dmax, k);
+        for (int j = r.x; j < r.y; j += g.chunk) out[w++] = Task{i, j, min(g.chunk, r.y - j), 0};
     }
 }
 

@@ -134,6 +134,7 @@ def lib():
         L.m2_ctx_device.argtypes = [vp]
         L.m2_ctx_device.restype = C.c_int
         L.m2_grad.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
+        L.m2_grad_ex.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint, _dp]
         for name in ("m2_grad", "m2_df_setup", "m2_jk_df", "m2_jk_df_device", "m2_df_timing", "m2_eigh", "m2_gemm", "m2_rhf"):
             getattr(L, name).restype = C.c_int
         for name in ("m2_set_profiling", "m2_get_profile", "m2_reset_profile", "m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
@@ -355,14 +356,17 @@ class Engine:
         self._chk(self.L.m2_eigh(self.h, n, A.ctypes.data_as(_dp), _p(w), V.ctypes.data_as(_dp)))
         return w, V
 
-    def grad(self, mol, D, W, precision=1e-12):
-        """dE/dR (natom x 3) for density D and energy-weighted density W."""
+    def grad(self, mol, D, W, precision=1e-12, mixed=False):
+        """dE/dR (natom x 3) for density D and energy-weighted density W.
+        mixed: register-class derivative quartets below the FP32 threshold
+        (set_mixed_threshold) in FP32 (m2_grad_ex with M2_MIXED)."""
         Z = np.asarray(mol.Z, float)
         X = np.ascontiguousarray(mol.xyz, float)
         D = np.ascontiguousarray(D, float)
         W = np.ascontiguousarray(W, float)
         g = np.zeros(mol.natom * 3)
-        self._chk(self.L.m2_grad(self.h, mol.natom, _p(Z), _p(X), _p(D), _p(W), precision, _p(g)))
+        self._chk(self.L.m2_grad_ex(self.h, mol.natom, _p(Z), _p(X), _p(D), _p(W), precision,
+                                    M2_MIXED if mixed else 0, _p(g)))
         return g.reshape(mol.natom, 3)
 
     def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, gradient=False, guess="hcore", **kw):

@@ -54,3 +54,27 @@ def test_gradient_finite_difference_on_device(engine):
         ep = engine.rhf(mol.displaced(atom, axis, h), **kw)["energy"]
         em = engine.rhf(mol.displaced(atom, axis, -h), **kw)["energy"]
         assert abs(g[atom, axis] - (ep - em) / (2 * h)) < 2e-7
+
+
+def test_mixed_gradient_against_fp64(engine):
+    """M2_MIXED: register-class derivative quartets below the FP32 threshold
+    (1e-4) in FP32.  Same density: within 1e-7 Eh/bohr of the FP64 gradient;
+    through a mixed-precision SCF: within the 1e-6 Eh/bohr target."""
+    mol = M.water_cluster(4, seed=4)
+    print("case: (H2O)4 mixed-precision gradient", flush=True)
+    kw = dict(enconverge=1e-10, densconverge=1e-8, maxiter=80)
+    r = engine.rhf(mol, gradient=True, **kw)
+    nocc = mol.nel // 2
+    Co = r["C"][:, :nocc]
+    W = Co @ np.diag(r["eps"][:nocc]) @ Co.T
+    engine.set_mixed_threshold(1e-4)
+    g64 = engine.grad(mol, r["D"], W)
+    g32 = engine.grad(mol, r["D"], W, mixed=True)
+    assert np.max(np.abs(g64 - r["grad"])) < 1e-10
+    d = np.max(np.abs(g32 - g64))
+    print("same density: max |dg| %.2e" % d, flush=True)
+    assert 0 < d < 1e-7
+    rm = engine.rhf(mol, gradient=True, mixed=1, **kw)
+    dm = np.max(np.abs(rm["grad"] - r["grad"]))
+    print("mixed SCF + gradient: max |dg| %.2e" % dm, flush=True)
+    assert dm < 1e-6
