@@ -215,8 +215,8 @@ def main():
             "scf_speedup": round((r64["total_seconds"] - r64["grad_seconds"]) /
                                  max(1e-9, rmx["total_seconds"] - rmx["grad_seconds"]), 3),
             "grad_speedup": round(r64["grad_seconds"] / max(1e-9, rmx["grad_seconds"]), 3),
-            "note": "the mixed gradient takes register-class derivative quartets below 1e-4 in FP32 "
-                    "(program classes and everything above in FP64), each on its own SCF's density",
+            "note": "the mixed gradient takes the derivative quartets whose bound is below 1e-4 in FP32, "
+                    "each gradient on its own SCF's density",
         }
     t = time.perf_counter()
     eng.set_basis(shells, pure=True)

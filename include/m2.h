@@ -139,8 +139,8 @@ m2_status m2_df_timing(m2_ctx* ctx, double* ms4);
  * `precision`).  Z / xyz are the nuclear charges and positions (bohr). */
 m2_status m2_grad(m2_ctx* ctx, int natom, const double* Z, const double* xyz, const double* D,
                   const double* W, double precision, double* g);
-/* m2_grad with flags: M2_MIXED computes the register-class derivative
- * quartets whose bound Q_ab Q_cd max|D| lies below the context's FP32
+/* m2_grad with flags: M2_MIXED computes the derivative quartets whose
+ * bound Q_ab Q_cd max|D| lies below the context's FP32
  * threshold (m2_set_mixed_threshold) in FP32 arithmetic, summed in FP64 (the
  * mixed-precision counterpart of the reference's FP64 compute_forces,
  * src/rhfgrad.cpp:15-89); 0 is m2_grad. */

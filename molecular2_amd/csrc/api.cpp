@@ -788,8 +788,7 @@ hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, bool mi
             g.chunk = reg ? 64 : GRAD_PROG_CHUNK;
             g.precision = precision;
             g.dmax = Dmax;
-            // program classes run in FP64 only
-            g.split = mixed && reg ? std::max(precision, ctx->fp32_below) : precision;
+            g.split = mixed ? std::max(precision, ctx->fp32_below) : precision;
             for (int part = 0; part < 2; ++part) {
                 g.part = part;
                 args.push_back(g);

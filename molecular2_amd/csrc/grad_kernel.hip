@@ -43,8 +43,8 @@ struct DerivProg {
 
 // The VRR phase of one primitive quartet from its pass-ordered words, the
 // next pass's words loaded while the current one computes.
-__device__ __forceinline__ void vrr_phase_passes(const DerivProg& P, double* __restrict__ X,
-                                                 const double* __restrict__ S) {
+template <typename R>
+__device__ __forceinline__ void vrr_phase_passes(const DerivProg& P, R* __restrict__ X, const R* __restrict__ S) {
     constexpr int NW = 7;
     const int lane = threadIdx.x;
     const int* wp = P.vw + lane;
@@ -61,13 +61,13 @@ __device__ __forceinline__ void vrr_phase_passes(const DerivProg& P, double* __r
         if (h >= 0) {
             if ((kind & 3) == 0) {
                 const int nt = h >> 16;
-                double v = 0.0;
+                R v = R(0);
 #pragma unroll
                 for (int t = 0; t < NW - 1; ++t)
                     if (t < nt) {
                         const int tw = cur[1 + t];
                         const int c6 = (tw >> 19) & 63;
-                        v += double(c6 >= 32 ? c6 - 64 : c6) * S[(tw >> 14) & 31] * X[tw & 16383];
+                        v += R(c6 >= 32 ? c6 - 64 : c6) * S[(tw >> 14) & 31] * X[tw & 16383];
                     }
                 X[h & 65535] = v;
             } else {
@@ -80,34 +80,40 @@ __device__ __forceinline__ void vrr_phase_passes(const DerivProg& P, double* __r
     }
 }
 
-__device__ __forceinline__ void prog_level(const DerivProg& P, int i0, int i1, double* X, const double* S) {
+template <typename R>
+__device__ __forceinline__ void prog_level(const DerivProg& P, int i0, int i1, R* X, const R* S) {
     for (int j = i0 + (int)threadIdx.x; j < i1; j += 64) {
         const int* in = P.ins + 14 * j;
         const double* cf = P.coef + 6 * j;
         const int nt = in[1];
-        double v = 0.0;
-        for (int t = 0; t < nt; ++t) v += cf[t] * S[in[8 + t]] * X[in[2 + t]];
+        R v = R(0);
+        for (int t = 0; t < nt; ++t) v += R(cf[t]) * S[in[8 + t]] * X[in[2 + t]];
         X[in[0]] = v;
     }
 }
 
-__device__ __forceinline__ void prog_contract(const DerivProg& P, int c0, int c1, const double* X, const double* Gam,
+template <typename R>
+__device__ __forceinline__ void prog_contract(const DerivProg& P, int c0, int c1, const R* X, const R* Gam,
                                               double* g) {
     for (int j = c0 + (int)threadIdx.x; j < c1; j += 64) {
         const int* c = P.contr + 3 * j;
-        g[c[2]] += P.ccoef[j] * Gam[c[1]] * X[c[0]];
+        g[c[2]] += double(R(P.ccoef[j]) * Gam[c[1]] * X[c[0]]);
     }
 }
 
+// R: type of the LDS slots and of the VRR / HRR arithmetic (float for the
+// quartets a mixed-precision gradient leaves below its FP32 threshold); the
+// Boys values, scalars and density blocks are formed in double.
+template <typename R>
 __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
                                                   DerivProg P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int NA = ncart_d(P.la), NB = ncart_d(P.lb), NC = ncart_d(P.lc), ND = ncart_d(P.ld);
     const int ngam = NA * NB * NC * ND;
-    double* X = reinterpret_cast<double*>(smem);
-    double* S = X + P.nslots;
-    double* Gam = S + NSCALAR_D;
-    double* Dt = Gam + ngam;   // 6 Cartesian density blocks, 36 each
+    double* Dt = reinterpret_cast<double*>(smem);   // 6 Cartesian density blocks, 36 each
+    R* X = reinterpret_cast<R*>(Dt + 6 * 36);
+    R* S = X + P.nslots;
+    R* Gam = S + NSCALAR_D;
     // GRAD_PROG_SPLIT workgroups per task, each walking GRAD_PROG_KETS of its kets
     const int w = blockIdx.x / GRAD_PROG_SPLIT, part = blockIdx.x - w * GRAD_PROG_SPLIT;
     if (w >= ntasks) return;
@@ -139,11 +145,11 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
         const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);
         for (int t = lane; t < ngam; t += 64) {
             const int l = t % ND, k = (t / ND) % NC, j = (t / (ND * NC)) % NB, i = t / (ND * NC * NB);
-            Gam[t] = deg * (0.5 * Dt[0 * 36 + i * NB + j] * Dt[1 * 36 + k * ND + l] -
-                            0.125 * (Dt[2 * 36 + i * NC + k] * Dt[3 * 36 + j * ND + l] +
-                                     Dt[4 * 36 + i * ND + l] * Dt[5 * 36 + j * NC + k]));
+            Gam[t] = R(deg * (0.5 * Dt[0 * 36 + i * NB + j] * Dt[1 * 36 + k * ND + l] -
+                              0.125 * (Dt[2 * 36 + i * NC + k] * Dt[3 * 36 + j * ND + l] +
+                                       Dt[4 * 36 + i * ND + l] * Dt[5 * 36 + j * NC + k])));
         }
-        for (int t = lane; t < P.nacc; t += 64) X[t] = 0.0;
+        for (int t = lane; t < P.nacc; t += 64) X[t] = R(0);
         const PrimPair* bp = a.prims + a.pair_off[tk.bra];
         const PrimPair* kp = a.prims + a.pair_off[ket];
         const int nbp = a.pair_np[tk.bra], nkp = a.pair_np[ket];
@@ -175,20 +181,17 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
                         const double h = 0.5 / T;
                         for (int i = 0; i < lane; ++i) F = (2 * i + 1) * F * h;
                     }
-                    X[P.base + lane] = B.K * K.K * rs * F;
+                    X[P.base + lane] = R(B.K * K.K * rs * F);
                 }
                 if (lane == 0) {
                     const double rp = q * ipq, rq = p * ipq;
-                    S[0] = 1.0;
-                    S[1] = B.PA[0]; S[2] = B.PA[1]; S[3] = B.PA[2];
-                    S[4] = -rp * dx; S[5] = -rp * dy; S[6] = -rp * dz;
-                    S[7] = K.PA[0]; S[8] = K.PA[1]; S[9] = K.PA[2];
-                    S[10] = rq * dx; S[11] = rq * dy; S[12] = rq * dz;
-                    S[13] = B.inv2p; S[14] = K.inv2p; S[15] = 0.5 * ipq;
-                    S[16] = B.inv2p * rp; S[17] = K.inv2p * rq;
-                    S[18] = ABx; S[19] = ABy; S[20] = ABz;
-                    S[21] = CDx; S[22] = CDy; S[23] = CDz;
-                    S[24] = 2.0 * B.a; S[25] = 2.0 * (B.p - B.a); S[26] = 2.0 * K.a;
+                    const double sv[NSCALAR_D] = {1.0, B.PA[0], B.PA[1], B.PA[2], -rp * dx, -rp * dy, -rp * dz,
+                                                  K.PA[0], K.PA[1], K.PA[2], rq * dx, rq * dy, rq * dz,
+                                                  B.inv2p, K.inv2p, 0.5 * ipq, B.inv2p * rp, K.inv2p * rq,
+                                                  ABx, ABy, ABz, CDx, CDy, CDz,
+                                                  2.0 * B.a, 2.0 * (B.p - B.a), 2.0 * K.a};
+#pragma unroll
+                    for (int q = 0; q < NSCALAR_D; ++q) S[q] = R(sv[q]);
                 }
                 __syncthreads();
                 if (P.vw) {
@@ -553,8 +556,7 @@ hipError_t deriv_upload(int device) {
 
 #include "generated/deriv_dispatch.inc"
 
-// fp32: FP32 arithmetic (register classes only; the host gives the program
-// classes FP64 tasks alone)
+// fp32: FP32 arithmetic for the tasks' integrals
 hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const GradArgs& a, const BoysTables& tab,
                           const Task* t, int nt, int natom, int fp32, hipStream_t st) {
     if (nt <= 0) return hipSuccess;
@@ -600,8 +602,13 @@ hipError_t launch_grad_2e(int device, int la, int lb, int lc, int ld, const Grad
             }
         }
         const int ngam = d[14] * d[15] * d[16] * d[17];
-        const size_t sm = sizeof(double) * (P.nslots + NSCALAR_D + ngam + 6 * 36);
-        hipLaunchKernelGGL(k_grad_prog, dim3((unsigned)nt * GRAD_PROG_SPLIT), dim3(64), sm, st, a, tab, t, nt, P);
+        const size_t slots = (size_t)P.nslots + NSCALAR_D + ngam;
+        if (fp32)
+            hipLaunchKernelGGL(k_grad_prog<float>, dim3((unsigned)nt * GRAD_PROG_SPLIT), dim3(64),
+                               sizeof(double) * 6 * 36 + sizeof(float) * slots, st, a, tab, t, nt, P);
+        else
+            hipLaunchKernelGGL(k_grad_prog<double>, dim3((unsigned)nt * GRAD_PROG_SPLIT), dim3(64),
+                               sizeof(double) * (6 * 36 + slots), st, a, tab, t, nt, P);
     }
     return hipGetLastError();
 }

@@ -358,7 +358,7 @@ class Engine:
 
     def grad(self, mol, D, W, precision=1e-12, mixed=False):
         """dE/dR (natom x 3) for density D and energy-weighted density W.
-        mixed: register-class derivative quartets below the FP32 threshold
+        mixed: derivative quartets below the FP32 threshold
         (set_mixed_threshold) in FP32 (m2_grad_ex with M2_MIXED)."""
         Z = np.asarray(mol.Z, float)
         X = np.ascontiguousarray(mol.xyz, float)

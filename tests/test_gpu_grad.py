@@ -57,7 +57,7 @@ def test_gradient_finite_difference_on_device(engine):
 
 
 def test_mixed_gradient_against_fp64(engine):
-    """M2_MIXED: register-class derivative quartets below the FP32 threshold
+    """M2_MIXED: derivative quartets below the FP32 threshold
     (1e-4) in FP32.  Same density: within 1e-7 Eh/bohr of the FP64 gradient;
     through a mixed-precision SCF: within the 1e-6 Eh/bohr target."""
     mol = M.water_cluster(4, seed=4)
