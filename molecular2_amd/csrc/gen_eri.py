@@ -1407,18 +1407,27 @@ def write_deriv_programs(outdir):
 
 DERIV_REGISTER = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0), (1, 1, 1, 0), (2, 0, 1, 0),
                   (2, 1, 0, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0), (2, 2, 0, 0),
-                  (2, 2, 1, 0)]
+                  (2, 2, 1, 0), (2, 1, 1, 1)]
 # classes whose three derivative centres run as three lighter passes
 DERIV_SPLIT = {(1, 1, 1, 0), (2, 0, 1, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0),
-               (2, 2, 0, 0), (2, 2, 1, 0)}
+               (2, 2, 0, 0), (2, 2, 1, 0), (2, 1, 1, 1)}
+# split classes whose every centre pass is split again over the c components
+# of Gamma (fewer accumulators per pass, each pass with its own VRR)
+DERIV_CSPLIT = {(2, 1, 1, 1)}
 
 
 def gen_deriv_class(la, lb, lc, ld, mask=7):
     """Register-resident derivative adjoint of one small class:
     EriDeriv<la,lb,lc,ld,mask>::run<R>(..., gam_fill, g) adds to g[0..8] the
     contraction of d/dA (mask & 1), d/dB (mask & 2), d/dC (mask & 4) of
-    (ab|cd) (Cartesian) with Gamma; the other entries of g are left alone."""
+    (ab|cd) (Cartesian) with Gamma; the other entries of g are left alone.
+    mask >> 3 = k > 0 keeps only the Gamma entries whose c component is
+    comps(lc)[k - 1] (one of the passes of a class split over its c
+    components): the HRR is generated on demand for those outputs, and only
+    the accumulators it reads are kept."""
     Lp = la + lb + lc + ld + 1
+    part = mask >> 3
+    cmask = mask & 7
 
     def rng(lo, hi):
         return [c for L_ in range(max(lo, 0), hi + 1) for c in comps(L_)]
@@ -1426,28 +1435,118 @@ def gen_deriv_class(la, lb, lc, ld, mask=7):
     def rect(E, F):
         return [(e, f) for e in E for f in F]
 
-    # unweighted targets: only those the lowered-index terms -a(a-1), -b(b-1),
-    # -c(c-1) reach through the HRR
-    s0 = []
-    if la > 0 and mask & 1:
-        s0 += rect(rng(la - 1, la - 1 + lb), rng(lc, lc + ld))
-    if lb > 0 and mask & 2:
-        s0 += rect(rng(la, la + lb - 1), rng(lc, lc + ld))
-    if lc > 0 and mask & 4:
-        s0 += rect(rng(la, la + lb), rng(lc - 1, lc - 1 + ld))
-    s0 = sorted(set(s0))
-    sets = {"s0": (s0, None)}
-    if mask & 1:
-        sets["sa"] = (rect(rng(la + 1, la + lb + 1), rng(lc, lc + ld)), "wA")
-    if mask & 2:
-        sets["sb"] = (rect(rng(la, la + lb + 1), rng(lc, lc + ld)), "wB")
-    if mask & 4:
-        sets["sc"] = (rect(rng(la, la + lb), rng(lc + 1, lc + ld + 1)), "wC")
+    ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
+    gidx = {}
+    for ia, a in enumerate(ca):
+        for ib, b in enumerate(cb):
+            for ic, c in enumerate(cc):
+                for idd, d in enumerate(cd):
+                    gidx[(a, b, c, d)] = ((ia * len(cb) + ib) * len(cc) + ic) * len(cd) + idd
+    keep_c = cc[part - 1] if part else None
+
+    # post phase first (into its own emitter): HRR per term, on demand, then
+    # the contraction; records the accumulators it reads
+    post = Emitter()
+    used = set()
+    defined = set()
+
+    def hrr_get(acc, sa, sb, sc, sd, quad):
+        memo = {}
+
+        def kget(e, c, d):
+            if sum(d) == 0:
+                used.add((acc, e, c))
+                return "%s_%s_%s" % (acc, cname(e), cname(c))
+            key = ("k", e, c, d)
+            if key in memo:
+                return memo[key]
+            i = next(q for q in range(3) if d[q] > 0)
+            n1 = kget(e, add(c, i), sub(d, i))
+            n0 = kget(e, c, sub(d, i))
+            nm = "k%s_%d%d%d%d_%s_%s_%s" % (acc, sa, sb, sc, sd, cname(e), cname(c), cname(d))
+            if nm not in defined:
+                post.out("    const R %s = %s + CD%s * %s;" % (nm, n1, AX[i], n0))
+                defined.add(nm)
+            memo[key] = nm
+            return nm
+
+        def bget(a, b, c, d):
+            if sum(b) == 0:
+                return kget(a, c, d)
+            key = ("b", a, b, c, d)
+            if key in memo:
+                return memo[key]
+            i = next(q for q in range(3) if b[q] > 0)
+            n1 = bget(add(a, i), sub(b, i), c, d)
+            n0 = bget(a, sub(b, i), c, d)
+            nm = "h%s_%d%d%d%d_%s_%s_%s_%s" % (acc, sa, sb, sc, sd, cname(a), cname(b), cname(c), cname(d))
+            if nm not in defined:
+                post.out("    const R %s = %s + AB%s * %s;" % (nm, n1, AX[i], n0))
+                defined.add(nm)
+            memo[key] = nm
+            return nm
+
+        return bget(*quad)
+
+    terms = []
+    if cmask & 1:
+        terms.append(("sa", (la + 1, lb, lc, ld), 0, +1))
+    if cmask & 2:
+        terms.append(("sb", (la, lb + 1, lc, ld), 1, +1))
+    if cmask & 4:
+        terms.append(("sc", (la, lb, lc + 1, ld), 2, +1))
+    if la > 0 and cmask & 1:
+        terms.append(("s0", (la - 1, lb, lc, ld), 0, -1))
+    if lb > 0 and cmask & 2:
+        terms.append(("s0", (la, lb - 1, lc, ld), 1, -1))
+    if lc > 0 and cmask & 4:
+        terms.append(("s0", (la, lb, lc - 1, ld), 2, -1))
+    sums = {k: [] for k in range(9)}
+    for acc, cls, ctr, sgn in terms:
+        for a in comps(cls[0]):
+            for b in comps(cls[1]):
+                for c in comps(cls[2]):
+                    for d in comps(cls[3]):
+                        quad = [a, b, c, d]
+                        for i in range(3):
+                            q = list(quad[ctr])
+                            if sgn > 0:
+                                if q[i] == 0:
+                                    continue
+                                q[i] -= 1
+                                coef = None
+                            else:
+                                q[i] += 1
+                                coef = float(q[i])
+                            src = list(quad)
+                            src[ctr] = tuple(q)
+                            if keep_c is not None and src[2] != keep_c:
+                                continue
+                            nm = hrr_get(acc, *cls, (a, b, c, d))
+                            sums[3 * ctr + i].append(("+" if coef is None else "-", coef, gidx[tuple(src)], nm))
+    for k in range(9):
+        if not sums[k]:
+            continue
+        parts = []
+        for sg, coef, gi, nm in sums[k]:
+            if coef is None:
+                parts.append("+ Gam[%d] * %s" % (gi, nm))
+            else:
+                parts.append("- %s * Gam[%d] * %s" % (lit(coef), gi, nm))
+        post.out("    g[%d] += R(0) %s;" % (k, " ".join(parts)))
+
+    # accumulator sets, pruned to what the post phase reads
+    weight = {"s0": None, "sa": "wA", "sb": "wB", "sc": "wC"}
+    sets = {}
+    for nm in ("s0", "sa", "sb", "sc"):
+        T_ = sorted((e, f) for a_, e, f in used if a_ == nm)
+        if T_:
+            sets[nm] = (T_, weight[nm])
     targets = sorted({t for T_, _ in sets.values() for t in T_})
     order, nodes = build_vrr(targets, Lp)
     em = Emitter()
     em.out("template <> struct EriDeriv<%d, %d, %d, %d, %d> {" % (la, lb, lc, ld, mask))
-    ngam = len(comps(la)) * len(comps(lb)) * len(comps(lc)) * len(comps(ld))
+    ngam = len(ca) * len(cb) * len(cc) * len(cd)
     em.out("  static constexpr int NACC = %d;" % sum(len(T_) for T_, _ in sets.values()))
     em.out("  template <typename R, typename GamFill>")
     em.out("  static __device__ __forceinline__ void run(const PrimPair* __restrict__ bra, int nbra,")
@@ -1498,101 +1597,7 @@ def gen_deriv_class(la, lb, lc, ld, mask=7):
     # Gamma is formed only now, so that it does not hold registers in the loop
     em.out("    R Gam[%d];" % ngam)
     em.out("    gam_fill(Gam);")
-    # post: HRR per term, then contraction
-    ca, cb, cc, cd = comps(la), comps(lb), comps(lc), comps(ld)
-    gidx = {}
-    for ia, a in enumerate(ca):
-        for ib, b in enumerate(cb):
-            for ic, c in enumerate(cc):
-                for idd, d in enumerate(cd):
-                    gidx[(a, b, c, d)] = ((ia * len(cb) + ib) * len(cc) + ic) * len(cd) + idd
-    defined = set()
-
-    def hrr(acc, sa, sb, sc, sd):
-        vals = {}
-        memo = {}
-
-        def kget(e, c, d):
-            if sum(d) == 0:
-                return "%s_%s_%s" % (acc, cname(e), cname(c))
-            key = ("k", e, c, d)
-            if key in memo:
-                return memo[key]
-            i = next(q for q in range(3) if d[q] > 0)
-            n1 = kget(e, add(c, i), sub(d, i))
-            n0 = kget(e, c, sub(d, i))
-            nm = "k%s_%d%d%d%d_%s_%s_%s" % (acc, sa, sb, sc, sd, cname(e), cname(c), cname(d))
-            if nm not in defined:
-                em.out("    const R %s = %s + CD%s * %s;" % (nm, n1, AX[i], n0))
-                defined.add(nm)
-            memo[key] = nm
-            return nm
-
-        def bget(a, b, c, d):
-            if sum(b) == 0:
-                return kget(a, c, d)
-            key = ("b", a, b, c, d)
-            if key in memo:
-                return memo[key]
-            i = next(q for q in range(3) if b[q] > 0)
-            n1 = bget(add(a, i), sub(b, i), c, d)
-            n0 = bget(a, sub(b, i), c, d)
-            nm = "h%s_%d%d%d%d_%s_%s_%s_%s" % (acc, sa, sb, sc, sd, cname(a), cname(b), cname(c), cname(d))
-            if nm not in defined:
-                em.out("    const R %s = %s + AB%s * %s;" % (nm, n1, AX[i], n0))
-                defined.add(nm)
-            memo[key] = nm
-            return nm
-
-        for a in comps(sa):
-            for b in comps(sb):
-                for c in comps(sc):
-                    for d in comps(sd):
-                        vals[(a, b, c, d)] = bget(a, b, c, d)
-        return vals
-
-    terms = []
-    if mask & 1:
-        terms.append(("sa", (la + 1, lb, lc, ld), 0, +1))
-    if mask & 2:
-        terms.append(("sb", (la, lb + 1, lc, ld), 1, +1))
-    if mask & 4:
-        terms.append(("sc", (la, lb, lc + 1, ld), 2, +1))
-    if la > 0 and mask & 1:
-        terms.append(("s0", (la - 1, lb, lc, ld), 0, -1))
-    if lb > 0 and mask & 2:
-        terms.append(("s0", (la, lb - 1, lc, ld), 1, -1))
-    if lc > 0 and mask & 4:
-        terms.append(("s0", (la, lb, lc - 1, ld), 2, -1))
-    sums = {k: [] for k in range(9)}
-    for acc, cls, ctr, sgn in terms:
-        vals = hrr(acc, *cls)
-        for (a, b, c, d), nm in vals.items():
-            quad = [a, b, c, d]
-            for i in range(3):
-                q = list(quad[ctr])
-                if sgn > 0:
-                    if q[i] == 0:
-                        continue
-                    q[i] -= 1
-                    coef = None
-                else:
-                    q[i] += 1
-                    coef = float(q[i])
-                src = list(quad)
-                src[ctr] = tuple(q)
-                gi = gidx[tuple(src)]
-                sums[3 * ctr + i].append(("+" if coef is None else "-", coef, gi, nm))
-    for k in range(9):
-        if not sums[k]:
-            continue
-        parts = []
-        for sg, coef, gi, nm in sums[k]:
-            if coef is None:
-                parts.append("+ Gam[%d] * %s" % (gi, nm))
-            else:
-                parts.append("- %s * Gam[%d] * %s" % (lit(coef), gi, nm))
-        em.out("    g[%d] += R(0) %s;" % (k, " ".join(parts)))
+    em.lines.extend(post.lines)
     em.out("  }")
     em.out("};")
     return "\n".join(em.lines)
@@ -1612,6 +1617,8 @@ def write_deriv_register(outdir):
     for c in DERIV_REGISTER:
         tag = "%d%d%d%d" % c
         masks = (1, 2, 4) if c in DERIV_SPLIT else (7,)
+        if c in DERIV_CSPLIT:
+            masks = tuple(m | (k << 3) for m in masks for k in range(1, len(comps(c[2])) + 1))
         body = [head, '#include "../grad_reg.hpp"', "namespace m2 {"]
         for m in masks:
             body.append(gen_deriv_class(*c, mask=m))
