@@ -1407,13 +1407,13 @@ def write_deriv_programs(outdir):
 
 DERIV_REGISTER = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0), (1, 1, 1, 0), (2, 0, 1, 0),
                   (2, 1, 0, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0), (2, 2, 0, 0),
-                  (2, 2, 1, 0), (2, 1, 1, 1)]
+                  (2, 2, 1, 0), (2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0)]
 # classes whose three derivative centres run as three lighter passes
 DERIV_SPLIT = {(1, 1, 1, 0), (2, 0, 1, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0),
-               (2, 2, 0, 0), (2, 2, 1, 0), (2, 1, 1, 1)}
+               (2, 2, 0, 0), (2, 2, 1, 0), (2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0)}
 # split classes whose every centre pass is split again over the c components
 # of Gamma (fewer accumulators per pass, each pass with its own VRR)
-DERIV_CSPLIT = {(2, 1, 1, 1)}
+DERIV_CSPLIT = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0)}
 
 
 def gen_deriv_class(la, lb, lc, ld, mask=7):
