@@ -219,3 +219,29 @@ def test_heavy_pass_words_encode_the_programs():
                 got.append(cur)
                 cur = []
         assert got == [[(dst, [tuple(x) for x in t]) for dst, t in lvl] for lvl in P["post_levels"]]
+
+
+def _contraction_terms(code):
+    """(g component, Gamma index, sign, coefficient) of every term of the
+    g[k] += lines"""
+    out = []
+    for line in code.split("\n"):
+        m = re.match(r"\s+g\[(\d)\] \+= R\(0\) (.*);", line)
+        if not m:
+            continue
+        for sg, coef, gi in re.findall(r"([+-]) (?:(R\([-0-9.e]+\)) \* )?Gam\[(\d+)\]", m.group(2)):
+            out.append((int(m.group(1)), int(gi), sg, coef))
+    return sorted(out)
+
+
+def test_derivative_class_split_passes_partition_the_contraction():
+    """The passes of a class split over the c components of Gamma together
+    contract exactly the terms of the unsplit centre pass."""
+    import gen_eri as G
+    for c in sorted(G.DERIV_CSPLIT):
+        for m in (1, 2, 4):
+            whole = _contraction_terms(G.gen_deriv_class(*c, mask=m))
+            parts = []
+            for k in range(1, len(G.comps(c[2])) + 1):
+                parts += _contraction_terms(G.gen_deriv_class(*c, mask=m | (k << 3)))
+            assert whole and any(t[2] == "-" for t in whole) and sorted(parts) == whole, (c, m)
