@@ -134,15 +134,18 @@ def df_bench(eng, mol, Cocc, n, steps, torch, dev):
 
 
 def batch_bench(eng, nmol=8):
-    """Batched SCF (m2_rhf_batch) on the first `nmol` molecules of the seeded
-    256-molecule H/C/N/O batch (3-30 atoms, cc-pVDZ), FP64 and mixed;
-    molecules/s counts every molecule of the sample,
-    converged or not.  maxiter is raised to 100 (the random clusters need
-    50-100 iterations); the other options are the reference's defaults."""
+    """Batched SCF (m2_rhf_batch_min) on the first `nmol` molecules of the
+    seeded 256-molecule H/C/N/O batch (3-30 atoms, cc-pVDZ), FP64 and mixed,
+    from the SOAD guess (the reference's default); molecules/s counts every
+    molecule of the sample, converged or not.  maxiter is raised to 100 (the
+    random clusters need 50-100 iterations); the other options are the
+    reference's defaults.  A core-Hamiltonian-guess FP64 run is listed beside
+    them."""
     from molecular2_amd import molecules as M
     mols = M.random_batch(256, seed=256)[:nmol]
-    r64, w64 = eng.rhf_batch(mols, nworkers=8, maxiter=100)
-    rmx, wmx = eng.rhf_batch(mols, nworkers=8, maxiter=100, mixed=1)
+    r64, w64 = eng.rhf_batch(mols, nworkers=8, maxiter=100, guess="soad")
+    rmx, wmx = eng.rhf_batch(mols, nworkers=8, maxiter=100, mixed=1, guess="soad")
+    rhc, whc = eng.rhf_batch(mols, nworkers=8, maxiter=100, guess="hcore")
     both = [(a["energy"], b["energy"]) for a, b in zip(r64, rmx) if a["converged"] and b["converged"]]
     dE = max((abs(x - y) for x, y in both), default=None)
     return {"sample": "first %d of random_batch(256, seed=256): %s atoms" % (nmol, [m.natom for m in mols]),
@@ -151,6 +154,9 @@ def batch_bench(eng, nmol=8):
                      "converged": sum(r["converged"] for r in r64), "iterations": sum(r["iterations"] for r in r64)},
             "mixed": {"molecules_per_s": round(nmol / wmx, 3), "wall_s": round(wmx, 2),
                       "converged": sum(r["converged"] for r in rmx), "iterations": sum(r["iterations"] for r in rmx)},
+            "fp64_core_guess": {"molecules_per_s": round(nmol / whc, 3), "wall_s": round(whc, 2),
+                                "converged": sum(r["converged"] for r in rhc),
+                                "iterations": sum(r["iterations"] for r in rhc)},
             "max_abs_dE_Eh_converged": dE}
 
 

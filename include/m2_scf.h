@@ -85,12 +85,25 @@ int m2_uhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
  * stream, so the SCFs of different molecules overlap on the GPU; molecules
  * are taken largest basis first.  res[m] is m2_rhf's result for molecule m
  * (iterations = -1 if its run failed).  Core-Hamiltonian guess only
- * (o->guess must be 0).  Returns 0 when every molecule ran (converged or
+ * (o->guess must be 0; m2_rhf_batch_min below takes the SOAD guess).
+ * Returns 0 when every molecule ran (converged or
  * not); otherwise m2_rhf_last_error() names the first failure. */
 int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int* charge, const double* Z,
                  const double* xyz, int pure, const int* nsh, const int* l, const int* nprim, const double* exps,
                  const double* coefs, const double* centers, const int* atoms, const m2_scf_options_c* o,
                  m2_scf_result_c* res);
+
+/* m2_rhf_batch with each molecule's minimal (STO-3G) shells for the SOAD
+ * guess (o->guess = 1, the reference's default, src/ProgramController.cpp:307):
+ * nsh_min[m] shells per molecule, concatenated like the orbital shells and
+ * grouped by atom in atom order (atoms_min local to the molecule).  With
+ * nsh_min = NULL it is m2_rhf_batch. */
+int m2_rhf_batch_min(int device, int nworkers, int nmol, const int* natom, const int* charge, const double* Z,
+                     const double* xyz, int pure, const int* nsh, const int* l, const int* nprim, const double* exps,
+                     const double* coefs, const double* centers, const int* atoms, const int* nsh_min,
+                     const int* l_min, const int* nprim_min, const double* exps_min, const double* coefs_min,
+                     const double* centers_min, const int* atoms_min, const m2_scf_options_c* o,
+                     m2_scf_result_c* res);
 
 /* Fock::compute_hessian_numeric (src/rhfgrad.cpp:362-416): the RHF nuclear
  * Hessian (3*natom x 3*natom, Eh/bohr^2) as central differences of the

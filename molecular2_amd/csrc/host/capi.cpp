@@ -219,19 +219,39 @@ int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int
                  const double* xyz, int pure, const int* nsh, const int* l, const int* nprim, const double* exps,
                  const double* coefs, const double* centers, const int* atoms, const m2_scf_options_c* o,
                  m2_scf_result_c* res) {
+    if (o && o->guess != 0) {
+        g_rhf_error = "m2_rhf_batch: the SOAD guess needs the minimal shells of m2_rhf_batch_min";
+        return 1;
+    }
+    return m2_rhf_batch_min(device, nworkers, nmol, natom, charge, Z, xyz, pure, nsh, l, nprim, exps, coefs, centers,
+                            atoms, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, o, res);
+}
+
+int m2_rhf_batch_min(int device, int nworkers, int nmol, const int* natom, const int* charge, const double* Z,
+                     const double* xyz, int pure, const int* nsh, const int* l, const int* nprim, const double* exps,
+                     const double* coefs, const double* centers, const int* atoms, const int* nsh_min,
+                     const int* l_min, const int* nprim_min, const double* exps_min, const double* coefs_min,
+                     const double* centers_min, const int* atoms_min, const m2_scf_options_c* o,
+                     m2_scf_result_c* res) {
     g_rhf_error.clear();
     if (nmol < 0 || (nmol > 0 && (!natom || !Z || !xyz || !nsh || !l || !nprim || !exps || !coefs || !centers ||
                                   !atoms || !o || !res))) {
         g_rhf_error = "m2_rhf_batch: null argument";
         return 1;
     }
-    if (o->guess != 0) {
-        g_rhf_error = "m2_rhf_batch: only the core-Hamiltonian guess (guess = 0) is available in batch mode";
+    const bool minimal = nsh_min != nullptr;
+    if (minimal && (!l_min || !nprim_min || !exps_min || !coefs_min || !centers_min || !atoms_min)) {
+        g_rhf_error = "m2_rhf_batch_min: null minimal-shell argument";
+        return 1;
+    }
+    if (o->guess != 0 && !minimal) {
+        g_rhf_error = "m2_rhf_batch_min: the SOAD guess (guess = 1) needs the minimal shells";
         return 1;
     }
     if (nmol == 0) return 0;
     // offsets of every molecule into the concatenated arrays
     std::vector<long> a_off(nmol + 1, 0), s_off(nmol + 1, 0), p_off(nmol + 1, 0);
+    std::vector<long> ms_off(nmol + 1, 0), mp_off(nmol + 1, 0);
     std::vector<long> nbf(nmol, 0);
     for (int m = 0; m < nmol; ++m) {
         if (natom[m] <= 0 || nsh[m] <= 0) {
@@ -246,6 +266,12 @@ int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int
             nbf[m] += pure ? 2 * l[s] + 1 : (l[s] + 1) * (l[s] + 2) / 2;
         }
         p_off[m + 1] = p_off[m] + np;
+        if (minimal) {
+            ms_off[m + 1] = ms_off[m] + nsh_min[m];
+            long mp = 0;
+            for (long s = ms_off[m]; s < ms_off[m + 1]; ++s) mp += nprim_min[s];
+            mp_off[m + 1] = mp_off[m] + mp;
+        }
     }
     std::vector<int> order(nmol);
     std::iota(order.begin(), order.end(), 0);
@@ -268,10 +294,14 @@ int m2_rhf_batch(int device, int nworkers, int nmol, const int* natom, const int
         for (int k; (k = next.fetch_add(1)) < nmol;) {
             const int m = order[k];
             const long a0 = a_off[m], s0 = s_off[m], p0 = p_off[m];
+            const long ms0 = ms_off[m], mp0 = mp_off[m];
             const int st = m2_rhf(ctx, natom[m], Z + a0, xyz + 3 * a0, charge ? charge[m] : 0, 1, pure, nsh[m], l + s0,
                                   nprim + s0, exps + p0, coefs + p0, centers + 3 * s0, atoms + s0, 0, nullptr, nullptr,
-                                  nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                  nullptr, o, &res[m], nullptr, nullptr, nullptr, nullptr, nullptr);
+                                  nullptr, nullptr, nullptr, nullptr, minimal ? nsh_min[m] : 0,
+                                  minimal ? l_min + ms0 : nullptr, minimal ? nprim_min + ms0 : nullptr,
+                                  minimal ? exps_min + mp0 : nullptr, minimal ? coefs_min + mp0 : nullptr,
+                                  minimal ? centers_min + 3 * ms0 : nullptr, minimal ? atoms_min + ms0 : nullptr, o,
+                                  &res[m], nullptr, nullptr, nullptr, nullptr, nullptr);
             if (st != 0) {
                 std::lock_guard<std::mutex> lk(mu);
                 if (first_error.empty()) first_error = "m2_rhf_batch: molecule " + std::to_string(m) + ": " + g_rhf_error;

@@ -119,6 +119,10 @@ def lib():
         L.m2_rhf_batch.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp,
                                    _dp, _ip, C.POINTER(SCFOptions), C.POINTER(SCFResult)]
         L.m2_rhf_batch.restype = C.c_int
+        L.m2_rhf_batch_min.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp,
+                                       _dp, _dp, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip, C.POINTER(SCFOptions),
+                                       C.POINTER(SCFResult)]
+        L.m2_rhf_batch_min.restype = C.c_int
         L.m2_rhf_hessian.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
                                      C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
                                      C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
@@ -405,16 +409,20 @@ class Engine:
                 "log": log[:res.iterations], "grad": g.reshape(mol.natom, 3) if gradient else None,
                 "grad_seconds": res.grad_seconds}
 
-    def rhf_batch(self, mols, basis="cc-pvdz", pure=True, nworkers=8, **kw):
+    def rhf_batch(self, mols, basis="cc-pvdz", pure=True, nworkers=8, guess="hcore", **kw):
         """Closed-shell SCF::rhf of every molecule in `mols` through
-        m2_rhf_batch: nworkers host threads, each with its own engine context
-        and HIP stream on this engine's device, take molecules largest basis
-        first.  Core-Hamiltonian guess.  Returns one result dict per molecule
+        m2_rhf_batch (guess "hcore") or m2_rhf_batch_min (guess "soad", with
+        each molecule's STO-3G shells): nworkers host threads, each with its
+        own engine context and HIP stream on this engine's device, take
+        molecules largest basis first.  Returns one result dict per molecule
         (energy, iterations, converged, n_full, n_incremental, n_fp32,
         jk_seconds, total_seconds) and the wall seconds of the whole batch."""
         import time
+        if guess not in ("hcore", "soad"):
+            raise ValueError("guess must be 'hcore' or 'soad'")
         o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
-                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=0, diis_start=2)
+                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=1 if guess == "soad" else 0,
+                       diis_start=2)
         for k, v in kw.items():
             setattr(o, k, v)
         nmol = len(mols)
@@ -428,9 +436,19 @@ class Engine:
         res = (SCFResult * nmol)()
         ip = lambda a: a.ctypes.data_as(_ip)
         t = time.perf_counter()
-        st = self.L.m2_rhf_batch(self.L.m2_ctx_device(self.h), int(nworkers), nmol, ip(natom), ip(charge), _p(Z),
-                                 _p(X), int(pure), ip(nsh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen), ip(at),
-                                 C.byref(o), res)
+        if guess == "soad":
+            marr = [shell_arrays(m.shells("sto-3g")) for m in mols]
+            nshm = np.array([len(a[0]) for a in marr], np.int32)
+            lm, npm, exm, cfm, cenm, atm = (np.ascontiguousarray(np.concatenate([a[k] for a in marr]))
+                                            for k in range(6))
+            st = self.L.m2_rhf_batch_min(self.L.m2_ctx_device(self.h), int(nworkers), nmol, ip(natom), ip(charge),
+                                         _p(Z), _p(X), int(pure), ip(nsh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen),
+                                         ip(at), ip(nshm), ip(lm), ip(npm), _p(exm), _p(cfm), _p(cenm), ip(atm),
+                                         C.byref(o), res)
+        else:
+            st = self.L.m2_rhf_batch(self.L.m2_ctx_device(self.h), int(nworkers), nmol, ip(natom), ip(charge),
+                                     _p(Z), _p(X), int(pure), ip(nsh), ip(l), ip(npr), _p(ex), _p(cf), _p(cen),
+                                     ip(at), C.byref(o), res)
         wall = time.perf_counter() - t
         if st != 0:
             raise M2Error(st, self.L.m2_rhf_last_error().decode())

@@ -43,7 +43,35 @@ def test_batch_mixed_within_one_microhartree(engine):
     assert nconv >= 4
 
 
-def test_batch_rejects_soad_guess(engine):
-    from molecular2_amd.engine import M2Error
-    with pytest.raises(M2Error, match="guess"):
-        engine.rhf_batch(_small_batch()[:1], guess=1)
+def test_batch_soad_guess_matches_single_runs(engine):
+    """m2_rhf_batch_min with the STO-3G shells: the SOAD-started batch gives
+    the single SOAD-started runs' energies (the iteration counts of the slowly
+    converging molecule vary with FP64 atomics noise, so they are not
+    compared)."""
+    mols = _small_batch()[:4]
+    # one of them needs ~100 iterations from this guess: room enough that no
+    # run ends at the limit, where FP64 atomics noise decides convergence
+    kw = dict(KW, maxiter=160)
+    print("case: SOAD batch of %d molecules, 4 workers" % len(mols), flush=True)
+    out, _ = engine.rhf_batch(mols, nworkers=4, guess="soad", **kw)
+    for m, r in zip(mols, out):
+        s = engine.rhf(m, guess="soad", **kw)
+        assert r["converged"] and s["converged"], (m.natom, r["iterations"], s["iterations"])
+        assert abs(r["energy"] - s["energy"]) < 5e-9, (m.natom, r["energy"], s["energy"])
+
+
+def test_batch_without_minimal_shells_rejects_soad(engine):
+    """m2_rhf_batch has no minimal shells: guess = 1 is refused before any run"""
+    import ctypes as C
+    from molecular2_amd.engine import SCFOptions, SCFResult
+    L = engine.L
+    o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12, df=0,
+                   mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=1, diis_start=2)
+    res = (SCFResult * 1)()
+    z = np.zeros(8, np.int32)
+    d = np.zeros(8)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    st = L.m2_rhf_batch(0, 1, 1, ip(z), ip(z), dp(d), dp(d), 1, ip(z), ip(z), ip(z), dp(d), dp(d), dp(d), ip(z),
+                        C.byref(o), res)
+    assert st != 0 and b"minimal shells" in L.m2_rhf_last_error()
