@@ -98,9 +98,24 @@ m2_status m2_grad_ex(m2_ctx* ctx, int natom, const double* Z, const double* xyz,
             }
         sig[p] = std::sqrt(nrm) >= 1e-12 ? 1 : 0;
     }
-    DevBuf<double> dD, dW, dG, dZ, dX, dsph, dex, dcf;
-    DevBuf<int> datom, dl, dnp, dpoff;
-    DevBuf<unsigned char> dsig;
+    // the call's device buffers, freed on every return path
+    struct Bufs {
+        DevBuf<double> dD, dW, dG, dZ, dX, dsph, dex, dcf, dDc;
+        DevBuf<int> datom, dl, dnp, dpoff, dcfo;
+        DevBuf<unsigned char> dsig;
+        DevBuf<int2> dp1;
+        ~Bufs() {
+            for (auto* x : {&dD, &dW, &dG, &dZ, &dX, &dsph, &dex, &dcf, &dDc}) x->release();
+            for (auto* x : {&datom, &dl, &dnp, &dpoff, &dcfo}) x->release();
+            dsig.release();
+            dp1.release();
+        }
+    } bufs;
+    auto &dD = bufs.dD, &dW = bufs.dW, &dG = bufs.dG, &dZ = bufs.dZ, &dX = bufs.dX, &dsph = bufs.dsph,
+         &dex = bufs.dex, &dcf = bufs.dcf, &dDc = bufs.dDc;
+    auto &datom = bufs.datom, &dl = bufs.dl, &dnp = bufs.dnp, &dpoff = bufs.dpoff, &dcfo = bufs.dcfo;
+    auto& dsig = bufs.dsig;
+    auto& dp1 = bufs.dp1;
     HIPCHK(dD.reserve((size_t)n * n));
     HIPCHK(dW.reserve((size_t)n * n));
     HIPCHK(dG.reserve((size_t)natom * 3));
@@ -176,8 +191,6 @@ m2_status m2_grad_ex(m2_ctx* ctx, int natom, const double* Z, const double* xyz,
     for (int r = 0; r < nc; ++r)
         for (int c = 0; c < n; ++c) YT[(size_t)c * nc + r] = Y[(size_t)r * n + c];
     for (int s = 0; s < nsh; ++s) to_cart(s, YT.data(), nc, Dct.data() + (size_t)cfo[s] * nc, nc, nc);   // (T D T^T)^T
-    DevBuf<double> dDc;
-    DevBuf<int> dcfo;
     HIPCHK(dDc.reserve((size_t)nc * nc));
     HIPCHK(dcfo.reserve(nsh));
     HIPCHK(hipMemcpy(dDc.p, Dct.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));   // symmetric
@@ -242,7 +255,6 @@ m2_status m2_grad_ex(m2_ctx* ctx, int natom, const double* Z, const double* xyz,
                     p1.push_back(make_int2(s1, s2));
                     ++cnt1[c];
                 }
-    DevBuf<int2> dp1;
     HIPCHK(dp1.reserve(p1.size()));
     HIPCHK(hipMemcpy(dp1.p, p1.data(), sizeof(int2) * p1.size(), hipMemcpyHostToDevice));
     HIPCHK(launch_grad_1e(g1, dp1.p, cnt1, ctx->stream));
@@ -262,12 +274,6 @@ m2_status m2_grad_ex(m2_ctx* ctx, int natom, const double* Z, const double* xyz,
                 g[3 * a2 + k] += d[k] * f;
             }
         }
-    for (auto* x : {&dD, &dW, &dG, &dZ, &dX, &dsph, &dex, &dcf}) x->release();
-    for (auto* x : {&datom, &dl, &dnp, &dpoff}) x->release();
-    dsig.release();
-    dDc.release();
-    dcfo.release();
-    dp1.release();
     return M2_OK;
 }
 
