@@ -1619,16 +1619,32 @@ def write_deriv_register(outdir):
         masks = (1, 2, 4) if c in DERIV_SPLIT else (7,)
         if c in DERIV_CSPLIT:
             masks = tuple(m | (k << 3) for m in masks for k in range(1, len(comps(c[2])) + 1))
-        body = [head, '#include "../grad_reg.hpp"', "namespace m2 {"]
-        for m in masks:
-            body.append(gen_deriv_class(*c, mask=m))
-        body.append("void launch_grad_reg_%s(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom,"
-                    " int fp32, hipStream_t st) {" % tag)
-        for m in masks:
-            body.append("  if (fp32) launch_reg<%d, %d, %d, %d, %d, float>(a, tab, t, nt, natom, st);" % (c + (m,)))
-            body.append("  else launch_reg<%d, %d, %d, %d, %d, double>(a, tab, t, nt, natom, st);" % (c + (m,)))
-        body += ["}", "}  // namespace m2", ""]
-        put(os.path.join(outdir, "grad_reg_%s.hip" % tag), "\n".join(body))
+        # classes split over the c components get one translation unit per
+        # derivative centre (grad_reg_ABCD_k.hip), so they compile in parallel
+        groups = [masks] if c not in DERIV_CSPLIT else [tuple(m for m in masks if m & 7 == ctr) for ctr in (1, 2, 4)]
+        sig = "(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom, int fp32, hipStream_t st)"
+        for gi, grp in enumerate(groups):
+            body = [head, '#include "../grad_reg.hpp"', "namespace m2 {"]
+            for m in grp:
+                body.append(gen_deriv_class(*c, mask=m))
+            fn = "launch_grad_reg_%s" % tag if len(groups) == 1 else "launch_grad_reg_%s_%d" % (tag, gi)
+            body.append("void %s%s {" % (fn, sig))
+            for m in grp:
+                body.append("  if (fp32) launch_reg<%d, %d, %d, %d, %d, float>(a, tab, t, nt, natom, st);" % (c + (m,)))
+                body.append("  else launch_reg<%d, %d, %d, %d, %d, double>(a, tab, t, nt, natom, st);" % (c + (m,)))
+            body.append("}")
+            if len(groups) > 1 and gi == len(groups) - 1:
+                for k in range(len(groups) - 1):
+                    body.append("void launch_grad_reg_%s_%d%s;" % (tag, k, sig))
+                body.append("void launch_grad_reg_%s%s {" % (tag, sig))
+                for k in range(len(groups)):
+                    body.append("  launch_grad_reg_%s_%d(a, tab, t, nt, natom, fp32, st);" % (tag, k))
+                body.append("}")
+            body += ["}  // namespace m2", ""]
+            fname = "grad_reg_%s.hip" % tag if len(groups) == 1 else "grad_reg_%s_%d.hip" % (tag, gi)
+            put(os.path.join(outdir, fname), "\n".join(body))
+        if len(groups) > 1 and os.path.exists(os.path.join(outdir, "grad_reg_%s.hip" % tag)):
+            os.remove(os.path.join(outdir, "grad_reg_%s.hip" % tag))
         names.append((c, tag))
     disp = [head, "#pragma once"]
     for c, tag in names:
