@@ -151,6 +151,14 @@ template <int LA, int LB, int LC, int LD, typename R>
 struct JKWaves {
     static constexpr int value = EriClass<LA, LB, LC, LD>::NACC * (int)sizeof(R) <= 280 ? 8 : 4;
 };
+// (ps|ss) in FP64 runs 4-wave workgroups held to 5 waves per SIMD (its 102
+// VGPRs fit 5; 8-wave workgroups reach only 4); elsewhere no minimum
+template <int LA, int LB, int LC, int LD, typename R>
+struct JKOccupancy {
+    static constexpr bool ps = LA == 1 && LB == 0 && LC == 0 && LD == 0 && sizeof(R) == 8;
+    static constexpr int waves = ps ? 4 : JKWaves<LA, LB, LC, LD, R>::value;
+    static constexpr int min_per_eu = ps ? 5 : 1;
+};
 
 // Quartets are computed when lo <= bound < hi (the launchers pass
 // [precision, inf); precision is chosen per task list).
@@ -158,11 +166,12 @@ struct JKWaves {
 // NDEN = 2 (unrestricted build): every quartet is digested with D and with D2
 // into (J, K) and (J2, K2); the LDS then holds the K rows of both.
 template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN>
-__global__ __launch_bounds__((JKWaves<LA, LB, LC, LD, R>::value * 64)) void k_jk(JKArgs a, BoysTables tab, double lo,
+__global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
+                             (JKOccupancy<LA, LB, LC, LD, R>::min_per_eu)) void k_jk(JKArgs a, BoysTables tab, double lo,
                                                                                  double hi) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NC = NFun<LC, PURE>::value, ND = NFun<LD, PURE>::value;
-    constexpr int W = JKWaves<LA, LB, LC, LD, R>::value;
+    constexpr int W = JKOccupancy<LA, LB, LC, LD, R>::waves;
     typedef const __attribute__((address_space(4))) int cint;
     // workgroup -> bra pair and its ket segments (written by the list kernels)
     const int bl = ((cint*)a.blk_bra)[blockIdx.x];
@@ -482,10 +491,10 @@ void launch_jk_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode,
     const size_t lds = a.k_lds ? kbytes : 0;
     if (mode == 2)
         hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE, NDEN>), grid,
-                           dim3(64 * JKWaves<LA, LB, LC, LD, float>::value), lds, st, a, tab, a.precision, inf);
+                           dim3(64 * JKOccupancy<LA, LB, LC, LD, float>::waves), lds, st, a, tab, a.precision, inf);
     else
         hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE, NDEN>), grid,
-                           dim3(64 * JKWaves<LA, LB, LC, LD, double>::value), lds, st, a, tab, a.precision, inf);
+                           dim3(64 * JKOccupancy<LA, LB, LC, LD, double>::waves), lds, st, a, tab, a.precision, inf);
 }
 
 template <int LA, int LB, int LC, int LD, bool PURE>
