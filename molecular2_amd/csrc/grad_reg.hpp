@@ -66,9 +66,17 @@ __device__ __forceinline__ double wave_sum_g(double v) {
 // R: arithmetic of the integrals and their contraction (float for the
 // quartets a mixed-precision gradient leaves below its FP32 threshold); the
 // per-quartet sums are added up in double.
+// FP64 (ss|ss) and (ps|ss) are held one wave per SIMD above the occupancy
+// their register counts (100, 134) would otherwise give; elsewhere no minimum
 template <int LA, int LB, int LC, int LD, int MASK, typename R>
-__global__ __launch_bounds__(256) void k_grad_reg(GradArgs a, BoysTables tab, const Task* __restrict__ tasks,
-                                                  int ntasks, int natom) {
+struct GradOccupancy {
+    static constexpr bool d = sizeof(R) == 8 && LB == 0 && LC == 0 && LD == 0 && MASK == 7;
+    static constexpr int min_per_eu = d && LA == 0 ? 5 : d && LA == 1 ? 4 : 1;
+};
+
+template <int LA, int LB, int LC, int LD, int MASK, typename R>
+__global__ __launch_bounds__(256, (GradOccupancy<LA, LB, LC, LD, MASK, R>::min_per_eu)) void k_grad_reg(
+    GradArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks, int natom) {
     constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
     constexpr int NC = (LC + 1) * (LC + 2) / 2, ND = (LD + 1) * (LD + 2) / 2;
     extern __shared__ double bins[];   // natom * 3
