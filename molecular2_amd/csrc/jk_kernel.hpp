@@ -311,11 +311,15 @@ struct FamPrimOf<float> {
 
 template <int LA, int LB, typename R>
 struct JKWavesFam {
-    static constexpr int value = EriClass<LA, LB, 0, 0>::NACC_FAM * (int)sizeof(R) <= 280 ? 8 : 4;
+    // (ss|ss) in FP64 (91 VGPRs) runs 4-wave workgroups held to 5 waves per SIMD
+    static constexpr bool ss = LA == 0 && LB == 0 && sizeof(R) == 8;
+    static constexpr int value = ss ? 4 : EriClass<LA, LB, 0, 0>::NACC_FAM * (int)sizeof(R) <= 280 ? 8 : 4;
+    static constexpr int min_per_eu = ss ? 5 : 1;
 };
 
 template <int LA, int LB, typename R, bool PURE, int NDEN>
-__global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64)) void k_jk_fam(JKArgs a, BoysTables tab, double lo,
+__global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
+                             (JKWavesFam<LA, LB, R>::min_per_eu)) void k_jk_fam(JKArgs a, BoysTables tab, double lo,
                                                                                double hi) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NCART = EriClass<LA, LB, 0, 0>::NCART;
