@@ -61,6 +61,7 @@ def main(ref):
     for lst in ("basis.list", "jkfit.list"):
         shutil.copyfile(os.path.join(ref, "basislibrary", lst), os.path.join(lib, lst))
     build = os.path.join(lib, "bases", "build")
+    os.makedirs(build, exist_ok=True)
     cut_xml(os.path.join(ref, "basislibrary/bases/build/basis11.xml"), os.path.join(build, "basis11.xml"))
     cut_xml(os.path.join(ref, "basislibrary/bases/build/jkfit4.xml"), os.path.join(build, "jkfit4.xml"))
     dz = convert_basis.parse_molpro(os.path.join(ref, "basissets/CC-PVDZfirst.basis"), {s.capitalize() for s in KEEP})
