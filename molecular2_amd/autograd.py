@@ -42,12 +42,23 @@ def basis_tensors(mol, basis: str = "cc-pvdz"):
 
 def rhf_energy(R: torch.Tensor, mol, basis: str = "cc-pvdz", pure: bool = True, precision: float = 1e-12,
                enconverge: float = 1e-10, densconverge: float = 1e-8, maxiter: int = 80,
-               mixed: bool = False) -> torch.Tensor:
+               mixed: bool = False, guess: str = "soad") -> torch.Tensor:
     """E_RHF at positions R (natom x 3, bohr; FP64) for the atoms, charge and
-    basis of `mol` (a molecules.Molecule).  Differentiable in R (first order)."""
+    basis of `mol` (a molecules.Molecule).  Differentiable in R (first order).
+    guess: "soad" (superposition of STO-3G atomic densities, the reference's
+    default) or "hcore"."""
+    if guess not in ("soad", "hcore"):
+        raise ValueError("guess must be 'soad' or 'hcore'")
     load()
     Z = torch.tensor(np.asarray(mol.Z, float), dtype=torch.float64)
     l, nprim, exps, coefs, atoms = basis_tensors(mol, basis)
+    if guess == "soad":
+        mins = basis_tensors(mol, "sto-3g")
+    else:
+        mins = (torch.zeros(0, dtype=torch.int32), torch.zeros(0, dtype=torch.int32),
+                torch.zeros(0, dtype=torch.float64), torch.zeros(0, dtype=torch.float64),
+                torch.zeros(0, dtype=torch.int32))
     return torch.ops.m2.rhf_energy(R, Z, l, nprim, exps, coefs, atoms, int(pure), int(mol.charge),
                                    int(mol.multiplicity), float(precision), float(enconverge),
-                                   float(densconverge), int(maxiter), int(mixed))
+                                   float(densconverge), int(maxiter), int(mixed), 1 if guess == "soad" else 0,
+                                   *mins)

@@ -911,7 +911,8 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_counters.release();
     for (auto* x : {&ctx->d_B64, &ctx->d_dfV, &ctx->d_dfC, &ctx->d_dfX, &ctx->d_dfG, &ctx->d_dfJ, &ctx->d_dfK, &ctx->d_dfKp})
         x->release();
-    for (auto* x : {&ctx->d_B32, &ctx->d_dfC32, &ctx->d_dfX32, &ctx->d_dfG32, &ctx->d_dfJ32, &ctx->d_dfK32, &ctx->d_dfKp32})
+    ctx->d_df_jblk.release();
+    for (auto* x : {&ctx->d_B32, &ctx->d_dfC32, &ctx->d_dfX32, &ctx->d_dfKp32})
         x->release();
     for (auto* x : {&ctx->d_aux_off, &ctx->d_aux_np, &ctx->d_aux_lists}) x->release();
     ctx->d_aux_prims.release();
@@ -927,38 +928,78 @@ void* m2_ctx_stream(m2_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int m2_ctx_device(const m2_ctx* ctx) { return ctx ? ctx->device : -1; }
 
+// Empty the basis `which` (and for the orbital basis its shell pairs), so
+// that every later call needing it returns M2_ERR_ARG.
+static void clear_basis(m2_ctx* ctx, int which) {
+    BasisData& b = (which == M2_ORBITAL) ? ctx->obs : ctx->aux;
+    b.nsh = 0;
+    b.nbf = 0;
+    b.sh.clear();
+    ctx->df_ready = false;
+    if (which != M2_ORBITAL) return;
+    ctx->pair_sh.clear();
+    ctx->pair_off.clear();
+    ctx->pair_np.clear();
+    ctx->pair_Q.clear();
+    ctx->Qsh.clear();
+    for (auto& c : ctx->cls) c = PairClassSeg{};
+    for (auto& L : ctx->lists) L = JKList{};
+    ctx->fam_on = false;
+    ctx->fam_seg = PairClassSeg{};
+}
+
+// Every input is checked before the context changes.  A call that fails
+// (bad input, device upload, pair construction) leaves the basis empty
+// (nbf = nsh = 0): later calls that need it return M2_ERR_ARG instead of
+// mixing new shell arrays with old pair data or old sizes.
 m2_status m2_basis_set(m2_ctx* ctx, int which, int nsh, const int* l, const int* nprim, const double* exps,
                        const double* coefs, const double* centers, const int* atoms, int pure) {
     if (!ctx) return M2_ERR_ARG;
     ARGCHK(which == M2_ORBITAL || which == M2_AUXILIARY, "m2_basis_set: which must be M2_ORBITAL or M2_AUXILIARY");
-    ARGCHK(nsh > 0 && l && nprim && exps && coefs && centers, "m2_basis_set: null or empty input");
-    HIPCHK(hipSetDevice(ctx->device));
-    BasisData& b = (which == M2_ORBITAL) ? ctx->obs : ctx->aux;
-    ctx->df_ready = false;   // the fitted tensor depends on both bases
-    b.nsh = nsh;
-    b.pure = pure ? 1 : 0;
-    b.sh.assign(nsh, HostShell{});
+    std::vector<HostShell> shells;
+    auto fail = [&](const char* msg) {
+        clear_basis(ctx, which);
+        ctx->err = msg;
+        return M2_ERR_ARG;
+    };
+    if (!(nsh > 0 && l && nprim && exps && coefs && centers)) return fail("m2_basis_set: null or empty input");
+    const int lmax = which == M2_ORBITAL ? 2 : 3;
+    shells.resize(nsh);
     int off = 0, bf = 0;
     for (int s = 0; s < nsh; ++s) {
-        HostShell& S = b.sh[s];
-        ARGCHK(l[s] >= 0 && l[s] <= 4 && nprim[s] > 0, "m2_basis_set: shell l must be 0..4 and nprim > 0");
+        HostShell& S = shells[s];
+        if (nprim[s] <= 0) return fail("m2_basis_set: every shell needs nprim > 0");
+        if (l[s] < 0 || l[s] > lmax)
+            return fail(which == M2_ORBITAL ? "m2_basis_set: orbital shells must have l <= 2"
+                                            : "m2_basis_set: auxiliary shells must have l <= 3");
         S.l = l[s];
         S.atom = atoms ? atoms[s] : -1;
         for (int k = 0; k < 3; ++k) S.O[k] = centers[3 * s + k];
         S.a.assign(exps + off, exps + off + nprim[s]);
         S.c.assign(coefs + off, coefs + off + nprim[s]);
-        for (double x : S.a) ARGCHK(x > 0.0, "m2_basis_set: exponents must be positive");
+        for (double x : S.a)
+            if (!(x > 0.0)) return fail("m2_basis_set: exponents must be positive");
         off += nprim[s];
         normalise(S);
-        S.nbf = b.pure ? 2 * S.l + 1 : (S.l + 1) * (S.l + 2) / 2;
+        S.nbf = pure ? 2 * S.l + 1 : (S.l + 1) * (S.l + 2) / 2;
         S.bf = bf;
         bf += S.nbf;
     }
+    clear_basis(ctx, which);
+    HIPCHK(hipSetDevice(ctx->device));
+    BasisData& b = (which == M2_ORBITAL) ? ctx->obs : ctx->aux;
+    b.pure = pure ? 1 : 0;
+    b.sh.swap(shells);
+    b.nsh = nsh;
     b.nbf = bf;
     m2_status st = upload_basis(ctx, b);
-    if (st) return st;
-    if (which == M2_ORBITAL) return build_pairs(ctx);
-    return M2_OK;
+    if (st == M2_OK && which == M2_ORBITAL) st = build_pairs(ctx);
+    if (st != M2_OK) {
+        const std::string why = ctx->err;
+        clear_basis(ctx, which);
+        ctx->err = why;
+    }
+    return st;
 }
 
 m2_status m2_basis_info(m2_ctx* ctx, int which, int* nbf, int* nsh) {

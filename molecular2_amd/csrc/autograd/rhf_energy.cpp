@@ -4,6 +4,11 @@
 // forward:  SCF::rhf (src/scf.cpp:90-185) on the device through m2_rhf at the
 //           nuclear positions R (natom x 3, bohr); returns E (0-dim FP64) and
 //           keeps D = 2 C_occ C_occ^T and W = C_occ diag(eps_occ) C_occ^T.
+//           guess 1 starts from the superposition of atomic densities over
+//           the minimal (STO-3G) shells given as (lm, nprim_m, exps_m,
+//           coefs_m, atoms_m), the reference's default
+//           (src/ProgramController.cpp:307); guess 0 from the core
+//           Hamiltonian (the minimal tensors may then be empty).
 // backward: dE/dR = Fock::compute_forces (src/rhfgrad.cpp:15-89) from D and
 //           W through m2_grad, whose two-electron part is the derivative-ERI
 //           adjoint kernel (integrals recomputed, never stored), scaled by
@@ -90,8 +95,15 @@ class RHFEnergy : public torch::autograd::Function<RHFEnergy> {
     static Tensor forward(AutogradContext* ctx, const Tensor& R, const Tensor& Z, const Tensor& l,
                           const Tensor& nprim, const Tensor& exps, const Tensor& coefs, const Tensor& atoms,
                           int64_t pure, int64_t charge, int64_t multiplicity, double precision, double enconverge,
-                          double densconverge, int64_t maxiter, int64_t mixed) {
+                          double densconverge, int64_t maxiter, int64_t mixed, int64_t guess, const Tensor& lm,
+                          const Tensor& nprim_m, const Tensor& exps_m, const Tensor& coefs_m, const Tensor& atoms_m) {
         Host h = host_inputs(R, Z, l, nprim, exps, coefs, atoms, pure != 0);
+        TORCH_CHECK(guess == 0 || guess == 1, "m2.rhf_energy: guess must be 0 (core) or 1 (SOAD)");
+        Host hm;
+        if (guess == 1) {
+            TORCH_CHECK(lm.numel() > 0, "m2.rhf_energy: the SOAD guess needs the minimal (STO-3G) shells");
+            hm = host_inputs(R, Z, lm, nprim_m, exps_m, coefs_m, atoms_m, pure != 0);
+        }
         const int device = R.is_cuda() ? R.get_device() : 0;
         const int n = h.n;
         std::vector<double> D((size_t)n * n), C((size_t)n * n), eps(n);
@@ -106,7 +118,7 @@ class RHFEnergy : public torch::autograd::Function<RHFEnergy> {
         o.mixed = (int)mixed;
         o.mixed_threshold = 1e-4;
         o.fp64_final = 2;
-        o.guess = 0;
+        o.guess = (int)guess;
         o.diis_start = 2;
         m2_scf_result_c res{};
         {
@@ -114,9 +126,9 @@ class RHFEnergy : public torch::autograd::Function<RHFEnergy> {
             m2_ctx* c = context_for(device);
             const int st = m2_rhf(c, h.natom, h.Z.data(), h.xyz.data(), (int)charge, (int)multiplicity, (int)pure,
                                   h.nsh, h.l.data(), h.nprim.data(), h.exps.data(), h.coefs.data(), h.centers.data(),
-                                  h.atoms.data(), 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr,
-                                  nullptr, nullptr, nullptr, nullptr, nullptr, &o, &res, D.data(), C.data(),
-                                  eps.data(), nullptr, nullptr);
+                                  h.atoms.data(), 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, hm.nsh,
+                                  hm.l.data(), hm.nprim.data(), hm.exps.data(), hm.coefs.data(), hm.centers.data(),
+                                  hm.atoms.data(), &o, &res, D.data(), C.data(), eps.data(), nullptr, nullptr);
             TORCH_CHECK(st == 0, "m2.rhf_energy: ", m2_rhf_last_error());
         }
         TORCH_CHECK(res.converged, "m2.rhf_energy: SCF did not converge in ", maxiter, " iterations");
@@ -158,7 +170,7 @@ class RHFEnergy : public torch::autograd::Function<RHFEnergy> {
         Tensor gt = torch::from_blob(g.data(), {h.natom, 3}, torch::TensorOptions().dtype(torch::kFloat64))
                         .clone()
                         .to(R.device(), R.scalar_type());
-        variable_list out(15);
+        variable_list out(21);
         out[0] = grad_out[0].to(gt.scalar_type()) * gt;
         return out;
     }
@@ -166,9 +178,11 @@ class RHFEnergy : public torch::autograd::Function<RHFEnergy> {
 
 Tensor rhf_energy(const Tensor& R, const Tensor& Z, const Tensor& l, const Tensor& nprim, const Tensor& exps,
                   const Tensor& coefs, const Tensor& atoms, int64_t pure, int64_t charge, int64_t multiplicity,
-                  double precision, double enconverge, double densconverge, int64_t maxiter, int64_t mixed) {
+                  double precision, double enconverge, double densconverge, int64_t maxiter, int64_t mixed,
+                  int64_t guess, const Tensor& lm, const Tensor& nprim_m, const Tensor& exps_m, const Tensor& coefs_m,
+                  const Tensor& atoms_m) {
     return RHFEnergy::apply(R, Z, l, nprim, exps, coefs, atoms, pure, charge, multiplicity, precision, enconverge,
-                            densconverge, maxiter, mixed);
+                            densconverge, maxiter, mixed, guess, lm, nprim_m, exps_m, coefs_m, atoms_m);
 }
 
 }  // namespace
@@ -177,6 +191,6 @@ TORCH_LIBRARY(m2, m) {
     m.def(
         "rhf_energy(Tensor R, Tensor Z, Tensor l, Tensor nprim, Tensor exps, Tensor coefs, Tensor atoms, int pure, "
         "int charge, int multiplicity, float precision, float enconverge, float densconverge, int maxiter, "
-        "int mixed) -> Tensor",
+        "int mixed, int guess, Tensor lm, Tensor nprim_m, Tensor exps_m, Tensor coefs_m, Tensor atoms_m) -> Tensor",
         &rhf_energy);
 }

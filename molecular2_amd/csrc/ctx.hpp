@@ -23,10 +23,16 @@ struct HostShell {
     std::vector<double> a, c;
 };
 
+// Device buffer owned by its holder: grown by reserve, freed by release or
+// when the holder goes out of scope (so early error returns free it too).
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
     hipError_t reserve(size_t cnt) {
         if (cnt <= n && p) return hipSuccess;
         if (p) (void)hipFree(p);
@@ -48,6 +54,10 @@ template <typename T>
 struct PinnedBuf {
     T* p = nullptr;
     size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
     hipError_t reserve(size_t cnt) {
         if (cnt <= n && p) return hipSuccess;
         if (p) (void)hipHostFree(p);
@@ -144,7 +154,9 @@ struct m2_ctx {
     m2::DevBuf<m2::PrimPair> d_aux_prims;
     m2::DevBuf<int> d_aux_off, d_aux_np, d_aux_lists;
     m2::DevBuf<double> d_B64, d_dfV, d_dfC, d_dfX, d_dfG, d_dfJ, d_dfK, d_dfKp;
-    m2::DevBuf<float> d_B32, d_dfC32, d_dfX32, d_dfG32, d_dfJ32, d_dfK32, d_dfKp32;   // Kp: split-K partial slices
+    m2::DevBuf<float> d_B32, d_dfC32, d_dfX32, d_dfKp32;   // Kp: split-K partial slices
+    m2::DevBuf<int> d_df_jblk;        // (x, y0) per workgroup of the Coulomb contraction
+    int df_jblocks = 0;
     bool df_ready = false;
     double df_last_ms[4] = {0, 0, 0, 0};   // X = C^T B, K = X^T X, J, total (profiling)
     // dense linear algebra (rocBLAS / rocSOLVER)

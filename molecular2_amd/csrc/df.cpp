@@ -47,10 +47,11 @@ __global__ void k_df_reorder(const double* __restrict__ M, double* __restrict__ 
     }
 }
 
-// g[K] = sum_{x,i} X[x][K][i] C[x + i*n]   (X as written by the X = C^T B GEMM)
+// g[K] = sum_{x,i} X[x][K][i] C[x + i*n]   (X as written by the X = C^T B GEMM),
+// accumulated and stored in FP64 for either input precision
 template <typename R>
-__global__ __launch_bounds__(256) void k_df_gamma(const R* __restrict__ X, const R* __restrict__ C, R* __restrict__ g,
-                                                  int n, int naux, int nocc) {
+__global__ __launch_bounds__(256) void k_df_gamma(const R* __restrict__ X, const R* __restrict__ C,
+                                                  double* __restrict__ g, int n, int naux, int nocc) {
     const int K = blockIdx.x;
     __shared__ double red[256];
     double s = 0.0;
@@ -64,27 +65,53 @@ __global__ __launch_bounds__(256) void k_df_gamma(const R* __restrict__ X, const
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) g[K] = R(red[0]);
+    if (threadIdx.x == 0) g[K] = red[0];
 }
 
-// out[i] = sum_s part[s * m + i]
+// out[i] = sum_s part[s * m + i], summed and written in FP64
 template <typename R>
-__global__ void k_sum_slices(const R* __restrict__ part, R* __restrict__ out, size_t m, int ns) {
+__global__ void k_sum_slices(const R* __restrict__ part, double* __restrict__ out, size_t m, int ns) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    R v = R(0);
-    for (int s = 0; s < ns; ++s) v += part[(size_t)s * m + i];
+    double v = 0.0;
+    for (int s = 0; s < ns; ++s) v += double(part[(size_t)s * m + i]);
     out[i] = v;
+}
+
+// Coulomb part of compute_2body_fock_df (src/fockbuild.cpp:652-659),
+// J(x, y) = sum_K B(xy, K) g_K, over the lower triangle y <= x of the fitted
+// tensor stored Bx[x][K][y] (each unique pair read once), mirrored into J.
+// One workgroup per (x, 256-wide y chunk); lanes over y read B coalesced,
+// the loop runs over K with g staged in LDS; FP64 accumulation whatever the
+// storage precision of B.
+constexpr int DF_J_CHUNK = 256;
+template <typename R>
+__global__ __launch_bounds__(DF_J_CHUNK) void k_df_coulomb(const R* __restrict__ B, const double* __restrict__ g,
+                                                           const int* __restrict__ blk_x, double* __restrict__ J,
+                                                           int n, int naux) {
+    extern __shared__ double sg[];
+    const int x = blk_x[2 * blockIdx.x], y0 = blk_x[2 * blockIdx.x + 1];
+    for (int k = threadIdx.x; k < naux; k += blockDim.x) sg[k] = g[k];
+    __syncthreads();
+    const int y = y0 + threadIdx.x;
+    if (y > x) return;
+    const R* b = B + (size_t)x * naux * n + y;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    int k = 0;
+    for (; k + 4 <= naux; k += 4) {
+        acc0 += double(b[(size_t)k * n]) * sg[k];
+        acc1 += double(b[(size_t)(k + 1) * n]) * sg[k + 1];
+        acc2 += double(b[(size_t)(k + 2) * n]) * sg[k + 2];
+        acc3 += double(b[(size_t)(k + 3) * n]) * sg[k + 3];
+    }
+    for (; k < naux; ++k) acc0 += double(b[(size_t)k * n]) * sg[k];
+    const double v = (acc0 + acc1) + (acc2 + acc3);
+    J[(size_t)x * n + y] = v;
+    J[(size_t)y * n + x] = v;
 }
 
 // chunks of the auxiliary index of the exchange GEMM
 constexpr int DF_KSPLIT = 48;
-
-template <typename R>
-__global__ void k_to_double(const R* __restrict__ a, double* __restrict__ b, size_t n) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-        b[t] = double(a[t]);
-}
 
 __global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, size_t n) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
@@ -230,6 +257,16 @@ m2_status df_setup(m2_ctx* ctx) {
     HIPCHK(ctx->d_B32.reserve(nB));
     hipLaunchKernelGGL(k_df_reorder, dim3(8192), dim3(256), 0, ctx->stream, M.p, ctx->d_B64.p, ctx->d_B32.p, n, naux);
     HIPCHK(hipGetLastError());
+    // workgroups of the Coulomb contraction: (x, first y) over y <= x
+    std::vector<int> jb;
+    for (int x = 0; x < n; ++x)
+        for (int y0 = 0; y0 <= x; y0 += DF_J_CHUNK) {
+            jb.push_back(x);
+            jb.push_back(y0);
+        }
+    ctx->df_jblocks = (int)jb.size() / 2;
+    HIPCHK(ctx->d_df_jblk.reserve(jb.size()));
+    HIPCHK(hipMemcpy(ctx->d_df_jblk.p, jb.data(), sizeof(int) * jb.size(), hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     M.release();
     dt.release();
@@ -256,12 +293,6 @@ struct Blas<float> {
         e = ctx->d_dfKp32.reserve(cnt);
         return ctx->d_dfKp32.p;
     }
-    static rocblas_status gemv_sb(rocblas_handle h, int m, int n, const float* al, const float* A, int lda,
-                                  rocblas_stride sA, const float* x, rocblas_stride sx, const float* be, float* y,
-                                  rocblas_stride sy, int batch) {
-        return rocblas_sgemv_strided_batched(h, rocblas_operation_none, m, n, al, A, lda, sA, x, 1, sx, be, y, 1, sy,
-                                             batch);
-    }
 };
 template <>
 struct Blas<double> {
@@ -280,16 +311,10 @@ struct Blas<double> {
         e = ctx->d_dfKp.reserve(cnt);
         return ctx->d_dfKp.p;
     }
-    static rocblas_status gemv_sb(rocblas_handle h, int m, int n, const double* al, const double* A, int lda,
-                                  rocblas_stride sA, const double* x, rocblas_stride sx, const double* be, double* y,
-                                  rocblas_stride sy, int batch) {
-        return rocblas_dgemv_strided_batched(h, rocblas_operation_none, m, n, al, A, lda, sA, x, 1, sx, be, y, 1, sy,
-                                             batch);
-    }
 };
 
 template <typename R>
-m2_status jk_df_core(m2_ctx* ctx, const R* B, const R* C, int nocc, R* X, R* g, R* J, R* K) {
+m2_status jk_df_core(m2_ctx* ctx, const R* B, const R* C, int nocc, R* X, double* g, double* J, double* K) {
     const int n = ctx->obs.nbf, naux = ctx->aux.nbf;
     void* hv = nullptr;
     m2_status st = la_handle(ctx, &hv);
@@ -339,12 +364,10 @@ m2_status jk_df_core(m2_ctx* ctx, const R* B, const R* C, int nocc, R* X, R* g, 
     if (ctx->prof) HIPCHK(hipEventRecord(ev[2], ctx->stream));
     hipLaunchKernelGGL(k_df_gamma<R>, dim3(naux), dim3(256), 0, ctx->stream, X, C, g, n, naux, nocc);
     HIPCHK(hipGetLastError());
-    // J(:, x) = B(:, x*naux .. x*naux+naux) g
-    if (Blas<R>::gemv_sb(h, n, naux, &one, B, n, (rocblas_stride)n * naux, g, 0, &zero, J, n, n) !=
-        rocblas_status_success) {
-        ctx->err = "m2_jk_df: strided GEMV for J failed";
-        return M2_ERR_HIP;
-    }
+    // J over the lower triangle of B, mirrored
+    hipLaunchKernelGGL(k_df_coulomb<R>, dim3(ctx->df_jblocks), dim3(DF_J_CHUNK), sizeof(double) * naux, ctx->stream, B,
+                       g, ctx->d_df_jblk.p, J, n, naux);
+    HIPCHK(hipGetLastError());
     if (ctx->prof) {
         HIPCHK(hipEventRecord(ev[3], ctx->stream));
         HIPCHK(hipEventSynchronize(ev[3]));
@@ -366,21 +389,17 @@ m2_status jk_df_device(m2_ctx* ctx, const double* dC, int nocc, int fp32, double
         if (st) return st;
     }
     const size_t nx = (size_t)nocc * n * naux;
+    HIPCHK(ctx->d_dfG.reserve(naux));
     if (fp32) {
+        // FP32 storage and MFMA GEMMs; gamma, the slice sum of K and the
+        // Coulomb contraction accumulate in FP64 and write J, K directly
         HIPCHK(ctx->d_dfC32.reserve((size_t)n * nocc));
         HIPCHK(ctx->d_dfX32.reserve(nx));
-        HIPCHK(ctx->d_dfG32.reserve(naux));
-        HIPCHK(ctx->d_dfJ32.reserve((size_t)n * n));
-        HIPCHK(ctx->d_dfK32.reserve((size_t)n * n));
         hipLaunchKernelGGL(k_to_float, dim3(256), dim3(256), 0, ctx->stream, dC, ctx->d_dfC32.p, (size_t)n * nocc);
-        m2_status st = jk_df_core<float>(ctx, ctx->d_B32.p, ctx->d_dfC32.p, nocc, ctx->d_dfX32.p, ctx->d_dfG32.p,
-                                         ctx->d_dfJ32.p, ctx->d_dfK32.p);
+        m2_status st = jk_df_core<float>(ctx, ctx->d_B32.p, ctx->d_dfC32.p, nocc, ctx->d_dfX32.p, ctx->d_dfG.p, dJ, dK);
         if (st) return st;
-        hipLaunchKernelGGL(k_to_double<float>, dim3(256), dim3(256), 0, ctx->stream, ctx->d_dfJ32.p, dJ, (size_t)n * n);
-        hipLaunchKernelGGL(k_to_double<float>, dim3(256), dim3(256), 0, ctx->stream, ctx->d_dfK32.p, dK, (size_t)n * n);
     } else {
         HIPCHK(ctx->d_dfX.reserve(nx));
-        HIPCHK(ctx->d_dfG.reserve(naux));
         m2_status st = jk_df_core<double>(ctx, ctx->d_B64.p, dC, nocc, ctx->d_dfX.p, ctx->d_dfG.p, dJ, dK);
         if (st) return st;
     }

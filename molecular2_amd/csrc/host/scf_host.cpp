@@ -309,7 +309,10 @@ void Fock::makeJK(Matrix& P, double multiplier) {
 }
 
 void Fock::formJKdirect(const Matrix& Schwarz, Matrix& D, double multiplier) {
-    (void)Schwarz;   // the engine holds the device copy of the same Schwarz bounds
+    // The engine holds the device copy of the same Schwarz bounds; an empty
+    // Schwarz matrix switches the quartet screening off, as in the reference
+    // (src/fockbuild.cpp:74,114): every unique quartet is computed.
+    const double screen = Schwarz.rows() != 0 && Schwarz.cols() != 0 ? precision : 0.0;
     m2_ctx* ctx = integrals.context();
     unsigned flags = M2_FP64;
     if (opt.mixed && !fp64_forced) {
@@ -324,7 +327,7 @@ void Fock::formJKdirect(const Matrix& Schwarz, Matrix& D, double multiplier) {
     jints = Matrix(nbfs, nbfs);
     kints = Matrix(nbfs, nbfs);
     auto t0 = std::chrono::steady_clock::now();
-    check(ctx, m2_jk_direct(ctx, D.data(), multiplier, precision, flags, jints.data(), kints.data()), "m2_jk_direct");
+    check(ctx, m2_jk_direct(ctx, D.data(), multiplier, screen, flags, jints.data(), kints.data()), "m2_jk_direct");
     jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // leave the context's split where the caller set it (an all-FP32 Delta-D build changes it)
     if (flags == M2_MIXED) check(ctx, m2_set_mixed_threshold(ctx, opt.mixed_threshold), "m2_set_mixed_threshold");
@@ -336,8 +339,9 @@ void Fock::formJKdf(Matrix& Cocc, double multiplier) {
     jints = Matrix(nbfs, nbfs);
     kints = Matrix(nbfs, nbfs);
     auto t0 = std::chrono::steady_clock::now();
-    check(ctx, m2_jk_df(ctx, Cocc.data(), Cocc.cols(), opt.mixed && !fp64_forced ? 1 : 0, jints.data(), kints.data()),
-          "m2_jk_df");
+    const int fp32 = opt.mixed && !fp64_forced ? 1 : 0;
+    n_fp32_builds += fp32;
+    check(ctx, m2_jk_df(ctx, Cocc.data(), Cocc.cols(), fp32, jints.data(), kints.data()), "m2_jk_df");
     jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     jkints = multiplier * (2.0 * jints - kints);
 }
@@ -702,6 +706,9 @@ UnrestrictedFock::UnrestrictedFock(const Options& o, IntegralEngine& ints, const
     nbeta = m.nbeta();
     if (nalpha < 0 || nbeta < 0 || nalpha > nbfs || (m.getNel() - m.multiplicity + 1) % 2 != 0)
         throw Error("UHF", "electron count and multiplicity do not fit");
+    // mixed precision (FP32 quartets / fits with an FP64 hand-over) is built
+    // for the restricted SCF only; the reference's UHF is FP64 throughout
+    if (o.mixed) throw Error("UHF", "mixed precision is available for RHF only");
 }
 
 // src/fock.cpp:305-323
@@ -752,14 +759,15 @@ void UnrestrictedFock::makeJK() {
 }
 
 void UnrestrictedFock::formJKdirect(const Matrix& Schwarz, Matrix& Da, Matrix& Db, double multiplier) {
-    (void)Schwarz;   // the engine holds the device copy of the same Schwarz bounds
+    // empty Schwarz matrix: no quartet screening (src/fockbuild.cpp:74,114)
+    const double screen = Schwarz.rows() != 0 && Schwarz.cols() != 0 ? precision : 0.0;
     m2_ctx* ctx = integrals.context();
     jints_alpha = Matrix(nbfs, nbfs);
     kints_alpha = Matrix(nbfs, nbfs);
     jints_beta = Matrix(nbfs, nbfs);
     kints_beta = Matrix(nbfs, nbfs);
     auto t0 = std::chrono::steady_clock::now();
-    check(ctx, m2_jk_direct_uhf(ctx, Da.data(), Db.data(), multiplier, precision, M2_FP64, jints_alpha.data(),
+    check(ctx, m2_jk_direct_uhf(ctx, Da.data(), Db.data(), multiplier, screen, M2_FP64, jints_alpha.data(),
                                 kints_alpha.data(), jints_beta.data(), kints_beta.data()),
           "m2_jk_direct_uhf");
     jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -773,7 +781,7 @@ void UnrestrictedFock::formJKdf(Matrix& ca_occ, Matrix& cb_occ, double multiplie
         J = Matrix(nbfs, nbfs);
         K = Matrix(nbfs, nbfs);
         if (c.cols() == 0) return;
-        check(ctx, m2_jk_df(ctx, c.data(), c.cols(), opt.mixed ? 1 : 0, J.data(), K.data()), "m2_jk_df");
+        check(ctx, m2_jk_df(ctx, c.data(), c.cols(), 0, J.data(), K.data()), "m2_jk_df");
         J = multiplier * J;
         K = multiplier * K;
     };

@@ -101,7 +101,9 @@ struct Options {
     double precision = 1e-12;
     bool df = false;
     bool direct = true;
-    std::string guess = "hcore";      // or "soad" (superposition of atomic densities)
+    // "soad" (superposition of atomic densities, the reference's default for
+    // RHF and UHF, src/ProgramController.cpp:307,322) or "hcore"
+    std::string guess = "soad";
     // DIIS extrapolation from SCF iteration diis_start + 1 on (2: the current
     // reference source, src/scf.cpp:148,154; 0 reproduces the build that
     // wrote the reference's recorded logs)

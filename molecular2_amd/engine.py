@@ -373,7 +373,7 @@ class Engine:
                                     M2_MIXED if mixed else 0, _p(g)))
         return g.reshape(mol.natom, 3)
 
-    def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, gradient=False, guess="hcore", **kw):
+    def rhf(self, mol, basis="cc-pvdz", pure=True, aux=None, gradient=False, guess="soad", **kw):
         """Run the host C++ SCF (molecular::SCF::rhf) on `mol`.  guess:
         "hcore" or "soad" (STO-3G superposition of atomic densities, the
         reference's default).  Keyword options as SCFOptions (defaults: the
@@ -409,7 +409,7 @@ class Engine:
                 "log": log[:res.iterations], "grad": g.reshape(mol.natom, 3) if gradient else None,
                 "grad_seconds": res.grad_seconds}
 
-    def rhf_batch(self, mols, basis="cc-pvdz", pure=True, nworkers=8, guess="hcore", **kw):
+    def rhf_batch(self, mols, basis="cc-pvdz", pure=True, nworkers=8, guess="soad", **kw):
         """Closed-shell SCF::rhf of every molecule in `mols` through
         m2_rhf_batch (guess "hcore") or m2_rhf_batch_min (guess "soad", with
         each molecule's STO-3G shells): nworkers host threads, each with its
@@ -457,7 +457,7 @@ class Engine:
                 "jk_seconds": r.jk_seconds, "total_seconds": r.total_seconds} for r in res]
         return out, wall
 
-    def rhf_hessian(self, mol, basis="cc-pvdz", pure=True, aux=None, guess="hcore", step=0.01, nworkers=8, **kw):
+    def rhf_hessian(self, mol, basis="cc-pvdz", pure=True, aux=None, guess="soad", step=0.01, nworkers=8, **kw):
         """Fock::compute_hessian_numeric through m2_rhf_hessian: the RHF
         nuclear Hessian (3N x 3N, Eh/bohr^2) from central differences (step
         in bohr) of the analytic gradient over the 6N displaced geometries,
@@ -512,7 +512,7 @@ class Engine:
                                           *[_p(x) for x in out]))
         return tuple(out)
 
-    def uhf(self, mol, basis="cc-pvdz", pure=True, aux=None, guess="hcore", **kw):
+    def uhf(self, mol, basis="cc-pvdz", pure=True, aux=None, guess="soad", **kw):
         """Run the host C++ unrestricted SCF (molecular::SCF::uhf)."""
         if guess not in ("hcore", "soad"):
             raise ValueError("guess must be 'hcore' or 'soad'")

@@ -27,7 +27,7 @@ def test_autograd_energy_and_backward_water(engine):
     R = torch.tensor(mol.xyz, dtype=torch.float64, requires_grad=True)
     E = rhf_energy(R, mol, **TIGHT)
     assert E.dim() == 0 and E.dtype == torch.float64
-    r = engine.rhf(mol, gradient=True, **TIGHT)
+    r = engine.rhf(mol, gradient=True, **TIGHT, guess="soad")   # the operator's default guess
     assert abs(E.item() - r["energy"]) < 1e-10
     (2.0 * E).backward()
     g = R.grad.numpy() / 2.0
@@ -48,13 +48,15 @@ def test_autograd_on_device_tensor_and_no_grad(engine):
     mol = _distorted_water()
     print("case: autograd water, R on cuda:0", flush=True)
     R = torch.tensor(mol.xyz, dtype=torch.float64, device="cuda:0", requires_grad=True)
-    E = rhf_energy(R, mol, **TIGHT)
+    E = rhf_energy(R, mol, guess="hcore", **TIGHT)
     assert E.device == R.device
     E.backward()
     assert R.grad.device == R.device and R.grad.shape == (3, 3)
     assert abs(R.grad.sum(dim=0)).max().item() < 1e-8          # translational invariance
     with torch.no_grad():
-        E2 = rhf_energy(R, mol, **TIGHT)
+        E2 = rhf_energy(R, mol, guess="hcore", **TIGHT)
+        E3 = rhf_energy(R, mol, guess="soad", **TIGHT)
+    assert abs(E3.item() - E.item()) < 1e-10                   # converged energy does not depend on the guess
     assert abs(E2.item() - E.item()) < 1e-11
 
 
@@ -63,7 +65,7 @@ def test_random_chain_gradient(oracle, engine):
     compute_forces at the device density, and vs central differences."""
     mol = M.random_chain(12, seed=12)
     print("case: random chain gradient natom=12", flush=True)
-    r = engine.rhf(mol, pure=True, gradient=True, **CHAIN)
+    r = engine.rhf(mol, pure=True, gradient=True, **CHAIN, guess="hcore")
     assert r["converged"]
     b = oracle.Basis(mol.shells("cc-pvdz"), pure=True)
     nocc = mol.nel // 2
@@ -75,8 +77,8 @@ def test_random_chain_gradient(oracle, engine):
     assert np.max(np.abs(g.sum(axis=0))) < 1e-7
     h = 1e-4
     for atom, axis in ((0, 0), (5, 1), (11, 2)):
-        ep = engine.rhf(mol.displaced(atom, axis, h), **CHAIN)["energy"]
-        em = engine.rhf(mol.displaced(atom, axis, -h), **CHAIN)["energy"]
+        ep = engine.rhf(mol.displaced(atom, axis, h), **CHAIN, guess="hcore")["energy"]
+        em = engine.rhf(mol.displaced(atom, axis, -h), **CHAIN, guess="hcore")["energy"]
         assert abs(g[atom, axis] - (ep - em) / (2 * h)) < 5e-7, (atom, axis)
 
 
@@ -86,8 +88,8 @@ def test_random_chain_mixed_precision_error(engine):
     mol = M.random_chain(12, seed=12)
     print("case: random chain mixed vs fp64", flush=True)
     kw = dict(enconverge=1e-10, densconverge=1e-8, maxiter=120)
-    r64 = engine.rhf(mol, gradient=True, **kw)
-    rmx = engine.rhf(mol, gradient=True, mixed=1, **kw)
+    r64 = engine.rhf(mol, gradient=True, **kw, guess="hcore")
+    rmx = engine.rhf(mol, gradient=True, mixed=1, **kw, guess="hcore")
     assert r64["converged"] and rmx["converged"]
     assert abs(rmx["energy"] - r64["energy"]) < 1e-6
     assert np.max(np.abs(rmx["grad"] - r64["grad"])) < 1e-6

@@ -21,10 +21,10 @@ def _small_batch():
 def test_batch_matches_single_runs(engine):
     mols = _small_batch()
     print("case: batch of %d molecules, 4 workers" % len(mols), flush=True)
-    out, wall = engine.rhf_batch(mols, nworkers=4, **KW)
+    out, wall = engine.rhf_batch(mols, nworkers=4, **KW, guess="hcore")
     assert wall > 0 and len(out) == len(mols)
     for m, r in zip(mols, out):
-        s = engine.rhf(m, **KW)
+        s = engine.rhf(m, **KW, guess="hcore")
         assert r["converged"] == s["converged"]
         if s["converged"]:
             assert abs(r["energy"] - s["energy"]) < 1e-9, (m.natom, r["energy"], s["energy"])
@@ -33,8 +33,8 @@ def test_batch_matches_single_runs(engine):
 def test_batch_mixed_within_one_microhartree(engine):
     mols = _small_batch()
     print("case: batch mixed vs fp64", flush=True)
-    r64, _ = engine.rhf_batch(mols, nworkers=4, **KW)
-    rmx, _ = engine.rhf_batch(mols, nworkers=4, mixed=1, **KW)
+    r64, _ = engine.rhf_batch(mols, nworkers=4, **KW, guess="hcore")
+    rmx, _ = engine.rhf_batch(mols, nworkers=4, mixed=1, **KW, guess="hcore")
     nconv = 0
     for a, b in zip(r64, rmx):
         if a["converged"] and b["converged"]:

@@ -38,9 +38,9 @@ def test_df_scf_fp32_vs_fp64(engine):
     """DF-SCF with FP32-MFMA contractions vs FP64 contractions: |dE| < 1 uEh."""
     mol = M.water()
     print("case: water DF-SCF fp32 vs fp64", flush=True)
-    r64 = engine.rhf(mol, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-7)
-    r32 = engine.rhf(mol, aux="cc-pvdz-jkfit", df=1, mixed=1, enconverge=1e-10, densconverge=1e-7)
-    direct = engine.rhf(mol, enconverge=1e-10, densconverge=1e-7)
+    r64 = engine.rhf(mol, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-7, guess="hcore")
+    r32 = engine.rhf(mol, aux="cc-pvdz-jkfit", df=1, mixed=1, enconverge=1e-10, densconverge=1e-7, guess="hcore")
+    direct = engine.rhf(mol, enconverge=1e-10, densconverge=1e-7, guess="hcore")
     assert r64["converged"] and r32["converged"]
     assert abs(r32["energy"] - r64["energy"]) < 1e-6
     # DF fitting error vs exact J/K is well above 1 uEh but small
@@ -58,7 +58,7 @@ def test_df_scf_matches_recorded_monomer_energies(engine):
     print("case: DF-SCF of the 20 recorded monomers", flush=True)
     for k, m in enumerate(mono):
         mol = M.Molecule(list(m["Z"]), np.array(m["xyz_angstrom"]) * M.TOBOHR, 0, 1, "monomer")
-        r = engine.rhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-8)
+        r = engine.rhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-8, guess="hcore")
         assert r["converged"] and abs(r["energy"] - m["energy"]) < 1e-6, (k, r["energy"], m["energy"])
-        r32 = engine.rhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, mixed=1, enconverge=1e-10, densconverge=1e-8)
+        r32 = engine.rhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, mixed=1, enconverge=1e-10, densconverge=1e-8, guess="hcore")
         assert r32["converged"] and abs(r32["energy"] - m["energy"]) < 1e-6, (k, r32["energy"], m["energy"])

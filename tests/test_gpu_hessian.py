@@ -29,7 +29,7 @@ def _oracle_grad(oracle, mol, pure):
 def test_water_hessian_matches_oracle(oracle, engine, pure):
     mol = M.Molecule([8, 1, 1], WATER)
     print("case: water cc-pVDZ numerical Hessian pure=%s" % pure, flush=True)
-    H, runs = engine.rhf_hessian(mol, pure=pure, **TIGHT)
+    H, runs = engine.rhf_hessian(mol, pure=pure, **TIGHT, guess="hcore")
     assert len(runs) == 18 and all(r["converged"] for r in runs)
     Ho = oracle.hessian_numeric(lambda x: _oracle_grad(oracle, M.Molecule([8, 1, 1], x), pure), WATER)
     assert np.max(np.abs(H - Ho)) < 2e-6, np.max(np.abs(H - Ho))

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 def test_water_6d_energy_pin(engine):
     """test/h2otest.out_1:124: E(RHF/cc-pVDZ, 6 Cartesian d) = -75.990178761701."""
     print("case: C++ SCF water 6d", flush=True)
-    r = engine.rhf(M.water_h2otest(), pure=False, enconverge=1e-11, densconverge=1e-9)
+    r = engine.rhf(M.water_h2otest(), pure=False, enconverge=1e-11, densconverge=1e-9, guess="hcore")
     assert r["converged"]
     assert abs(r["energy"] - (-75.990178761701)) < 1e-8
     # D S D = 2 D and tr(D S) = N_el for the converged closed-shell density
@@ -24,7 +24,7 @@ def test_water_6d_energy_pin(engine):
 def test_benzene_6d_energy_pin(engine):
     """test/benzene.out_1:178: E = -230.723053 (printed to 1e-6)."""
     print("case: C++ SCF benzene 6d", flush=True)
-    r = engine.rhf(M.benzene_dz_ref(), pure=False, enconverge=1e-9, densconverge=1e-6)
+    r = engine.rhf(M.benzene_dz_ref(), pure=False, enconverge=1e-9, densconverge=1e-6, guess="hcore")
     assert r["converged"]
     assert abs(r["energy"] - (-230.723053)) < 1.5e-6
 
@@ -32,7 +32,7 @@ def test_benzene_6d_energy_pin(engine):
 def test_water_5d_matches_oracle_scf(oracle, engine):
     mol = M.water()
     print("case: C++ SCF water 5d vs oracle", flush=True)
-    r = engine.rhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8)
+    r = engine.rhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8, guess="hcore")
     ob = oracle.Basis(mol.shells("cc-pvdz"), pure=True)
     S, T, V = ob.one_body(mol.Z, mol.xyz)
     Q = ob.schwarz()
@@ -47,8 +47,8 @@ def test_mixed_precision_scf_benzene(engine):
     FP64 iterations) reaches the FP64 energy to 1 microhartree."""
     mol = M.benzene()
     print("case: mixed-precision SCF benzene", flush=True)
-    r64 = engine.rhf(mol, pure=True, enconverge=1e-9, densconverge=1e-6)
-    rmx = engine.rhf(mol, pure=True, enconverge=1e-9, densconverge=1e-6, mixed=1, mixed_threshold=1e-4)
+    r64 = engine.rhf(mol, pure=True, enconverge=1e-9, densconverge=1e-6, guess="hcore")
+    rmx = engine.rhf(mol, pure=True, enconverge=1e-9, densconverge=1e-6, mixed=1, mixed_threshold=1e-4, guess="hcore")
     assert r64["converged"] and rmx["converged"]
     assert rmx["n_fp32"] > 0
     assert abs(rmx["energy"] - r64["energy"]) < 1e-6
@@ -57,7 +57,7 @@ def test_mixed_precision_scf_benzene(engine):
 def test_odd_electrons_rejected(engine):
     mol = M.Molecule([8, 1], np.array([[0, 0, 0], [0, 0, 1.8]]))
     with pytest.raises(Exception, match="odd number of electrons"):
-        engine.rhf(mol, pure=True)
+        engine.rhf(mol, pure=True, guess="hcore")
 
 
 def _golden_logs():
@@ -73,7 +73,7 @@ def test_scf_trajectory_matches_recorded_water_log(engine):
     (diis_start = 0)."""
     rows = _golden_logs()["h2otest"]["rows"]
     print("case: C++ SCF trajectory vs test/h2otest.out_1", flush=True)
-    r = engine.rhf(M.water_h2otest(), pure=False, enconverge=1e-8, densconverge=1e-8, diis_start=0)
+    r = engine.rhf(M.water_h2otest(), pure=False, enconverge=1e-8, densconverge=1e-8, diis_start=0, guess="hcore")
     assert r["converged"] and r["iterations"] == len(rows)
     log = r["log"]
     for k, (it_r, E_r, dE_r, dD_r) in enumerate(rows):
@@ -87,7 +87,7 @@ def test_scf_first_iterations_match_recorded_benzene_log(engine):
     """test/benzene.out_1 (direct run): iterations 0-2, before any DIIS."""
     rows = _golden_logs()["benzene"]["rows"]
     print("case: C++ SCF benzene first iterations vs test/benzene.out_1", flush=True)
-    r = engine.rhf(M.benzene_dz_ref(), pure=False, maxiter=3)
+    r = engine.rhf(M.benzene_dz_ref(), pure=False, maxiter=3, guess="hcore")
     log = r["log"]
     assert len(log) == 3
     for k, (it_r, E_r, dE_r, dD_r) in enumerate(rows[:3]):

@@ -35,7 +35,7 @@ def test_jk_uhf_matches_oracle(oracle, engine, pure):
 def test_uhf_energy_matches_oracle(oracle, engine, mol_fn, na, nb):
     mol = mol_fn()
     print("case: C++ UHF %s" % mol.name, flush=True)
-    r = engine.uhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8, maxiter=80)
+    r = engine.uhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8, maxiter=80, guess="hcore")
     ob = oracle.Basis(mol.shells("cc-pvdz"), pure=True)
     S, T, V = ob.one_body(mol.Z, mol.xyz)
     Q = ob.schwarz()
@@ -54,14 +54,14 @@ def test_uhf_energy_matches_oracle(oracle, engine, mol_fn, na, nb):
 def test_closed_shell_uhf_equals_rhf_device(engine):
     mol = M.water()
     print("case: C++ UHF closed shell vs RHF", flush=True)
-    ru = engine.uhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8)
-    rr = engine.rhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8)
+    ru = engine.uhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8, guess="hcore")
+    rr = engine.rhf(mol, pure=True, enconverge=1e-10, densconverge=1e-8, guess="hcore")
     assert ru["converged"] and abs(ru["energy"] - rr["energy"]) < 1e-9
 
 
 def test_df_uhf_closed_shell_equals_df_rhf(engine):
     mol = M.water()
     print("case: C++ DF-UHF closed shell vs DF-RHF", flush=True)
-    ru = engine.uhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-8)
-    rr = engine.rhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-8)
+    ru = engine.uhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-8, guess="hcore")
+    rr = engine.rhf(mol, pure=True, aux="cc-pvdz-jkfit", df=1, enconverge=1e-10, densconverge=1e-8, guess="hcore")
     assert ru["converged"] and abs(ru["energy"] - rr["energy"]) < 1e-9

@@ -138,3 +138,88 @@ def test_one_body_identities(oracle):
     _, _, Vf = b.one_body([1.0], np.array([[R, 0.0, 0.0]]))
     # multipole expansion: -S/R plus dipole terms O(1/R^2)
     np.testing.assert_allclose(Vf, -S / R, atol=1e-9)
+
+
+def _tiny_cartesian_basis(seed):
+    """s, p, d shells on two centres, one or two primitives each."""
+    rng = np.random.default_rng(seed)
+    sh, raw = [], []
+    for i, l in enumerate((0, 1, 2, 0, 1)):
+        npr = 2 if l < 2 else 1
+        ex = list(rng.uniform(0.4, 2.5, npr))
+        cf = list(rng.uniform(0.3, 1.0, npr))
+        cen = (0.0, 0.0, 0.0) if i < 3 else (0.3, -0.4, 1.1)
+        sh.append(M.Shell(l, ex, cf, cen, 0 if i < 3 else 1))
+        raw.append((l, ex, cf, cen))
+    return sh, raw
+
+
+def test_schwarz_is_sqrt_of_max_diagonal(oracle):
+    """compute_schwarz_ints with use_2norm = false (src/integrals.cpp:471-513):
+    Q(s1,s2) = sqrt(max_i |(ab|ab)_ii|) over the diagonal of the n12 x n12
+    shell-pair block, symmetric.  The (ab|ab) blocks come from the
+    independent McMurchie-Davidson integrals of tests/md_ref.py.  The 2-norm
+    variant (sqrt of the diagonal's Euclidean norm) is shown to differ, so
+    that mistake fails.  (The max over all block elements cannot differ: the
+    (ab|ab) block is a Coulomb metric, positive semidefinite, so its largest
+    element lies on the diagonal; the test checks that too.)"""
+    import md_ref
+    sh, raw = _tiny_cartesian_basis(5)
+    b = oracle.Basis(sh, pure=False)
+    Q = b.schwarz()
+    np.testing.assert_array_equal(Q, Q.T)
+    differs_2norm = 0
+    for s1 in range(len(sh)):
+        for s2 in range(s1 + 1):
+            blk = md_ref.shell_quartet([raw[s1], raw[s2], raw[s1], raw[s2]])
+            n12 = blk.shape[0] * blk.shape[1]
+            m = blk.reshape(n12, n12)
+            diag = np.abs(np.diag(m))
+            ref = np.sqrt(diag.max())
+            assert abs(Q[s1, s2] - ref) <= 1e-12 * ref, (s1, s2, Q[s1, s2], ref)
+            if abs(np.sqrt(np.linalg.norm(diag)) - ref) > 1e-6 * ref:
+                differs_2norm += 1
+            assert np.abs(m).max() <= diag.max() * (1 + 1e-14)
+    assert differs_2norm >= 5
+
+
+def test_schwarz_pure_from_cartesian_diagonal(oracle):
+    """Pure shells: Q is taken over the solid-harmonic (ab|ab) block, whose
+    diagonal is the Cartesian block transformed by the real solid harmonics."""
+    import md_ref
+    sh, raw = _tiny_cartesian_basis(6)
+    b = oracle.Basis(sh, pure=True)
+    Q = b.schwarz()
+    s1, s2 = 2, 1                      # (d, p) on one centre
+    blk = md_ref.shell_quartet([raw[s1], raw[s2], raw[s1], raw[s2]])
+    Td = np.array([[oracle.lib().or_sph_coef(2, m, ic) for ic in range(6)] for m in range(-2, 3)])
+    pure = np.einsum("ai,bj,ck,dl,ijkl->abcd", Td, np.eye(3), Td, np.eye(3), blk)
+    ref = np.sqrt(np.abs(np.diag(pure.reshape(15, 15))).max())
+    assert abs(Q[s1, s2] - ref) <= 1e-11 * ref
+    # pure and Cartesian bounds of a d pair differ
+    assert abs(oracle.Basis(sh, pure=False).schwarz()[s1, s2] - ref) > 1e-6 * ref
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_shellblock_norm_is_block_max(oracle, pure):
+    """compute_shellblock_norm (src/integrals.cpp:451-469): entry (s1, s2) is
+    the infinity norm (max |x|) of the block rows of shell s1, columns of
+    shell s2 of a row-major matrix.  Pinned three ways: numpy block maxima
+    via np.maximum.reduceat over the shell offsets; a single non-zero element
+    lands in exactly one entry, at (shell of its row, shell of its column);
+    a non-symmetric matrix gives the transpose for the transpose."""
+    mol = M.water()
+    b = oracle.Basis(mol.shells("cc-pvdz"), pure=pure)
+    n, off = b.nbf, b.offsets
+    rng = np.random.default_rng(11)
+    A = rng.normal(size=(n, n))
+    ref = np.maximum.reduceat(np.maximum.reduceat(np.abs(A), off, axis=0), off, axis=1)
+    np.testing.assert_array_equal(b.shellblock_norm(A), ref)
+    np.testing.assert_array_equal(b.shellblock_norm(A.T), ref.T)
+    shell_of = np.repeat(np.arange(b.nsh), b.sizes)
+    for i, j in ((0, n - 1), (n // 2, 3), (7, 7)):
+        E = np.zeros((n, n))
+        E[i, j] = -2.5
+        out = b.shellblock_norm(E)
+        assert out[shell_of[i], shell_of[j]] == 2.5
+        assert np.count_nonzero(out) == 1
