@@ -91,6 +91,22 @@ m2_status m2_jk_direct(m2_ctx* ctx, const double* D, double multiplier, double p
 /* same on device buffers (n*n doubles each) */
 m2_status m2_jk_direct_device(m2_ctx* ctx, const double* dD, double multiplier, double precision,
                               unsigned flags, double* dJ, double* dK);
+/* m2_jk_direct_device without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and screening as m2_jk_direct_device;
+ * the context's basis must already be set (m2_basis_set allocates).  The
+ * per-build work memory (shell-block norms, unsymmetrised J/K accumulators,
+ * work-list totals, heavy-class tasks) lives in one static device array of
+ * M2_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), so the
+ * basis may have at most ~2000 functions (2 n^2 doubles plus the tasks must
+ * fit); larger inputs return M2_ERR_ARG with the size needed.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (two contexts on
+ * two host threads) race and must be serialised by the caller.  Profiling
+ * (m2_set_profiling) is ignored on this path. */
+#define M2_NOALLOC_SCRATCH_BYTES (64u << 20)
+m2_status m2_jk_direct_device_noalloc(m2_ctx* ctx, const double* dD, double multiplier, double precision,
+                                      unsigned flags, double* dJ, double* dK);
 /* UnrestrictedFock::formJKdirect (src/fockbuild.cpp:267-364): J and K of the
  * alpha and beta densities (Da = Ca_occ Ca_occ^T, Db likewise), every quartet
  * screened with the block norms of Da + Db; the unrestricted Fock matrices

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -621,31 +622,57 @@ std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, 
 // dD2 (optional): a second density digested from the same integrals into
 // dJ2, dK2 (the unrestricted build); dScreen: the density whose block norms
 // screen (default dD).
+// Device memory one direct J/K build works in: the shell-block norms, the
+// unsymmetrised J/K accumulators, the work-list totals (then max|D| as a
+// double) and the heavy-class tasks.  Either the context's own buffers
+// (grown on demand) or, for the allocation-free entry point, fixed slices of
+// the static scratch array (nothing is allocated; the tasks must fit).
+struct JKWork {
+    double* Dn = nullptr;
+    double* J = nullptr;
+    double* K = nullptr;
+    int* lists = nullptr;
+    Task* tasks = nullptr;
+    size_t task_cap = 0;
+    bool fixed = false;
+};
+
 m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, double precision, unsigned flags,
                          double* dJ, double* dK, const double* dScreen = nullptr, const double* dD2 = nullptr,
-                         double* dJ2 = nullptr, double* dK2 = nullptr) {
+                         double* dJ2 = nullptr, double* dK2 = nullptr, const JKWork* fixed = nullptr) {
     BasisData& b = ctx->obs;
     const int n = b.nbf, nsh = b.nsh;
     ARGCHK(n > 0, "m2_jk_direct: no orbital basis set");
     ARGCHK(flags <= M2_MIXED, "m2_jk_direct: unknown flags");
     const bool mixed = flags == M2_MIXED;
-    HIPCHK(ctx->d_Dn.reserve((size_t)nsh * nsh));
     const int nden = dD2 ? 2 : 1;
-    HIPCHK(ctx->d_J.reserve((size_t)nden * n * n));
-    HIPCHK(ctx->d_K.reserve((size_t)nden * n * n));
-    HIPCHK(ctx->d_lists.reserve(2 * NCOMBO + 2));
+    JKWork w;
+    if (fixed) {
+        w = *fixed;
+    } else {
+        HIPCHK(ctx->d_Dn.reserve((size_t)nsh * nsh));
+        HIPCHK(ctx->d_J.reserve((size_t)nden * n * n));
+        HIPCHK(ctx->d_K.reserve((size_t)nden * n * n));
+        HIPCHK(ctx->d_lists.reserve(2 * NCOMBO + 2));
+        w.Dn = ctx->d_Dn.p;
+        w.J = ctx->d_J.p;
+        w.K = ctx->d_K.p;
+        w.lists = ctx->d_lists.p;
+    }
+    // profiling creates events and a counter buffer: not in the fixed-memory build
+    const bool prof = ctx->prof && !w.fixed;
     // shell-block norms, max|D| and the register-class work lists, on the device
-    launch_blocknorm(dScreen ? dScreen : dD, n, nsh, b.d_bf.p, b.d_nbf.p, ctx->d_Dn.p, ctx->stream);
-    double* d_dmax = reinterpret_cast<double*>(ctx->d_lists.p + 2 * NCOMBO);
-    launch_dmax(ctx->d_Dn.p, nsh * nsh, d_dmax, ctx->stream);
+    launch_blocknorm(dScreen ? dScreen : dD, n, nsh, b.d_bf.p, b.d_nbf.p, w.Dn, ctx->stream);
+    double* d_dmax = reinterpret_cast<double*>(w.lists + 2 * NCOMBO);
+    launch_dmax(w.Dn, nsh * nsh, d_dmax, ctx->stream);
     for (int ci = 0; ci < NCOMBO; ++ci) {
         const JKList& L = ctx->lists[ci];
         if (L.nbra > 0)
             launch_task_lists(L, ctx->d_pair_Q.p, d_dmax, precision, ctx->fp32_below, mixed ? 1 : 0,
-                              ctx->d_lists.p + 2 * ci, ctx->stream);
+                              w.lists + 2 * ci, ctx->stream);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(ctx->h_lists, ctx->d_lists.p, sizeof(int) * (2 * NCOMBO + 2), hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(ctx->h_lists, w.lists, sizeof(int) * (2 * NCOMBO + 2), hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     double Dmax;
@@ -653,7 +680,27 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     // host tasks of the heavy classes
     std::vector<ComboRange> heavy = build_tasks(ctx, precision, Dmax, mixed, false, true);
     long long units = (long long)ctx->h_tasks.size();
-    HIPCHK(upload_task_list(ctx));
+    if (!w.fixed) {
+        HIPCHK(upload_task_list(ctx));
+        w.tasks = ctx->d_tasks.p;
+    } else if (!ctx->h_tasks.empty()) {
+        const size_t nt = ctx->h_tasks.size();
+        if (nt > w.task_cap) {
+            ctx->err = "m2_jk_direct_device_noalloc: " + std::to_string(nt) + " heavy-class tasks exceed the " +
+                       std::to_string(w.task_cap) + " that fit in the static scratch buffer";
+            return M2_ERR_ARG;
+        }
+        // through the pinned staging buffer when it is large enough (it is
+        // never grown here), else straight from pageable host memory
+        if (nt <= ctx->h_tasks_pin.n) {
+            std::memcpy(ctx->h_tasks_pin.p, ctx->h_tasks.data(), sizeof(Task) * nt);
+            HIPCHK(hipMemcpyAsync(w.tasks, ctx->h_tasks_pin.p, sizeof(Task) * nt, hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(w.tasks, ctx->h_tasks.data(), sizeof(Task) * nt, hipMemcpyHostToDevice,
+                                  ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
+    }
     // launch plan in class order: register combos from their lists, heavy ones from host tasks
     struct Launch {
         int X, Y, ci, hv;   // ci: list index (register) or -1; hv: index into `heavy` or -1
@@ -674,11 +721,12 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
             }
         }
     ctx->last_tasks = units;
-    launch_zero(ctx->d_J.p, (size_t)nden * n * n, ctx->stream);
-    launch_zero(ctx->d_K.p, (size_t)nden * n * n, ctx->stream);
+    launch_zero(w.J, (size_t)nden * n * n, ctx->stream);
+    launch_zero(w.K, (size_t)nden * n * n, ctx->stream);
     JKArgs a = jk_args(ctx);
+    a.Dn = w.Dn;
     a.counters = nullptr;
-    if (ctx->prof) {
+    if (prof) {
         HIPCHK(ctx->d_counters.reserve(72));
         launch_zero((double*)ctx->d_counters.p, 72, ctx->stream);
         if (ctx->prof_counters) a.counters = ctx->d_counters.p;
@@ -689,19 +737,19 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
         }
     }
     a.D = dD;
-    a.J = ctx->d_J.p;
-    a.K = ctx->d_K.p;
+    a.J = w.J;
+    a.K = w.K;
     a.nden = nden;
     a.D2 = dD2;
-    a.J2 = nden == 2 ? ctx->d_J.p + (size_t)n * n : nullptr;
-    a.K2 = nden == 2 ? ctx->d_K.p + (size_t)n * n : nullptr;
+    a.J2 = nden == 2 ? w.J + (size_t)n * n : nullptr;
+    a.K2 = nden == 2 ? w.K + (size_t)n * n : nullptr;
     a.precision = precision;
     a.fp32_below = ctx->fp32_below;
     a.scale = multiplier;
     for (size_t pi = 0; pi < plan.size(); ++pi) {
         const Launch& pl = plan[pi];
         a.combo = 6 * pl.X + pl.Y;
-        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi], ctx->stream));
+        if (prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi], ctx->stream));
         a.fam_kets = ctx->fam_on && fam_combo(pl.X, pl.Y, ctx->fam_wide) ? 1 : 0;
         if (pl.ci >= 0) {
             const JKList& L = ctx->lists[pl.ci];
@@ -725,16 +773,16 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
                 const int b0 = mode == 0 ? r.t0 : r.tmid, b1 = mode == 0 ? r.tmid : r.t1;
                 if (b1 <= b0) continue;
                 HIPCHK(launch_jk_heavy(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1],
-                                       b.pure, a, tables(ctx), ctx->d_tasks.p + b0, b1 - b0, mode, ctx->stream));
+                                       b.pure, a, tables(ctx), w.tasks + b0, b1 - b0, mode, ctx->stream));
             }
         }
-        if (ctx->prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi + 1], ctx->stream));
+        if (prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi + 1], ctx->stream));
     }
-    launch_symmetrize(ctx->d_J.p, ctx->d_K.p, dJ, dK, n, ctx->stream);
-    if (nden == 2) launch_symmetrize(ctx->d_J.p + (size_t)n * n, ctx->d_K.p + (size_t)n * n, dJ2, dK2, n, ctx->stream);
+    launch_symmetrize(w.J, w.K, dJ, dK, n, ctx->stream);
+    if (nden == 2) launch_symmetrize(w.J + (size_t)n * n, w.K + (size_t)n * n, dJ2, dK2, n, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (ctx->prof) {
+    if (prof) {
         unsigned long long cnt[72];
         HIPCHK(hipMemcpy(cnt, ctx->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
         for (size_t pi = 0; pi < plan.size(); ++pi) {
@@ -834,6 +882,37 @@ hipError_t grad_tasks_device(m2_ctx* ctx, double precision, double Dmax, bool mi
 
 }  // namespace m2
 
+namespace {
+
+// Device memory of m2_jk_direct_device_noalloc: a file-scope array, one copy
+// per GPU the library runs on, held until the process ends.  Host code never
+// uses its name as an address (that is a host-side stand-in); each device's
+// copy is reached through the address hipGetSymbolAddress returns with that
+// device current, kept per device in g_scratch.
+static __device__ __align__(256) unsigned char scratch[M2_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch;
+
+// once per device, with `device` current
+hipError_t scratch_address(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it == g_scratch.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        it = g_scratch.emplace(device, static_cast<unsigned char*>(p)).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
 extern "C" {
 
 m2_status m2_ctx_create(m2_ctx** out, int device) {
@@ -850,6 +929,11 @@ m2_status m2_ctx_create(m2_ctx** out, int device) {
     }
     if (hipHostMalloc((void**)&ctx->h_lists, sizeof(int) * (2 * NCOMBO + 2)) != hipSuccess) {
         std::fprintf(stderr, "m2_ctx_create: pinned allocation failed\n");
+        m2_ctx_destroy(ctx);
+        return M2_ERR_HIP;
+    }
+    if ((e = scratch_address(device, &ctx->scratch)) != hipSuccess) {
+        std::fprintf(stderr, "m2_ctx_create: hipGetSymbolAddress of the scratch array: %s\n", hipGetErrorString(e));
         m2_ctx_destroy(ctx);
         return M2_ERR_HIP;
     }
@@ -1123,6 +1207,39 @@ m2_status m2_jk_direct_device(m2_ctx* ctx, const double* dD, double multiplier, 
     ARGCHK(dD && dJ && dK, "m2_jk_direct_device: null buffer");
     HIPCHK(hipSetDevice(ctx->device));
     return jk_device_core(ctx, dD, multiplier, precision, flags, dJ, dK);
+}
+
+m2_status m2_jk_direct_device_noalloc(m2_ctx* ctx, const double* dD, double multiplier, double precision,
+                                      unsigned flags, double* dJ, double* dK) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(dD && dJ && dK, "m2_jk_direct_device_noalloc: null buffer");
+    ARGCHK(ctx->obs.nbf > 0, "m2_jk_direct_device_noalloc: no orbital basis set");
+    if (!ctx->scratch) {
+        ctx->err = "m2_jk_direct_device_noalloc: the scratch array has no device address";
+        return M2_ERR_STATE;
+    }
+    const size_t n = ctx->obs.nbf, nsh = ctx->obs.nsh;
+    // slices of the scratch array: norms | list totals + max|D| | J | K | tasks
+    const size_t o_lists = align256(sizeof(double) * nsh * nsh);
+    const size_t o_J = o_lists + align256(sizeof(int) * (2 * NCOMBO + 2));
+    const size_t o_K = o_J + align256(sizeof(double) * n * n);
+    const size_t o_T = o_K + align256(sizeof(double) * n * n);
+    if (o_T > M2_NOALLOC_SCRATCH_BYTES) {
+        ctx->err = "m2_jk_direct_device_noalloc: " + std::to_string(n) + " basis functions need " +
+                   std::to_string(o_T) + " bytes of scratch; the static buffer holds " +
+                   std::to_string((size_t)M2_NOALLOC_SCRATCH_BYTES);
+        return M2_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    JKWork w;
+    w.fixed = true;
+    w.Dn = reinterpret_cast<double*>(ctx->scratch);
+    w.lists = reinterpret_cast<int*>(ctx->scratch + o_lists);
+    w.J = reinterpret_cast<double*>(ctx->scratch + o_J);
+    w.K = reinterpret_cast<double*>(ctx->scratch + o_K);
+    w.tasks = reinterpret_cast<Task*>(ctx->scratch + o_T);
+    w.task_cap = (M2_NOALLOC_SCRATCH_BYTES - o_T) / sizeof(Task);
+    return jk_device_core(ctx, dD, multiplier, precision, flags, dJ, dK, nullptr, nullptr, nullptr, nullptr, &w);
 }
 
 m2_status m2_jk_direct(m2_ctx* ctx, const double* D, double multiplier, double precision, unsigned flags, double* J,

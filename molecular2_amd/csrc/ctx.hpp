@@ -159,6 +159,9 @@ struct m2_ctx {
     int df_jblocks = 0;
     bool df_ready = false;
     double df_last_ms[4] = {0, 0, 0, 0};   // X = C^T B, K = X^T X, J, total (profiling)
+    // this device's copy of the static scratch array of the allocation-free
+    // J/K entry point (hipGetSymbolAddress at context creation)
+    unsigned char* scratch = nullptr;
     // dense linear algebra (rocBLAS / rocSOLVER)
     void* blas = nullptr;
     m2::DevBuf<double> la_a, la_b, la_c, la_w;

@@ -83,6 +83,7 @@ def lib():
         L.m2_one_body.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.m2_jk_direct.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_uint, _dp, _dp]
         L.m2_jk_direct_device.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint, vp, vp]
+        L.m2_jk_direct_device_noalloc.argtypes = [vp, vp, C.c_double, C.c_double, C.c_uint, vp, vp]
         L.m2_set_mixed_threshold.argtypes = [vp, C.c_double]
         L.m2_set_jk_families.argtypes = [vp, C.c_int]
         L.m2_set_jk_families.restype = C.c_int
@@ -143,6 +144,7 @@ def lib():
             getattr(L, name).restype = C.c_int
         for name in ("m2_set_profiling", "m2_get_profile", "m2_reset_profile", "m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
                      "m2_shellblock_norm", "m2_one_body", "m2_jk_direct", "m2_jk_direct_device",
+                     "m2_jk_direct_device_noalloc",
                      "m2_set_mixed_threshold", "m2_last_stats"):
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -307,6 +309,13 @@ class Engine:
         self._chk(self.L.m2_jk_direct_device(self.h, C.c_void_p(dD_ptr), multiplier, precision,
                                              M2_MIXED if mixed else M2_FP64, C.c_void_p(dJ_ptr),
                                              C.c_void_p(dK_ptr)))
+
+    def jk_device_noalloc(self, dD_ptr, dJ_ptr, dK_ptr, multiplier=1.0, precision=1e-12, mixed=False):
+        """jk_device through m2_jk_direct_device_noalloc: the library allocates
+        nothing (its work memory is one static device array per GPU)."""
+        self._chk(self.L.m2_jk_direct_device_noalloc(self.h, C.c_void_p(dD_ptr), multiplier, precision,
+                                                     M2_MIXED if mixed else M2_FP64, C.c_void_p(dJ_ptr),
+                                                     C.c_void_p(dK_ptr)))
 
     def set_mixed_threshold(self, t):
         self._chk(self.L.m2_set_mixed_threshold(self.h, t))
