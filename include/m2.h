@@ -126,6 +126,16 @@ m2_status m2_set_mixed_threshold(m2_ctx* ctx, double fp32_below);
  * always and for (ps|ss) and (ds|ss) when most family members are
  * significant together; 2: on for all four. */
 m2_status m2_set_jk_families(m2_ctx* ctx, int mode);
+/* Primitive screening of the direct J/K build.  Inside a shell quartet that
+ * survives the screening of src/fockbuild.cpp:98-115, a primitive quartet is
+ * skipped when q_ab q_cd max|D_blocks| < relative * precision, where q is the
+ * primitive pair's own bound sqrt(max |(ab|ab)|) (so |[ab|cd]| <= q_ab q_cd).
+ * The reference drops primitive quartets inside libint2 the same way, below
+ * its engine precision min(precision/max|D|, eps)/K_max^4
+ * (src/fockbuild.cpp:77-83).  Default relative = 1e-6 (each skipped
+ * primitive quartet moves a J/K element by < 1e-18 at precision 1e-12);
+ * 0 computes every primitive quartet. */
+m2_status m2_set_prim_screening(m2_ctx* ctx, double relative);
 /* statistics of the last J/K build: computed quartets (FP64, FP32), tasks */
 m2_status m2_last_stats(m2_ctx* ctx, long long* q64, long long* q32, long long* ntasks);
 
@@ -185,6 +195,10 @@ m2_status m2_set_profiling(m2_ctx* ctx, int on);
 m2_status m2_get_profile(m2_ctx* ctx, double* ms, long long* launches, long long* quartets,
                          long long* prim_quartets);
 m2_status m2_reset_profile(m2_ctx* ctx);
+/* primitive quartets actually computed per class combination (after
+ * primitive screening; m2_get_profile's prim_quartets counts all of them),
+ * accumulated with the device counters (on = 2) since the last reset */
+m2_status m2_get_profile_computed(m2_ctx* ctx, long long* computed);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,9 @@ JKArgs jk_args(m2_ctx* ctx) {
     a.sh_O = ctx->obs.d_O.p;
     a.sh_bf = ctx->obs.d_bf.p;
     a.Dn = ctx->d_Dn.p;
+    a.prim_q = ctx->d_prim_q.p;
+    a.fam_q = ctx->d_fam_q.p;
+    a.prim_tau = 0.0;
     a.n = ctx->obs.nbf;
     a.nsh = ctx->obs.nsh;
     if (ctx->fam_on) {
@@ -185,6 +188,7 @@ static m2_status build_families(m2_ctx* ctx) {
         double Q = 0.0;
         std::vector<PrimPair> pp;
         std::vector<double> km;
+        std::vector<float> q;     // primitive bounds
     };
     std::map<std::pair<int, int>, int> index;
     std::vector<FP> fps;
@@ -256,6 +260,29 @@ static m2_status build_families(m2_ctx* ctx) {
                     fp.km.push_back(c);
                 }
             }
+        // primitive bounds: an s-s primitive pair's self-repulsion is
+        // K^2 / sqrt(2p) (T = 0), so q = max_m |K_m| / (2p)^(1/4) over the
+        // members' factors; primitives ordered by q descending
+        const size_t np = fp.pp.size();
+        std::vector<float> q(np);
+        for (size_t k = 0; k < np; ++k) {
+            double mx = 0.0;
+            for (int m = 0; m < FAM_MAXMEM; ++m) mx = std::max(mx, std::fabs(fp.km[FAM_MAXMEM * k + m]));
+            q[k] = (float)(mx / std::pow(2.0 * fp.pp[k].p, 0.25));
+        }
+        std::vector<int> idx(np);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return q[x] > q[y]; });
+        std::vector<PrimPair> pp(np);
+        std::vector<double> km(fp.km.size());
+        fp.q.resize(np);
+        for (size_t k = 0; k < np; ++k) {
+            pp[k] = fp.pp[idx[k]];
+            fp.q[k] = q[idx[k]];
+            for (int m = 0; m < FAM_MAXMEM; ++m) km[FAM_MAXMEM * k + m] = fp.km[FAM_MAXMEM * idx[k] + m];
+        }
+        fp.pp.swap(pp);
+        fp.km.swap(km);
     }
     std::vector<int> order(nfp);
     std::iota(order.begin(), order.end(), 0);
@@ -268,7 +295,7 @@ static m2_status build_families(m2_ctx* ctx) {
     std::vector<PrimPair> prims;
     std::vector<BraPair<float>> prims_f;
     std::vector<double> km, fQ(nfp);
-    std::vector<float> km_f;
+    std::vector<float> km_f, fq;
     std::vector<int> off(nfp), np(nfp), pfam(npair, -1), pmem(npair, -1);
     std::vector<int4> mem(nfp);
     for (int r = 0; r < nfp; ++r) {
@@ -296,8 +323,11 @@ static m2_status build_families(m2_ctx* ctx) {
             prims_f.push_back(f);
         }
         km.insert(km.end(), fp.km.begin(), fp.km.end());
+        fq.insert(fq.end(), fp.q.begin(), fp.q.end());
     }
     km_f.assign(km.begin(), km.end());
+    HIPCHK(ctx->d_fam_q.reserve(std::max<size_t>(fq.size(), 1)));
+    if (!fq.empty()) HIPCHK(hipMemcpy(ctx->d_fam_q.p, fq.data(), sizeof(float) * fq.size(), hipMemcpyHostToDevice));
     HIPCHK(ctx->d_fam_prims.reserve(std::max<size_t>(prims.size(), 1)));
     HIPCHK(ctx->d_fam_prims_f.reserve(std::max<size_t>(prims.size(), 1)));
     HIPCHK(ctx->d_fam_km.reserve(std::max<size_t>(km.size(), 1)));
@@ -341,6 +371,7 @@ m2_status build_pairs(m2_ctx* ctx) {
     struct P {
         int a, b, cls;
         std::vector<PrimPair> pp;
+        std::vector<float> q;   // primitive-pair bounds (after the Schwarz pass)
     };
     std::vector<P> pairs;
     const double sqrt2pi52 = std::sqrt(2.0 * std::pow(M_PI, 2.5));
@@ -416,6 +447,12 @@ m2_status build_pairs(m2_ctx* ctx) {
         HIPCHK(hipMemcpy(ctx->d_pair_off.p, ctx->pair_off.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_pair_np.p, ctx->pair_np.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_pair_sh.p, ctx->pair_sh.data(), sizeof(int2) * npair, hipMemcpyHostToDevice));
+        // primitive bounds once they exist (the first upload precedes them)
+        HIPCHK(ctx->d_prim_q.reserve(std::max<size_t>(prims.size(), 1)));
+        std::vector<float> q;
+        for (int i = 0; i < npair; ++i) q.insert(q.end(), pairs[i].q.begin(), pairs[i].q.end());
+        if (q.size() == prims.size() && !q.empty())
+            HIPCHK(hipMemcpy(ctx->d_prim_q.p, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice));
         return M2_OK;
     };
     m2_status st = upload();
@@ -445,6 +482,72 @@ m2_status build_pairs(m2_ctx* ctx) {
     for (int i = 0; i < npair; ++i) {
         ctx->Qsh[(size_t)pairs[i].a * b.nsh + pairs[i].b] = Q[i];
         ctx->Qsh[(size_t)pairs[i].b * b.nsh + pairs[i].a] = Q[i];
+    }
+    // Primitive-pair bounds q = sqrt(max |(ab|ab)|) of each primitive pair on
+    // its own (contraction coefficients included), by the same Schwarz
+    // kernels run over one-primitive "pairs"; by Cauchy-Schwarz every
+    // primitive quartet satisfies |[ab|cd]| <= q_ab q_cd.  Each pair's
+    // primitives are then ordered by q descending, which lets the J/K loops
+    // stop at the first primitive below the screening threshold.
+    std::vector<double> qprim;
+    {
+        const int nprim = npair ? ctx->pair_off[npair - 1] + ctx->pair_np[npair - 1] : 0;
+        std::vector<int2> tsh(nprim);
+        std::vector<int> toff(nprim), tnp(nprim, 1);
+        for (int i = 0; i < npair; ++i)
+            for (int k = 0; k < ctx->pair_np[i]; ++k) {
+                tsh[ctx->pair_off[i] + k] = ctx->pair_sh[i];
+                toff[ctx->pair_off[i] + k] = ctx->pair_off[i] + k;
+            }
+        DevBuf<int2> d_tsh;
+        DevBuf<int> d_toff, d_tnp;
+        DevBuf<double> d_tq;
+        HIPCHK(d_tsh.reserve(nprim));
+        HIPCHK(d_toff.reserve(nprim));
+        HIPCHK(d_tnp.reserve(nprim));
+        HIPCHK(d_tq.reserve(nprim));
+        qprim.assign(nprim, 0.0);
+        if (nprim > 0) {
+            HIPCHK(hipMemcpy(d_tsh.p, tsh.data(), sizeof(int2) * nprim, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_toff.p, toff.data(), sizeof(int) * nprim, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_tnp.p, tnp.data(), sizeof(int) * nprim, hipMemcpyHostToDevice));
+            JKArgs a = jk_args(ctx);
+            a.pair_sh = d_tsh.p;
+            a.pair_off = d_toff.p;
+            a.pair_np = d_tnp.p;
+            int i0 = 0;
+            while (i0 < npair) {
+                int c = pairs[i0].cls, i1 = i0;
+                while (i1 < npair && pairs[i1].cls == c) ++i1;
+                const int p0 = ctx->pair_off[i0], p1 = ctx->pair_off[i1 - 1] + ctx->pair_np[i1 - 1];
+                if (p1 > p0) {
+                    if (SW_LAUNCH[c]) {
+                        SW_LAUNCH[c](b.pure, a, tables(ctx), p0, p1 - p0, d_tq.p, ctx->stream);
+                        HIPCHK(hipGetLastError());
+                    } else {
+                        HIPCHK(launch_schwarz_heavy(ctx->device, PAIR_L[c][0], PAIR_L[c][1], b.pure, a, tables(ctx),
+                                                    p0, p1 - p0, d_tq.p, ctx->stream));
+                    }
+                }
+                i0 = i1;
+            }
+            HIPCHK(hipMemcpyAsync(qprim.data(), d_tq.p, sizeof(double) * nprim, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
+        for (int i = 0; i < npair; ++i) {
+            P& pr = pairs[i];
+            const int o = ctx->pair_off[i];
+            std::vector<int> idx(pr.pp.size());
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return qprim[o + x] > qprim[o + y]; });
+            std::vector<PrimPair> pp(pr.pp.size());
+            pr.q.resize(pr.pp.size());
+            for (size_t k = 0; k < idx.size(); ++k) {
+                pp[k] = pr.pp[idx[k]];
+                pr.q[k] = (float)qprim[o + idx[k]];
+            }
+            pr.pp.swap(pp);
+        }
     }
     // final layout: class, np bucket, Q descending
     std::vector<int> order(npair);
@@ -727,8 +830,8 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     a.Dn = w.Dn;
     a.counters = nullptr;
     if (prof) {
-        HIPCHK(ctx->d_counters.reserve(72));
-        launch_zero((double*)ctx->d_counters.p, 72, ctx->stream);
+        HIPCHK(ctx->d_counters.reserve(108));
+        launch_zero((double*)ctx->d_counters.p, 108, ctx->stream);
         if (ctx->prof_counters) a.counters = ctx->d_counters.p;
         while (ctx->ev.size() < 2 * plan.size() + 2) {
             hipEvent_t e;
@@ -746,6 +849,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     a.precision = precision;
     a.fp32_below = ctx->fp32_below;
     a.scale = multiplier;
+    a.prim_tau = precision * ctx->prim_screen;
     for (size_t pi = 0; pi < plan.size(); ++pi) {
         const Launch& pl = plan[pi];
         a.combo = 6 * pl.X + pl.Y;
@@ -783,7 +887,7 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (prof) {
-        unsigned long long cnt[72];
+        unsigned long long cnt[108];
         HIPCHK(hipMemcpy(cnt, ctx->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
         for (size_t pi = 0; pi < plan.size(); ++pi) {
             float ms = 0.f;
@@ -793,8 +897,9 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
             ctx->prof_launch[c] += 1;
         }
         for (int c = 0; c < 36; ++c) {
-            ctx->prof_q[c] += (long long)cnt[2 * c];
-            ctx->prof_pq[c] += (long long)cnt[2 * c + 1];
+            ctx->prof_q[c] += (long long)cnt[3 * c];
+            ctx->prof_pq[c] += (long long)cnt[3 * c + 1];
+            ctx->prof_cpq[c] += (long long)cnt[3 * c + 2];
         }
     }
     return M2_OK;
@@ -1321,11 +1426,24 @@ m2_status m2_get_profile(m2_ctx* ctx, double* ms, long long* launches, long long
     return M2_OK;
 }
 
+m2_status m2_get_profile_computed(m2_ctx* ctx, long long* computed) {
+    if (!ctx || !computed) return M2_ERR_ARG;
+    for (int c = 0; c < 36; ++c) computed[c] = ctx->prof_cpq[c];
+    return M2_OK;
+}
+
+m2_status m2_set_prim_screening(m2_ctx* ctx, double relative) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(relative >= 0.0 && relative <= 1.0, "m2_set_prim_screening: relative threshold must lie in [0, 1]");
+    ctx->prim_screen = relative;
+    return M2_OK;
+}
+
 m2_status m2_reset_profile(m2_ctx* ctx) {
     if (!ctx) return M2_ERR_ARG;
     for (int c = 0; c < 36; ++c) {
         ctx->prof_ms[c] = 0.0;
-        ctx->prof_launch[c] = ctx->prof_q[c] = ctx->prof_pq[c] = 0;
+        ctx->prof_launch[c] = ctx->prof_q[c] = ctx->prof_pq[c] = ctx->prof_cpq[c] = 0;
     }
     return M2_OK;
 }

@@ -112,6 +112,10 @@ struct m2_ctx {
     m2::DevBuf<int> d_pair_off, d_pair_np;
     m2::DevBuf<int2> d_pair_sh;
     m2::DevBuf<double> d_pair_Q;
+    // primitive-pair bounds sqrt(max |(ab|ab)|), aligned with d_prims / d_fam_prims,
+    // and the primitive screening threshold relative to the quartet precision
+    m2::DevBuf<float> d_prim_q, d_fam_q;
+    double prim_screen = 1e-6;
     // s-s ket family pairs (shells of one atom with identical exponents share
     // their primitive pairs), ordered like pairs: np bucket, Q descending
     int fam_mode = 1;                  // m2_set_jk_families
@@ -147,7 +151,7 @@ struct m2_ctx {
     std::vector<hipEvent_t> ev;
     double prof_ms[36] = {0};
     long long prof_launch[36] = {0};
-    long long prof_q[36] = {0}, prof_pq[36] = {0};
+    long long prof_q[36] = {0}, prof_pq[36] = {0}, prof_cpq[36] = {0};
     m2::DevBuf<unsigned long long> d_counters;
     // density fitting
     std::vector<m2::PrimPair> h_aux_prims;

@@ -251,23 +251,34 @@ def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
     em.out("  static constexpr int L = %d;" % L)
     em.out("  static constexpr int NCART = %d;" % (na * nb * nc * nd))
     em.out("  static constexpr int NACC = %d;   // VRR accumulators per lane" % (len(E) * len(F)))
-    em.out("  template <typename R, bool PURE, typename BPtr, typename KPtr>")
+    em.out("  template <typename R, bool PURE, typename BPtr, typename KPtr, typename QBPtr = const float*>")
     em.out("  static __device__ __forceinline__ void run(BPtr bra, int nbra,")
     em.out("      KPtr ket, int nket, const R ABx, const R ABy, const R ABz,")
-    em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, R* __restrict__ out) {")
+    em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, R* __restrict__ out,")
+    em.out("      QBPtr bq = nullptr, const float* __restrict__ kq = nullptr, float cut = 0.f) {")
     for e in E:
         for f in F:
             em.out("    R a%s_%s = R(0);" % (cname(e), cname(f)))
     # ket primitive pairs (different per lane) outside, bra primitive pairs
-    # (the same for the whole wavefront: scalar loads) inside
+    # (the same for the whole wavefront: scalar loads) inside.  Primitive
+    # screening (cut > 0): bq / kq bound the primitive pairs, both sorted
+    # descending, so a primitive quartet can be dropped when bq kq < cut, the
+    # lane's bra loop shrinks as kq falls and the lane stops at the first ket
+    # primitive for which no bra primitive is left.
+    em.out("    int nb = nbra;")
     em.out("    for (int ik = 0; ik < nket; ++ik) {")
+    em.out("      if (cut > 0.f) {")
+    em.out("        const float qk = kq[ik];")
+    em.out("        while (nb > 0 && bq[nb - 1] * qk < cut) --nb;")
+    em.out("        if (nb == 0) break;")
+    em.out("      }")
     em.out("      const auto kp = ket[ik];")
     em.out("      const R q = R(kp.p), Kcd = R(kp.K);")
     em.out("      const R Qx = R(kp.P[0]), Qy = R(kp.P[1]), Qz = R(kp.P[2]);")
     if lc + ld > 0:
         em.out("      const R inv2q = R(kp.inv2p);")
         em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
-    em.out("      for (int ib = 0; ib < nbra; ++ib) {")
+    em.out("      for (int ib = 0; ib < nb; ++ib) {")
     em.out("        const auto& bp = bra[ib];")
     em.out("        const R p = R(bp.p), Kab = R(bp.K);")
     if la + lb > 0:
@@ -420,6 +431,7 @@ def gen_class_fam(la, lb):
     E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
     accs = ["a%s_%s" % (cname(e), cname((0, 0, 0))) for e in E]
     i_loop = next(i for i, t in enumerate(lines) if t.strip().startswith("for (int ik = 0; ik < nket; ++ik)"))
+    i_kp = next(i for i in range(i_loop, len(lines)) if lines[i].strip().startswith("const auto kp = ket[ik];"))
     i_bra = next(i for i in range(i_loop, len(lines)) if lines[i].strip().startswith("for (int ib = 0"))
     i_acc = next(i for i in range(i_bra, len(lines)) if lines[i].strip().startswith(accs[0] + " +="))
     i_end = i_acc + len(accs)
@@ -428,19 +440,22 @@ def gen_class_fam(la, lb):
     # drop the closing of run() and of the struct from the post block
     assert post[-1] == "};" and post[-2].strip() == "}"
     post = post[:-2]
+    screen = lines[i_loop + 1:i_kp]
     ket_head = [t.replace("const R q = R(kp.p), Kcd = R(kp.K);", "const R q = R(kp.p);")
-                for t in lines[i_loop + 1:i_bra]]
+                for t in lines[i_kp:i_bra]]
     bra_body = [t.replace("Kab * Kcd * rs_pq", "Kab * rs_pq") for t in lines[i_bra:i_acc]]
     em = []
     em.append("  static constexpr int NACC_FAM = %d;   // accumulators of run_fam" % (len(accs) * (FAM_MAXMEM + 1)))
-    em.append("  template <typename R, bool PURE, typename BPtr, typename KPtr, typename CPtr>")
+    em.append("  template <typename R, bool PURE, typename BPtr, typename KPtr, typename CPtr, typename QBPtr>")
     em.append("  static __device__ __forceinline__ void run_fam(BPtr bra, int nbra,")
     em.append("      KPtr ket, CPtr km, int nket, const R ABx, const R ABy, const R ABz,")
-    em.append("      const BoysTables& tab, R* __restrict__ out) {")
+    em.append("      const BoysTables& tab, R* __restrict__ out, QBPtr bq, const float* __restrict__ kq, float cut) {")
     for m in range(FAM_MAXMEM):
         for a in accs:
             em.append("    R %s_m%d = R(0);" % (a, m))
+    em.append("    int nb = nbra;")
     em.append("    for (int ik = 0; ik < nket; ++ik) {")
+    em.extend(screen)
     em.extend(ket_head)
     for a in accs:
         em.append("      R t%s = R(0);" % a[1:])

@@ -380,8 +380,10 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
         const double bound = dn * a.pair_Q[tk.bra] * a.pair_Q[ket];
         if (!(bound >= lo && bound < hi)) continue;
         if (a.counters && lane == 0) {
-            atomicAdd(&a.counters[2 * a.combo], 1ull);
-            atomicAdd(&a.counters[2 * a.combo + 1], (unsigned long long)a.pair_np[tk.bra] * a.pair_np[ket]);
+            const unsigned long long npq = (unsigned long long)a.pair_np[tk.bra] * a.pair_np[ket];
+            atomicAdd(&a.counters[3 * a.combo], 1ull);
+            atomicAdd(&a.counters[3 * a.combo + 1], npq);
+            atomicAdd(&a.counters[3 * a.combo + 2], npq);   // no primitive screening here
         }
         const double CDx = O[3 * sc] - O[3 * sd], CDy = O[3 * sc + 1] - O[3 * sd + 1], CDz = O[3 * sc + 2] - O[3 * sd + 2];
         const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];

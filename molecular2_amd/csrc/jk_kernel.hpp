@@ -131,6 +131,21 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
         }
 }
 
+// Primitive quartets one lane computes under primitive screening (the loop
+// of the generated run functions, counted for profiling only).
+template <typename QB>
+__device__ __forceinline__ int screened_prims(QB bq, int nbra, const float* kq, int nket, float cut) {
+    if (!(cut > 0.f)) return nbra * nket;
+    int nb = nbra, tot = 0;
+    for (int ik = 0; ik < nket; ++ik) {
+        const float qk = kq[ik];
+        while (nb > 0 && bq[nb - 1] * qk < cut) --nb;
+        if (nb == 0) break;
+        tot += nb;
+    }
+    return tot;
+}
+
 template <typename R>
 struct PrimPairOf;
 template <>
@@ -199,6 +214,8 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
     typedef const __attribute__((address_space(4))) PP cPP;
     const int nbp = a.pair_np[bra];
     cPP* bprims = (cPP*)(gp + a.pair_off[bra]);
+    typedef const __attribute__((address_space(4))) float cfl;
+    cfl* bq = (cfl*)(a.prim_q + a.pair_off[bra]);
     extern __shared__ double sK[];
     if (lds)
         for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
@@ -224,7 +241,7 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
     const double dnab = Dn[sa * nsh + sb];
     const double qab = a.pair_Q[bra];
     const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
-    unsigned long long cnt_q = 0, cnt_pq = 0;   // profiling counters of this wave
+    unsigned long long cnt_q = 0, cnt_pq = 0, cnt_cpq = 0;   // profiling counters of this wave
     for (int ch = c0 + wave; ch < c1; ch += W) {
         // chunk -> ket group segment
         int ket0 = 0, nket = 0, rem = ch;
@@ -248,15 +265,18 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
         const double bound = dn * qab * a.pair_Q[ket];
         const bool active = lane < nket && bound >= lo && bound < hi;
         if (!__any(active)) continue;
+        const float* kq = a.prim_q + a.pair_off[ket];
+        const float cut = float(a.prim_tau / dn);
+        const int nkp = active ? a.pair_np[ket] : 0;
         if (a.counters) {
             cnt_q += __popcll(__ballot(active));
             cnt_pq += (unsigned long long)(wave_sum(active ? double(a.pair_np[ket]) : 0.0) * nbp + 0.5);
+            cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
         const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
         R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
-        const int nkp = active ? a.pair_np[ket] : 0;
         EriClass<LA, LB, LC, LD>::template run<R, PURE>(bprims, nbp, gp + a.pair_off[ket], nkp,
-                                                        ABx, ABy, ABz, CDx, CDy, CDz, tab, v);
+                                                        ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq, cut);
         if (active) {
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
             digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK);
@@ -266,8 +286,9 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
         }
     }
     if (a.counters && lane == 0 && cnt_q) {
-        atomicAdd(&a.counters[2 * a.combo], cnt_q);
-        atomicAdd(&a.counters[2 * a.combo + 1], cnt_pq);
+        atomicAdd(&a.counters[3 * a.combo], cnt_q);
+        atomicAdd(&a.counters[3 * a.combo + 1], cnt_pq);
+        atomicAdd(&a.counters[3 * a.combo + 2], cnt_cpq);
     }
     // J(ab): one wave sum per function
 #pragma unroll
@@ -346,6 +367,8 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
     typedef const __attribute__((address_space(4))) PP cPP;
     const int nbp = a.pair_np[bra];
     cPP* bprims = (cPP*)(PrimPairOf<R>::get(a) + a.pair_off[bra]);
+    typedef const __attribute__((address_space(4))) float cfl;
+    cfl* bq = (cfl*)(a.prim_q + a.pair_off[bra]);
     const auto* fprims = FamPrimOf<R>::prims(a);
     const auto* fkm = FamPrimOf<R>::km(a);
     // unique pairs of pairs inside class 0: the bra's own family and slot
@@ -376,7 +399,7 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
     const double dnab = Dn[sa * nsh + sb];
     const double qab = a.pair_Q[bra];
     const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
-    unsigned long long cnt_q = 0, cnt_pq = 0;
+    unsigned long long cnt_q = 0, cnt_pq = 0, cnt_cpq = 0;
     for (int ch = c0 + wave; ch < c1; ch += W) {
         int ket0 = 0, nket = 0, rem = ch;
         for (int g = 0; g < a.ngrp; ++g) {
@@ -396,6 +419,7 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
         bool act[FAM_MAXMEM];
         bool active = false;
         int nact = 0;
+        double dnmax = 0.0;
 #pragma unroll
         for (int m = 0; m < FAM_MAXMEM; ++m) {
             act[m] = false;
@@ -407,20 +431,24 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
                                                                            fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
                 const double bound = dn * qab * a.pair_Q[j];
                 act[m] = bound >= lo && bound < hi;
+                if (act[m]) dnmax = fmax(dnmax, dn);
             }
             active = active || act[m];
             nact += act[m] ? 1 : 0;
         }
         if (!__any(active)) continue;
+        const int nkp = active ? a.fam_np[ket] : 0;
+        const int off = a.fam_off[ket];
+        const float* kq = a.fam_q + off;
+        const float cut = float(a.prim_tau / dnmax);
         if (a.counters) {   // algorithmic counts: every member quartet with its own primitive quartets
             cnt_q += (unsigned long long)(wave_sum(double(nact)) + 0.5);
             cnt_pq += (unsigned long long)(wave_sum(double(nact) * a.fam_np[ket]) * nbp + 0.5);
+            cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
         R v[FAM_MAXMEM * NCART];
-        const int nkp = active ? a.fam_np[ket] : 0;
-        const int off = a.fam_off[ket];
         EriClass<LA, LB, 0, 0>::template run_fam<R, PURE>(bprims, nbp, fprims + off, fkm + FAM_MAXMEM * off, nkp,
-                                                          ABx, ABy, ABz, tab, v);
+                                                          ABx, ABy, ABz, tab, v, bq, kq, cut);
 #pragma unroll
         for (int m = 0; m < FAM_MAXMEM; ++m)
             if (act[m]) {
@@ -435,8 +463,9 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
             }
     }
     if (a.counters && lane == 0 && cnt_q) {
-        atomicAdd(&a.counters[2 * a.combo], cnt_q);
-        atomicAdd(&a.counters[2 * a.combo + 1], cnt_pq);
+        atomicAdd(&a.counters[3 * a.combo], cnt_q);
+        atomicAdd(&a.counters[3 * a.combo + 1], cnt_pq);
+        atomicAdd(&a.counters[3 * a.combo + 2], cnt_cpq);
     }
 #pragma unroll
     for (int x = 0; x < NA * NB; ++x) {

@@ -64,7 +64,7 @@ struct JKArgs {
     double precision;        // quartet screening threshold
     double fp32_below;       // quartets with bound < this run in FP32 (mixed mode)
     double scale;            // multiplier
-    unsigned long long* counters;  // profiling only: per class combo {quartets, primitive quartets}
+    unsigned long long* counters;  // profiling only: per class combo {quartets, primitive quartets, computed ones}
     int combo;               // 6 * X + Y of the launch
     int k_lds;               // set by the launcher: K rows of the bra shells accumulated in LDS
     // device work list of the launch (tasks.hip): workgroup -> bra, and the
@@ -93,6 +93,13 @@ struct JKArgs {
     const int4* fam_mem;     // member shell-pair indices (-1 pads)
     const int* pair_fam;     // class-0 shell pair -> its family pair
     const int* pair_mem;     // class-0 shell pair -> its member slot
+    // primitive screening: bounds sqrt(max |(ab|ab)|) of every primitive pair
+    // (sorted descending within each shell pair) and of every family
+    // primitive pair (largest member factor); a primitive quartet is dropped
+    // when bound_ab bound_cd max|D_blocks| < prim_tau (0: none dropped)
+    const float* prim_q;
+    const float* fam_q;
+    double prim_tau;
 };
 
 constexpr int FAM_MAXMEM = 4;

@@ -92,6 +92,8 @@ def lib():
         _ll = C.POINTER(C.c_longlong)
         L.m2_get_profile.argtypes = [vp, _dp, _ll, _ll, _ll]
         L.m2_reset_profile.argtypes = [vp]
+        L.m2_get_profile_computed.argtypes = [vp, _ll]
+        L.m2_set_prim_screening.argtypes = [vp, C.c_double]
         L.m2_df_setup.argtypes = [vp]
         L.m2_jk_df.argtypes = [vp, _dp, C.c_int, C.c_int, _dp, _dp]
         L.m2_jk_df_device.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp]
@@ -144,7 +146,7 @@ def lib():
             getattr(L, name).restype = C.c_int
         for name in ("m2_set_profiling", "m2_get_profile", "m2_reset_profile", "m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
                      "m2_shellblock_norm", "m2_one_body", "m2_jk_direct", "m2_jk_direct_device",
-                     "m2_jk_direct_device_noalloc",
+                     "m2_jk_direct_device_noalloc", "m2_get_profile_computed", "m2_set_prim_screening",
                      "m2_set_mixed_threshold", "m2_last_stats"):
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -334,13 +336,21 @@ class Engine:
         self._chk(self.L.m2_reset_profile(self.h))
 
     def profile(self):
-        """dict combo 'X,Y' -> (ms, launches, quartets, prim_quartets)"""
+        """dict combo (X, Y) -> (ms, launches, quartets, prim_quartets,
+        computed_prim_quartets); the counts need set_profiling(counters=True)"""
         ms = np.zeros(36)
-        la, q, pq = (np.zeros(36, np.int64) for _ in range(3))
+        la, q, pq, cpq = (np.zeros(36, np.int64) for _ in range(4))
         P = C.POINTER(C.c_longlong)
         self._chk(self.L.m2_get_profile(self.h, _p(ms), la.ctypes.data_as(P), q.ctypes.data_as(P),
                                         pq.ctypes.data_as(P)))
-        return {(c // 6, c % 6): (ms[c], int(la[c]), int(q[c]), int(pq[c])) for c in range(36) if la[c] > 0}
+        self._chk(self.L.m2_get_profile_computed(self.h, cpq.ctypes.data_as(P)))
+        return {(c // 6, c % 6): (ms[c], int(la[c]), int(q[c]), int(pq[c]), int(cpq[c])) for c in range(36)
+                if la[c] > 0}
+
+    def set_prim_screening(self, relative):
+        """primitive quartets below relative * precision (bound q_ab q_cd
+        max|D|) are skipped; 0 computes all of them (default 1e-6)"""
+        self._chk(self.L.m2_set_prim_screening(self.h, float(relative)))
 
     def df_setup(self):
         self._chk(self.L.m2_df_setup(self.h))
