@@ -251,7 +251,8 @@ def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
     em.out("  static constexpr int L = %d;" % L)
     em.out("  static constexpr int NCART = %d;" % (na * nb * nc * nd))
     em.out("  static constexpr int NACC = %d;   // VRR accumulators per lane" % (len(E) * len(F)))
-    em.out("  template <typename R, bool PURE, typename BPtr, typename KPtr, typename QBPtr = const float*>")
+    em.out("  template <typename R, bool PURE, bool SCR = false, typename BPtr, typename KPtr,")
+    em.out("            typename QBPtr = const float*>")
     em.out("  static __device__ __forceinline__ void run(BPtr bra, int nbra,")
     em.out("      KPtr ket, int nket, const R ABx, const R ABy, const R ABz,")
     em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, R* __restrict__ out,")
@@ -261,24 +262,24 @@ def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
             em.out("    R a%s_%s = R(0);" % (cname(e), cname(f)))
     # ket primitive pairs (different per lane) outside, bra primitive pairs
     # (the same for the whole wavefront: scalar loads) inside.  Primitive
-    # screening (cut > 0): bq / kq bound the primitive pairs, both sorted
-    # descending, so a primitive quartet can be dropped when bq kq < cut, the
-    # lane's bra loop shrinks as kq falls and the lane stops at the first ket
-    # primitive for which no bra primitive is left.
-    em.out("    int nb = nbra;")
+    # screening (SCR): bq / kq bound the primitive pairs, both sorted
+    # descending, and a primitive quartet may be dropped when bq kq < cut
+    # (the lane's own cut).  A lane stops at the first ket primitive whose
+    # product with the largest bra bound is below its cut; the bra loop of
+    # the wavefront stops at the first bra primitive below every lane's cut.
+    # Quartets between those limits are computed, which only adds accuracy.
+    em.out("    const float bq0 = SCR ? float(bq[0]) : 0.f;")
     em.out("    for (int ik = 0; ik < nket; ++ik) {")
-    em.out("      if (cut > 0.f) {")
-    em.out("        const float qk = kq[ik];")
-    em.out("        while (nb > 0 && bq[nb - 1] * qk < cut) --nb;")
-    em.out("        if (nb == 0) break;")
-    em.out("      }")
+    em.out("      const float qk = SCR ? kq[ik] : 0.f;")
+    em.out("      if (SCR && qk * bq0 < cut) break;")
     em.out("      const auto kp = ket[ik];")
     em.out("      const R q = R(kp.p), Kcd = R(kp.K);")
     em.out("      const R Qx = R(kp.P[0]), Qy = R(kp.P[1]), Qz = R(kp.P[2]);")
     if lc + ld > 0:
         em.out("      const R inv2q = R(kp.inv2p);")
         em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
-    em.out("      for (int ib = 0; ib < nb; ++ib) {")
+    em.out("      for (int ib = 0; ib < nbra; ++ib) {")
+    em.out("        if (SCR && !__any(float(bq[ib]) * qk >= cut)) break;")
     em.out("        const auto& bp = bra[ib];")
     em.out("        const R p = R(bp.p), Kab = R(bp.K);")
     if la + lb > 0:
@@ -446,14 +447,14 @@ def gen_class_fam(la, lb):
     bra_body = [t.replace("Kab * Kcd * rs_pq", "Kab * rs_pq") for t in lines[i_bra:i_acc]]
     em = []
     em.append("  static constexpr int NACC_FAM = %d;   // accumulators of run_fam" % (len(accs) * (FAM_MAXMEM + 1)))
-    em.append("  template <typename R, bool PURE, typename BPtr, typename KPtr, typename CPtr, typename QBPtr>")
+    em.append("  template <typename R, bool PURE, bool SCR, typename BPtr, typename KPtr, typename CPtr, typename QBPtr>")
     em.append("  static __device__ __forceinline__ void run_fam(BPtr bra, int nbra,")
     em.append("      KPtr ket, CPtr km, int nket, const R ABx, const R ABy, const R ABz,")
     em.append("      const BoysTables& tab, R* __restrict__ out, QBPtr bq, const float* __restrict__ kq, float cut) {")
     for m in range(FAM_MAXMEM):
         for a in accs:
             em.append("    R %s_m%d = R(0);" % (a, m))
-    em.append("    int nb = nbra;")
+    em.append("    const float bq0 = SCR ? float(bq[0]) : 0.f;")
     em.append("    for (int ik = 0; ik < nket; ++ik) {")
     em.extend(screen)
     em.extend(ket_head)
@@ -518,8 +519,9 @@ def main(outdir):
     written.append(write_programs(outdir))
     written.append(write_deriv_programs(outdir))
     written.append(write_deriv_register(outdir))
-    written.append(write_flop_model(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                                                 "flop_model.json")))
+    # The op-count model (molecular2_amd/flop_model.json) is committed and
+    # fixed; the build does not rewrite it.  `gen_eri.py --flop-model PATH`
+    # regenerates it from the textbook recurrence DAG.
     return written
 
 
@@ -1681,4 +1683,8 @@ def write_deriv_register(outdir):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(os.path.abspath(__file__)), "generated"))
+    if len(sys.argv) > 2 and sys.argv[1] == "--flop-model":
+        write_flop_model(sys.argv[2])
+    else:
+        main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                                "generated"))

@@ -131,11 +131,11 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
         }
 }
 
-// Primitive quartets one lane computes under primitive screening (the loop
-// of the generated run functions, counted for profiling only).
+// Primitive quartets whose bound passes the lane's cut (profiling only; the
+// generated loops compute these and, where lanes of a wavefront differ, a
+// few more).
 template <typename QB>
 __device__ __forceinline__ int screened_prims(QB bq, int nbra, const float* kq, int nket, float cut) {
-    if (!(cut > 0.f)) return nbra * nket;
     int nb = nbra, tot = 0;
     for (int ik = 0; ik < nket; ++ik) {
         const float qk = kq[ik];
@@ -275,8 +275,8 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
         }
         const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
         R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
-        EriClass<LA, LB, LC, LD>::template run<R, PURE>(bprims, nbp, gp + a.pair_off[ket], nkp,
-                                                        ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq, cut);
+        EriClass<LA, LB, LC, LD>::template run<R, PURE, true>(bprims, nbp, gp + a.pair_off[ket], nkp,
+                                                              ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq, cut);
         if (active) {
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
             digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK);
@@ -447,8 +447,8 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
             cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
         R v[FAM_MAXMEM * NCART];
-        EriClass<LA, LB, 0, 0>::template run_fam<R, PURE>(bprims, nbp, fprims + off, fkm + FAM_MAXMEM * off, nkp,
-                                                          ABx, ABy, ABz, tab, v, bq, kq, cut);
+        EriClass<LA, LB, 0, 0>::template run_fam<R, PURE, true>(bprims, nbp, fprims + off, fkm + FAM_MAXMEM * off,
+                                                                nkp, ABx, ABy, ABz, tab, v, bq, kq, cut);
 #pragma unroll
         for (int m = 0; m < FAM_MAXMEM; ++m)
             if (act[m]) {
