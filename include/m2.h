@@ -107,6 +107,15 @@ m2_status m2_jk_direct_device(m2_ctx* ctx, const double* dD, double multiplier, 
 #define M2_NOALLOC_SCRATCH_BYTES (64u << 20)
 m2_status m2_jk_direct_device_noalloc(m2_ctx* ctx, const double* dD, double multiplier, double precision,
                                       unsigned flags, double* dJ, double* dK);
+/* The two-electron integral tensor in the unique packed layout the
+ * reference's in-core path stores (IntegralEngine::compute_eris,
+ * src/eris.cpp:18-87, read by Fock::formJK, src/fockbuild.cpp:36-51, and
+ * printed by IntegralEngine::printERI, src/integrals.cpp:202-233):
+ * out[p(ij) * np + p(kl)] = (ij|kl) with p(ij) = i(i+1)/2 + j for i >= j
+ * and np = n(n+1)/2, so out holds np^2 doubles.  Every quartet is computed
+ * in FP64 without screening, one column (..|kl) per direct J build of a unit
+ * density on the device.  For small bases (n <= 256). */
+m2_status m2_eri_packed(m2_ctx* ctx, double* out);
 /* UnrestrictedFock::formJKdirect (src/fockbuild.cpp:267-364): J and K of the
  * alpha and beta densities (Da = Ca_occ Ca_occ^T, Db likewise), every quartet
  * screened with the block norms of Da + Db; the unrestricted Fock matrices

@@ -68,6 +68,28 @@ typedef struct {
 int m2_deck_run(const char* deck_text, const char* basislib, int device, int pure, m2_deck_result* res, int maxres,
                 int* nres);
 
+/* IntegralEngine::printERI (src/integrals.cpp:202-233): writes the
+ * two-electron integrals of the context's orbital basis to `path` in the
+ * reference's text format (header, one line "c1 c2 c3 c4 w*(c1c2|c3c4)" per
+ * c1 >= c2, c1 >= c3 >= c4, 1-based, w the permutational weight, zero below
+ * thrint; then the integral and insignificant counts).  The integrals come
+ * from m2_eri_packed (so n <= 256).  nints / ninsig may be NULL.  A deck
+ * with "printeris, true" writes this file (name "intfile", default
+ * eris.ints; threshold "thrint", default 1e-12) for each molecule an hf /
+ * rhf / uhf command runs.  Returns 0, or non-zero with m2_rhf_last_error(). */
+int m2_print_eri(m2_ctx* ctx, const char* path, double thrint, long long* nints, long long* ninsig);
+
+/* Shells of the atoms Z (positions xyz, bohr) from a Molpro basis text file
+ * such as the reference's basissets/CC-PVDZfirst.basis: per atom in order,
+ * the element's "c, i.j, ..." contraction lines in file order, one segmented
+ * shell each, over the exponents of the preceding "<l>, <element>, ..."
+ * line (the shells the reference's basis library holds for that file).
+ * Call with out = NULL (or too small *nsh / *nprim) for the sizes; then with
+ * arrays of those sizes.  Deck basis names ending in ".basis" are read the
+ * same way.  Returns 0, or non-zero with m2_rhf_last_error() (BASISLIB). */
+int m2_basis_file_shells(const char* path, int natom, const int* Z, const double* xyz, int* nsh, int* nprim,
+                         m2_shells_out* out);
+
 #ifdef __cplusplus
 }
 #endif

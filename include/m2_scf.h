@@ -29,6 +29,10 @@ typedef struct {
     int fp64_final;
     int guess;           /* 0: core Hamiltonian, 1: SOAD (needs the minimal basis) */
     int diis_start;      /* DIIS from iteration diis_start + 1 on (reference source: 2) */
+    int incore;          /* 1: the reference's in-core path ("direct, false"): the
+                            unique ERIs are stored once (m2_eri_packed) and J/K
+                            formed from them (Fock::formJK, src/fockbuild.cpp:36-51)
+                            for bases of at most 256 functions, else direct */
 } m2_scf_options_c;
 
 /* energy = one_e + two_e + enuc (SCF::calcE, src/scf.cpp:29-48); counts of

@@ -1347,6 +1347,40 @@ m2_status m2_jk_direct_device_noalloc(m2_ctx* ctx, const double* dD, double mult
     return jk_device_core(ctx, dD, multiplier, precision, flags, dJ, dK, nullptr, nullptr, nullptr, nullptr, &w);
 }
 
+m2_status m2_eri_packed(m2_ctx* ctx, double* out) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(out, "m2_eri_packed: null output");
+    const int n = ctx->obs.nbf;
+    ARGCHK(n > 0, "m2_eri_packed: no orbital basis set");
+    ARGCHK(n <= 256, "m2_eri_packed: the packed tensor is for bases of at most 256 functions");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)n * n, np = (size_t)n * (n + 1) / 2;
+    HIPCHK(ctx->d_D.reserve(n2));
+    HIPCHK(ctx->d_Jo.reserve(n2));
+    HIPCHK(ctx->d_Ko.reserve(n2));
+    std::vector<double> D(n2, 0.0), J(n2);
+    // column (.. | k l) of the tensor = J of the symmetric unit density with
+    // D_kl = D_lk = 1/2 (1 on the diagonal), every quartet computed (no
+    // screening, FP64): J_ab = sum_cd (ab|cd) D_cd = (ab|kl)
+    for (int k = 0; k < n; ++k)
+        for (int l = 0; l <= k; ++l) {
+            const double v = k == l ? 1.0 : 0.5;
+            D[(size_t)k * n + l] = v;
+            D[(size_t)l * n + k] = v;
+            HIPCHK(hipMemcpyAsync(ctx->d_D.p, D.data(), sizeof(double) * n2, hipMemcpyHostToDevice, ctx->stream));
+            m2_status st = jk_device_core(ctx, ctx->d_D.p, 1.0, 0.0, M2_FP64, ctx->d_Jo.p, ctx->d_Ko.p);
+            if (st) return st;
+            HIPCHK(hipMemcpyAsync(J.data(), ctx->d_Jo.p, sizeof(double) * n2, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            D[(size_t)k * n + l] = 0.0;
+            D[(size_t)l * n + k] = 0.0;
+            const size_t col = (size_t)k * (k + 1) / 2 + l;
+            for (int a = 0; a < n; ++a)
+                for (int b = 0; b <= a; ++b) out[((size_t)a * (a + 1) / 2 + b) * np + col] = J[(size_t)a * n + b];
+        }
+    return M2_OK;
+}
+
 m2_status m2_jk_direct(m2_ctx* ctx, const double* D, double multiplier, double precision, unsigned flags, double* J,
                        double* K) {
     if (!ctx) return M2_ERR_ARG;

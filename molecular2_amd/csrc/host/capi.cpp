@@ -6,6 +6,9 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <fstream>
+#include <iomanip>
+#include <iterator>
 #include <mutex>
 #include <numeric>
 #include <stdexcept>
@@ -35,9 +38,118 @@ static void load_shells(int ns, const int* ll, const int* np, const double* ex, 
     }
 }
 
+// IntegralEngine::printERI (src/integrals.cpp:202-233): c1 >= c2, c1 >= c3 >=
+// c4 (1-based), each line the weight (1/8 times the number of distinct
+// index permutations of the quartet) times (c1 c2|c3 c4), zero (and counted
+// as insignificant) below thrint; iostream default formatting as in the
+// reference.
+static void write_eri_file(std::ostream& out, int n, const std::vector<double>& packed, double thrint,
+                           long long* nints, long long* ninsig) {
+    const size_t np = (size_t)n * (n + 1) / 2;
+    auto pidx = [](int i, int j) { return i >= j ? (size_t)i * (i + 1) / 2 + j : (size_t)j * (j + 1) / 2 + i; };
+    out << " TWO ELECTRON INTEGRALS: \n";
+    long long icount = 0, scount = 0;
+    for (int c1 = 0; c1 < n; ++c1)
+        for (int c2 = 0; c2 <= c1; ++c2)
+            for (int c3 = 0; c3 <= c1; ++c3)
+                for (int c4 = 0; c4 <= c3; ++c4) {
+                    ++icount;
+                    double w = 0.125;
+                    if (c1 != c2) w *= 2.0;
+                    if (c3 != c4) w *= 2.0;
+                    if ((c1 + c2) != (c3 + c4)) w *= 2.0;
+                    else if (c1 != c3 && c1 != c4) w *= 2.0;
+                    else if (c2 != c3 && c2 != c4) w *= 2.0;
+                    const double v = packed[pidx(c1, c2) * np + pidx(c3, c4)];
+                    if (std::fabs(v) < thrint) {
+                        ++scount;
+                        w = 0.0;
+                    }
+                    out << std::setw(6) << c1 + 1 << std::setw(6) << c2 + 1 << std::setw(6) << c3 + 1
+                        << std::setw(6) << c4 + 1 << std::setw(20) << w * v << "\n";
+                }
+    out << "N 2e Ints: " << icount << "\n";
+    out << "N insig. Ints: " << scount << "\n";
+    if (nints) *nints = icount;
+    if (ninsig) *ninsig = scount;
+}
+
+static void print_eri_ctx(m2_ctx* ctx, const std::string& path, double thrint, long long* nints,
+                          long long* ninsig) {
+    int n = 0, nsh = 0;
+    if (m2_basis_info(ctx, M2_ORBITAL, &n, &nsh) != 0 || n <= 0)
+        throw std::runtime_error("m2_print_eri: no orbital basis set");
+    const size_t np = (size_t)n * (n + 1) / 2;
+    std::vector<double> packed(np * np);
+    if (m2_eri_packed(ctx, packed.data()) != 0) throw std::runtime_error(std::string("m2_print_eri: ") + m2_last_error(ctx));
+    std::ofstream out(path);
+    if (!out) throw std::runtime_error("m2_print_eri: cannot open " + path);
+    write_eri_file(out, n, packed, thrint, nints, ninsig);
+    if (!out) throw std::runtime_error("m2_print_eri: write to " + path + " failed");
+}
+
 extern "C" {
 
 const char* m2_rhf_last_error(void) { return g_rhf_error.c_str(); }
+
+int m2_print_eri(m2_ctx* ctx, const char* path, double thrint, long long* nints, long long* ninsig) {
+    g_rhf_error.clear();
+    try {
+        if (!ctx || !path) throw std::runtime_error("m2_print_eri: null argument");
+        print_eri_ctx(ctx, path, thrint, nints, ninsig);
+        return 0;
+    } catch (const std::exception& e) {
+        g_rhf_error = e.what();
+        return 1;
+    }
+}
+
+int m2_basis_file_shells(const char* path, int natom, const int* Z, const double* xyz, int* nsh, int* nprim,
+                         m2_shells_out* out) {
+    using namespace molecular;
+    g_rhf_error.clear();
+    try {
+        if (!path || !Z || !xyz || !nsh || !nprim) throw std::runtime_error("m2_basis_file_shells: null argument");
+        std::ifstream f(path);
+        if (!f) throw std::runtime_error(std::string("BASISLIB: Could not open basis file ") + path);
+        const std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        std::vector<RawShell> all;
+        for (int a = 0; a < natom; ++a) {
+            std::vector<RawShell> sh = molpro_element_shells(text, element_symbol(Z[a]));
+            if (sh.empty())
+                throw std::runtime_error(std::string("BASISLIB: Could not read basis ") + path + " for atom " +
+                                         element_symbol(Z[a]));
+            for (RawShell& s : sh) {
+                s.atom = a;
+                for (int k = 0; k < 3; ++k) s.O[k] = xyz[3 * a + k];
+                all.push_back(s);
+            }
+        }
+        int np = 0;
+        for (const RawShell& s : all) np += (int)s.exps.size();
+        const bool fill = out && *nsh >= (int)all.size() && *nprim >= np;
+        *nsh = (int)all.size();
+        *nprim = np;
+        if (!fill) return 0;
+        int off = 0;
+        for (size_t i = 0; i < all.size(); ++i) {
+            const RawShell& s = all[i];
+            if (out->l) out->l[i] = s.l;
+            if (out->nprim) out->nprim[i] = (int)s.exps.size();
+            if (out->atoms) out->atoms[i] = s.atom;
+            if (out->centers)
+                for (int k = 0; k < 3; ++k) out->centers[3 * i + k] = s.O[k];
+            for (size_t k = 0; k < s.exps.size(); ++k, ++off) {
+                if (out->exps) out->exps[off] = s.exps[k];
+                if (out->coefs) out->coefs[off] = s.coefs[k];
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_rhf_error = e.what();
+        return 1;
+    }
+}
 
 int m2_soad_fock(int device, int natom, const double* Z, const double* xyz, int pure, int nsh, const int* l,
                  const int* nprim, const double* exps, const double* coefs, const double* centers, const int* atoms,
@@ -74,6 +186,7 @@ static molecular::Options options_from(const m2_scf_options_c* o) {
     opt.fp64_final = o->fp64_final;
     opt.guess = o->guess == 1 ? "soad" : "hcore";
     opt.diis_start = o->diis_start;
+    opt.direct = o->incore == 0;
     return opt;
 }
 
@@ -512,6 +625,27 @@ int m2_deck_run(const char* deck_text, const char* basislib, int device, int pur
             r.method = -1;
             if ((cmd.name == "hf" || cmd.name == "rhf" || cmd.name == "uhf") && cmd.molecule >= 0) {
                 const DeckData d = load_deck_molecule(deck_text, cmd.molecule, basislib);
+                // printeris / intfile / thrint (src/ProgramController.cpp:114-118):
+                // the molecule's two-electron integrals in the printERI format
+                const std::string* pe = deck.global("printeris");
+                if (pe && (*pe == "true" || *pe == "1")) {
+                    const std::string* fname = deck.global("intfile");
+                    const std::string* thr = deck.global("thrint");
+                    std::vector<int> l, np, at;
+                    std::vector<double> ex, cf, cen;
+                    for (const RawShell& s : d.orb) {
+                        l.push_back(s.l);
+                        np.push_back((int)s.exps.size());
+                        at.push_back(s.atom);
+                        ex.insert(ex.end(), s.exps.begin(), s.exps.end());
+                        cf.insert(cf.end(), s.coefs.begin(), s.coefs.end());
+                        cen.insert(cen.end(), {s.O[0], s.O[1], s.O[2]});
+                    }
+                    if (m2_basis_set(ctx, M2_ORBITAL, (int)l.size(), l.data(), np.data(), ex.data(), cf.data(),
+                                     cen.data(), at.data(), pure) != 0)
+                        throw std::runtime_error(std::string("m2_deck_run: ") + m2_last_error(ctx));
+                    print_eri_ctx(ctx, fname ? *fname : "eris.ints", thr ? std::stod(*thr) : 1e-12, nullptr, nullptr);
+                }
                 int nel = -d.mol.charge;
                 for (const Atom& a : d.mol.atoms) nel += a.Z;
                 // ProgramController::call_hf (src/ProgramController.cpp:288-293)
@@ -538,6 +672,9 @@ int m2_deck_run(const char* deck_text, const char* basislib, int device, int pur
                 if (const std::string* v = cmd.find("precision")) o.precision = std::stod(*v);
                 if (const std::string* v = cmd.find("df")) o.df = std::stoi(*v) != 0;
                 if (const std::string* v = cmd.find("guess")) o.guess = *v == "soad" ? 1 : 0;
+                // "direct, false" (the reference's default, src/ProgramController.cpp:116)
+                // selects the in-core path; this driver runs direct unless the deck says so
+                if (const std::string* v = deck.global("direct")) o.incore = *v == "false" || *v == "0";
                 if (o.df && d.aux.empty()) throw std::runtime_error("m2_deck_run: df needs a jkbasis construct");
                 std::vector<double> Z, xyz;
                 for (const Atom& a : d.mol.atoms) {

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cctype>
+#include <dirent.h>
+
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -278,10 +280,100 @@ std::map<std::string, int> read_basis_list(const std::string& libdir, const std:
     return out;
 }
 
+bool is_molpro_name(const std::string& name) {
+    return name.size() > 6 && name.compare(name.size() - 6, 6, ".basis") == 0;
+}
+
+std::string resolve_basis_file(const std::string& libdir, const std::string& name) {
+    // as given, beside the XML library, or in the basissets/ directory that
+    // sits beside basislibrary/ in the reference's tree.  Deck text is read
+    // in lower case (as the reference parses it), so a directory entry whose
+    // name matches apart from case is taken as well.
+    auto lower = [](std::string x) {
+        std::transform(x.begin(), x.end(), x.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+        return x;
+    };
+    for (const std::string& p : {name, libdir + "/" + name, libdir + "/../basissets/" + name}) {
+        std::ifstream f(p);
+        if (f.good()) return p;
+    }
+    const size_t slash = name.find_last_of('/');
+    const std::string base = lower(slash == std::string::npos ? name : name.substr(slash + 1));
+    const std::string given = slash == std::string::npos ? std::string(".") : name.substr(0, slash);
+    for (const std::string& dir : {given, libdir, libdir + "/../basissets"}) {
+        DIR* d = opendir(dir.c_str());
+        if (!d) continue;
+        std::string hit;
+        while (dirent* e = readdir(d))
+            if (lower(e->d_name) == base) {
+                hit = dir + "/" + e->d_name;
+                break;
+            }
+        closedir(d);
+        if (!hit.empty()) return hit;
+    }
+    throw std::runtime_error("BASISLIB: Could not open basis file " + name);
+}
+
+// Molpro basis text: "<l>, <element>, e1, e2, ..." gives the exponents of an
+// angular momentum; each following "c, i.j, c_i, ..., c_j" line is one
+// segmented shell over exponents i..j (1-based), which is how the
+// reference's basis library splits general contractions into shells
+// (src/basisreader.cpp:62-86 reads the XML built from this text with one
+// <Shell> per contraction line).  Lines starting with '!' are comments.
+std::vector<RawShell> molpro_element_shells(const std::string& text, const std::string& symbol) {
+    auto lower = [](std::string x) {
+        std::transform(x.begin(), x.end(), x.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+        return x;
+    };
+    const std::string want = lower(symbol);
+    std::vector<RawShell> out;
+    std::vector<double> exps;
+    int l = -1;
+    bool mine = false;
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t b = line.find_first_not_of(" \t\r");
+        if (b == std::string::npos || line[b] == '!' || line[b] == '}' || lower(line.substr(b, 5)) == "basis")
+            continue;
+        std::vector<std::string> f;
+        std::stringstream ss(line.substr(b));
+        std::string item;
+        while (std::getline(ss, item, ',')) f.push_back(strip_blanks(item));
+        if (f.size() < 2) continue;
+        const std::string head = lower(f[0]);
+        if (head == "c") {
+            if (!mine) continue;
+            const size_t dot = f[1].find('.');
+            if (l < 0 || dot == std::string::npos) throw std::runtime_error("BASISLIB: malformed Molpro line: " + line);
+            const int lo = std::stoi(f[1].substr(0, dot)), hi = std::stoi(f[1].substr(dot + 1));
+            if (lo < 1 || hi > (int)exps.size() || hi < lo || (int)f.size() - 2 != hi - lo + 1)
+                throw std::runtime_error("BASISLIB: contraction does not fit its exponents: " + line);
+            RawShell s;
+            s.l = l;
+            for (int k = lo; k <= hi; ++k) {
+                s.exps.push_back(exps[k - 1]);
+                s.coefs.push_back(std::stod(f[2 + k - lo]));
+            }
+            out.push_back(s);
+        } else if (head.size() == 1 && std::string("spdfghi").find(head[0]) != std::string::npos) {
+            l = (int)std::string("spdfghi").find(head[0]);
+            mine = lower(f[1]) == want;
+            exps.clear();
+            if (mine)
+                for (size_t k = 2; k < f.size(); ++k)
+                    if (!f[k].empty()) exps.push_back(std::stod(f[k]));
+        }
+    }
+    return out;
+}
+
 std::vector<RawShell> library_shells(const std::string& libdir, const std::string& kind,
                                      const std::vector<Atom>& atoms, const std::map<int, std::string>& names) {
     const std::map<std::string, int> list = read_basis_list(libdir, kind);
     std::map<int, std::string> docs;   // file number -> text
+    std::map<std::string, std::string> molpro;   // Molpro file name -> text
     std::vector<RawShell> out;
     for (size_t ia = 0; ia < atoms.size(); ++ia) {
         const Atom& a = atoms[ia];
@@ -289,8 +381,22 @@ std::vector<RawShell> library_shells(const std::string& libdir, const std::strin
         auto it = names.find(a.Z);
         if (it != names.end()) name = it->second;
         else if ((it = names.find(0)) != names.end()) name = it->second;
-        std::transform(name.begin(), name.end(), name.begin(), [](unsigned char c) { return (char)std::tolower(c); });
         const std::string sym = element_symbol(a.Z);
+        if (is_molpro_name(name)) {
+            // a Molpro basis file named directly in the deck
+            if (!molpro.count(name)) molpro[name] = read_file(resolve_basis_file(libdir, name));
+            std::vector<RawShell> sh = molpro_element_shells(molpro[name], sym);
+            if (sh.empty()) throw std::runtime_error("BASISLIB: Could not read basis " + name + " for atom " + sym);
+            for (RawShell& s : sh) {
+                s.atom = (int)ia;
+                s.O[0] = a.x;
+                s.O[1] = a.y;
+                s.O[2] = a.z;
+                out.push_back(s);
+            }
+            continue;
+        }
+        std::transform(name.begin(), name.end(), name.begin(), [](unsigned char c) { return (char)std::tolower(c); });
         size_t before = out.size();
         auto li = list.find(name);
         if (li != list.end()) {

@@ -68,5 +68,13 @@ std::map<std::string, int> read_basis_list(const std::string& libdir, const std:
 // <libdir>/bases/build/<kind>N.xml; throws BASISLIB when an atom has no shells
 std::vector<RawShell> library_shells(const std::string& libdir, const std::string& kind,
                                      const std::vector<Atom>& atoms, const std::map<int, std::string>& names);
+// A basis name ending in ".basis" names a Molpro basis text file instead of
+// a library entry; it is found as given, in <libdir>, or in
+// <libdir>/../basissets (the reference's basissets/ directory).
+bool is_molpro_name(const std::string& name);
+std::string resolve_basis_file(const std::string& libdir, const std::string& name);
+// the shells of one element from Molpro basis text, in file order, one
+// segmented shell per contraction line (no atom or centre set)
+std::vector<RawShell> molpro_element_shells(const std::string& text, const std::string& symbol);
 
 }  // namespace molecular

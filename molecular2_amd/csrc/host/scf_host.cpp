@@ -275,7 +275,10 @@ Vector DIISEngine::compute(std::vector<Vector>& errors) {
 Fock::Fock(const Options& o, IntegralEngine& ints, const Molecule& m) : opt(o), integrals(ints), molecule(m) {
     formHCore();
     formOrthog();
-    direct = o.direct;
+    // the in-core path stores n(n+1)/2 squared integrals; larger bases run
+    // direct, as the reference does when the estimate exceeds its memory
+    // (src/integrals.cpp:86-93)
+    direct = o.direct || nbfs > 256;
     density_fitted = o.df;
     diis = o.diis;
     iter = 0;
@@ -305,7 +308,41 @@ void Fock::formOrthog() {
 
 void Fock::makeJK(Matrix& P, double multiplier) {
     if (density_fitted) formJKdf(P, multiplier);
+    else if (!direct) formJK(P, multiplier);
     else formJKdirect(integrals.getPrescreen(), P, multiplier);
+}
+
+// src/fockbuild.cpp:36-51 over the packed tensor: J(u,v) = m sum_sl P(s,l)
+// (uv|sl), K(u,v) = m sum_sl P(s,l) (ul|sv).  The tensor is computed once
+// (m2_eri_packed: every quartet in FP64 on the device, like the reference's
+// compute_eris, src/eris.cpp:18-87).
+void Fock::formJK(Matrix& P, double multiplier) {
+    m2_ctx* ctx = integrals.context();
+    const int n = nbfs;
+    const size_t np = (size_t)n * (n + 1) / 2;
+    auto t0 = std::chrono::steady_clock::now();
+    if (twoints.empty()) {
+        twoints.resize(np * np);
+        check(ctx, m2_eri_packed(ctx, twoints.data()), "m2_eri_packed");
+    }
+    auto pidx = [](int i, int j) { return i >= j ? (size_t)i * (i + 1) / 2 + j : (size_t)j * (j + 1) / 2 + i; };
+    jints = Matrix(n, n);
+    kints = Matrix(n, n);
+    for (int u = 0; u < n; ++u)
+        for (int v = 0; v < n; ++v) {
+            double j = 0.0, k = 0.0;
+            const double* row_uv = twoints.data() + pidx(u, v) * np;
+            for (int s = 0; s < n; ++s)
+                for (int l = 0; l < n; ++l) {
+                    const double p = P(s, l);
+                    j += p * row_uv[pidx(s, l)];
+                    k += p * twoints[pidx(u, l) * np + pidx(s, v)];
+                }
+            jints(u, v) = multiplier * j;
+            kints(u, v) = multiplier * k;
+        }
+    jk_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    jkints = jints - 0.5 * kints;
 }
 
 void Fock::formJKdirect(const Matrix& Schwarz, Matrix& D, double multiplier) {

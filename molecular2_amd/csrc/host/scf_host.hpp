@@ -178,6 +178,9 @@ public:
     void makeJK(Matrix& P, double multiplier = 1.0);
     void formJKdirect(const Matrix& Schwarz, Matrix& P, double multiplier = 1.0);
     void formJKdf(Matrix& Cocc, double multiplier = 1.0);
+    // Fock::formJK (src/fockbuild.cpp:36-51): J and K from the stored unique
+    // integrals (the in-core path, Options::direct false)
+    void formJK(Matrix& P, double multiplier = 1.0);
     void makeFock();
     void makeFock(Matrix& P, double multiplier = 1.0);
     void addDiis();
@@ -222,6 +225,7 @@ protected:
     std::vector<Matrix> focks;
     int nbfs = 0, nocc = 0, iter = 0, MAX = 8;
     bool diis, density_fitted, direct;
+    std::vector<double> twoints;   // packed unique (ij|kl) of the in-core path
     double precision;
     bool last_build_incremental = false;
 };

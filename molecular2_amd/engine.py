@@ -26,7 +26,7 @@ class SCFOptions(C.Structure):
     _fields_ = [("diis", C.c_int), ("maxdiis", C.c_int), ("maxiter", C.c_int),
                 ("enconverge", C.c_double), ("densconverge", C.c_double), ("precision", C.c_double),
                 ("df", C.c_int), ("mixed", C.c_int), ("mixed_threshold", C.c_double), ("fp64_final", C.c_int),
-                ("guess", C.c_int), ("diis_start", C.c_int)]
+                ("guess", C.c_int), ("diis_start", C.c_int), ("incore", C.c_int)]
 
 
 class SCFResult(C.Structure):
@@ -135,6 +135,12 @@ def lib():
         L.m2_deck_molecule.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.POINTER(DeckInfo), _dp, _dp,
                                        C.POINTER(ShellsOut), C.POINTER(ShellsOut), C.POINTER(ShellsOut)]
         L.m2_deck_molecule.restype = C.c_int
+        L.m2_basis_file_shells.argtypes = [C.c_char_p, C.c_int, _ip, _dp, _ip, _ip, C.POINTER(ShellsOut)]
+        L.m2_basis_file_shells.restype = C.c_int
+        L.m2_print_eri.argtypes = [vp, C.c_char_p, C.c_double, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.m2_print_eri.restype = C.c_int
+        L.m2_eri_packed.argtypes = [vp, _dp]
+        L.m2_eri_packed.restype = C.c_int
         L.m2_deck_run.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(DeckResult), C.c_int,
                                   C.POINTER(C.c_int)]
         L.m2_deck_run.restype = C.c_int
@@ -208,6 +214,23 @@ def _shells_from(b):
                          tuple(b["centers"][3 * s:3 * s + 3]), int(b["atoms"][s])))
         p += k
     return out
+
+
+def basis_file_shells(path: str, mol):
+    """Shells of `mol` from a Molpro basis text file (m2_basis_file_shells,
+    include/m2_input.h; host code, no GPU)."""
+    L = lib()
+    Z = np.ascontiguousarray(mol.Z, np.int32)
+    X = np.ascontiguousarray(mol.xyz, float).reshape(-1)
+    nsh, npr = C.c_int(0), C.c_int(0)
+    if L.m2_basis_file_shells(path.encode(), mol.natom, Z.ctypes.data_as(_ip), _p(X), C.byref(nsh), C.byref(npr),
+                              None) != 0:
+        raise M2Error(1, L.m2_rhf_last_error().decode())
+    b, o = _shells_buffers(nsh.value, npr.value)
+    if L.m2_basis_file_shells(path.encode(), mol.natom, Z.ctypes.data_as(_ip), _p(X), C.byref(nsh), C.byref(npr),
+                              C.byref(o)) != 0:
+        raise M2Error(1, L.m2_rhf_last_error().decode())
+    return _shells_from(b)
 
 
 def deck_molecule(deck_text: str, imol: int = 0, basislib: str = ""):
@@ -318,6 +341,23 @@ class Engine:
         self._chk(self.L.m2_jk_direct_device_noalloc(self.h, C.c_void_p(dD_ptr), multiplier, precision,
                                                      M2_MIXED if mixed else M2_FP64, C.c_void_p(dJ_ptr),
                                                      C.c_void_p(dK_ptr)))
+
+    def eri_packed(self):
+        """(ij|kl) of the orbital basis, packed: [p(ij), p(kl)] with
+        p(ij) = i(i+1)/2 + j, i >= j (m2_eri_packed)."""
+        n = self.nbf
+        np_ = n * (n + 1) // 2
+        out = np.zeros((np_, np_))
+        self._chk(self.L.m2_eri_packed(self.h, _p(out)))
+        return out
+
+    def print_eri(self, path, thrint=1e-12):
+        """IntegralEngine::printERI format written to `path` (m2_print_eri);
+        returns (integral count, insignificant count)."""
+        a, b = C.c_longlong(), C.c_longlong()
+        if self.L.m2_print_eri(self.h, path.encode(), thrint, C.byref(a), C.byref(b)) != 0:
+            raise M2Error(1, self.L.m2_rhf_last_error().decode())
+        return a.value, b.value
 
     def set_mixed_threshold(self, t):
         self._chk(self.L.m2_set_mixed_threshold(self.h, t))

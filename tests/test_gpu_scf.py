@@ -125,3 +125,17 @@ def test_scf_soad_guess(engine):
     assert abs(r1["energy"] - (-75.990178761701)) < 2e-9
     assert r1["log"][0, 0] < -75.0
     assert r1["iterations"] < r0["iterations"]
+
+
+def test_incore_path_reproduces_recorded_water_energy(engine):
+    """The in-core path (Fock::formJK over the stored unique integrals,
+    src/fockbuild.cpp:36-51; "direct, false" as in test/h2otest.mol) gives
+    test/h2otest.out_1:124, -75.990178761701 (water, 6d), and agrees with the
+    direct path."""
+    print("case: in-core SCF water 6d", flush=True)
+    mol = M.water_h2otest()
+    ri = engine.rhf(mol, pure=False, enconverge=1e-11, densconverge=1e-9, guess="hcore", incore=1)
+    rd = engine.rhf(mol, pure=False, enconverge=1e-11, densconverge=1e-9, guess="hcore")
+    assert ri["converged"] and rd["converged"]
+    assert abs(ri["energy"] - (-75.990178761701)) < 1e-8
+    assert abs(ri["energy"] - rd["energy"]) < 1e-9
