@@ -145,7 +145,49 @@ m2_status m2_set_jk_families(m2_ctx* ctx, int mode);
  * primitive quartet moves a J/K element by < 1e-18 at precision 1e-12);
  * 0 computes every primitive quartet. */
 m2_status m2_set_prim_screening(m2_ctx* ctx, double relative);
-/* statistics of the last J/K build: computed quartets (FP64, FP32), tasks */
+/* Launch shape of the direct J/K class kernels (no counterpart in the
+ * reference, whose Fock::formJKdirect walks the shell quartets on the host,
+ * src/fockbuild.cpp:64-164).  host_totals = 0 (default): each class kernel's
+ * grid is a bound on its work-item count that depends on the basis only,
+ * and workgroups past the count the device list kernels wrote exit at once,
+ * so a build reads nothing back before its end (and can be captured into a
+ * graph).  host_totals = 1: the counts are read back first and every grid
+ * is exact.  The results agree to rounding (the FP64 atomics sum in
+ * another order). */
+m2_status m2_set_jk_launch(m2_ctx* ctx, int host_totals);
+/* Graph replay of the direct J/K builds (off by default; no counterpart in
+ * the reference).  When on, a build with the bounded-grid launch and
+ * profiling off is captured once into a HIP graph per key -- its device
+ * pointers, multiplier, precision, flags, mixed threshold and primitive
+ * screening -- and later builds with the same key launch that graph (one
+ * launch instead of ~30).  Up to 8 graphs are kept, the least recently used
+ * dropped first; a new orbital basis, a chunks-per-item change or turning
+ * replay off drops them all.  The allocation-free entry point never uses
+ * graphs.  Results equal those of the eager build to rounding. */
+m2_status m2_set_jk_graphs(m2_ctx* ctx, int on);
+/* graph captures and replays so far, and graphs held */
+m2_status m2_jk_graph_stats(m2_ctx* ctx, long long* captures, long long* replays, int* cached);
+/* Ket chunks per work item of class combination (X | Y), pair classes
+ * ss=0 ps=1 pp=2 ds=3 dp=4 dd=5, 0 <= Y <= X; 0 restores the default (64
+ * chunks of 64 kets for the register classes, 1 chunk of 4 kets for the
+ * table-driven ones); values below the smallest the list capacity holds
+ * (16 for the register classes) are raised to it.  Kept across basis sets. */
+m2_status m2_set_jk_block_tasks(m2_ctx* ctx, int X, int Y, int bt);
+/* Autotuner of the chunks per work item (no counterpart in the reference):
+ * times `repeats` builds of the device density dD at precision/flags for
+ * each of 16, 32, 64, 128, 256 chunks per item (register classes; 1, 2, 4,
+ * 8, 16 for the table-driven ones), keeps the fastest per class
+ * combination (as m2_set_jk_block_tasks would) and reports the mean build
+ * time before and after (ms).  Uses the context's J/K output buffers;
+ * profiling and graph settings are restored, the profile is cleared. */
+m2_status m2_jk_autotune(m2_ctx* ctx, const double* dD, double precision, unsigned flags, int repeats,
+                         double* ms_before, double* ms_after);
+/* Current launch shape; block_tasks and grid_bound (optional, 36 entries,
+ * index 6 X + Y): chunks per item and grid bound of every combination with
+ * bra pairs in the current basis, else 0. */
+m2_status m2_get_jk_launch(m2_ctx* ctx, int* host_totals, int* block_tasks, int* grid_bound);
+/* statistics of the last J/K build: computed quartets (FP64, FP32; -1: not
+ * counted), work items of the class kernels */
 m2_status m2_last_stats(m2_ctx* ctx, long long* q64, long long* q32, long long* ntasks);
 
 /* Density-fitted J/K: Fock::formJKdf / compute_2body_fock_df

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -16,6 +17,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace m2 {
@@ -27,7 +29,7 @@ hipError_t heavy_upload(int device);
 void la_release(m2_ctx* ctx);
 bool heavy_has(int la, int lb, int lc, int ld);
 hipError_t launch_jk_heavy(int device, int la, int lb, int lc, int ld, int pure, const JKArgs& a,
-                           const BoysTables& tab, const Task* tasks, int ntasks, int mode, hipStream_t st);
+                           const BoysTables& tab, int nitems, int mode, hipStream_t st);
 hipError_t launch_schwarz_heavy(int device, int la, int lb, int pure, const JKArgs& a, const BoysTables& tab,
                                 int p0, int np, double* Q, hipStream_t st);
 }  // namespace m2
@@ -350,6 +352,7 @@ static m2_status build_families(m2_ctx* ctx) {
     HIPCHK(hipMemcpy(ctx->d_fam_np.p, np.data(), sizeof(int) * nfp, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_fam_mem.p, mem.data(), sizeof(int4) * nfp, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_pair_fam.p, pfam.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
+    ctx->h_pair_fam = pfam;
     HIPCHK(hipMemcpy(ctx->d_pair_mem.p, pmem.data(), sizeof(int) * npair, hipMemcpyHostToDevice));
     PairClassSeg& seg = ctx->fam_seg;
     seg.begin = 0;
@@ -366,6 +369,45 @@ static m2_status build_families(m2_ctx* ctx) {
 // Build canonical shell pairs (la >= lb; equal l: a >= b), their primitive
 // pair records, Schwarz bounds (device), then order each class by
 // (primitive-count bucket, Q descending).
+// Drops every captured J/K graph (their kernel arguments name this basis's
+// buffers and launch shape) and moves to a new key generation.
+void drop_graphs(m2_ctx* ctx) {
+    for (JKGraph& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
+    ++ctx->graph_gen;
+}
+
+// Grid bound of list ci at its current chunks per item: every bra with a
+// nonzero Schwarz bound with all the kets it can pair with (kets up to its
+// own position when bra and ket class agree), so no density gives more
+// items (an unscreened build gives exactly this many).
+void list_bound(m2_ctx* ctx, int ci) {
+    JKList& L = ctx->lists[ci];
+    long long items = 0;
+    for (int i = 0; i < L.nbra; ++i) {
+        const int bra = L.bra0 + i;
+        if (!(ctx->pair_Q[bra] > 0.0)) continue;   // the list kernels give such bras no items
+        const int lim = L.same ? (L.bra_fam ? ctx->h_pair_fam[bra] : bra) + 1 : INT_MAX;
+        long long ch = 0;
+        for (int g = 0; g < L.ngrp; ++g) {
+            const int len = std::min(L.gend[g], lim) - L.gbeg[g];
+            if (len > 0) ch += (len + L.kc - 1) / L.kc;
+        }
+        items += (ch + L.bt - 1) / L.bt;
+    }
+    L.bound = (int)std::min<long long>(items, L.cap);
+}
+
+// the list descriptors to the device (read by the list kernels); after a
+// basis change or a new chunks-per-item choice
+m2_status upload_jklists(m2_ctx* ctx) {
+    for (int ci = 0; ci < NCOMBO; ++ci)
+        if (ctx->lists[ci].nbra > 0) list_bound(ctx, ci);
+    HIPCHK(ctx->d_jklists.reserve(NCOMBO));
+    HIPCHK(hipMemcpy(ctx->d_jklists.p, ctx->lists, sizeof(JKList) * NCOMBO, hipMemcpyHostToDevice));
+    return M2_OK;
+}
+
 m2_status build_pairs(m2_ctx* ctx) {
     BasisData& b = ctx->obs;
     struct P {
@@ -587,15 +629,17 @@ m2_status build_pairs(m2_ctx* ctx) {
     }
     st = build_families(ctx);
     if (st) return st;
-    // device work-list buffers of the register-class combinations (sizes
-    // depend on the basis only)
+    // device work-list buffers of every class combination (sizes depend on
+    // the basis only)
+    ctx->max_nbra = 0;
     for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
         for (int Y = 0; Y <= X; ++Y, ++ci) {
             JKList& L = ctx->lists[ci];
             L = JKList{};
-            const bool fam = ctx->fam_on && fam_combo(X, Y, ctx->fam_wide);
+            const bool reg = JK_COMBO[X][Y] != nullptr;
+            const bool fam = reg && ctx->fam_on && fam_combo(X, Y, ctx->fam_wide);
             const PairClassSeg &sx = ctx->cls[X], &sy = fam ? ctx->fam_seg : ctx->cls[Y];
-            if (!JK_COMBO[X][Y] || sx.end <= sx.begin || sy.end <= sy.begin) continue;
+            if (sx.end <= sx.begin || sy.end <= sy.begin) continue;
             ARGCHK((int)sy.groups.size() <= JK_MAXG, "m2_basis_set: too many primitive-count groups");
             L.bra0 = sx.begin;
             L.nbra = sx.end - sx.begin;
@@ -603,13 +647,23 @@ m2_status build_pairs(m2_ctx* ctx) {
             L.Qk = fam ? ctx->d_fam_Q.p : ctx->d_pair_Q.p;
             L.bra_fam = fam && X == Y ? ctx->d_pair_fam.p : nullptr;
             L.ngrp = (int)sy.groups.size();
-            int maxch = 0;
+            L.kc = reg ? 64 : JK_HEAVY_KETS;
+            long long maxch = 0;   // chunks of the bra with the most kets
             for (int g = 0; g < L.ngrp; ++g) {
                 L.gbeg[g] = sy.groups[g].begin;
                 L.gend[g] = sy.groups[g].end;
-                maxch += (sy.groups[g].end - sy.groups[g].begin + 63) / 64;
+                maxch += (sy.groups[g].end - sy.groups[g].begin + L.kc - 1) / L.kc;
             }
-            L.cap = L.nbra * ((maxch + JK_BLOCK_TASKS - 1) / JK_BLOCK_TASKS);
+            // capacity: every bra at its largest chunk count, in items of
+            // bt_min chunks (16 for the register classes, 1 for the heavy
+            // ones), more when the list would otherwise pass
+            // JK_LIST_MAX_ITEMS items
+            L.bt_min = (int)std::max<long long>(reg ? JK_BLOCK_TASKS_MIN : 1,
+                                                ((long long)L.nbra * maxch + JK_LIST_MAX_ITEMS - 1) / JK_LIST_MAX_ITEMS);
+            const long long cap = (long long)L.nbra * ((maxch + L.bt_min - 1) / L.bt_min);
+            ARGCHK(cap <= 2LL * JK_LIST_MAX_ITEMS, "m2_basis_set: work list too large");
+            L.cap = (int)cap;
+            L.bt = std::max(L.bt_min, ctx->tune_bt[ci] > 0 ? ctx->tune_bt[ci] : reg ? JK_BLOCK_TASKS : L.bt_min);
             DevBuf<int2>& cb = ctx->list_cut[ci];
             DevBuf<int>& ob = ctx->list_off[ci];
             DevBuf<int>& bb = ctx->list_bra[ci];
@@ -619,34 +673,11 @@ m2_status build_pairs(m2_ctx* ctx) {
             L.cut = cb.p;
             L.off = ob.p;
             L.bra_of = bb.p;
+            ctx->max_nbra = std::max(ctx->max_nbra, L.nbra);
         }
-    return M2_OK;
+    return upload_jklists(ctx);
 }
 
-// number of leading pairs of [g.begin, g.end) (Q descending) with Q >= thr
-int count_ge(const std::vector<double>& Q, int b, int e, double thr) {
-    int lo = b, hi = e;
-    while (lo < hi) {
-        int mid = (lo + hi) / 2;
-        if (Q[mid] >= thr) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo - b;
-}
-
-struct ComboRange {
-    int X, Y, t0, tmid, t1;   // FP64 tasks [t0, tmid), FP32 tasks [tmid, t1)
-};
-
-// Work list: for every bra class X >= ket class Y, every bra pair i and every
-// primitive-count group of Y, the ket pairs j whose Q_j clears
-// precision / (Q_i Dmax) -- a superset of the exactly screened quartets,
-// with j <= i inside one class (unique pairs of pairs, src/fockbuild.cpp:88-109).
-// Mixed precision: the ket prefix with Q_i Q_j Dmax >= fp32_below stays FP64,
-// the rest (every quartet bound < fp32_below) goes to the FP32 list.
-// Register classes take up to 64 kets per task (one per lane); table-driven
-// heavy classes take HEAVY_CHUNK (one wavefront walks them in turn).
-constexpr int HEAVY_CHUNK = 4;
 // kets per task of the gradient's program classes (split over workgroups)
 constexpr int GRAD_PROG_CHUNK = GRAD_PROG_SPLIT * GRAD_PROG_KETS;
 
@@ -658,172 +689,82 @@ static bool deriv_register(int la, int lb, int lc, int ld) {
     return false;
 }
 
-// heavy_only: the J/K path builds host tasks only for the table-driven heavy
-// classes (the register classes use the device work lists of tasks.hip).
-// h_tasks -> d_tasks on the context's stream through the pinned staging
-// buffer.  Callers synchronise the stream before building the next task list,
-// so the staging buffer is never rewritten while a copy from it is pending.
-hipError_t upload_task_list(m2_ctx* ctx) {
-    const size_t nt = ctx->h_tasks.size();
-    if (nt == 0) return hipSuccess;
-    hipError_t e = ctx->d_tasks.reserve(nt);
-    if (e != hipSuccess) return e;
-    if (nt > ctx->h_tasks_pin.n) {
-        if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
-        if ((e = ctx->h_tasks_pin.reserve(nt)) != hipSuccess) return e;
-    }
-    std::memcpy(ctx->h_tasks_pin.p, ctx->h_tasks.data(), sizeof(Task) * nt);
-    return hipMemcpyAsync(ctx->d_tasks.p, ctx->h_tasks_pin.p, sizeof(Task) * nt, hipMemcpyHostToDevice, ctx->stream);
-}
-
-std::vector<ComboRange> build_tasks(m2_ctx* ctx, double precision, double Dmax, bool mixed, bool gradient,
-                                    bool heavy_only) {
-    std::vector<ComboRange> ranges;
-    auto& T = ctx->h_tasks;
-    T.clear();
-    const auto& Q = ctx->pair_Q;
-    std::vector<Task> t32;
-    for (int X = 0; X < NPAIRCLASS; ++X) {
-        const PairClassSeg& sx = ctx->cls[X];
-        for (int Y = 0; Y <= X; ++Y) {
-            const PairClassSeg& sy = ctx->cls[Y];
-            // gradient: the five smallest classes run one quartet per lane
-            const bool reg = gradient ? deriv_register(PAIR_L[X][0], PAIR_L[X][1], PAIR_L[Y][0], PAIR_L[Y][1])
-                                      : JK_COMBO[X][Y] != nullptr;
-            if (heavy_only && reg) continue;
-            const int chunk = reg ? 64 : gradient ? GRAD_PROG_CHUNK : HEAVY_CHUNK;
-            ComboRange r{X, Y, (int)T.size(), 0, 0};
-            t32.clear();
-            for (int i = sx.begin; i < sx.end; ++i) {
-                const double qi = Q[i];
-                if (qi <= 0.0 || Dmax <= 0.0) continue;
-                const double thr = precision / (qi * Dmax);
-                const double thr64 = mixed ? std::max(precision, ctx->fp32_below) / (qi * Dmax) : thr;
-                for (const PairGroup& g : sy.groups) {
-                    int end = g.begin + count_ge(Q, g.begin, g.end, thr);
-                    int mid = g.begin + count_ge(Q, g.begin, g.end, thr64);
-                    if (X == Y) {
-                        end = std::min(end, i + 1);
-                        mid = std::min(mid, i + 1);
-                    }
-                    mid = std::min(mid, end);
-                    for (int j = g.begin; j < mid; j += chunk) T.push_back(Task{i, j, std::min(chunk, mid - j), 0});
-                    for (int j = mid; j < end; j += chunk) t32.push_back(Task{i, j, std::min(chunk, end - j), 0});
-                }
-            }
-            r.tmid = (int)T.size();
-            T.insert(T.end(), t32.begin(), t32.end());
-            r.t1 = (int)T.size();
-            if (r.t1 > r.t0) ranges.push_back(r);
-        }
-    }
-    return ranges;
-}
-
-// dScreen (optional): the density whose shell-block norms screen the quartets
-// (UnrestrictedFock::formJKdirect screens both spin builds with Da + Db).
-// dD2 (optional): a second density digested from the same integrals into
-// dJ2, dK2 (the unrestricted build); dScreen: the density whose block norms
-// screen (default dD).
 // Device memory one direct J/K build works in: the shell-block norms, the
-// unsymmetrised J/K accumulators, the work-list totals (then max|D| as a
-// double) and the heavy-class tasks.  Either the context's own buffers
-// (grown on demand) or, for the allocation-free entry point, fixed slices of
-// the static scratch array (nothing is allocated; the tasks must fit).
+// unsymmetrised J/K accumulators and the work-list totals (then max|D| as a
+// double).  Either the context's own buffers (grown on demand) or, for the
+// allocation-free entry point, fixed slices of the static scratch array
+// (nothing is allocated).
 struct JKWork {
     double* Dn = nullptr;
     double* J = nullptr;
     double* K = nullptr;
     int* lists = nullptr;
-    Task* tasks = nullptr;
-    size_t task_cap = 0;
     bool fixed = false;
 };
 
-m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, double precision, unsigned flags,
-                         double* dJ, double* dK, const double* dScreen = nullptr, const double* dD2 = nullptr,
-                         double* dJ2 = nullptr, double* dK2 = nullptr, const JKWork* fixed = nullptr) {
+// The arguments of one build.  dScreen (optional): the density whose
+// shell-block norms screen the quartets (UnrestrictedFock::formJKdirect
+// screens both spin builds with Da + Db).  dD2 (optional): a second density
+// digested from the same integrals into dJ2, dK2 (the unrestricted build).
+struct JKCall {
+    const double* dD;
+    double multiplier, precision;
+    unsigned flags;
+    double* dJ;
+    double* dK;
+    const double* dScreen;
+    const double* dD2;
+    double* dJ2;
+    double* dK2;
+};
+
+// class combinations launched by one build, in class order
+struct JKLaunch {
+    int X, Y, ci;
+};
+
+// On the device: shell-block norms and max|D|, then every class
+// combination's work list (three launches for all of them), then one class
+// kernel per combination and precision, then the symmetrisation.  By
+// default the host reads nothing back: each class kernel's grid is the
+// list's bound (a function of the basis), and workgroups past the total the
+// list kernels wrote exit at once, so the whole build can also be captured
+// into a graph.  With host_totals the totals are read back first (a stream
+// synchronisation) and each grid is the exact item count.
+m2_status jk_enqueue(m2_ctx* ctx, const JKCall& c, const JKWork& w, bool prof, std::vector<JKLaunch>& plan) {
     BasisData& b = ctx->obs;
     const int n = b.nbf, nsh = b.nsh;
-    ARGCHK(n > 0, "m2_jk_direct: no orbital basis set");
-    ARGCHK(flags <= M2_MIXED, "m2_jk_direct: unknown flags");
-    const bool mixed = flags == M2_MIXED;
-    const int nden = dD2 ? 2 : 1;
-    JKWork w;
-    if (fixed) {
-        w = *fixed;
-    } else {
-        HIPCHK(ctx->d_Dn.reserve((size_t)nsh * nsh));
-        HIPCHK(ctx->d_J.reserve((size_t)nden * n * n));
-        HIPCHK(ctx->d_K.reserve((size_t)nden * n * n));
-        HIPCHK(ctx->d_lists.reserve(2 * NCOMBO + 2));
-        w.Dn = ctx->d_Dn.p;
-        w.J = ctx->d_J.p;
-        w.K = ctx->d_K.p;
-        w.lists = ctx->d_lists.p;
-    }
-    // profiling creates events and a counter buffer: not in the fixed-memory build
-    const bool prof = ctx->prof && !w.fixed;
-    // shell-block norms, max|D| and the register-class work lists, on the device
-    launch_blocknorm(dScreen ? dScreen : dD, n, nsh, b.d_bf.p, b.d_nbf.p, w.Dn, ctx->stream);
+    const bool mixed = c.flags == M2_MIXED;
+    const int nden = c.dD2 ? 2 : 1;
+    const bool bounded = !ctx->host_totals;
+    launch_blocknorm(c.dScreen ? c.dScreen : c.dD, n, nsh, b.d_bf.p, b.d_nbf.p, w.Dn, ctx->stream);
     double* d_dmax = reinterpret_cast<double*>(w.lists + 2 * NCOMBO);
     launch_dmax(w.Dn, nsh * nsh, d_dmax, ctx->stream);
-    for (int ci = 0; ci < NCOMBO; ++ci) {
-        const JKList& L = ctx->lists[ci];
-        if (L.nbra > 0)
-            launch_task_lists(L, ctx->d_pair_Q.p, d_dmax, precision, ctx->fp32_below, mixed ? 1 : 0,
-                              w.lists + 2 * ci, ctx->stream);
-    }
+    launch_task_lists(ctx->d_jklists.p, NCOMBO, ctx->max_nbra, ctx->d_pair_Q.p, d_dmax, c.precision,
+                      ctx->fp32_below, mixed ? 1 : 0, w.lists, ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(ctx->h_lists, w.lists, sizeof(int) * (2 * NCOMBO + 2), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    double Dmax;
-    std::memcpy(&Dmax, ctx->h_lists + 2 * NCOMBO, sizeof(double));
-    // host tasks of the heavy classes
-    std::vector<ComboRange> heavy = build_tasks(ctx, precision, Dmax, mixed, false, true);
-    long long units = (long long)ctx->h_tasks.size();
-    if (!w.fixed) {
-        HIPCHK(upload_task_list(ctx));
-        w.tasks = ctx->d_tasks.p;
-    } else if (!ctx->h_tasks.empty()) {
-        const size_t nt = ctx->h_tasks.size();
-        if (nt > w.task_cap) {
-            ctx->err = "m2_jk_direct_device_noalloc: " + std::to_string(nt) + " heavy-class tasks exceed the " +
-                       std::to_string(w.task_cap) + " that fit in the static scratch buffer";
-            return M2_ERR_ARG;
-        }
-        // through the pinned staging buffer when it is large enough (it is
-        // never grown here), else straight from pageable host memory
-        if (nt <= ctx->h_tasks_pin.n) {
-            std::memcpy(ctx->h_tasks_pin.p, ctx->h_tasks.data(), sizeof(Task) * nt);
-            HIPCHK(hipMemcpyAsync(w.tasks, ctx->h_tasks_pin.p, sizeof(Task) * nt, hipMemcpyHostToDevice, ctx->stream));
-        } else {
-            HIPCHK(hipMemcpyAsync(w.tasks, ctx->h_tasks.data(), sizeof(Task) * nt, hipMemcpyHostToDevice,
-                                  ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-        }
+    if (!bounded) {
+        HIPCHK(hipMemcpyAsync(ctx->h_lists, w.lists, sizeof(int) * (2 * NCOMBO + 2), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
     }
-    // launch plan in class order: register combos from their lists, heavy ones from host tasks
-    struct Launch {
-        int X, Y, ci, hv;   // ci: list index (register) or -1; hv: index into `heavy` or -1
-    };
-    std::vector<Launch> plan;
+    // launch plan: every combination with bra pairs (with host-read totals:
+    // those with work items)
+    plan.clear();
+    long long units = 0;
     for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
         for (int Y = 0; Y <= X; ++Y, ++ci) {
-            if (JK_COMBO[X][Y]) {
-                // combinations without bra pairs launch no list kernels: their totals are not written
-                const bool any = ctx->lists[ci].nbra > 0;
-                const int t64 = any ? ctx->h_lists[2 * ci] : 0, t32 = any ? ctx->h_lists[2 * ci + 1] : 0;
-                ARGCHK(t64 <= ctx->lists[ci].cap && t32 <= ctx->lists[ci].cap, "m2_jk_direct: work list overflow");
+            const JKList& L = ctx->lists[ci];
+            if (L.nbra <= 0 || L.bound <= 0) continue;
+            if (!bounded) {
+                const int t64 = ctx->h_lists[2 * ci], t32 = ctx->h_lists[2 * ci + 1];
                 units += t64 + t32;
-                if (t64 + t32 > 0) plan.push_back(Launch{X, Y, ci, -1});
-            } else {
-                for (size_t h = 0; h < heavy.size(); ++h)
-                    if (heavy[h].X == X && heavy[h].Y == Y) plan.push_back(Launch{X, Y, -1, (int)h});
+                if (t64 + t32 == 0) continue;
             }
+            plan.push_back(JKLaunch{X, Y, ci});
         }
-    ctx->last_tasks = units;
+    ctx->last_tasks = bounded ? -1 : units;
+    ctx->last_tot = bounded ? w.lists : nullptr;
     launch_zero(w.J, (size_t)nden * n * n, ctx->stream);
     launch_zero(w.K, (size_t)nden * n * n, ctx->stream);
     JKArgs a = jk_args(ctx);
@@ -839,69 +780,183 @@ m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, doubl
             ctx->ev.push_back(e);
         }
     }
-    a.D = dD;
+    a.D = c.dD;
     a.J = w.J;
     a.K = w.K;
     a.nden = nden;
-    a.D2 = dD2;
+    a.D2 = c.dD2;
     a.J2 = nden == 2 ? w.J + (size_t)n * n : nullptr;
     a.K2 = nden == 2 ? w.K + (size_t)n * n : nullptr;
-    a.precision = precision;
+    a.precision = c.precision;
     a.fp32_below = ctx->fp32_below;
-    a.scale = multiplier;
-    a.prim_tau = precision * ctx->prim_screen;
+    a.scale = c.multiplier;
+    a.prim_tau = c.precision * ctx->prim_screen;
     for (size_t pi = 0; pi < plan.size(); ++pi) {
-        const Launch& pl = plan[pi];
+        const JKLaunch& pl = plan[pi];
+        const JKList& L = ctx->lists[pl.ci];
         a.combo = 6 * pl.X + pl.Y;
         if (prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi], ctx->stream));
         a.fam_kets = ctx->fam_on && fam_combo(pl.X, pl.Y, ctx->fam_wide) ? 1 : 0;
-        if (pl.ci >= 0) {
-            const JKList& L = ctx->lists[pl.ci];
-            a.cut = L.cut;
-            a.bra0 = L.bra0;
-            a.ngrp = L.ngrp;
-            for (int g = 0; g < JK_MAXG; ++g) a.gbeg[g] = L.gbeg[g];
-            // FP64 heads, then FP32 tails
-            for (int tail = 0; tail < 2; ++tail) {
-                const int nblk = ctx->h_lists[2 * pl.ci + tail];
+        a.cut = L.cut;
+        a.bra0 = L.bra0;
+        a.ngrp = L.ngrp;
+        for (int g = 0; g < JK_MAXG; ++g) a.gbeg[g] = L.gbeg[g];
+        a.block_tasks = L.bt;
+        // FP64 heads, then FP32 tails (only mixed builds have tails)
+        for (int tail = 0; tail < (mixed ? 2 : 1); ++tail) {
+            int nblk;
+            if (bounded) {
+                nblk = L.bound;
+                a.blk_total = w.lists + 2 * pl.ci + tail;
+            } else {
+                nblk = ctx->h_lists[2 * pl.ci + tail];
                 if (nblk <= 0) continue;
-                a.tail = tail;
-                a.blk_off = L.off + tail * (L.nbra + 1);
-                a.blk_bra = L.bra_of + (size_t)tail * L.cap;
+                a.blk_total = nullptr;
+            }
+            a.tail = tail;
+            a.blk_off = L.off + tail * (L.nbra + 1);
+            a.blk_bra = L.bra_of + (size_t)tail * L.cap;
+            if (JK_COMBO[pl.X][pl.Y]) {
                 JK_COMBO[pl.X][pl.Y](b.pure, a, tables(ctx), nblk, tail ? 2 : 0, ctx->stream);
                 HIPCHK(hipGetLastError());
-            }
-        } else {
-            const ComboRange& r = heavy[pl.hv];
-            for (int mode : {0, 2}) {
-                const int b0 = mode == 0 ? r.t0 : r.tmid, b1 = mode == 0 ? r.tmid : r.t1;
-                if (b1 <= b0) continue;
-                HIPCHK(launch_jk_heavy(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1],
-                                       b.pure, a, tables(ctx), w.tasks + b0, b1 - b0, mode, ctx->stream));
+            } else {
+                HIPCHK(launch_jk_heavy(ctx->device, PAIR_L[pl.X][0], PAIR_L[pl.X][1], PAIR_L[pl.Y][0],
+                                       PAIR_L[pl.Y][1], b.pure, a, tables(ctx), nblk, tail ? 2 : 0, ctx->stream));
             }
         }
         if (prof) HIPCHK(hipEventRecord(ctx->ev[2 * pi + 1], ctx->stream));
     }
-    launch_symmetrize(w.J, w.K, dJ, dK, n, ctx->stream);
-    if (nden == 2) launch_symmetrize(w.J + (size_t)n * n, w.K + (size_t)n * n, dJ2, dK2, n, ctx->stream);
+    launch_symmetrize(w.J, w.K, c.dJ, c.dK, n, ctx->stream);
+    if (nden == 2) launch_symmetrize(w.J + (size_t)n * n, w.K + (size_t)n * n, c.dJ2, c.dK2, n, ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (prof) {
-        unsigned long long cnt[108];
-        HIPCHK(hipMemcpy(cnt, ctx->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
-        for (size_t pi = 0; pi < plan.size(); ++pi) {
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * pi], ctx->ev[2 * pi + 1]));
-            const int c = 6 * plan[pi].X + plan[pi].Y;
-            ctx->prof_ms[c] += ms;
-            ctx->prof_launch[c] += 1;
-        }
-        for (int c = 0; c < 36; ++c) {
-            ctx->prof_q[c] += (long long)cnt[3 * c];
-            ctx->prof_pq[c] += (long long)cnt[3 * c + 1];
-            ctx->prof_cpq[c] += (long long)cnt[3 * c + 2];
-        }
+    return M2_OK;
+}
+
+// per-combination kernel times and counters of a profiled build (after its stream synchronisation)
+m2_status jk_profile_readout(m2_ctx* ctx, const std::vector<JKLaunch>& plan) {
+    unsigned long long cnt[108];
+    HIPCHK(hipMemcpy(cnt, ctx->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    for (size_t pi = 0; pi < plan.size(); ++pi) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev[2 * pi], ctx->ev[2 * pi + 1]));
+        const int c = 6 * plan[pi].X + plan[pi].Y;
+        ctx->prof_ms[c] += ms;
+        ctx->prof_launch[c] += 1;
     }
+    for (int c = 0; c < 36; ++c) {
+        ctx->prof_q[c] += (long long)cnt[3 * c];
+        ctx->prof_pq[c] += (long long)cnt[3 * c + 1];
+        ctx->prof_cpq[c] += (long long)cnt[3 * c + 2];
+    }
+    return M2_OK;
+}
+
+// Graph replay (m2_set_jk_graphs): a build is captured once per key -- the
+// call's pointers and scalars, the context's settings that enter the kernel
+// arguments, its work buffers, and a generation counter bumped by every
+// change of basis or launch shape -- and later builds with the same key
+// launch the instantiated graph.  The capture holds only kernel launches
+// (the accumulators are cleared by a fill kernel, not a memset node).
+bool jk_graph_key_eq(const JKGraphKey& x, const JKGraphKey& y) { return std::memcmp(&x, &y, sizeof(x)) == 0; }
+
+m2_status jk_graph_run(m2_ctx* ctx, const JKCall& c, const JKWork& w) {
+    JKGraphKey k;
+    std::memset(&k, 0, sizeof(k));   // padding too: keys compare bytewise
+    k.dD = c.dD;
+    k.dJ = c.dJ;
+    k.dK = c.dK;
+    k.dScreen = c.dScreen;
+    k.dD2 = c.dD2;
+    k.dJ2 = c.dJ2;
+    k.dK2 = c.dK2;
+    k.multiplier = c.multiplier;
+    k.precision = c.precision;
+    k.flags = c.flags;
+    k.fp32_below = ctx->fp32_below;
+    k.prim_screen = ctx->prim_screen;
+    k.Dn = w.Dn;
+    k.J = w.J;
+    k.K = w.K;
+    k.lists = w.lists;
+    k.gen = ctx->graph_gen;
+    for (size_t i = 0; i < ctx->graphs.size(); ++i) {
+        if (!jk_graph_key_eq(ctx->graphs[i].key, k)) continue;
+        std::rotate(ctx->graphs.begin(), ctx->graphs.begin() + i, ctx->graphs.begin() + i + 1);   // most recent first
+        HIPCHK(hipGraphLaunch(ctx->graphs[0].exec, ctx->stream));
+        ctx->last_tasks = -1;
+        ctx->last_tot = w.lists;
+        ++ctx->graph_replays;
+        return M2_OK;
+    }
+    // capture (nothing runs), instantiate, then launch
+    std::vector<JKLaunch> plan;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const m2_status st = jk_enqueue(ctx, c, w, false, plan);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (st != M2_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        ctx->err = std::string("m2_jk_direct: graph capture: ") + hipGetErrorString(ec);
+        return M2_ERR_HIP;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        ctx->err = std::string("m2_jk_direct: graph instantiation: ") + hipGetErrorString(ei);
+        return M2_ERR_HIP;
+    }
+    if (ctx->graphs.size() >= JK_GRAPH_CACHE) {
+        (void)hipGraphExecDestroy(ctx->graphs.back().exec);
+        ctx->graphs.pop_back();
+    }
+    ctx->graphs.insert(ctx->graphs.begin(), JKGraph{k, ex});
+    ++ctx->graph_captures;
+    HIPCHK(hipGraphLaunch(ex, ctx->stream));
+    return M2_OK;
+}
+
+// One direct J/K build, enqueued on the context's stream and waited for.
+m2_status jk_device_core(m2_ctx* ctx, const double* dD, double multiplier, double precision, unsigned flags,
+                         double* dJ, double* dK, const double* dScreen = nullptr, const double* dD2 = nullptr,
+                         double* dJ2 = nullptr, double* dK2 = nullptr, const JKWork* fixed = nullptr) {
+    BasisData& b = ctx->obs;
+    const int n = b.nbf, nsh = b.nsh;
+    ARGCHK(n > 0, "m2_jk_direct: no orbital basis set");
+    ARGCHK(flags <= M2_MIXED, "m2_jk_direct: unknown flags");
+    const int nden = dD2 ? 2 : 1;
+    JKWork w;
+    if (fixed) {
+        w = *fixed;
+    } else {
+        HIPCHK(ctx->d_Dn.reserve((size_t)nsh * nsh));
+        HIPCHK(ctx->d_J.reserve((size_t)nden * n * n));
+        HIPCHK(ctx->d_K.reserve((size_t)nden * n * n));
+        HIPCHK(ctx->d_lists.reserve(2 * NCOMBO + 2));
+        w.Dn = ctx->d_Dn.p;
+        w.J = ctx->d_J.p;
+        w.K = ctx->d_K.p;
+        w.lists = ctx->d_lists.p;
+    }
+    const JKCall c{dD, multiplier, precision, flags, dJ, dK, dScreen, dD2, dJ2, dK2};
+    // profiling creates events and a counter buffer: not in the fixed-memory build
+    const bool prof = ctx->prof && !w.fixed;
+    // (the allocation-free build never instantiates a graph: that allocates)
+    if (ctx->use_graphs && !prof && !ctx->host_totals && !w.fixed) {
+        const m2_status st = jk_graph_run(ctx, c, w);
+        if (st) return st;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return M2_OK;
+    }
+    std::vector<JKLaunch> plan;
+    const m2_status st = jk_enqueue(ctx, c, w, prof, plan);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (prof) return jk_profile_readout(ctx, plan);
     return M2_OK;
 }
 
@@ -1067,6 +1122,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_graphs(ctx);
     for (BasisData* b : {&ctx->obs, &ctx->aux}) {
         b->d_O.release();
         b->d_bf.release();
@@ -1087,7 +1143,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_Ko.release();
     ctx->d_Dn.release();
     ctx->d_tasks.release();
-    ctx->h_tasks_pin.release();
+    ctx->d_jklists.release();
     for (int ci = 0; ci < NCOMBO; ++ci) {
         ctx->list_cut[ci].release();
         ctx->list_off[ci].release();
@@ -1145,6 +1201,11 @@ m2_status m2_basis_set(m2_ctx* ctx, int which, int nsh, const int* l, const int*
                        const double* coefs, const double* centers, const int* atoms, int pure) {
     if (!ctx) return M2_ERR_ARG;
     ARGCHK(which == M2_ORBITAL || which == M2_AUXILIARY, "m2_basis_set: which must be M2_ORBITAL or M2_AUXILIARY");
+    if (which == M2_ORBITAL) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        drop_graphs(ctx);
+    }
     std::vector<HostShell> shells;
     auto fail = [&](const char* msg) {
         clear_basis(ctx, which);
@@ -1324,14 +1385,14 @@ m2_status m2_jk_direct_device_noalloc(m2_ctx* ctx, const double* dD, double mult
         return M2_ERR_STATE;
     }
     const size_t n = ctx->obs.nbf, nsh = ctx->obs.nsh;
-    // slices of the scratch array: norms | list totals + max|D| | J | K | tasks
+    // slices of the scratch array: norms | list totals + max|D| | J | K
     const size_t o_lists = align256(sizeof(double) * nsh * nsh);
     const size_t o_J = o_lists + align256(sizeof(int) * (2 * NCOMBO + 2));
     const size_t o_K = o_J + align256(sizeof(double) * n * n);
-    const size_t o_T = o_K + align256(sizeof(double) * n * n);
-    if (o_T > M2_NOALLOC_SCRATCH_BYTES) {
+    const size_t o_end = o_K + align256(sizeof(double) * n * n);
+    if (o_end > M2_NOALLOC_SCRATCH_BYTES) {
         ctx->err = "m2_jk_direct_device_noalloc: " + std::to_string(n) + " basis functions need " +
-                   std::to_string(o_T) + " bytes of scratch; the static buffer holds " +
+                   std::to_string(o_end) + " bytes of scratch; the static buffer holds " +
                    std::to_string((size_t)M2_NOALLOC_SCRATCH_BYTES);
         return M2_ERR_ARG;
     }
@@ -1342,8 +1403,6 @@ m2_status m2_jk_direct_device_noalloc(m2_ctx* ctx, const double* dD, double mult
     w.lists = reinterpret_cast<int*>(ctx->scratch + o_lists);
     w.J = reinterpret_cast<double*>(ctx->scratch + o_J);
     w.K = reinterpret_cast<double*>(ctx->scratch + o_K);
-    w.tasks = reinterpret_cast<Task*>(ctx->scratch + o_T);
-    w.task_cap = (M2_NOALLOC_SCRATCH_BYTES - o_T) / sizeof(Task);
     return jk_device_core(ctx, dD, multiplier, precision, flags, dJ, dK, nullptr, nullptr, nullptr, nullptr, &w);
 }
 
@@ -1473,6 +1532,149 @@ m2_status m2_set_prim_screening(m2_ctx* ctx, double relative) {
     return M2_OK;
 }
 
+m2_status m2_set_jk_launch(m2_ctx* ctx, int host_totals) {
+    if (!ctx) return M2_ERR_ARG;
+    ctx->host_totals = host_totals != 0;
+    return M2_OK;
+}
+
+m2_status m2_set_jk_graphs(m2_ctx* ctx, int on) {
+    if (!ctx) return M2_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->use_graphs = on != 0;
+    if (!ctx->use_graphs) drop_graphs(ctx);
+    return M2_OK;
+}
+
+m2_status m2_jk_graph_stats(m2_ctx* ctx, long long* captures, long long* replays, int* cached) {
+    if (!ctx) return M2_ERR_ARG;
+    if (captures) *captures = ctx->graph_captures;
+    if (replays) *replays = ctx->graph_replays;
+    if (cached) *cached = (int)ctx->graphs.size();
+    return M2_OK;
+}
+
+m2_status m2_set_jk_block_tasks(m2_ctx* ctx, int X, int Y, int bt) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(X >= 0 && X < NPAIRCLASS && Y >= 0 && Y <= X, "m2_set_jk_block_tasks: need 0 <= Y <= X < 6");
+    ARGCHK(bt >= 0 && bt <= 4096, "m2_set_jk_block_tasks: chunks per work item must lie in [0, 4096]");
+    const int ci = X * (X + 1) / 2 + Y;
+    ctx->tune_bt[ci] = bt;
+    JKList& L = ctx->lists[ci];
+    if (L.nbra > 0) {
+        HIPCHK(hipSetDevice(ctx->device));
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // no build reads the old descriptors any more
+        drop_graphs(ctx);
+        L.bt = std::max(L.bt_min, bt > 0 ? bt : JK_COMBO[X][Y] ? JK_BLOCK_TASKS : L.bt_min);
+        return upload_jklists(ctx);
+    }
+    return M2_OK;
+}
+
+m2_status m2_jk_autotune(m2_ctx* ctx, const double* dD, double precision, unsigned flags, int repeats,
+                         double* ms_before, double* ms_after) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(dD, "m2_jk_autotune: null density");
+    ARGCHK(ctx->obs.nbf > 0, "m2_jk_autotune: no orbital basis set");
+    ARGCHK(repeats >= 1 && repeats <= 100, "m2_jk_autotune: repeats must lie in [1, 100]");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)ctx->obs.nbf * ctx->obs.nbf;
+    HIPCHK(ctx->d_Jo.reserve(n2));
+    HIPCHK(ctx->d_Ko.reserve(n2));
+    double* dJ = ctx->d_Jo.p;
+    double* dK = ctx->d_Ko.p;
+    const bool prof0 = ctx->prof, cnt0 = ctx->prof_counters, graphs0 = ctx->use_graphs;
+    ctx->use_graphs = false;
+    auto restore = [&]() {
+        ctx->prof = prof0;
+        ctx->prof_counters = cnt0;
+        ctx->use_graphs = graphs0;
+        (void)m2_reset_profile(ctx);
+    };
+    // wall time of `repeats` builds between two events on the stream
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto timed = [&](double* ms) -> m2_status {
+        ctx->prof = false;
+        m2_status st = jk_device_core(ctx, dD, 1.0, precision, flags, dJ, dK);   // warm-up
+        if (st) return st;
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < repeats; ++r)
+            if ((st = jk_device_core(ctx, dD, 1.0, precision, flags, dJ, dK))) return st;
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t / repeats;
+        return M2_OK;
+    };
+    const int REG[] = {16, 32, 64, 128, 256}, HEAVY[] = {1, 2, 4, 8, 16};
+    constexpr int NCAND = 5;
+    double best_ms[NCOMBO];
+    int best_bt[NCOMBO];
+    for (int ci = 0; ci < NCOMBO; ++ci) {
+        best_ms[ci] = 1e300;
+        best_bt[ci] = ctx->lists[ci].bt;
+    }
+    double before = 0.0, after = 0.0;
+    m2_status st = timed(&before);
+    for (int k = 0; k < NCAND && st == M2_OK; ++k) {
+        for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
+            for (int Y = 0; Y <= X; ++Y, ++ci) {
+                JKList& L = ctx->lists[ci];
+                if (L.nbra > 0) L.bt = std::max(L.bt_min, JK_COMBO[X][Y] ? REG[k] : HEAVY[k]);
+            }
+        drop_graphs(ctx);
+        if ((st = upload_jklists(ctx))) break;
+        ctx->prof = true;
+        ctx->prof_counters = false;
+        if ((st = jk_device_core(ctx, dD, 1.0, precision, flags, dJ, dK))) break;   // warm-up
+        (void)m2_reset_profile(ctx);
+        for (int r = 0; r < repeats && st == M2_OK; ++r) st = jk_device_core(ctx, dD, 1.0, precision, flags, dJ, dK);
+        for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
+            for (int Y = 0; Y <= X; ++Y, ++ci) {
+                const double ms = ctx->prof_ms[6 * X + Y] / repeats;
+                if (ctx->lists[ci].nbra > 0 && ms < best_ms[ci]) {
+                    best_ms[ci] = ms;
+                    best_bt[ci] = ctx->lists[ci].bt;
+                }
+            }
+    }
+    // keep the fastest per combination (also for later bases)
+    for (int ci = 0; ci < NCOMBO; ++ci) {
+        JKList& L = ctx->lists[ci];
+        if (L.nbra <= 0) continue;
+        L.bt = best_bt[ci];
+        ctx->tune_bt[ci] = best_bt[ci];
+    }
+    drop_graphs(ctx);
+    m2_status su = upload_jklists(ctx);
+    if (st == M2_OK) st = su;
+    if (st == M2_OK) st = timed(&after);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    restore();
+    if (st) return st;
+    if (ms_before) *ms_before = before;
+    if (ms_after) *ms_after = after;
+    return M2_OK;
+}
+
+m2_status m2_get_jk_launch(m2_ctx* ctx, int* host_totals, int* block_tasks, int* grid_bound) {
+    if (!ctx) return M2_ERR_ARG;
+    if (host_totals) *host_totals = ctx->host_totals ? 1 : 0;
+    for (int X = 0, ci = 0; X < NPAIRCLASS; ++X)
+        for (int Y = 0; Y < NPAIRCLASS; ++Y) {
+            const bool on = Y <= X && ctx->lists[ci].nbra > 0;
+            if (block_tasks) block_tasks[6 * X + Y] = on ? ctx->lists[ci].bt : 0;
+            if (grid_bound) grid_bound[6 * X + Y] = on ? ctx->lists[ci].bound : 0;
+            if (Y <= X) ++ci;
+        }
+    return M2_OK;
+}
+
 m2_status m2_reset_profile(m2_ctx* ctx) {
     if (!ctx) return M2_ERR_ARG;
     for (int c = 0; c < 36; ++c) {
@@ -1486,7 +1688,21 @@ m2_status m2_last_stats(m2_ctx* ctx, long long* q64, long long* q32, long long* 
     if (!ctx) return M2_ERR_ARG;
     if (q64) *q64 = -1;
     if (q32) *q32 = -1;
-    if (ntasks) *ntasks = ctx->last_tasks;
+    if (ntasks) {
+        // a bounded-grid build leaves its work-item totals on the device
+        if (ctx->last_tot) {
+            int tot[2 * NCOMBO];
+            HIPCHK(hipSetDevice(ctx->device));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            HIPCHK(hipMemcpy(tot, ctx->last_tot, sizeof(tot), hipMemcpyDeviceToHost));
+            long long units = 0;
+            for (int ci = 0; ci < NCOMBO; ++ci)
+                if (ctx->lists[ci].nbra > 0) units += tot[2 * ci] + tot[2 * ci + 1];
+            ctx->last_tasks = units;
+            ctx->last_tot = nullptr;
+        }
+        *ntasks = ctx->last_tasks;
+    }
     return M2_OK;
 }
 

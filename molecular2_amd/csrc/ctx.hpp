@@ -49,32 +49,6 @@ struct DevBuf {
     }
 };
 
-// Pinned host staging buffer (grown, never shrunk; hipHostMalloc'd)
-template <typename T>
-struct PinnedBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    PinnedBuf() = default;
-    PinnedBuf(const PinnedBuf&) = delete;
-    PinnedBuf& operator=(const PinnedBuf&) = delete;
-    ~PinnedBuf() { release(); }
-    hipError_t reserve(size_t cnt) {
-        if (cnt <= n && p) return hipSuccess;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-        const size_t want = std::max<size_t>(cnt + cnt / 2, 1024);
-        hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T));
-        if (e == hipSuccess) n = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
 struct PairGroup {
     int begin, end;   // global pair index range, Q descending inside
 };
@@ -93,6 +67,29 @@ struct BasisData {
     m2::DevBuf<int> d_bf, d_nbf;
 };
 
+
+// a captured direct J/K build: its key and executable graph
+struct JKGraphKey {
+    const void* dD;
+    const void* dJ;
+    const void* dK;
+    const void* dScreen;
+    const void* dD2;
+    const void* dJ2;
+    const void* dK2;
+    double multiplier, precision, fp32_below, prim_screen;
+    const void* Dn;
+    const void* J;
+    const void* K;
+    const void* lists;
+    unsigned long long gen;
+    unsigned flags;
+};
+struct JKGraph {
+    JKGraphKey key;
+    hipGraphExec_t exec;
+};
+constexpr size_t JK_GRAPH_CACHE = 8;
 
 }  // namespace m2
 
@@ -127,21 +124,38 @@ struct m2_ctx {
     m2::DevBuf<double> d_fam_km, d_fam_Q;
     m2::DevBuf<float> d_fam_km_f;
     m2::DevBuf<int> d_fam_off, d_fam_np, d_pair_fam, d_pair_mem;
+    std::vector<int> h_pair_fam;
     m2::DevBuf<int4> d_fam_mem;
     // Boys tables
     m2::DevBuf<double> d_boys;
     m2::DevBuf<float> d_boys_f;
     // J/K work buffers
     m2::DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
-    m2::DevBuf<m2::Task> d_tasks;
-    std::vector<m2::Task> h_tasks;   // heavy-class (and gradient) tasks
-    m2::PinnedBuf<m2::Task> h_tasks_pin;   // their staging copy for the upload
-    // device work lists of the register-class combinations (index over X >= Y)
+    m2::DevBuf<m2::Task> d_tasks;    // the gradient's task list
+    // device work lists of the class combinations (index over X >= Y), their
+    // device copy (read by the list kernels) and the largest bra count
     m2::JKList lists[m2::NCOMBO];
     m2::DevBuf<int2> list_cut[m2::NCOMBO];
     m2::DevBuf<int> list_off[m2::NCOMBO], list_bra[m2::NCOMBO];
-    m2::DevBuf<int> d_lists;          // per combination workgroup totals (FP64, FP32), then max|D|
+    m2::DevBuf<m2::JKList> d_jklists;
+    int max_nbra = 0;
+    m2::DevBuf<int> d_lists;          // per combination work-item totals (FP64, FP32), then max|D|
     int* h_lists = nullptr;           // pinned copy of d_lists
+    // launch shape (m2_set_jk_launch, m2_set_jk_block_tasks): grids sized
+    // from the lists' bounds with no host round trip (default), or from the
+    // totals read back; chunks per work item per combination (0: default)
+    bool host_totals = false;
+    int tune_bt[m2::NCOMBO] = {0};
+    // totals of the last bounded build, read back only when asked for
+    // (m2_last_stats)
+    const int* last_tot = nullptr;
+    // graph replay of the direct J/K build (m2_set_jk_graphs): the most
+    // recently used graphs first; graph_gen changes with the basis and the
+    // launch shape
+    bool use_graphs = false;
+    std::vector<m2::JKGraph> graphs;
+    unsigned long long graph_gen = 0;
+    long long graph_captures = 0, graph_replays = 0;
     std::vector<double> h_Dn;
     double fp32_below = 1e-4;
     long long last_tasks = 0;

@@ -335,24 +335,19 @@ __device__ __forceinline__ void heavy_digest(const double* __restrict__ Y, const
     }
 }
 
-// NDEN = 2 (unrestricted build): each quartet is digested with D into (J, K)
-// and with D2 into (J2, K2).
+// One task: bra pair `bra` with the kets [ket0, ket0 + nket), one quartet at a
+// time.  NDEN = 2 (unrestricted build): each quartet is digested with D into
+// (J, K) and with D2 into (J2, K2).  Ends with a barrier: the LDS it uses is
+// free for the next task.
 template <typename R, int NA, int NB, int NC, int ND, int NAP, int NDEN>
-__global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks,
-                                                 double lo, double hi, HeavyProg P) {
+__device__ __forceinline__ void heavy_task(const JKArgs& a, const BoysTables& tab, const HeavyProg& P, int bra,
+                                           int ket0, int nket, double lo, double hi, R* X, R* S, double* Y,
+                                           double* blk) {
     using B = HeavyBlocks<NA, NB, NC, ND>;
-    extern __shared__ __align__(16) unsigned char smem[];
-    R* X = reinterpret_cast<R*>(smem);
-    R* S = X + P.nslots;
     constexpr int NOUT = NA * NB * NC * ND;
-    double* Y = reinterpret_cast<double*>(smem + heavy_y_offset<R>(P.nslots));
-    double* blk = Y + NOUT;   // NDEN sets of HeavyBlocks
-    const int w = blockIdx.x;
-    if (w >= ntasks) return;
     const int lane = threadIdx.x;
-    const Task tk = tasks[w];
     const int nsh = a.nsh, n = a.n;
-    const int2 sab = a.pair_sh[tk.bra];
+    const int2 sab = a.pair_sh[bra];
     const int sa = sab.x, sb = sab.y;
     const double* O = a.sh_O;
     const double ABx = O[3 * sa] - O[3 * sb], ABy = O[3 * sa + 1] - O[3 * sb + 1], ABz = O[3 * sa + 2] - O[3 * sb + 2];
@@ -370,17 +365,17 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
         accw[q][0] = t < P.nacc ? P.acc[2 * t] : -1;
         accw[q][1] = t < P.nacc ? P.acc[2 * t + 1] : 0;
     }
-    for (int kk = 0; kk < tk.nket; ++kk) {
-        const int ket = tk.ket0 + kk;
+    for (int kk = 0; kk < nket; ++kk) {
+        const int ket = ket0 + kk;
         const int2 scd = a.pair_sh[ket];
         const int sc = scd.x, sd = scd.y;
         const double* Dn = a.Dn;
         const double dn = fmax(fmax(Dn[sa * nsh + sb], Dn[sc * nsh + sd]),
                                fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
-        const double bound = dn * a.pair_Q[tk.bra] * a.pair_Q[ket];
+        const double bound = dn * a.pair_Q[bra] * a.pair_Q[ket];
         if (!(bound >= lo && bound < hi)) continue;
         if (a.counters && lane == 0) {
-            const unsigned long long npq = (unsigned long long)a.pair_np[tk.bra] * a.pair_np[ket];
+            const unsigned long long npq = (unsigned long long)a.pair_np[bra] * a.pair_np[ket];
             atomicAdd(&a.counters[3 * a.combo], 1ull);
             atomicAdd(&a.counters[3 * a.combo + 1], npq);
             atomicAdd(&a.counters[3 * a.combo + 2], npq);   // no primitive screening here
@@ -392,16 +387,49 @@ __global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, const
 #pragma unroll
         for (int d = 0; d < NDEN; ++d)
             heavy_stage_ket<NA, NB, NC, ND>(d ? a.D2 : a.D, blk + d * B::SIZE, fa, fb, fc, fd, n, lane);
-        heavy_quartet<R, NAP>(P, a, tab, tk.bra, ket, ABx, ABy, ABz, CDx, CDy, CDz, X, S, accw);
+        heavy_quartet<R, NAP>(P, a, tab, bra, ket, ABx, ABy, ABz, CDx, CDy, CDz, X, S, accw);
         for (int o = lane; o < NOUT; o += 64) Y[o] = double(X[P.out[o]]);
         __syncthreads();
-        const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (tk.bra == ket ? 1.0 : 2.0);
+        const double deg = (sa == sb ? 1.0 : 2.0) * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
         const double f = deg * a.scale;
 #pragma unroll
         for (int d = 0; d < NDEN; ++d)
             heavy_digest<NA, NB, NC, ND>(Y, blk + d * B::SIZE, d ? a.J2 : a.J, d ? a.K2 : a.K, f, fa, fb, fc, fd, n,
                                          lane);
         __syncthreads();   // Y and the density blocks consumed before the next quartet
+    }
+    __syncthreads();       // the bra's density block and coefficients consumed before the next task
+}
+
+// Heavy-class J/K over the device work list: one work item per workgroup, a
+// bra and a.block_tasks chunks of JK_HEAVY_KETS kets (FP64 head or FP32
+// tail by a.tail).  With a.blk_total set the grid is a bound on the item
+// count and workgroups past the count exit at once.  The chunks of an item
+// that fall in one primitive-count group form one contiguous ket range.
+template <typename R, int NA, int NB, int NC, int ND, int NAP, int NDEN>
+__global__ __launch_bounds__(64) void k_jk_heavy(JKArgs a, BoysTables tab, double lo, double hi, HeavyProg P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    R* X = reinterpret_cast<R*>(smem);
+    R* S = X + P.nslots;
+    double* Y = reinterpret_cast<double*>(smem + heavy_y_offset<R>(P.nslots));
+    double* blk = Y + NA * NB * NC * ND;   // NDEN sets of HeavyBlocks
+    const int w = blockIdx.x;
+    if (a.blk_total && w >= *a.blk_total) return;
+    const int bl = a.blk_bra[w];
+    const int c0 = (w - a.blk_off[bl]) * a.block_tasks, c1 = c0 + a.block_tasks;
+    const int bra = a.bra0 + bl;
+    const int2* cut = a.cut + bl * JK_MAXG;   // (mid, end) per group
+    int cb = 0;                               // chunks of the groups before g
+    for (int g = 0; g < a.ngrp && cb < c1; ++g) {
+        const int2 me = cut[g];
+        const int k0 = a.tail ? me.x : a.gbeg[g], k1 = a.tail ? me.y : me.x;
+        const int nch = (k1 - k0 + JK_HEAVY_KETS - 1) / JK_HEAVY_KETS;
+        const int s0 = max(c0, cb), s1 = min(c1, cb + nch);
+        if (s0 < s1) {
+            const int j0 = k0 + (s0 - cb) * JK_HEAVY_KETS, j1 = min(k1, k0 + (s1 - cb) * JK_HEAVY_KETS);
+            heavy_task<R, NA, NB, NC, ND, NAP, NDEN>(a, tab, P, bra, j0, j1 - j0, lo, hi, X, S, Y, blk);
+        }
+        cb += nch;
     }
 }
 
@@ -536,25 +564,26 @@ bool heavy_has(int la, int lb, int lc, int ld) {
     return false;
 }
 
+// nitems: the grid (the item count, or a bound on it when a.blk_total is set)
 template <int NA, int NB, int NC, int ND, int NAP, int NDEN>
-static void launch_heavy_dims(const HeavyProg& P, const JKArgs& a, const BoysTables& tab, const Task* tasks,
-                              int ntasks, int mode, hipStream_t st) {
+static void launch_heavy_dims(const HeavyProg& P, const JKArgs& a, const BoysTables& tab, int nitems, int mode,
+                              hipStream_t st) {
     const double inf = __builtin_huge_val();
     const size_t extra = sizeof(double) * (NA * NB * NC * ND + NDEN * HeavyBlocks<NA, NB, NC, ND>::SIZE);
     if (mode == 2) {
         const size_t sm = heavy_y_offset<float>(P.nslots) + extra;
-        hipLaunchKernelGGL((k_jk_heavy<float, NA, NB, NC, ND, NAP, NDEN>), dim3(ntasks), dim3(64), sm, st, a, tab,
-                           tasks, ntasks, a.precision, inf, P);
+        hipLaunchKernelGGL((k_jk_heavy<float, NA, NB, NC, ND, NAP, NDEN>), dim3(nitems), dim3(64), sm, st, a, tab,
+                           a.precision, inf, P);
     } else {
         const size_t sm = heavy_y_offset<double>(P.nslots) + extra;
-        hipLaunchKernelGGL((k_jk_heavy<double, NA, NB, NC, ND, NAP, NDEN>), dim3(ntasks), dim3(64), sm, st, a, tab,
-                           tasks, ntasks, a.precision, inf, P);
+        hipLaunchKernelGGL((k_jk_heavy<double, NA, NB, NC, ND, NAP, NDEN>), dim3(nitems), dim3(64), sm, st, a, tab,
+                           a.precision, inf, P);
     }
 }
 
 template <int LA, int LB, int LC, int LD>
-static void launch_heavy_class(int pure, const HeavyProg& P, const JKArgs& a, const BoysTables& tab,
-                               const Task* tasks, int ntasks, int mode, hipStream_t st) {
+static void launch_heavy_class(int pure, const HeavyProg& P, const JKArgs& a, const BoysTables& tab, int nitems,
+                               int mode, hipStream_t st) {
     constexpr int PA = 2 * LA + 1, PB = 2 * LB + 1, PC = 2 * LC + 1, PD = 2 * LD + 1;
     constexpr int CA = (LA + 1) * (LA + 2) / 2, CB = (LB + 1) * (LB + 2) / 2, CC = (LC + 1) * (LC + 2) / 2,
                   CD = (LD + 1) * (LD + 2) / 2;
@@ -566,26 +595,26 @@ static void launch_heavy_class(int pure, const HeavyProg& P, const JKArgs& a, co
     };
     constexpr int NAP = (ncart_sum(LA, LA + LB) * ncart_sum(LC, LC + LD) + 63) / 64;
     if (a.nden > 1) {
-        if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP, 2>(P, a, tab, tasks, ntasks, mode, st);
-        else launch_heavy_dims<CA, CB, CC, CD, NAP, 2>(P, a, tab, tasks, ntasks, mode, st);
+        if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP, 2>(P, a, tab, nitems, mode, st);
+        else launch_heavy_dims<CA, CB, CC, CD, NAP, 2>(P, a, tab, nitems, mode, st);
     } else {
-        if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP, 1>(P, a, tab, tasks, ntasks, mode, st);
-        else launch_heavy_dims<CA, CB, CC, CD, NAP, 1>(P, a, tab, tasks, ntasks, mode, st);
+        if (pure) launch_heavy_dims<PA, PB, PC, PD, NAP, 1>(P, a, tab, nitems, mode, st);
+        else launch_heavy_dims<CA, CB, CC, CD, NAP, 1>(P, a, tab, nitems, mode, st);
     }
 }
 
 hipError_t launch_jk_heavy(int device, int la, int lb, int lc, int ld, int pure, const JKArgs& a,
-                           const BoysTables& tab, const Task* tasks, int ntasks, int mode, hipStream_t st) {
-    if (ntasks <= 0) return hipSuccess;
+                           const BoysTables& tab, int nitems, int mode, hipStream_t st) {
+    if (nitems <= 0) return hipSuccess;
     HeavyProg P;
     if (!find_prog(device, la, lb, lc, ld, pure, P)) return hipErrorInvalidValue;
     const int key = ((la * 3 + lb) * 3 + lc) * 3 + ld;
     switch (key) {
-        case ((2 * 3 + 1) * 3 + 2) * 3 + 1: launch_heavy_class<2, 1, 2, 1>(pure, P, a, tab, tasks, ntasks, mode, st); break;
-        case ((2 * 3 + 2) * 3 + 1) * 3 + 1: launch_heavy_class<2, 2, 1, 1>(pure, P, a, tab, tasks, ntasks, mode, st); break;
-        case ((2 * 3 + 2) * 3 + 2) * 3 + 0: launch_heavy_class<2, 2, 2, 0>(pure, P, a, tab, tasks, ntasks, mode, st); break;
-        case ((2 * 3 + 2) * 3 + 2) * 3 + 1: launch_heavy_class<2, 2, 2, 1>(pure, P, a, tab, tasks, ntasks, mode, st); break;
-        case ((2 * 3 + 2) * 3 + 2) * 3 + 2: launch_heavy_class<2, 2, 2, 2>(pure, P, a, tab, tasks, ntasks, mode, st); break;
+        case ((2 * 3 + 1) * 3 + 2) * 3 + 1: launch_heavy_class<2, 1, 2, 1>(pure, P, a, tab, nitems, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 1) * 3 + 1: launch_heavy_class<2, 2, 1, 1>(pure, P, a, tab, nitems, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 2) * 3 + 0: launch_heavy_class<2, 2, 2, 0>(pure, P, a, tab, nitems, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 2) * 3 + 1: launch_heavy_class<2, 2, 2, 1>(pure, P, a, tab, nitems, mode, st); break;
+        case ((2 * 3 + 2) * 3 + 2) * 3 + 2: launch_heavy_class<2, 2, 2, 2>(pure, P, a, tab, nitems, mode, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

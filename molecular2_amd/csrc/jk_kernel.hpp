@@ -3,9 +3,12 @@
 //
 // Work unit: one chunk = one bra shell pair x up to 64 consecutive ket shell
 // pairs of one class and one primitive-count group (so every lane runs the
-// same primitive loop); one lane per ket pair.  A workgroup owns one bra
-// pair and up to JK_BLOCK_TASKS of its chunks, which its waves take in turn;
-// the chunks come from the device work list of tasks.hip.
+// same primitive loop); one lane per ket pair.  A workgroup owns one work
+// item: a bra pair and up to a.block_tasks of its chunks, which its waves
+// take in turn; the items come from the device work list of tasks.hip.  The
+// grid is either the list's item count (read back by the host) or a bound on
+// it that depends on the basis only (no host round trip): workgroups past
+// the count the list kernels wrote exit at once.
 // Each lane applies the exact screening of Fock::formJKdirect
 // (src/fockbuild.cpp:98-115): max of the six shell-block norms of D times
 // Q_ab Q_cd against `precision`, computes its contracted quartet with the
@@ -180,17 +183,16 @@ struct JKOccupancy {
 // Dynamic LDS (when a.k_lds): the (NA + NB) x n FP64 K rows of shells a and b.
 // NDEN = 2 (unrestricted build): every quartet is digested with D and with D2
 // into (J, K) and (J2, K2); the LDS then holds the K rows of both.
+// One work item of k_jk: bra pair and up to a.block_tasks of its ket chunks.
 template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN>
-__global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
-                             (JKOccupancy<LA, LB, LC, LD, R>::min_per_eu)) void k_jk(JKArgs a, BoysTables tab, double lo,
-                                                                                 double hi) {
+__device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, double lo, double hi, int witem) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NC = NFun<LC, PURE>::value, ND = NFun<LD, PURE>::value;
     constexpr int W = JKOccupancy<LA, LB, LC, LD, R>::waves;
     typedef const __attribute__((address_space(4))) int cint;
-    // workgroup -> bra pair and its ket segments (written by the list kernels)
-    const int bl = ((cint*)a.blk_bra)[blockIdx.x];
-    const int lb = blockIdx.x - ((cint*)a.blk_off)[bl];
+    // work item -> bra pair and its ket segments (written by the list kernels)
+    const int bl = ((cint*)a.blk_bra)[witem];
+    const int lb = witem - ((cint*)a.blk_off)[bl];
     const int bra = a.bra0 + bl;
     cint* cut = (cint*)a.cut + 2 * bl * JK_MAXG;   // (mid, end) per group
     int nch = 0;
@@ -198,7 +200,7 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
         const int mid = cut[2 * g], end = cut[2 * g + 1];
         nch += ((a.tail ? end - mid : mid - a.gbeg[g]) + 63) >> 6;
     }
-    const int c0 = lb * JK_BLOCK_TASKS, c1 = min(c0 + JK_BLOCK_TASKS, nch);
+    const int c0 = lb * a.block_tasks, c1 = min(c0 + a.block_tasks, nch);
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
     const int2 sab = a.pair_sh[bra];
@@ -312,6 +314,14 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
     }
 }
 
+template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN>
+__global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
+                             (JKOccupancy<LA, LB, LC, LD, R>::min_per_eu)) void k_jk(JKArgs a, BoysTables tab, double lo,
+                                                                                 double hi) {
+    if (a.blk_total && (int)blockIdx.x >= *a.blk_total) return;   // grid sized from a bound: past the total
+    jk_item<LA, LB, LC, LD, R, PURE, NDEN>(a, tab, lo, hi, blockIdx.x);
+}
+
 // (ab|ss) with s-s family kets: one lane per ket family pair.  Its primitive
 // pairs are shared by up to FAM_MAXMEM contracted member pairs (cd); each
 // member keeps the exact screening of src/fockbuild.cpp:98-115 and, for the
@@ -339,15 +349,14 @@ struct JKWavesFam {
 };
 
 template <int LA, int LB, typename R, bool PURE, int NDEN>
-__global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
-                             (JKWavesFam<LA, LB, R>::min_per_eu)) void k_jk_fam(JKArgs a, BoysTables tab, double lo,
-                                                                               double hi) {
+__device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& tab, double lo, double hi,
+                                            int witem) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NCART = EriClass<LA, LB, 0, 0>::NCART;
     constexpr int W = JKWavesFam<LA, LB, R>::value;
     typedef const __attribute__((address_space(4))) int cint;
-    const int bl = ((cint*)a.blk_bra)[blockIdx.x];
-    const int lb = blockIdx.x - ((cint*)a.blk_off)[bl];
+    const int bl = ((cint*)a.blk_bra)[witem];
+    const int lb = witem - ((cint*)a.blk_off)[bl];
     const int bra = a.bra0 + bl;
     cint* cut = (cint*)a.cut + 2 * bl * JK_MAXG;
     int nch = 0;
@@ -355,7 +364,7 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
         const int mid = cut[2 * g], end = cut[2 * g + 1];
         nch += ((a.tail ? end - mid : mid - a.gbeg[g]) + 63) >> 6;
     }
-    const int c0 = lb * JK_BLOCK_TASKS, c1 = min(c0 + JK_BLOCK_TASKS, nch);
+    const int c0 = lb * a.block_tasks, c1 = min(c0 + a.block_tasks, nch);
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
     const int2 sab = a.pair_sh[bra];
@@ -488,6 +497,14 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
     }
 }
 
+template <int LA, int LB, typename R, bool PURE, int NDEN>
+__global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
+                             (JKWavesFam<LA, LB, R>::min_per_eu)) void k_jk_fam(JKArgs a, BoysTables tab, double lo,
+                                                                               double hi) {
+    if (a.blk_total && (int)blockIdx.x >= *a.blk_total) return;
+    jk_fam_item<LA, LB, R, PURE, NDEN>(a, tab, lo, hi, blockIdx.x);
+}
+
 // Schwarz: Q(pair) = sqrt(max_ab |(ab|ab)|), one lane per pair
 // (IntegralEngine::compute_schwarz_ints, src/integrals.cpp:471-513).
 template <int LA, int LB, bool PURE>
@@ -520,14 +537,16 @@ void launch_jk_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode,
     const size_t kbytes = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
     a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
     const double inf = __builtin_huge_val();
-    const dim3 grid(nblocks);
     const size_t lds = a.k_lds ? kbytes : 0;
-    if (mode == 2)
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, float, PURE, NDEN>), grid,
-                           dim3(64 * JKOccupancy<LA, LB, LC, LD, float>::waves), lds, st, a, tab, a.precision, inf);
-    else
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, double, PURE, NDEN>), grid,
-                           dim3(64 * JKOccupancy<LA, LB, LC, LD, double>::waves), lds, st, a, tab, a.precision, inf);
+    if (mode == 2) {
+        auto k = k_jk<LA, LB, LC, LD, float, PURE, NDEN>;
+        const int th = 64 * JKOccupancy<LA, LB, LC, LD, float>::waves;
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+    } else {
+        auto k = k_jk<LA, LB, LC, LD, double, PURE, NDEN>;
+        const int th = 64 * JKOccupancy<LA, LB, LC, LD, double>::waves;
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+    }
 }
 
 template <int LA, int LB, int LC, int LD, bool PURE>
@@ -545,12 +564,15 @@ void launch_jk_fam_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int m
     a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
     const double inf = __builtin_huge_val();
     const size_t lds = a.k_lds ? kbytes : 0;
-    if (mode == 2)
-        hipLaunchKernelGGL((k_jk_fam<LA, LB, float, PURE, NDEN>), dim3(nblocks),
-                           dim3(64 * JKWavesFam<LA, LB, float>::value), lds, st, a, tab, a.precision, inf);
-    else
-        hipLaunchKernelGGL((k_jk_fam<LA, LB, double, PURE, NDEN>), dim3(nblocks),
-                           dim3(64 * JKWavesFam<LA, LB, double>::value), lds, st, a, tab, a.precision, inf);
+    if (mode == 2) {
+        auto k = k_jk_fam<LA, LB, float, PURE, NDEN>;
+        const int th = 64 * JKWavesFam<LA, LB, float>::value;
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+    } else {
+        auto k = k_jk_fam<LA, LB, double, PURE, NDEN>;
+        const int th = 64 * JKWavesFam<LA, LB, double>::value;
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+    }
 }
 
 template <int LA, int LB, bool PURE>

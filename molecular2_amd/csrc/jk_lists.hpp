@@ -1,4 +1,4 @@
-// Device work lists of one direct J/K class combination (tasks.hip).
+// Device work lists of the direct J/K class combinations (tasks.hip).
 #pragma once
 #include "m2_common.hpp"
 
@@ -13,6 +13,10 @@ struct JKList {
     int* off;                 // [2][nbra + 1] workgroups per bra -> exclusive offsets (FP64 head, FP32 tail)
     int* bra_of;              // [2][cap] workgroup -> bra (local index)
     int cap;                  // capacity of each half of bra_of
+    int kc;                   // kets per chunk: 64 (register classes) or JK_HEAVY_KETS (heavy classes)
+    int bt;                   // ket chunks per work item
+    int bt_min;               // the smallest bt the capacity holds
+    int bound;                // most work items any density gives at this bt (per precision)
     const double* Qk;         // Schwarz bounds the ket ranges are sorted by (pairs or family pairs)
     const int* bra_fam;       // same class with family kets: bra pair -> its family pair (else null)
 };
@@ -28,13 +32,16 @@ struct GradTaskArgs {
     double precision, dmax;
     int part;                 // 0: kets with Q_i Q_j max|D| >= split (FP64), 1: the rest (FP32)
     double split;             // = precision when nothing runs in FP32
+    const double* dmax_dev;   // device max|D| (read instead of dmax when set)
 };
 void launch_grad_task_count(const GradTaskArgs& g, int* counts, hipStream_t st);
 void launch_scan_excl(const int* in, int* out, int n, hipStream_t st);
 void launch_grad_task_fill(const GradTaskArgs& g, const int* offs, Task* out, hipStream_t st);
 
 void launch_dmax(const double* Dn, int m, double* out, hipStream_t st);
-void launch_task_lists(const JKList& c, const double* Q, const double* dmax, double precision, double fp32_below,
-                       int mixed, int* tot, hipStream_t st);
+// the lists of all `ncombo` combinations (device array `lists`; max_nbra: the
+// largest nbra); totals to tot[2 * combo + mode]
+void launch_task_lists(const JKList* lists, int ncombo, int max_nbra, const double* Q, const double* dmax,
+                       double precision, double fp32_below, int mixed, int* tot, hipStream_t st);
 
 }  // namespace m2

@@ -36,10 +36,16 @@ constexpr int BOYS_NGRID = 601;
 constexpr double BOYS_STEP = 0.1;
 constexpr double BOYS_TMAX = 60.0;
 
-// ket chunks (of <= 64 kets) per workgroup of the direct J/K class kernels,
-// primitive-count groups per pair class, and the LDS budget of the K-row
-// accumulators
+// ket chunks (of <= 64 kets) per work item of the direct J/K register-class
+// kernels (default, and the smallest value the work-list capacity allows; the
+// autotuner picks per class combination); kets per chunk of the table-driven
+// heavy classes (one wavefront walks them in turn; one chunk per work item by
+// default); primitive-count groups per pair class, and the LDS budget of the
+// K-row accumulators
 constexpr int JK_BLOCK_TASKS = 64;
+constexpr int JK_BLOCK_TASKS_MIN = 16;
+constexpr int JK_HEAVY_KETS = 4;
+constexpr long long JK_LIST_MAX_ITEMS = 1LL << 24;   // work items per list and precision (64 MiB)
 // gradient program classes: a host task holds up to GRAD_PROG_SPLIT *
 // GRAD_PROG_KETS kets; GRAD_PROG_SPLIT workgroups take GRAD_PROG_KETS each
 constexpr int GRAD_PROG_KETS = 4, GRAD_PROG_SPLIT = 8;
@@ -72,6 +78,8 @@ struct JKArgs {
     const int2* cut;         // [bra][JK_MAXG] (mid, end)
     const int* blk_off;      // [bra] first workgroup of the bra
     const int* blk_bra;      // [workgroup] bra (index local to the class)
+    const int* blk_total;    // device count of work items (grid sized from a bound), or null (grid = count)
+    int block_tasks;         // ket chunks per work item
     int bra0;                // first pair of the bra class
     int ngrp;                // ket groups
     int tail;                // 0: FP64 head [gbeg, mid) of each group; 1: FP32 tail [mid, end)

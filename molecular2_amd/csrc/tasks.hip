@@ -1,4 +1,4 @@
-// Device-side work lists of the direct J/K register-class kernels.
+// Device-side work lists of the direct J/K class kernels.
 //
 // For one class combination (bra pair class X, ket pair class Y <= X) and
 // every bra pair i of class X, the kets of class Y are kept in
@@ -12,11 +12,16 @@
 // groups hold family pairs, sorted by the largest member Q; the triangle is
 // cut at the bra's own family and finished per member by the kernel.
 //
+// Every class combination's list is built by the same three launches (the
+// combination is a grid dimension):
 //   k_task_count: per bra, the (mid, end) cut of every group (two binary
-//                 searches) and the number of workgroups per precision;
-//   k_task_scan:  exclusive prefix sums of the workgroup counts;
-//   k_task_fill:  workgroup -> bra map.
-// The class kernels then take their ket chunks of 64 straight from the cuts.
+//                 searches) and the number of work items per precision;
+//   k_task_scan:  exclusive prefix sums of the work-item counts and their
+//                 totals;
+//   k_task_fill:  work item -> bra map.
+// A work item is a bra and up to bt of its ket chunks (64 kets per chunk for
+// the register classes, JK_HEAVY_KETS for the table-driven heavy classes);
+// the class kernels take their ket ranges straight from the cuts.
 #include "jk_lists.hpp"
 
 namespace m2 {
@@ -46,9 +51,11 @@ __global__ __launch_bounds__(1024) void k_dmax(const double* __restrict__ Dn, in
     }
 }
 
-__global__ __launch_bounds__(256) void k_task_count(JKList c, const double* __restrict__ Q,
+// blockIdx.y: the class combination (its list in `lists`)
+__global__ __launch_bounds__(256) void k_task_count(const JKList* __restrict__ lists, const double* __restrict__ Q,
                                                     const double* __restrict__ dmax, double precision,
                                                     double fp32_below, int mixed) {
+    const JKList& c = lists[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.nbra) return;
     const int bra = c.bra0 + i;
@@ -72,16 +79,22 @@ __global__ __launch_bounds__(256) void k_task_count(JKList c, const double* __re
             mid = max(min(mid, end), gb);
         }
         c.cut[i * JK_MAXG + g] = make_int2(mid, end);
-        ch64 += (mid - gb + 63) >> 6;
-        ch32 += (end - mid + 63) >> 6;
+        ch64 += (mid - gb + c.kc - 1) / c.kc;
+        ch32 += (end - mid + c.kc - 1) / c.kc;
     }
-    c.off[i] = (ch64 + JK_BLOCK_TASKS - 1) / JK_BLOCK_TASKS;
-    c.off[c.nbra + 1 + i] = (ch32 + JK_BLOCK_TASKS - 1) / JK_BLOCK_TASKS;
+    c.off[i] = (ch64 + c.bt - 1) / c.bt;
+    c.off[c.nbra + 1 + i] = (ch32 + c.bt - 1) / c.bt;
 }
 
 // exclusive scan in place of off[mode * (nbra + 1) + 0 .. nbra - 1]; the total
-// goes to off[mode * (nbra + 1) + nbra] and tot[mode].  blockIdx.x = mode.
-__global__ __launch_bounds__(1024) void k_task_scan(JKList c, int* __restrict__ tot) {
+// goes to off[mode * (nbra + 1) + nbra] and tot[2 * combo + mode].
+// blockIdx.x = mode, blockIdx.y = combo.  The list capacity holds every
+// count the basis allows (nbra x the largest chunk count of a bra / bt_min),
+// so the clamp to it never binds; it keeps the class kernels in bounds by
+// construction.
+__global__ __launch_bounds__(1024) void k_task_scan(const JKList* __restrict__ lists, int* __restrict__ tot) {
+    const JKList& c = lists[blockIdx.y];
+    if (c.nbra <= 0) return;
     __shared__ int s_w[16];
     __shared__ int s_carry;
     int* a = c.off + blockIdx.x * (c.nbra + 1);
@@ -115,13 +128,14 @@ __global__ __launch_bounds__(1024) void k_task_scan(JKList c, int* __restrict__ 
     }
     if (threadIdx.x == 0) {
         a[c.nbra] = s_carry;
-        tot[blockIdx.x] = s_carry;
+        tot[2 * blockIdx.y + blockIdx.x] = min(s_carry, c.cap);
     }
 }
 
-// workgroup -> bra (local index); blockIdx.y = mode.  Entries beyond the
-// capacity are not written (the host checks the totals against it).
-__global__ __launch_bounds__(256) void k_task_fill(JKList c) {
+// work item -> bra (local index); blockIdx.y = mode, blockIdx.z = combo.
+// Entries beyond the capacity are not written.
+__global__ __launch_bounds__(256) void k_task_fill(const JKList* __restrict__ lists) {
+    const JKList& c = lists[blockIdx.z];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.nbra) return;
     const int mode = blockIdx.y;
@@ -156,10 +170,11 @@ __global__ __launch_bounds__(256) void k_grad_task_count(GradTaskArgs g, int* __
     if (t >= g.nbra) return;
     const int i = g.x0 + t;
     const double qi = g.Q[i];
+    const double dmax = g.dmax_dev ? *g.dmax_dev : g.dmax;
     int n = 0;
-    if (qi > 0.0 && g.dmax > 0.0) {
+    if (qi > 0.0 && dmax > 0.0) {
         for (int k = 0; k < g.ngrp; ++k) {
-            const int2 r = grad_group_range(g, i, qi * g.dmax, k);
+            const int2 r = grad_group_range(g, i, qi * dmax, k);
             if (r.y > r.x) n += (r.y - r.x + g.chunk - 1) / g.chunk;
         }
     }
@@ -172,10 +187,11 @@ __global__ __launch_bounds__(256) void k_grad_task_fill(GradTaskArgs g, const in
     if (t >= g.nbra) return;
     const int i = g.x0 + t;
     const double qi = g.Q[i];
-    if (!(qi > 0.0 && g.dmax > 0.0)) return;
+    const double dmax = g.dmax_dev ? *g.dmax_dev : g.dmax;
+    if (!(qi > 0.0 && dmax > 0.0)) return;
     int w = offs[t];
     for (int k = 0; k < g.ngrp; ++k) {
-        const int2 r = grad_group_range(g, i, qi * g.dmax, k);
+        const int2 r = grad_group_range(g, i, qi * dmax, k);
         for (int j = r.x; j < r.y; j += g.chunk) out[w++] = Task{i, j, min(g.chunk, r.y - j), 0};
     }
 }
@@ -220,13 +236,14 @@ void launch_dmax(const double* Dn, int m, double* out, hipStream_t st) {
     hipLaunchKernelGGL(k_dmax, dim3(1), dim3(1024), 0, st, Dn, m, out);
 }
 
-void launch_task_lists(const JKList& c, const double* Q, const double* dmax, double precision, double fp32_below,
-                       int mixed, int* tot, hipStream_t st) {
-    if (c.nbra <= 0) return;
-    const int nb = (c.nbra + 255) / 256;
-    hipLaunchKernelGGL(k_task_count, dim3(nb), dim3(256), 0, st, c, Q, dmax, precision, fp32_below, mixed);
-    hipLaunchKernelGGL(k_task_scan, dim3(2), dim3(1024), 0, st, c, tot);
-    hipLaunchKernelGGL(k_task_fill, dim3(nb, 2), dim3(256), 0, st, c);
+void launch_task_lists(const JKList* lists, int ncombo, int max_nbra, const double* Q, const double* dmax,
+                       double precision, double fp32_below, int mixed, int* tot, hipStream_t st) {
+    if (ncombo <= 0 || max_nbra <= 0) return;
+    const int nb = (max_nbra + 255) / 256;
+    hipLaunchKernelGGL(k_task_count, dim3(nb, ncombo), dim3(256), 0, st, lists, Q, dmax, precision, fp32_below,
+                       mixed);
+    hipLaunchKernelGGL(k_task_scan, dim3(2, ncombo), dim3(1024), 0, st, lists, tot);
+    hipLaunchKernelGGL(k_task_fill, dim3(nb, 2, ncombo), dim3(256), 0, st, lists);
 }
 
 }  // namespace m2

@@ -150,6 +150,15 @@ def lib():
         L.m2_grad_ex.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint, _dp]
         for name in ("m2_grad", "m2_df_setup", "m2_jk_df", "m2_jk_df_device", "m2_df_timing", "m2_eigh", "m2_gemm", "m2_rhf"):
             getattr(L, name).restype = C.c_int
+        L.m2_set_jk_launch.argtypes = [vp, C.c_int]
+        L.m2_set_jk_graphs.argtypes = [vp, C.c_int]
+        L.m2_jk_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _ip]
+        L.m2_set_jk_block_tasks.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.m2_get_jk_launch.argtypes = [vp, _ip, _ip, _ip]
+        L.m2_jk_autotune.argtypes = [vp, vp, C.c_double, C.c_uint, C.c_int, _dp, _dp]
+        for name in ("m2_set_jk_launch", "m2_set_jk_graphs", "m2_jk_graph_stats", "m2_set_jk_block_tasks",
+                     "m2_get_jk_launch", "m2_jk_autotune"):
+            getattr(L, name).restype = C.c_int
         for name in ("m2_set_profiling", "m2_get_profile", "m2_reset_profile", "m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
                      "m2_shellblock_norm", "m2_one_body", "m2_jk_direct", "m2_jk_direct_device",
                      "m2_jk_direct_device_noalloc", "m2_get_profile_computed", "m2_set_prim_screening",
@@ -386,6 +395,43 @@ class Engine:
         self._chk(self.L.m2_get_profile_computed(self.h, cpq.ctypes.data_as(P)))
         return {(c // 6, c % 6): (ms[c], int(la[c]), int(q[c]), int(pq[c]), int(cpq[c])) for c in range(36)
                 if la[c] > 0}
+
+    def set_jk_launch(self, host_totals=False):
+        """grids of the J/K class kernels: bounds from the basis with no host
+        round trip (default), or the work-item totals read back first"""
+        self._chk(self.L.m2_set_jk_launch(self.h, int(bool(host_totals))))
+
+    def set_jk_graphs(self, on=True):
+        """capture each J/K build key once into a HIP graph and replay it"""
+        self._chk(self.L.m2_set_jk_graphs(self.h, int(bool(on))))
+
+    def jk_graph_stats(self):
+        """(captures, replays, graphs held)"""
+        a, b, c = C.c_longlong(), C.c_longlong(), C.c_int()
+        self._chk(self.L.m2_jk_graph_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_jk_block_tasks(self, X, Y, bt):
+        """ket chunks per work item of class combination (X | Y); 0: default"""
+        self._chk(self.L.m2_set_jk_block_tasks(self.h, int(X), int(Y), int(bt)))
+
+    def jk_launch(self):
+        """(host_totals, {(X, Y): (chunks per item, grid bound)}) of the
+        combinations with bra pairs"""
+        ht = C.c_int()
+        bt = np.zeros(36, np.int32)
+        gb = np.zeros(36, np.int32)
+        self._chk(self.L.m2_get_jk_launch(self.h, C.byref(ht), bt.ctypes.data_as(_ip), gb.ctypes.data_as(_ip)))
+        return bool(ht.value), {(c // 6, c % 6): (int(bt[c]), int(gb[c])) for c in range(36) if bt[c] > 0}
+
+    def jk_autotune(self, dD_ptr, precision=1e-12, mixed=False, repeats=3):
+        """times the chunks-per-item candidates on the device density at
+        dD_ptr, keeps the fastest per class combination; returns the mean
+        build time (ms) before and after"""
+        b, a = C.c_double(), C.c_double()
+        self._chk(self.L.m2_jk_autotune(self.h, C.c_void_p(dD_ptr), precision, M2_MIXED if mixed else M2_FP64,
+                                        int(repeats), C.byref(b), C.byref(a)))
+        return b.value, a.value
 
     def set_prim_screening(self, relative):
         """primitive quartets below relative * precision (bound q_ab q_cd
@@ -628,6 +674,8 @@ class Engine:
         return G
 
     def last_stats(self):
+        """(FP64 quartets, FP32 quartets, work items) of the last J/K build;
+        the quartet counts are -1 (not counted)"""
         a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
         self._chk(self.L.m2_last_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
