@@ -206,6 +206,38 @@ m2_status m2_jk_df_device(m2_ctx* ctx, const double* dCocc, int nocc, int fp32, 
  * K = X^T X, of the J part, and their sum */
 m2_status m2_df_timing(m2_ctx* ctx, double* ms4);
 
+/* One fragment of the fragment-local DF-JK (FragmentInfo, inc/fock.hpp:74-95):
+ * its orbital functions [start, start + nbfs) and shells (nshells, contiguous
+ * in basis order), auxiliary functions [auxstart, auxstart + naux) and shells
+ * (ndfshells), occupied orbitals occ (its LMOs are consecutive columns of
+ * Cocc, fragments in order), radius and centre of mass (bohr). */
+typedef struct {
+    int occ, nbfs, naux, start, auxstart, nshells, ndfshells;
+    double radius;
+    double com[3];
+} m2_fragment;
+/* Fock::compute_2body_fock_df_local (src/fockbuild.cpp:794-984), the ALMO
+ * SCF's Fock build: Coulomb J (= 4 sum_K (xy|K) [V^-1 Pk]_K, Pk the kept
+ * blocks of (xy|K) Pt) and exchange K (per LMO, fitted in its own fit
+ * domain and added over its AO domain).  The first call after
+ * m2_df_local_reset or a basis change builds the screened fragment blocks
+ * of (xy|K) from Pt and the Schwarz bounds (build_blocked_eris, :985-1047;
+ * block kept when 2 max|Q| max|Pt| sqrt(max|V_kk|) / r > thrint, or r < 1)
+ * and the LMO, AO and fit domains of Cocc sigmainv^(1/2) (build_domains,
+ * :662-792, with the reference's mo_thresh, fit_thresh and r_thresh); later
+ * calls reuse them, as the reference does once its xyK is set.  Fragments
+ * tile both bases in order.  Cocc: n x nocc column-major; sigmainv:
+ * nocc x nocc; Pt, J, K: n x n (host).  Unlike the reference, Cocc is not
+ * overwritten by Cocc sigmainv^(1/2). */
+m2_status m2_jk_df_local(m2_ctx* ctx, int nfrag, const m2_fragment* frags, double mo_thresh, double fit_thresh,
+                         double r_thresh, double thrint, const double* Cocc, int nocc, const double* sigmainv,
+                         const double* Pt, double* J, double* K);
+/* drops the fragment-local blocks and domains (the next call rebuilds them) */
+m2_status m2_df_local_reset(m2_ctx* ctx);
+/* the domains of the last set-up: which = 0 (LMO), 1 (AO), 2 (fit); per LMO
+ * i, counts[i] fragment centres in centres[i * nfrag ...] (rest -1) */
+m2_status m2_df_local_domains(m2_ctx* ctx, int which, int* centres, int* counts);
+
 /* RHF nuclear gradient dE/dR (natom x 3, Eh/bohr): Fock::compute_forces
  * (src/rhfgrad.cpp:15-89) = D:(dT + dV) - 2 W:dS + 1/4 G1[D]:D + dE_nuc,
  * with the two-electron part of compute_2body_fock_deriv<1>

@@ -1123,6 +1123,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graphs(ctx);
+    local_df_release(ctx);
     for (BasisData* b : {&ctx->obs, &ctx->aux}) {
         b->d_O.release();
         b->d_bf.release();
@@ -1201,11 +1202,10 @@ m2_status m2_basis_set(m2_ctx* ctx, int which, int nsh, const int* l, const int*
                        const double* coefs, const double* centers, const int* atoms, int pure) {
     if (!ctx) return M2_ERR_ARG;
     ARGCHK(which == M2_ORBITAL || which == M2_AUXILIARY, "m2_basis_set: which must be M2_ORBITAL or M2_AUXILIARY");
-    if (which == M2_ORBITAL) {
-        (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);
-        drop_graphs(ctx);
-    }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    local_df_release(ctx);
+    if (which == M2_ORBITAL) drop_graphs(ctx);
     std::vector<HostShell> shells;
     auto fail = [&](const char* msg) {
         clear_basis(ctx, which);

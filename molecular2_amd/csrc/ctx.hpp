@@ -68,6 +68,8 @@ struct BasisData {
 };
 
 
+struct LocalDF;
+
 // a captured direct J/K build: its key and executable graph
 struct JKGraphKey {
     const void* dD;
@@ -177,6 +179,8 @@ struct m2_ctx {
     int df_jblocks = 0;
     bool df_ready = false;
     double df_last_ms[4] = {0, 0, 0, 0};   // X = C^T B, K = X^T X, J, total (profiling)
+    // fragment-local DF-JK state (m2_jk_df_local), set up on first use
+    m2::LocalDF* ldf = nullptr;
     // this device's copy of the static scratch array of the allocation-free
     // J/K entry point (hipGetSymbolAddress at context creation)
     unsigned char* scratch = nullptr;
@@ -188,6 +192,7 @@ struct m2_ctx {
 
 namespace m2 {
 m2_status la_handle(m2_ctx* ctx, void** h);
+void local_df_release(m2_ctx* ctx);   // fragment-local DF state (df_local.cpp)
 }
 
 #define HIPCHK(expr)                                                                   \

@@ -126,7 +126,12 @@ namespace {
 
 BoysTables df_tables(m2_ctx* ctx) { return BoysTables{ctx->d_boys.p, ctx->d_boys_f.p}; }
 
-m2_status df_setup(m2_ctx* ctx) {
+}  // namespace
+
+namespace m2 {
+// The raw 3-centre integrals M[K][x][y] = (K|xy) (n * n * naux doubles) and
+// the Coulomb metric V = (K|L) (naux * naux), FP64, on the context's stream.
+m2_status df_raw(m2_ctx* ctx, double* Mout, double* Vout) {
     BasisData& ob = ctx->obs;
     BasisData& ab = ctx->aux;
     ARGCHK(ob.nbf > 0, "m2_df_setup: no orbital basis set");
@@ -172,9 +177,6 @@ m2_status df_setup(m2_ctx* ctx) {
     HIPCHK(hipMemcpy(ctx->d_aux_lists.p, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice));
 
     // 3-centre integrals into M[K][x][y]
-    const size_t nB = (size_t)n * n * naux;
-    DevBuf<double> M;
-    HIPCHK(M.reserve(nB));
     std::vector<Task> tasks;
     struct R3 {
         int lp, Y, t0, t1;
@@ -205,7 +207,7 @@ m2_status df_setup(m2_ctx* ctx) {
     a3.pair_sh = ctx->d_pair_sh.p;
     a3.sh_O = ob.d_O.p;
     a3.sh_bf = ob.d_bf.p;
-    a3.M = M.p;
+    a3.M = Mout;
     a3.n = n;
     a3.naux = naux;
     for (const R3& r : rr) {
@@ -213,7 +215,6 @@ m2_status df_setup(m2_ctx* ctx) {
         HIPCHK(hipGetLastError());
     }
     // 2-centre metric
-    HIPCHK(ctx->d_dfV.reserve((size_t)naux * naux));
     for (int lp = 0; lp <= 3; ++lp)
         for (int lq = 0; lq <= lp; ++lq) {
             DF2Args a2{};
@@ -225,11 +226,31 @@ m2_status df_setup(m2_ctx* ctx) {
             a2.nP = lbeg[lp + 1] - lbeg[lp];
             a2.Q_list = ctx->d_aux_lists.p + lbeg[lq];
             a2.nQ = lbeg[lq + 1] - lbeg[lq];
-            a2.V = ctx->d_dfV.p;
+            a2.V = Vout;
             a2.naux = naux;
             launch_df2(lp, lq, ab.pure, a2, df_tables(ctx), ctx->stream);
             HIPCHK(hipGetLastError());
         }
+    // the task list is freed once the kernels that read it have run
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return M2_OK;
+}
+}  // namespace m2
+
+namespace {
+
+m2_status df_setup(m2_ctx* ctx) {
+    BasisData& ob = ctx->obs;
+    BasisData& ab = ctx->aux;
+    ARGCHK(ob.nbf > 0, "m2_df_setup: no orbital basis set");
+    ARGCHK(ab.nbf > 0, "m2_df_setup: no auxiliary basis set");
+    const int n = ob.nbf, naux = ab.nbf;
+    const size_t nB = (size_t)n * n * naux;
+    DevBuf<double> M;
+    HIPCHK(M.reserve(nB));
+    HIPCHK(ctx->d_dfV.reserve((size_t)naux * naux));
+    m2_status st0 = df_raw(ctx, M.p, ctx->d_dfV.p);
+    if (st0) return st0;
     // Cholesky and fitting
     DevBuf<int> info;
     HIPCHK(info.reserve(1));
@@ -269,7 +290,6 @@ m2_status df_setup(m2_ctx* ctx) {
     HIPCHK(hipMemcpy(ctx->d_df_jblk.p, jb.data(), sizeof(int) * jb.size(), hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     M.release();
-    dt.release();
     info.release();
     ctx->df_ready = true;
     return M2_OK;

@@ -20,6 +20,12 @@ _ip = C.POINTER(C.c_int)
 _lib = None
 
 
+class M2Fragment(C.Structure):
+    """m2_fragment (include/m2.h): one fragment of the fragment-local DF-JK"""
+    _fields_ = [("occ", C.c_int), ("nbfs", C.c_int), ("naux", C.c_int), ("start", C.c_int), ("auxstart", C.c_int),
+                ("nshells", C.c_int), ("ndfshells", C.c_int), ("radius", C.c_double), ("com", C.c_double * 3)]
+
+
 class SCFOptions(C.Structure):
     """Options of the host C++ SCF (the reference's RHF defaults,
     src/ProgramController.cpp:295-307, plus the precision policy)."""
@@ -149,6 +155,12 @@ def lib():
         L.m2_grad.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
         L.m2_grad_ex.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_uint, _dp]
         for name in ("m2_grad", "m2_df_setup", "m2_jk_df", "m2_jk_df_device", "m2_df_timing", "m2_eigh", "m2_gemm", "m2_rhf"):
+            getattr(L, name).restype = C.c_int
+        L.m2_jk_df_local.argtypes = [vp, C.c_int, C.POINTER(M2Fragment), C.c_double, C.c_double, C.c_double,
+                                     C.c_double, _dp, C.c_int, _dp, _dp, _dp, _dp]
+        L.m2_df_local_reset.argtypes = [vp]
+        L.m2_df_local_domains.argtypes = [vp, C.c_int, _ip, _ip]
+        for name in ("m2_jk_df_local", "m2_df_local_reset", "m2_df_local_domains"):
             getattr(L, name).restype = C.c_int
         L.m2_set_jk_launch.argtypes = [vp, C.c_int]
         L.m2_set_jk_graphs.argtypes = [vp, C.c_int]
@@ -395,6 +407,38 @@ class Engine:
         self._chk(self.L.m2_get_profile_computed(self.h, cpq.ctypes.data_as(P)))
         return {(c // 6, c % 6): (ms[c], int(la[c]), int(q[c]), int(pq[c]), int(cpq[c])) for c in range(36)
                 if la[c] > 0}
+
+    def jk_df_local(self, frags, Cocc, sigmainv, Pt, mo_thresh=1e-6, fit_thresh=0.05, r_thresh=15.0,
+                    thrint=1e-12):
+        """Fock::compute_2body_fock_df_local through m2_jk_df_local.  frags:
+        dicts with occ, nbfs, naux, start, auxstart, nshells, ndfshells,
+        radius, com; the first call (or the first after df_local_reset)
+        builds the fragment blocks and the domains.  Returns J, K."""
+        arr = (M2Fragment * len(frags))()
+        for k, f in enumerate(frags):
+            for key in ("occ", "nbfs", "naux", "start", "auxstart", "nshells", "ndfshells"):
+                setattr(arr[k], key, int(f[key]))
+            arr[k].radius = float(f["radius"])
+            for d in range(3):
+                arr[k].com[d] = float(f["com"][d])
+        Cf = np.asfortranarray(Cocc, float)
+        n, nocc = Cf.shape
+        sig = np.asfortranarray(sigmainv, float)
+        P = np.ascontiguousarray(Pt, float)
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        self._chk(self.L.m2_jk_df_local(self.h, len(frags), arr, mo_thresh, fit_thresh, r_thresh, thrint,
+                                        Cf.ctypes.data_as(_dp), nocc, sig.ctypes.data_as(_dp), _p(P), _p(J), _p(K)))
+        return J, K
+
+    def df_local_reset(self):
+        self._chk(self.L.m2_df_local_reset(self.h))
+
+    def df_local_domains(self, which, nocc, nfrag):
+        """which: 0 LMO, 1 AO, 2 fit; list of fragment-centre lists per LMO"""
+        cen = np.zeros(nocc * nfrag, np.int32)
+        cnt = np.zeros(nocc, np.int32)
+        self._chk(self.L.m2_df_local_domains(self.h, int(which), cen.ctypes.data_as(_ip), cnt.ctypes.data_as(_ip)))
+        return [list(cen[i * nfrag:i * nfrag + cnt[i]]) for i in range(nocc)]
 
     def set_jk_launch(self, host_totals=False):
         """grids of the J/K class kernels: bounds from the basis with no host

@@ -190,6 +190,35 @@ def water_cluster(nwater: int = 20, seed: int = 20, spacing: float = 3.0,
         s += 1
 
 
+ATOMIC_MASS = {1: 1.00794, 6: 12.0107, 7: 14.0067, 8: 15.9994}
+
+
+def fragment_info(mol: Molecule, groups, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=True, occ=None,
+                  radius=None) -> list:
+    """FragmentInfo (inc/fock.hpp:74-95) of each atom group (lists of atom
+    indices, consecutive and in order, so each fragment's shells are one
+    contiguous run of the basis): functions, shells, auxiliary functions and
+    shells, occupied count (default nel/2 of the neutral group), centre of
+    mass (bohr) and radius (default: the largest atom distance from it plus
+    2 bohr)."""
+    obs, abs_ = mol.shells(basis), mol.shells(aux)
+    out, start, astart = [], 0, 0
+    for g, atoms in enumerate(groups):
+        atoms = list(atoms)
+        sh = [s for s in obs if s.atom in atoms]
+        ash = [s for s in abs_ if s.atom in atoms]
+        m = np.array([ATOMIC_MASS[mol.Z[a]] for a in atoms])
+        com = (m[:, None] * mol.xyz[atoms]).sum(axis=0) / m.sum()
+        rad = max(float(np.linalg.norm(mol.xyz[a] - com)) for a in atoms) + 2.0 if radius is None else radius
+        nb, na = nbf_of(sh, pure), nbf_of(ash, pure)
+        nel = int(sum(mol.Z[a] for a in atoms))
+        out.append(dict(occ=(nel // 2 if occ is None else occ[g]), nbfs=nb, naux=na, start=start, auxstart=astart,
+                        nshells=len(sh), ndfshells=len(ash), radius=rad, com=com))
+        start += nb
+        astart += na
+    return out
+
+
 def random_chain(natom: int = 12, seed: int = 12) -> Molecule:
     """Bonded random C/H/N/O chain: bond lengths U(1.0, 1.5) A, bond angles
     U(100, 125) deg and torsions U(-180, 180) deg from default_rng(seed);

@@ -9,7 +9,9 @@ Loads ``oracle/lib/libm2oracle.so`` (built from ``oracle/m2_oracle.c`` by
     updated from iteration 3 on);
   * Fock::makeFock incremental (Delta-D) logic, addDiis, average, transform,
     diagonalise, makeDens (src/fockbuild.cpp:176-222, src/fock.cpp:114-244);
-  * the density-fitted J/K of compute_2body_fock_df (src/fockbuild.cpp:585-660).
+  * the density-fitted J/K of compute_2body_fock_df (src/fockbuild.cpp:585-660);
+  * the fragment-local density-fitted J/K of compute_2body_fock_df_local with
+    build_blocked_eris and build_domains (src/fockbuild.cpp:662-1047).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import it.
 """
@@ -661,6 +663,147 @@ class DFJK:
         K = np.einsum("xiK,yiK->xy", xiK, xiK)
         gam = np.einsum("xiK,xi->K", xiK, Cocc)
         J = Bf @ gam
+        return J, K
+
+
+class LocalDFJK:
+    """Fragment-local density-fitted J/K: Fock::build_blocked_eris
+    (src/fockbuild.cpp:985-1047), Fock::build_domains (:662-792) and
+    Fock::compute_2body_fock_df_local (:794-984), restated over the full
+    raw tensor (xy|K) with the screened fragment blocks zeroed.
+
+    frags: list of dicts with start, nbfs, nshells (orbital functions and
+    shells, contiguous), auxstart, naux, ndfshells (auxiliary), occ, com (3,),
+    radius; mo_thresh, fit_thresh, r_thresh are the first fragment's (the
+    reference reads finfo[0]); thrint: the block screening threshold.
+    The first call (Cocc, sigmainv, Pt) builds the blocks and the domains,
+    as the reference does when its xyK is still empty."""
+
+    def __init__(self, obs, aux, frags, Q, S, mo_thresh=1e-6, fit_thresh=0.05, r_thresh=15.0, thrint=1e-12):
+        n = obs.nbf
+        xyK = obs.eri3_packed(aux)
+        iu = np.tril_indices(n)
+        idx = np.zeros((n, n), int)
+        idx[iu] = iu[0] * (iu[0] + 1) // 2 + iu[1]
+        idx = np.maximum(idx, idx.T)
+        self.M = xyK[idx]                         # n x n x naux, raw (xy|K)
+        self.V = aux.eri2()
+        self.Q = np.asarray(Q, float)             # shell-pair Schwarz (prescreen)
+        self.S = np.asarray(S, float)
+        self.frags = frags
+        self.mo_thresh, self.fit_thresh, self.r_thresh, self.thrint = mo_thresh, fit_thresh, r_thresh, thrint
+        self.n, self.naux = n, aux.nbf
+        self.ready = False
+
+    def _blocks(self, Pt):
+        """build_blocked_eris: which (f1 >= f2, fk) blocks are kept."""
+        F = self.frags
+        nf = len(F)
+        keep = np.zeros((nf, nf, nf), bool)
+        sh0 = np.cumsum([0] + [f["nshells"] for f in F])
+        for f1 in range(nf):
+            a = F[f1]
+            for f2 in range(f1 + 1):
+                b = F[f2]
+                infnorm = np.max(np.abs(self.Q[sh0[f1]:sh0[f1 + 1], sh0[f2]:sh0[f2 + 1]]))
+                ptnorm = np.max(np.abs(Pt[a["start"]:a["start"] + a["nbfs"], b["start"]:b["start"] + b["nbfs"]]))
+                for fk in range(nf):
+                    k = F[fk]
+                    Vb = self.V[k["auxstart"]:k["auxstart"] + k["naux"], k["auxstart"]:k["auxstart"] + k["naux"]]
+                    infjpq = np.sqrt(np.max(np.abs(Vb)))
+                    rxyz = 0.5 * (np.linalg.norm(np.subtract(a["com"], k["com"]))
+                                  + np.linalg.norm(np.subtract(b["com"], k["com"])))
+                    est = 1.0 if rxyz < 1.0 else 2.0 * infnorm * ptnorm * infjpq / rxyz
+                    if est > self.thrint:
+                        keep[f1, f2, fk] = keep[f2, f1, fk] = True
+        return keep
+
+    def _domains(self, C):
+        """build_domains on C = Cocc sigmainv^(1/2) (one LMO per column,
+        ordered by fragment)."""
+        F = self.frags
+        nf = len(F)
+        lmo, ao, fit = [], [], []
+        i = 0
+        for B in range(nf):
+            for _ in range(F[B]["occ"]):
+                cen, notc, off = [], [], 0
+                for A in range(nf):
+                    if A == B:
+                        add = True
+                    else:
+                        # the reference sums over occ(A) rows from the
+                        # fragment's first function (src/fockbuild.cpp:675-677)
+                        add = float(np.sum(C[off:off + F[A]["occ"], i] ** 2)) > self.mo_thresh
+                    (cen if add else notc).append(A)
+                    off += F[A]["nbfs"]
+                aoc = list(cen)
+                for A in cen:
+                    newnc = []
+                    for Cc in notc:
+                        sep = np.linalg.norm(np.subtract(F[Cc]["com"], F[A]["com"]))
+                        if F[Cc]["radius"] > sep:
+                            aoc.append(Cc)
+                        else:
+                            newnc.append(Cc)
+                    notc = newnc
+                lmo.append(cen)
+                ao.append(aoc)
+                i += 1
+        w, U = np.linalg.eigh(self.S)
+        SC = (U * np.sqrt(w)) @ U.T @ C
+        i = 0
+        for f1 in range(nf):
+            for _ in range(F[f1]["occ"]):
+                d = []
+                for f2 in range(nf):
+                    sep = np.linalg.norm(np.subtract(F[f2]["com"], F[f1]["com"]))
+                    s0 = F[f2]["start"]
+                    INi = float(np.sum(SC[s0:s0 + F[f2]["nbfs"], i] ** 2))
+                    if INi > self.fit_thresh or sep < self.r_thresh:
+                        d.append(f2)
+                fit.append(d)
+                i += 1
+        return lmo, ao, fit
+
+    def __call__(self, Cocc, sigmainv, Pt):
+        F = self.frags
+        n, naux = self.n, self.naux
+        w, U = np.linalg.eigh(sigmainv)
+        C = np.asarray(Cocc, float) @ ((U * np.sqrt(w)) @ U.T)
+        bf_frag = np.concatenate([[k] * f["nbfs"] for k, f in enumerate(F)])
+        aux_frag = np.concatenate([[k] * f["naux"] for k, f in enumerate(F)])
+        if not self.ready:
+            self.keep = self._blocks(Pt)
+            self.lmo, self.ao, self.fit = self._domains(C)
+            self.Mk = self.M * self.keep[bf_frag[:, None, None], bf_frag[None, :, None], aux_frag[None, None, :]]
+            self.ready = True
+        Mk = self.Mk
+        K = np.zeros((n, n))
+        def idx(frs, first, size):   # the functions of fragments frs, in domain order
+            return np.concatenate([np.arange(F[a][first], F[a][first] + F[a][size]) for a in frs]
+                                  + [np.zeros(0, int)])
+
+        for i in range(C.shape[1]):
+            aoi = idx(self.ao[i], "start", "nbfs")
+            moi = idx(self.lmo[i], "start", "nbfs")
+            fiti = idx(self.fit[i], "auxstart", "naux")
+            if fiti.size == 0:      # an empty fit domain adds no exchange
+                continue
+            uA = np.einsum("uvA,v->uA", Mk[np.ix_(aoi, moi, fiti)], C[moi, i])
+            L = np.linalg.cholesky(self.V[np.ix_(fiti, fiti)])
+            G = np.linalg.inv(L).T
+            Y = uA @ G
+            K[np.ix_(aoi, aoi)] += Y @ Y.T
+        Pk = np.zeros(naux)
+        for f1 in range(len(F)):
+            for f2 in range(f1 + 1):
+                a, b = F[f1], F[f2]
+                blk = Mk[a["start"]:a["start"] + a["nbfs"], b["start"]:b["start"] + b["nbfs"], :]
+                Pb = Pt[a["start"]:a["start"] + a["nbfs"], b["start"]:b["start"] + b["nbfs"]]
+                Pk += (0.5 if f1 == f2 else 1.0) * np.einsum("xyK,xy->K", blk, Pb)
+        Pk = np.linalg.solve(self.V, Pk)
+        J = 4.0 * np.einsum("xyK,K->xy", Mk, Pk)
         return J, K
 
 
