@@ -129,6 +129,25 @@ int m2_rhf_hessian(int device, int nworkers, int natom, const double* Z, const d
                    const double* coefs_min, const double* centers_min, const int* atoms_min,
                    const m2_scf_options_c* o, double step, double* hess, m2_scf_result_c* res);
 
+/* Fock::compute_hessian (src/rhfgrad.cpp:91-359) after SCF::rhf: the
+ * analytic RHF nuclear Hessian H = H_nr + 2 H_r (3N x 3N, Eh/bohr^2, into
+ * hess) with the coupled-perturbed Hartree-Fock response solved by the
+ * reference's fixed-point iteration u <- Q + V u (cphf_group_solver,
+ * src/cphf.cpp:4-34; the reference passes cphf_tol 1e-4, cphf_maxiter 30).
+ * The derivative integrals are central differences (step bohr, e.g. 1e-4) of
+ * the engine's analytic one-body and J/K integrals, and H_nr the central
+ * difference of the analytic gradient at fixed density and energy-weighted
+ * density; the MO-integral sums of the CPHF equations are direct J/K builds
+ * of the perturbed densities.  Shells as for m2_rhf (no auxiliary basis:
+ * the reference's Hessian needs the in-core SCF).  stats (optional, 2 ints):
+ * CPHF solves that stopped at cphf_maxiter, CPHF steps in total. */
+int m2_rhf_hessian_cphf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charge, int multiplicity,
+                        int pure, int nsh, const int* l, const int* nprim, const double* exps, const double* coefs,
+                        const double* centers, const int* atoms, int nsh_min, const int* l_min, const int* nprim_min,
+                        const double* exps_min, const double* coefs_min, const double* centers_min,
+                        const int* atoms_min, const m2_scf_options_c* o, double step, double cphf_tol,
+                        int cphf_maxiter, double* hess, m2_scf_result_c* res, int* stats);
+
 /* The two-electron part of the SOAD guess Fock matrix (Fock::compute_soad_guess,
  * src/fock.cpp:174-225, with compute_2body_fock_general, src/fockbuild.cpp:
  * 477-583): G (n x n, orbital basis) = sum_ls D(ls) [2 (uv|ls) - (ul|vs)]

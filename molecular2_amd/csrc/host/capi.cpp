@@ -325,6 +325,64 @@ int m2_uhf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charg
     }
 }
 
+// Fock::compute_hessian after SCF::rhf (the reference's "hessian" path,
+// src/optimiser.cpp:410): shells as for m2_rhf (the auxiliary basis is not
+// used); hess: 3N x 3N; stats (optional): CPHF solves that did not converge
+// and CPHF steps in total.
+int m2_rhf_hessian_cphf(m2_ctx* ctx, int natom, const double* Z, const double* xyz, int charge, int multiplicity,
+                        int pure, int nsh, const int* l, const int* nprim, const double* exps, const double* coefs,
+                        const double* centers, const int* atoms, int nsh_min, const int* l_min, const int* nprim_min,
+                        const double* exps_min, const double* coefs_min, const double* centers_min,
+                        const int* atoms_min, const m2_scf_options_c* o, double step, double cphf_tol,
+                        int cphf_maxiter, double* hess, m2_scf_result_c* res, int* stats) {
+    using namespace molecular;
+    g_rhf_error.clear();
+    try {
+        if (!(step > 0.0) || !(cphf_tol > 0.0) || cphf_maxiter < 1 || !hess)
+            throw Error("HESSIAN", "step and cphf_tol must be positive, cphf_maxiter at least 1, hess non-null");
+        auto t0 = std::chrono::steady_clock::now();
+        Molecule mol;
+        mol.pure = pure != 0;
+        mol.charge = charge;
+        mol.multiplicity = multiplicity;
+        for (int a = 0; a < natom; ++a) mol.atoms.push_back(Atom{(int)Z[a], xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]});
+        load_shells(nsh, l, nprim, exps, coefs, centers, atoms, mol.shells);
+        if (nsh_min > 0)
+            load_shells(nsh_min, l_min, nprim_min, exps_min, coefs_min, centers_min, atoms_min, mol.minshells);
+        Options opt = options_from(o);
+        if (opt.df) throw Error("HESSIAN", "the analytic Hessian needs the direct (non-DF) SCF");
+        IntegralEngine ints(ctx, mol);
+        Fock fock(opt, ints, mol);
+        SCF scf(opt, mol, fock);
+        scf.rhf(false);
+        fock.compute_hessian(mol.atoms, fock.getNocc(), step, cphf_tol, cphf_maxiter);
+        const int nc = 3 * natom;
+        for (int j = 0; j < nc; ++j)
+            for (int i = 0; i < nc; ++i) hess[(size_t)j * nc + i] = fock.getHessian()(i, j);
+        if (stats) {
+            stats[0] = fock.cphf_unconverged;
+            stats[1] = fock.cphf_iterations;
+        }
+        if (res) {
+            *res = m2_scf_result_c{};
+            res->energy = scf.getEnergy();
+            res->one_e = scf.getOneE();
+            res->two_e = scf.getTwoE();
+            res->enuc = mol.getEnuc();
+            res->iterations = scf.iterations;
+            res->converged = scf.converged ? 1 : 0;
+            res->n_full = fock.n_full;
+            res->n_incremental = fock.n_incremental;
+            res->jk_seconds = fock.jk_seconds;
+            res->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_rhf_error = e.what();
+        return 1;
+    }
+}
+
 // Batched RHF: molecule m runs m2_rhf on the context of whichever worker
 // takes it; workers take molecules from one queue ordered by basis size,
 // largest first, so the long runs start early and the short ones fill in.

@@ -907,4 +907,148 @@ void SCF::uhf(bool print) {
     iterations = iter;
 }
 
+// ------------------------------------------------------------ Hessian
+// Fock::compute_hessian (src/rhfgrad.cpp:91-359), the analytic RHF Hessian
+// H = H_nr + 2 H_r with the coupled-perturbed response of the orbitals, in
+// the reference's steps and with its CPHF solver (cphf_group_solver,
+// src/cphf.cpp:4-34: u <- Q + V u until ||du|| < tol, at most maxiter
+// steps).  Two things differ in how the pieces are obtained:
+//   * derivative integrals: the reference asks libint for first and second
+//     derivative integrals.  Here the first-derivative matrices S1, T1+V1
+//     and G1 = d(2J - K)[D]/dR (D = C_occ C_occ^T fixed) are central
+//     differences (step bohr) of the engine's analytic integrals at
+//     displaced geometries, and H_nr -- the second-derivative integrals
+//     contracted with D, D x D and W plus the nuclear term -- is the central
+//     difference of the engine's analytic gradient (m2_grad) taken at fixed
+//     D and W, which is exactly its definition;
+//   * MO integrals: the reference transforms the full ERI tensor (MP2
+//     class).  Every MO-integral sum it forms is a J/K build of an AO
+//     matrix contracted with MO coefficients: V u = [4 Cv^T J(X) Co - 2
+//     Cv^T K(X) Co] / (e_i - e_a) with X = sym(Cv u Co^T), and the S1 terms
+//     use J, K of Y = Co S1_oo Co^T; so each CPHF step is one direct J/K
+//     build on the device instead of an O(n^5) transformation.
+static Matrix sym_part(const Matrix& X) {
+    Matrix Y = X + X.transpose();
+    return Y * 0.5;
+}
+
+void Fock::compute_hessian(const std::vector<Atom>& atoms, int nocc_, double step, double cphf_tol, int cphf_maxiter) {
+    m2_ctx* ctx = integrals.context();
+    const int n = nbfs, N = (int)atoms.size(), nc = 3 * N, no = nocc_, nv = nbfs - nocc_;
+    if (no <= 0 || nv <= 0) throw Error("HESSIAN", "needs occupied and virtual orbitals");
+    const double hprec = std::min(precision, 1e-14);   // screening noise far below the difference quotients
+    Matrix Co = CP.block(0, 0, n, no), Cv = CP.block(0, no, n, nv);
+    Matrix Dh = matmul(ctx, Co, Co, false, true);       // dens / 2 (src/rhfgrad.cpp:93)
+    Matrix Ce(Co);
+    for (int j = 0; j < no; ++j)
+        for (int i = 0; i < n; ++i) Ce(i, j) *= eps[j];
+    Matrix W = matmul(ctx, Ce, Co, false, true);
+    Matrix Dfull = Dh * 2.0;
+    auto jk = [&](const Matrix& P, Matrix& J, Matrix& K) {
+        J = Matrix(n, n);
+        K = Matrix(n, n);
+        check(ctx, m2_jk_direct(ctx, P.data(), 1.0, hprec, M2_FP64, J.data(), K.data()), "m2_jk_direct");
+    };
+    // first-derivative matrices and H_nr by central differences
+    std::vector<Matrix> S1(nc, Matrix(n, n)), H1(nc, Matrix(n, n)), G1(nc, Matrix(n, n));
+    Matrix Hnr(nc, nc);
+    const double inv2h = 1.0 / (2.0 * step);
+    for (int c = 0; c < nc; ++c)
+        for (int sgn : {1, -1}) {
+            Molecule m(molecule);
+            const int a = c / 3, ax = c % 3;
+            double* pa = ax == 0 ? &m.atoms[a].x : ax == 1 ? &m.atoms[a].y : &m.atoms[a].z;
+            *pa += sgn * step;
+            for (auto& sh : m.shells)
+                if (sh.atom == a) sh.O[ax] += sgn * step;
+            m.jkshells.clear();
+            IntegralEngine ie(ctx, m);
+            const double f = sgn * inv2h;
+            S1[c] += ie.getOverlap() * f;
+            H1[c] += (ie.getKinetic() + ie.getNucAttract()) * f;
+            Matrix J, K;
+            jk(Dh, J, K);
+            G1[c] += (J * 2.0 - K) * f;
+            std::vector<double> Z, xyz, g(nc);
+            for (const auto& at : m.atoms) {
+                Z.push_back(at.Z);
+                xyz.push_back(at.x);
+                xyz.push_back(at.y);
+                xyz.push_back(at.z);
+            }
+            check(ctx, m2_grad(ctx, N, Z.data(), xyz.data(), Dfull.data(), W.data(), hprec, g.data()), "m2_grad");
+            for (int r = 0; r < nc; ++r) Hnr(r, c) += g[r] * f;
+        }
+    {
+        Molecule m(molecule);
+        m.jkshells.clear();
+        IntegralEngine restore(ctx, m);   // the context's basis back at the reference geometry
+    }
+    // CPHF per coordinate, then the perturbed density and energy-weighted density
+    std::vector<Matrix> P1(nc), W1(nc);
+    cphf_unconverged = 0;
+    cphf_iterations = 0;
+    for (int c = 0; c < nc; ++c) {
+        Matrix S1mo = matmul(ctx, matmul(ctx, CP, S1[c], true, false), CP);
+        Matrix H1mo = matmul(ctx, matmul(ctx, CP, H1[c], true, false), CP);
+        Matrix G1mo = matmul(ctx, matmul(ctx, CP, G1[c], true, false), CP);
+        Matrix Y = matmul(ctx, matmul(ctx, Co, S1mo.block(0, 0, no, no)), Co, false, true);
+        Matrix JY, KY;
+        jk(Y, JY, KY);
+        Matrix GYvo = matmul(ctx, matmul(ctx, Cv, JY * 2.0 - KY, true, false), Co);   // 2(ai|lk)S1 - (ak|li)S1
+        Matrix Q(nv, no);
+        for (int i = 0; i < no; ++i)
+            for (int a = 0; a < nv; ++a)
+                Q(a, i) = (H1mo(a + no, i) - S1mo(a + no, i) * eps[i] - GYvo(a, i) + G1mo(a + no, i)) /
+                          (eps[i] - eps[a + no]);
+        // u <- Q + V u (cphf_single_solver)
+        auto Vu = [&](const Matrix& U, Matrix& Jx, Matrix& Kx) {
+            Matrix X = sym_part(matmul(ctx, matmul(ctx, Cv, U), Co, false, true));
+            jk(X, Jx, Kx);
+            Matrix T = matmul(ctx, matmul(ctx, Cv, Jx * 4.0 - Kx * 2.0, true, false), Co);
+            for (int i = 0; i < no; ++i)
+                for (int a = 0; a < nv; ++a) T(a, i) /= eps[i] - eps[a + no];
+            return T;
+        };
+        Matrix U = Q, Jx, Kx;
+        bool conv = false;
+        int it = 0;
+        while (!conv && it < cphf_maxiter) {
+            Matrix U1 = Q + Vu(U, Jx, Kx);
+            const double delta = (U1 - U).norm();
+            U = U1;
+            conv = delta < cphf_tol;
+            ++it;
+        }
+        cphf_iterations += it;
+        if (!conv) ++cphf_unconverged;   // the reference prints "CPHF failed to converge" and goes on
+        // ftilde = H1 + G1 - C^T G(Y) C + C^T (4 J(X_u) - 2 K(X_u)) C over all MOs
+        Matrix Xu = sym_part(matmul(ctx, matmul(ctx, Cv, U), Co, false, true));
+        jk(Xu, Jx, Kx);
+        Matrix Ft = H1mo + G1mo - matmul(ctx, matmul(ctx, CP, JY * 2.0 - KY, true, false), CP) +
+                    matmul(ctx, matmul(ctx, CP, Jx * 4.0 - Kx * 2.0, true, false), CP);
+        Matrix pt(n, n), wt(n, n);
+        for (int i = 0; i < no; ++i) {
+            for (int j = 0; j < no; ++j) {
+                pt(i, j) = -S1mo(j, i);
+                wt(i, j) = Ft(j, i) - (eps[i] + eps[j]) * S1mo(j, i);
+            }
+            for (int a = no; a < n; ++a) {
+                pt(i, a) = 2.0 * U(a - no, i);
+                wt(i, a) = eps[i] * pt(i, a);
+            }
+        }
+        P1[c] = matmul(ctx, matmul(ctx, CP, pt), CP, false, true);
+        W1[c] = matmul(ctx, matmul(ctx, CP, wt), CP, false, true);
+    }
+    // H = H_nr + 2 H_r over the upper triangle, mirrored (src/rhfgrad.cpp:335-358)
+    hessian = Matrix(nc, nc);
+    for (int i = 0; i < nc; ++i)
+        for (int j = i; j < nc; ++j) {
+            const double hr = P1[j].dot(H1[i]) + P1[j].dot(G1[i]) - W1[j].dot(S1[i]);
+            hessian(i, j) = Hnr(i, j) + 2.0 * hr;
+            hessian(j, i) = hessian(i, j);
+        }
+}
+
 }  // namespace molecular

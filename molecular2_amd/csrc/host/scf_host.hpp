@@ -205,6 +205,15 @@ public:
     // Fock::compute_forces (src/rhfgrad.cpp:15-89): dE/dR, natom x 3
     void compute_forces(const std::vector<Atom>& atoms, int nocc);
     Matrix& getForces() { return forces; }
+    // Fock::compute_hessian (src/rhfgrad.cpp:91-359): the analytic RHF
+    // Hessian with CPHF response; derivative integrals by central differences
+    // of the engine's integrals (step bohr); the CPHF fixed point stops at
+    // ||du|| < cphf_tol or after cphf_maxiter steps (src/cphf.cpp, the
+    // reference passes 1e-4 and 30).  Leaves the context's basis at the
+    // molecule's geometry.
+    void compute_hessian(const std::vector<Atom>& atoms, int nocc, double step, double cphf_tol, int cphf_maxiter);
+    Matrix& getHessian() { return hessian; }
+    int cphf_unconverged = 0, cphf_iterations = 0;
     double grad_seconds = 0.0;
 
     // incremental Fock state (inc/fock.hpp:127-129)
@@ -220,7 +229,7 @@ protected:
     const Options opt;
     IntegralEngine& integrals;
     const Molecule& molecule;
-    Matrix hcore, orthog, fockm, focka, fockinc, CP, dens, dens_diff, jints, kints, jkints, forces;
+    Matrix hcore, orthog, fockm, focka, fockinc, CP, dens, dens_diff, jints, kints, jkints, forces, hessian;
     Vector eps;
     std::vector<Matrix> focks;
     int nbfs = 0, nocc = 0, iter = 0, MAX = 8;

@@ -162,6 +162,12 @@ def lib():
         L.m2_df_local_domains.argtypes = [vp, C.c_int, _ip, _ip]
         for name in ("m2_jk_df_local", "m2_df_local_reset", "m2_df_local_domains"):
             getattr(L, name).restype = C.c_int
+        L.m2_rhf_hessian_cphf.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                                          C.c_int, _ip, _ip, _dp, _dp, _dp, _ip,
+                                          C.POINTER(SCFOptions), C.c_double, C.c_double, C.c_int, _dp,
+                                          C.POINTER(SCFResult), _ip]
+        L.m2_rhf_hessian_cphf.restype = C.c_int
         L.m2_set_jk_launch.argtypes = [vp, C.c_int]
         L.m2_set_jk_graphs.argtypes = [vp, C.c_int]
         L.m2_jk_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _ip]
@@ -605,6 +611,36 @@ class Engine:
                 "n_full": r.n_full, "n_incremental": r.n_incremental, "n_fp32": r.n_fp32,
                 "jk_seconds": r.jk_seconds, "total_seconds": r.total_seconds} for r in res]
         return out, wall
+
+    def rhf_hessian_cphf(self, mol, basis="cc-pvdz", pure=True, guess="soad", step=1e-4, cphf_tol=1e-4,
+                         cphf_maxiter=30, **kw):
+        """Fock::compute_hessian through m2_rhf_hessian_cphf: SCF, then the
+        analytic RHF Hessian (3N x 3N, Eh/bohr^2) with the CPHF response
+        (the reference's fixed-point solver: cphf_tol 1e-4, 30 steps).
+        Returns (H, result dict with energy, converged, cphf_unconverged,
+        cphf_steps)."""
+        if guess not in ("hcore", "soad"):
+            raise ValueError("guess must be 'hcore' or 'soad'")
+        o = SCFOptions(diis=1, maxdiis=8, maxiter=40, enconverge=1e-7, densconverge=1e-4, precision=1e-12,
+                       df=0, mixed=0, mixed_threshold=1e-4, fp64_final=2, guess=1 if guess == "soad" else 0,
+                       diis_start=2)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        _, args = _scf_shells(mol, basis, None, guess)
+        n = 3 * mol.natom
+        H = np.zeros((n, n))
+        res = SCFResult()
+        stats = np.zeros(2, np.int32)
+        st = self.L.m2_rhf_hessian_cphf(self.h, mol.natom, _p(np.asarray(mol.Z, float)),
+                                        _p(np.ascontiguousarray(mol.xyz, float)), mol.charge, mol.multiplicity,
+                                        int(pure), *args[0:7], *args[14:21], C.byref(o), float(step),
+                                        float(cphf_tol), int(cphf_maxiter), _p(H), C.byref(res),
+                                        stats.ctypes.data_as(_ip))
+        if st != 0:
+            raise M2Error(st, self.L.m2_rhf_last_error().decode())
+        return H, {"energy": res.energy, "converged": bool(res.converged), "iterations": res.iterations,
+                   "cphf_unconverged": int(stats[0]), "cphf_steps": int(stats[1]),
+                   "total_seconds": res.total_seconds}
 
     def rhf_hessian(self, mol, basis="cc-pvdz", pure=True, aux=None, guess="soad", step=0.01, nworkers=8, **kw):
         """Fock::compute_hessian_numeric through m2_rhf_hessian: the RHF
