@@ -86,7 +86,7 @@ void boys_host(int mmax, double T, double* F) {
 
 namespace {
 
-BoysTables tables(m2_ctx* ctx) { return BoysTables{ctx->d_boys.p, ctx->d_boys_f.p}; }
+BoysTables tables(m2_ctx* ctx) { return boys_tables(ctx); }
 
 m2_status upload_basis(m2_ctx* ctx, BasisData& b) {
     b.h_bf.resize(b.nsh);
@@ -1107,9 +1107,21 @@ m2_status m2_ctx_create(m2_ctx** out, int device) {
     std::vector<float> tabf(tab.size());
     for (int k = 0; k < BOYS_NGRID; ++k) boys_host(BOYS_MTAB - 1, k * BOYS_STEP, &tab[(size_t)k * BOYS_MTAB]);
     for (size_t i = 0; i < tab.size(); ++i) tabf[i] = (float)tab[i];
+    // compact windows F_0 .. F_(M+7) of the low-m classes (same grid)
+    std::vector<double> tabw(BOYS_WSIZE);
+    for (int M = 0; M <= BOYS_DIRECT_MAX; ++M)
+        for (int k = 0; k <= BOYS_WK[M]; ++k)
+            for (int m = 0; m < M + 8; ++m) tabw[boys_woff(M) + (size_t)k * (M + 8) + m] = tab[(size_t)k * BOYS_MTAB + m];
+    std::vector<float> tabfw(BOYS_FWSIZE);
+    for (int M = 0; M <= BOYS_FW_MAX; ++M)
+        for (int k = 0; k < BOYS_NGRID; ++k)
+            for (int j = 0; j < 5; ++j) tabfw[((size_t)M * BOYS_NGRID + k) * 5 + j] = tabf[(size_t)k * BOYS_MTAB + M + j];
     if (ctx->d_boys.reserve(tab.size()) != hipSuccess || ctx->d_boys_f.reserve(tab.size()) != hipSuccess ||
+        ctx->d_boys_w.reserve(tabw.size()) != hipSuccess || ctx->d_boys_fw.reserve(tabfw.size()) != hipSuccess ||
+        hipMemcpy(ctx->d_boys_fw.p, tabfw.data(), sizeof(float) * tabfw.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(ctx->d_boys.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(ctx->d_boys_f.p, tabf.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(ctx->d_boys_f.p, tabf.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ctx->d_boys_w.p, tabw.data(), sizeof(double) * tabw.size(), hipMemcpyHostToDevice) != hipSuccess) {
         std::fprintf(stderr, "m2_ctx_create: device allocation failed\n");
         m2_ctx_destroy(ctx);
         return M2_ERR_HIP;
@@ -1137,6 +1149,8 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_pair_Q.release();
     ctx->d_boys.release();
     ctx->d_boys_f.release();
+    ctx->d_boys_w.release();
+    ctx->d_boys_fw.release();
     ctx->d_D.release();
     ctx->d_J.release();
     ctx->d_K.release();

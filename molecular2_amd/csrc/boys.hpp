@@ -17,7 +17,24 @@ namespace m2 {
 struct BoysTables {
     const double* d;   // [BOYS_NGRID][BOYS_MTAB]
     const float* f;
+    const double* w;   // the compact window tables of BoysEval<double, M <= 3>
+    const float* fw;   // the compact window tables of BoysEval<float, M <= BOYS_FW_MAX>
 };
+
+// Compact tables of the low-m classes: for M = 0..3, rows k = 0..BOYS_WK[M]
+// of F_0 .. F_(M+7) at T = k BOYS_STEP (M + 8 doubles a row), back to back
+// from BoysTables::w.  Beyond BOYS_WK[M] BOYS_STEP the asymptotic form
+// F_m = (2m-1)!! sqrt(pi) / (2^(m+1) T^(m+1/2)) is already within 1e-15
+// relative (the neglected part is ~e^-T T^(m-1/2) / Gamma(m+1/2) of it), so
+// the tables stop there: 21-39 KB each, against 125 KB for the full table,
+// and they stay in the vector L1 cache of the kernels that read them.
+constexpr int BOYS_WK[4] = {330, 380, 400, 440};
+constexpr int boys_woff(int M) { return M == 0 ? 0 : boys_woff(M - 1) + (BOYS_WK[M - 1] + 1) * (M - 1 + 8); }
+constexpr int BOYS_WSIZE = boys_woff(4);
+// FP32 windows F_M .. F_(M+4) (5 floats a row, all BOYS_NGRID rows) for
+// M = 0..BOYS_FW_MAX, back to back from BoysTables::fw: 12 KB each
+constexpr int BOYS_FW_MAX = 8;
+constexpr int BOYS_FWSIZE = (BOYS_FW_MAX + 1) * BOYS_NGRID * 5;
 
 // 1/sqrt(x) for finite x > 0: the hardware estimate (v_rsq_f64, ~2^-23
 // relative) refined by one third-order step of the iteration for x^(-1/2),
@@ -46,11 +63,12 @@ constexpr int BOYS_DIRECT_MAX = 3;
 template <int M>
 struct BoysEval<double, M> {
     static __device__ __forceinline__ void run(double T, double* F, const BoysTables& tab) {
-        if (T < BOYS_TMAX) {
+        constexpr double TTAB = M <= BOYS_DIRECT_MAX ? BOYS_WK[M <= BOYS_DIRECT_MAX ? M : 0] * BOYS_STEP : BOYS_TMAX;
+        if (T < TTAB) {
             const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
             const double d = k * BOYS_STEP - T;
             if constexpr (M <= BOYS_DIRECT_MAX) {
-                const double* r = tab.d + k * BOYS_MTAB;
+                const double* r = tab.w + boys_woff(M) + k * (M + 8);
                 const double d2 = d * 0.5, d3 = d * (1.0 / 3.0), d4 = d * 0.25, d5 = d * 0.2,
                              d6 = d * (1.0 / 6.0), d7 = d * (1.0 / 7.0);
 #pragma unroll
@@ -99,7 +117,7 @@ struct BoysEval<float, M> {
         if (T < float(BOYS_TMAX)) {
             const int k = __float2int_rn(T * float(1.0 / BOYS_STEP));
             const float d = k * float(BOYS_STEP) - T;
-            const float* r = tab.f + k * BOYS_MTAB + M;
+            const float* r = M <= BOYS_FW_MAX ? tab.fw + (M * BOYS_NGRID + k) * 5 : tab.f + k * BOYS_MTAB + M;
             float s = r[4];
             s = r[3] + d * (1.0f / 4.0f) * s;
             s = r[2] + d * (1.0f / 3.0f) * s;

@@ -128,9 +128,11 @@ struct m2_ctx {
     m2::DevBuf<int> d_fam_off, d_fam_np, d_pair_fam, d_pair_mem;
     std::vector<int> h_pair_fam;
     m2::DevBuf<int4> d_fam_mem;
-    // Boys tables
+    // Boys tables: full, FP32 copy, compact windows of the low-m classes
     m2::DevBuf<double> d_boys;
     m2::DevBuf<float> d_boys_f;
+    m2::DevBuf<double> d_boys_w;
+    m2::DevBuf<float> d_boys_fw;
     // J/K work buffers
     m2::DevBuf<double> d_D, d_J, d_K, d_Jo, d_Ko, d_Dn;
     m2::DevBuf<m2::Task> d_tasks;    // the gradient's task list
@@ -191,6 +193,9 @@ struct m2_ctx {
 };
 
 namespace m2 {
+inline BoysTables boys_tables(const m2_ctx* ctx) {
+    return BoysTables{ctx->d_boys.p, ctx->d_boys_f.p, ctx->d_boys_w.p, ctx->d_boys_fw.p};
+}
 m2_status la_handle(m2_ctx* ctx, void** h);
 void local_df_release(m2_ctx* ctx);   // fragment-local DF state (df_local.cpp)
 }

@@ -124,7 +124,7 @@ using namespace m2;
 
 namespace {
 
-BoysTables df_tables(m2_ctx* ctx) { return BoysTables{ctx->d_boys.p, ctx->d_boys_f.p}; }
+BoysTables df_tables(m2_ctx* ctx) { return boys_tables(ctx); }
 
 }  // namespace
 

@@ -198,7 +198,7 @@ m2_status m2_grad_ex(m2_ctx* ctx, int natom, const double* Z, const double* xyz,
     a.Dc = dDc.p;
     a.sh_cf = dcfo.p;
     a.nc = nc;
-    const BoysTables tab{ctx->d_boys.p, ctx->d_boys_f.p};
+    const BoysTables tab = boys_tables(ctx);
     for (const ComboRangeG& r : ranges) {
         // FP64 tasks [t0, tmid), FP32 tasks [tmid, t1) (mixed precision only)
         HIPCHK(launch_grad_2e(ctx->device, PAIR_L[r.X][0], PAIR_L[r.X][1], PAIR_L[r.Y][0], PAIR_L[r.Y][1], a, tab,
