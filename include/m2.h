@@ -145,6 +145,12 @@ m2_status m2_set_jk_families(m2_ctx* ctx, int mode);
  * primitive quartet moves a J/K element by < 1e-18 at precision 1e-12);
  * 0 computes every primitive quartet. */
 m2_status m2_set_prim_screening(m2_ctx* ctx, double relative);
+/* The same threshold for the FP32 quartets of an M2_MIXED build (no
+ * counterpart in the reference, which has no FP32 path).  Those quartets
+ * are computed with ~6e-8 relative rounding error, so dropping primitive
+ * quartets below relative * precision costs no accuracy they have.
+ * Default 0.1; 0 <= relative <= 1. */
+m2_status m2_set_prim_screening_fp32(m2_ctx* ctx, double relative);
 /* Launch shape of the direct J/K class kernels (no counterpart in the
  * reference, whose Fock::formJKdirect walks the shell quartets on the host,
  * src/fockbuild.cpp:64-164).  host_totals = 0 (default): each class kernel's

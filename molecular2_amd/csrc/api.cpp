@@ -130,6 +130,7 @@ JKArgs jk_args(m2_ctx* ctx) {
     a.prim_q = ctx->d_prim_q.p;
     a.fam_q = ctx->d_fam_q.p;
     a.prim_tau = 0.0;
+    a.d_lds_max = 2;
     a.n = ctx->obs.nbf;
     a.nsh = ctx->obs.nsh;
     if (ctx->fam_on) {
@@ -791,6 +792,7 @@ m2_status jk_enqueue(m2_ctx* ctx, const JKCall& c, const JKWork& w, bool prof, s
     a.fp32_below = ctx->fp32_below;
     a.scale = c.multiplier;
     a.prim_tau = c.precision * ctx->prim_screen;
+    const double prim_tau64 = a.prim_tau, prim_tau32 = c.precision * ctx->prim_screen32;
     for (size_t pi = 0; pi < plan.size(); ++pi) {
         const JKLaunch& pl = plan[pi];
         const JKList& L = ctx->lists[pl.ci];
@@ -814,6 +816,7 @@ m2_status jk_enqueue(m2_ctx* ctx, const JKCall& c, const JKWork& w, bool prof, s
                 a.blk_total = nullptr;
             }
             a.tail = tail;
+            a.prim_tau = tail ? prim_tau32 : prim_tau64;
             a.blk_off = L.off + tail * (L.nbra + 1);
             a.blk_bra = L.bra_of + (size_t)tail * L.cap;
             if (JK_COMBO[pl.X][pl.Y]) {
@@ -874,6 +877,7 @@ m2_status jk_graph_run(m2_ctx* ctx, const JKCall& c, const JKWork& w) {
     k.flags = c.flags;
     k.fp32_below = ctx->fp32_below;
     k.prim_screen = ctx->prim_screen;
+    k.prim_screen32 = ctx->prim_screen32;
     k.Dn = w.Dn;
     k.J = w.J;
     k.K = w.K;
@@ -1543,6 +1547,13 @@ m2_status m2_set_prim_screening(m2_ctx* ctx, double relative) {
     if (!ctx) return M2_ERR_ARG;
     ARGCHK(relative >= 0.0 && relative <= 1.0, "m2_set_prim_screening: relative threshold must lie in [0, 1]");
     ctx->prim_screen = relative;
+    return M2_OK;
+}
+
+m2_status m2_set_prim_screening_fp32(m2_ctx* ctx, double relative) {
+    if (!ctx) return M2_ERR_ARG;
+    ARGCHK(relative >= 0.0 && relative <= 1.0, "m2_set_prim_screening_fp32: relative threshold must lie in [0, 1]");
+    ctx->prim_screen32 = relative;
     return M2_OK;
 }
 

@@ -62,28 +62,59 @@ constexpr int BOYS_DIRECT_MAX = 3;
 
 template <int M>
 struct BoysEval<double, M> {
+    // the table branch covers T < TTAB; the asymptotic form the rest
+    static constexpr double TTAB = M <= BOYS_DIRECT_MAX ? BOYS_WK[M <= BOYS_DIRECT_MAX ? M : 0] * BOYS_STEP : BOYS_TMAX;
+    static constexpr int KMAX = M <= BOYS_DIRECT_MAX ? BOYS_WK[M <= BOYS_DIRECT_MAX ? M : 0] : BOYS_NGRID - 1;
+
     static __device__ __forceinline__ void run(double T, double* F, const BoysTables& tab) {
-        constexpr double TTAB = M <= BOYS_DIRECT_MAX ? BOYS_WK[M <= BOYS_DIRECT_MAX ? M : 0] * BOYS_STEP : BOYS_TMAX;
-        if (T < TTAB) {
-            const int k = __double2int_rn(T * (1.0 / BOYS_STEP));
-            const double d = k * BOYS_STEP - T;
-            if constexpr (M <= BOYS_DIRECT_MAX) {
-                const double* r = tab.w + boys_woff(M) + k * (M + 8);
-                const double d2 = d * 0.5, d3 = d * (1.0 / 3.0), d4 = d * 0.25, d5 = d * 0.2,
-                             d6 = d * (1.0 / 6.0), d7 = d * (1.0 / 7.0);
+        if (T < TTAB) table(T, F, tab);
+        else asym(T, F);
+    }
+
+    // T >= TTAB: F_0 = sqrt(pi/T)/2, F_(m+1) = (2m+1) F_m / (2T)
+    static __device__ __forceinline__ void asym(double T, double* F) {
+        const double rs = rsqrt_pos(T);
+        F[0] = 0.88622692545275801365 * rs;   // sqrt(pi)/2 / sqrt(T)
+        if constexpr (M > 0) {
+            const double h = 0.5 * rs * rs;
 #pragma unroll
-                for (int m = 0; m <= M; ++m) {
-                    double s = r[m + 6] + d7 * r[m + 7];
-                    s = r[m + 5] + d6 * s;
-                    s = r[m + 4] + d5 * s;
-                    s = r[m + 3] + d4 * s;
-                    s = r[m + 2] + d3 * s;
-                    s = r[m + 1] + d2 * s;
-                    F[m] = r[m] + d * s;
-                }
-                return;
+            for (int m = 0; m < M; ++m) F[m + 1] = (2 * m + 1) * F[m] * h;
+        }
+    }
+
+    // The table branch in two phases: load() gathers the row (and the offset
+    // d from the grid point), eval() sums the Taylor series.
+    static constexpr int NROW = M <= BOYS_DIRECT_MAX ? M + 8 : 8;
+    struct Row {
+        double r[NROW];
+        double d, T;
+    };
+    // 0 <= T <= TTAB (the row index is clamped to the table)
+    static __device__ __forceinline__ void load(double T, Row& w, const BoysTables& tab) {
+        const int k = min(__double2int_rn(T * (1.0 / BOYS_STEP)), KMAX);
+        w.d = k * BOYS_STEP - T;
+        w.T = T;
+        const double* r = M <= BOYS_DIRECT_MAX ? tab.w + boys_woff(M) + k * (M + 8) : tab.d + k * BOYS_MTAB + M;
+#pragma unroll
+        for (int i = 0; i < NROW; ++i) w.r[i] = r[i];
+    }
+    static __device__ __forceinline__ void eval(const Row& w, double* F) {
+        const double d = w.d;
+        const double* r = w.r;
+        if constexpr (M <= BOYS_DIRECT_MAX) {
+            const double d2 = d * 0.5, d3 = d * (1.0 / 3.0), d4 = d * 0.25, d5 = d * 0.2, d6 = d * (1.0 / 6.0),
+                         d7 = d * (1.0 / 7.0);
+#pragma unroll
+            for (int m = 0; m <= M; ++m) {
+                double s = r[m + 6] + d7 * r[m + 7];
+                s = r[m + 5] + d6 * s;
+                s = r[m + 4] + d5 * s;
+                s = r[m + 3] + d4 * s;
+                s = r[m + 2] + d3 * s;
+                s = r[m + 1] + d2 * s;
+                F[m] = r[m] + d * s;
             }
-            const double* r = tab.d + k * BOYS_MTAB + M;
+        } else {
             double s = r[7];
             s = r[6] + d * (1.0 / 7.0) * s;
             s = r[5] + d * (1.0 / 6.0) * s;
@@ -94,48 +125,71 @@ struct BoysEval<double, M> {
             s = r[0] + d * s;
             F[M] = s;
             if constexpr (M > 0) {
-                const double e = exp(-T);
-                const double t2 = 2.0 * T;
+                const double e = exp(-w.T);
+                const double t2 = 2.0 * w.T;
 #pragma unroll
                 for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0 / (2 * m + 1));
             }
-        } else {
-            const double rs = rsqrt_pos(T);
-            F[0] = 0.88622692545275801365 * rs;   // sqrt(pi)/2 / sqrt(T)
-            if constexpr (M > 0) {
-                const double h = 0.5 * rs * rs;
-#pragma unroll
-                for (int m = 0; m < M; ++m) F[m + 1] = (2 * m + 1) * F[m] * h;
-            }
         }
+    }
+    static __device__ __forceinline__ void table(double T, double* F, const BoysTables& tab) {
+        Row w;
+        load(T, w, tab);
+        eval(w, F);
     }
 };
 
 template <int M>
 struct BoysEval<float, M> {
+    static constexpr float TTAB = float(BOYS_TMAX);
+
     static __device__ __forceinline__ void run(float T, float* F, const BoysTables& tab) {
-        if (T < float(BOYS_TMAX)) {
-            const int k = __float2int_rn(T * float(1.0 / BOYS_STEP));
-            const float d = k * float(BOYS_STEP) - T;
-            const float* r = M <= BOYS_FW_MAX ? tab.fw + (M * BOYS_NGRID + k) * 5 : tab.f + k * BOYS_MTAB + M;
-            float s = r[4];
-            s = r[3] + d * (1.0f / 4.0f) * s;
-            s = r[2] + d * (1.0f / 3.0f) * s;
-            s = r[1] + d * (1.0f / 2.0f) * s;
-            s = r[0] + d * s;
-            F[M] = s;
-            if constexpr (M > 0) {
-                const float e = __expf(-T);
-                const float t2 = 2.0f * T;
+        if (T < TTAB) table(T, F, tab);
+        else asym(T, F);
+    }
+
+    // T >= BOYS_TMAX: the asymptotic branch in FP64
+    static __device__ __forceinline__ void asym(float T, float* F) {
+        double Fd[M + 1];
+        BoysEval<double, M>::asym(double(T), Fd);
 #pragma unroll
-                for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0f / (2 * m + 1));
-            }
-        } else {
-            double Fd[M + 1];
-            BoysEval<double, M>::run(double(T), Fd, tab);
+        for (int m = 0; m <= M; ++m) F[m] = float(Fd[m]);
+    }
+
+    static constexpr bool FW = M <= BOYS_FW_MAX;
+    struct Row {
+        float r[5];
+        float d, T;
+    };
+    // 0 <= T <= BOYS_TMAX: F_M by a 5-term Taylor series, the rest by the
+    // downward recursion
+    static __device__ __forceinline__ void load(float T, Row& w, const BoysTables& tab) {
+        const int k = min(__float2int_rn(T * float(1.0 / BOYS_STEP)), BOYS_NGRID - 1);
+        w.d = k * float(BOYS_STEP) - T;
+        w.T = T;
+        const float* r = FW ? tab.fw + (M * BOYS_NGRID + k) * 5 : tab.f + k * BOYS_MTAB + M;
 #pragma unroll
-            for (int m = 0; m <= M; ++m) F[m] = float(Fd[m]);
+        for (int i = 0; i < 5; ++i) w.r[i] = r[i];
+    }
+    static __device__ __forceinline__ void eval(const Row& w, float* F) {
+        const float d = w.d;
+        float s = w.r[4];
+        s = w.r[3] + d * (1.0f / 4.0f) * s;
+        s = w.r[2] + d * (1.0f / 3.0f) * s;
+        s = w.r[1] + d * (1.0f / 2.0f) * s;
+        s = w.r[0] + d * s;
+        F[M] = s;
+        if constexpr (M > 0) {
+            const float e = __expf(-w.T);
+            const float t2 = 2.0f * w.T;
+#pragma unroll
+            for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0f / (2 * m + 1));
         }
+    }
+    static __device__ __forceinline__ void table(float T, float* F, const BoysTables& tab) {
+        Row w;
+        load(T, w, tab);
+        eval(w, F);
     }
 };
 

@@ -79,7 +79,7 @@ struct JKGraphKey {
     const void* dD2;
     const void* dJ2;
     const void* dK2;
-    double multiplier, precision, fp32_below, prim_screen;
+    double multiplier, precision, fp32_below, prim_screen, prim_screen32;
     const void* Dn;
     const void* J;
     const void* K;
@@ -115,6 +115,10 @@ struct m2_ctx {
     // and the primitive screening threshold relative to the quartet precision
     m2::DevBuf<float> d_prim_q, d_fam_q;
     double prim_screen = 1e-6;
+    // the same for the FP32 quartets of a mixed build (their arithmetic
+    // carries ~6e-8 relative error, so skipped primitive quartets below
+    // precision * prim_screen32 stay within it)
+    double prim_screen32 = 0.1;
     // s-s ket family pairs (shells of one atom with identical exponents share
     // their primitive pairs), ordered like pairs: np bucket, Q descending
     int fam_mode = 1;                  // m2_set_jk_families

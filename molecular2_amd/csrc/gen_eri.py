@@ -239,13 +239,67 @@ def hrr_ket(la, lb, lc, ld):
     return node, defs
 
 
+def prim_body(la, lb, lc, ld, targets, interleave, pref, acc):
+    """Lines of one bra primitive of the bra loop (record `bp`), split into
+    (setup up to T, finish from the scaled F_m on); the Boys call sits in
+    between.  `acc(e, f)` names the accumulator of target (e, f)."""
+    L = la + lb + lc + ld
+    order, nodes = build_vrr(targets, L)
+    ind = "        "
+    setup = [ind + "const R p = R(bp.p), Kab = R(bp.K);"]
+    if la + lb > 0:
+        setup.append(ind + "const R inv2p = R(bp.inv2p);")
+        setup.append(ind + "const R PAx = R(bp.PA[0]), PAy = R(bp.PA[1]), PAz = R(bp.PA[2]);")
+    setup += [ind + "const R dx = R(bp.P[0]) - Qx, dy = R(bp.P[1]) - Qy, dz = R(bp.P[2]) - Qz;",
+              ind + "const R rs_pq = sqrt_inv<R>(p + q);",
+              ind + "const R inv_pq = rs_pq * rs_pq;",
+              ind + "const R rho = p * q * inv_pq;",
+              ind + "const R T = rho * (dx * dx + dy * dy + dz * dz);"]
+    fin = Emitter()
+    fin.out(ind + "const R pref = %s;" % pref)
+    fin.out("#pragma unroll")
+    fin.out(ind + "for (int m = 0; m <= %d; ++m) Fm[m] *= pref;" % L)
+    if L > 0:
+        fin.out(ind + "const R rho_p = q * inv_pq, rho_q = p * inv_pq, inv2pq = R(0.5) * inv_pq;")
+        fin.out(ind + "const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
+        fin.out(ind + "const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
+    if interleave:
+        # each target is accumulated right after its VRR node (shorter live ranges)
+        tset = set(targets)
+        for key in order:
+            emit_vrr(fin, [key], nodes, ind)
+            e, f, m = key
+            if m == 0 and (e, f) in tset:
+                fin.out(ind + "%s += %s;" % (acc(e, f), vrr_name(e, f, 0)))
+    else:
+        emit_vrr(fin, order, nodes, ind)
+        for e, f in targets:
+            fin.out(ind + "%s += %s;" % (acc(e, f), vrr_name(e, f, 0)))
+    return setup, fin.lines
+
+
+def emit_bra_loop(em, L, setup, fin):
+    """The bra primitive loop of one ket primitive; the wavefront stops at
+    the first bra primitive below every lane's cut (bra bounds are sorted).
+    (Two bra primitives per step, interleaved, measured slower on (H2O)20
+    cc-pVDZ for every class tried: the extra registers cost more occupancy
+    than the second chain hid latency.)"""
+    em.out("      for (int ib = 0; ib < nbra; ++ib) {")
+    em.out("        if (SCR && !__any(float(bq[ib]) * qk >= cut)) break;")
+    em.out("        const auto& bp = bra[ib];")
+    em.lines += setup
+    em.out("        R Fm[%d];" % (L + 1))
+    em.out("        BoysEval<R, %d>::run(T, Fm, tab);" % L)
+    em.lines += fin
+    em.out("      }")
+
+
 def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
     L = la + lb + lc + ld
     em = Emitter()
     E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
     F = [c for Lf in range(lc, lc + ld + 1) for c in comps(Lf)]
     targets = [(e, f) for e in E for f in F]
-    order, nodes = build_vrr(targets, L)
     na, nb, nc, nd = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
     em.out("template <> struct EriClass<%d, %d, %d, %d> {" % (la, lb, lc, ld))
     em.out("  static constexpr int L = %d;" % L)
@@ -278,41 +332,9 @@ def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
     if lc + ld > 0:
         em.out("      const R inv2q = R(kp.inv2p);")
         em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
-    em.out("      for (int ib = 0; ib < nbra; ++ib) {")
-    em.out("        if (SCR && !__any(float(bq[ib]) * qk >= cut)) break;")
-    em.out("        const auto& bp = bra[ib];")
-    em.out("        const R p = R(bp.p), Kab = R(bp.K);")
-    if la + lb > 0:
-        em.out("        const R inv2p = R(bp.inv2p);")
-        em.out("        const R PAx = R(bp.PA[0]), PAy = R(bp.PA[1]), PAz = R(bp.PA[2]);")
-    em.out("        const R dx = R(bp.P[0]) - Qx, dy = R(bp.P[1]) - Qy, dz = R(bp.P[2]) - Qz;")
-    em.out("        const R rs_pq = sqrt_inv<R>(p + q);")
-    em.out("        const R inv_pq = rs_pq * rs_pq;")
-    em.out("        const R rho = p * q * inv_pq;")
-    em.out("        const R T = rho * (dx * dx + dy * dy + dz * dz);")
-    em.out("        R Fm[%d];" % (L + 1))
-    em.out("        BoysEval<R, %d>::run(T, Fm, tab);" % L)
-    em.out("        const R pref = Kab * Kcd * rs_pq;")
-    em.out("#pragma unroll")
-    em.out("        for (int m = 0; m <= %d; ++m) Fm[m] *= pref;" % L)
-    if L > 0:
-        em.out("        const R rho_p = q * inv_pq, rho_q = p * inv_pq, inv2pq = R(0.5) * inv_pq;")
-        em.out("        const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
-        em.out("        const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
-    if interleave:
-        # each target is accumulated right after its VRR node (shorter live ranges)
-        tset = set(targets)
-        for key in order:
-            emit_vrr(em, [key], nodes, "        ")
-            e, f, m = key
-            if m == 0 and (e, f) in tset:
-                em.out("        a%s_%s += %s;" % (cname(e), cname(f), vrr_name(e, f, 0)))
-    else:
-        emit_vrr(em, order, nodes, "        ")
-        for e in E:
-            for f in F:
-                em.out("        a%s_%s += %s;" % (cname(e), cname(f), vrr_name(e, f, 0)))
-    em.out("      }")
+    setup, fin = prim_body(la, lb, lc, ld, targets, interleave, "Kab * Kcd * rs_pq",
+                           lambda e, f: "a%s_%s" % (cname(e), cname(f)))
+    emit_bra_loop(em, L, setup, fin)
     em.out("    }")
     # HRR: ket first for every e in E, then bra
     kn, kdefs = hrr_ket(la, lb, lc, ld)
@@ -444,7 +466,11 @@ def gen_class_fam(la, lb):
     screen = lines[i_loop + 1:i_kp]
     ket_head = [t.replace("const R q = R(kp.p), Kcd = R(kp.K);", "const R q = R(kp.p);")
                 for t in lines[i_kp:i_bra]]
-    bra_body = [t.replace("Kab * Kcd * rs_pq", "Kab * rs_pq") for t in lines[i_bra:i_acc]]
+    targets = [(e, (0, 0, 0)) for e in E]
+    setup, fin = prim_body(la, lb, 0, 0, targets, False, "Kab * rs_pq",
+                           lambda e, f: "t%s_%s" % (cname(e), cname(f)))
+    bra_loop = Emitter()
+    emit_bra_loop(bra_loop, la + lb, setup, fin)
     em = []
     em.append("  static constexpr int NACC_FAM = %d;   // accumulators of run_fam" % (len(accs) * (FAM_MAXMEM + 1)))
     em.append("  template <typename R, bool PURE, bool SCR, typename BPtr, typename KPtr, typename CPtr, typename QBPtr>")
@@ -460,10 +486,7 @@ def gen_class_fam(la, lb):
     em.extend(ket_head)
     for a in accs:
         em.append("      R t%s = R(0);" % a[1:])
-    em.extend(bra_body)
-    for e in E:
-        em.append("        t%s_%s += %s;" % (cname(e), cname((0, 0, 0)), vrr_name(e, (0, 0, 0), 0)))
-    em.append("      }")
+    em.extend(bra_loop.lines)
     for m in range(FAM_MAXMEM):
         em.append("      const R c%d = R(km[%d * ik + %d]);" % (m, FAM_MAXMEM, m))
     for m in range(FAM_MAXMEM):
@@ -479,8 +502,9 @@ def gen_class_fam(la, lb):
             em.append("  " + t.replace("out[", "outm["))
         em.append("    }")
     em.append("  }")
-    assert lines[-1] == "};"
-    return "\n".join(lines[:-1] + em + ["};"])
+    run = gen_class(la, lb, 0, 0, interleave=False).splitlines()
+    assert run[-1] == "};"
+    return "\n".join(run[:-1] + em + ["};"])
 
 
 def class_list(lmax_bra, lmax_ket):

@@ -55,11 +55,21 @@ __device__ __forceinline__ void krow_add(bool lds, double* __restrict__ sK, doub
         atomic_add_f64(&K[grow * n + col], v);
 }
 
+// D(row, col) for a row of the bra shells: rows r < NA are shell a's
+// (global row fa + r), the rest shell b's; from the LDS copy when staged
+template <int NA>
+__device__ __forceinline__ double drow(bool dl, const double* __restrict__ sD, const double* __restrict__ D, int n,
+                                       int fa, int fb, int r, int col) {
+    const double* row = dl ? sD + r * n : D + (r < NA ? fa + r : fb + r - NA) * n;
+    return row[col];
+}
+
 template <int NA, int NB, int NC, int ND, typename R>
 __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, int fc, int fd, double f,
                                        const double* __restrict__ Dab, double* __restrict__ jab,
                                        const double* __restrict__ D, int n, double* __restrict__ J,
-                                       double* __restrict__ K, bool lds, double* __restrict__ sK) {
+                                       double* __restrict__ K, bool lds, double* __restrict__ sK, bool dl,
+                                       const double* __restrict__ sD) {
     const double h = 0.5 * f;
     // J(ab) += f sum_cd D(cd) v  (lane registers)
 #pragma unroll
@@ -93,7 +103,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
-                for (int d = 0; d < ND; ++d) s += D[(fb + b) * n + fd + d] * double(v[((a * NB + b) * NC + c) * ND + d]);
+                for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, NA + b, fd + d) * double(v[((a * NB + b) * NC + c) * ND + d]);
             krow_add(lds, sK, K, n, a, fa + a, fc + c, h * s);
         }
     // K(bd) += h sum_ac D(ac) v
@@ -105,7 +115,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
-                for (int c = 0; c < NC; ++c) s += D[(fa + a) * n + fc + c] * double(v[((a * NB + b) * NC + c) * ND + d]);
+                for (int c = 0; c < NC; ++c) s += drow<NA>(dl, sD, D, n, fa, fb, a, fc + c) * double(v[((a * NB + b) * NC + c) * ND + d]);
             krow_add(lds, sK, K, n, NA + b, fb + b, fd + d, h * s);
         }
     // K(ad) += h sum_bc D(bc) v
@@ -117,7 +127,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
-                for (int c = 0; c < NC; ++c) s += D[(fb + b) * n + fc + c] * double(v[((a * NB + b) * NC + c) * ND + d]);
+                for (int c = 0; c < NC; ++c) s += drow<NA>(dl, sD, D, n, fa, fb, NA + b, fc + c) * double(v[((a * NB + b) * NC + c) * ND + d]);
             krow_add(lds, sK, K, n, a, fa + a, fd + d, h * s);
         }
     // K(bc) += h sum_ad D(ad) v
@@ -129,7 +139,7 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
 #pragma unroll
             for (int a = 0; a < NA; ++a)
 #pragma unroll
-                for (int d = 0; d < ND; ++d) s += D[(fa + a) * n + fd + d] * double(v[((a * NB + b) * NC + c) * ND + d]);
+                for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, a, fd + d) * double(v[((a * NB + b) * NC + c) * ND + d]);
             krow_add(lds, sK, K, n, NA + b, fb + b, fc + c, h * s);
         }
 }
@@ -178,6 +188,55 @@ struct JKOccupancy {
     static constexpr int min_per_eu = ps ? 5 : 1;
 };
 
+// Work item -> ket chunk `ch` of the bra's list: first ket and count (<= 64)
+// of the chunk; the chunks run over the ket groups' head segments [gbeg,
+// mid) (FP64) or tail segments [mid, end) (FP32).
+__device__ __forceinline__ void chunk_kets(const JKArgs& a, const __attribute__((address_space(4))) int* cut, int ch,
+                                           int& ket0, int& nket, int& grp) {
+    ket0 = 0;
+    nket = 0;
+    grp = -1;
+    int rem = ch;
+    for (int g = 0; g < a.ngrp; ++g) {
+        const int mid = cut[2 * g], end = cut[2 * g + 1];
+        const int s0 = a.tail ? mid : a.gbeg[g], s1 = a.tail ? end : mid;
+        const int nc = (s1 - s0 + 63) >> 6;
+        if (rem < nc) {
+            ket0 = s0 + 64 * rem;
+            nket = min(64, s1 - ket0);
+            grp = g;
+            return;
+        }
+        rem -= nc;
+    }
+}
+
+// Workgroup start: clear the K rows of the bra shells in LDS and stage the
+// bra shells' rows of the density norms (d_lds >= 1) and of D (and D2;
+// d_lds == 2) after them.  LDS layout: [K rows: NDEN (NA + NB) n]
+// [D rows: NDEN (NA + NB) n when d_lds == 2] [norm rows of a, b: 2 nsh].
+template <int NA, int NB, int NDEN>
+__device__ __forceinline__ void stage_bra_rows(const JKArgs& a, int sa, int sb, int fa, int fb, double* sK,
+                                               const double*& sD, const double*& sDn) {
+    const int n = a.n, nsh = a.nsh;
+    constexpr int NR = NA + NB;
+    const int nk = a.k_lds ? NDEN * NR * n : 0;
+    const int nd = a.d_lds == 2 ? NDEN * NR * n : 0;
+    double* d = sK + nk;
+    double* dn = d + nd;
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) sK[i] = 0.0;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+        const int set = i / (NR * n), r = i - set * NR * n;
+        const int row = r / n, col = r - row * n;
+        d[i] = (set ? a.D2 : a.D)[(row < NA ? fa + row : fb + row - NA) * n + col];
+    }
+    if (a.d_lds)
+        for (int i = threadIdx.x; i < 2 * nsh; i += blockDim.x) dn[i] = a.Dn[(i < nsh ? sa : sb) * nsh + i % nsh];
+    sD = d;
+    sDn = dn;
+    __syncthreads();
+}
+
 // Quartets are computed when lo <= bound < hi (the launchers pass
 // [precision, inf); precision is chosen per task list).
 // Dynamic LDS (when a.k_lds): the (NA + NB) x n FP64 K rows of shells a and b.
@@ -219,9 +278,9 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
     typedef const __attribute__((address_space(4))) float cfl;
     cfl* bq = (cfl*)(a.prim_q + a.pair_off[bra]);
     extern __shared__ double sK[];
-    if (lds)
-        for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
-    __syncthreads();
+    const double *sD, *sDn;
+    stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
+    const bool dl = a.d_lds == 2, dnl = a.d_lds != 0;
     const double* D = a.D;
     const double* D2 = a.D2;
     double Dab[NA * NB], jab[NA * NB];
@@ -245,25 +304,16 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
     const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
     unsigned long long cnt_q = 0, cnt_pq = 0, cnt_cpq = 0;   // profiling counters of this wave
     for (int ch = c0 + wave; ch < c1; ch += W) {
-        // chunk -> ket group segment
-        int ket0 = 0, nket = 0, rem = ch;
-        for (int g = 0; g < a.ngrp; ++g) {
-            const int mid = cut[2 * g], end = cut[2 * g + 1];
-            const int s0 = a.tail ? mid : a.gbeg[g], s1 = a.tail ? end : mid;
-            const int nc = (s1 - s0 + 63) >> 6;
-            if (rem < nc) {
-                ket0 = s0 + 64 * rem;
-                nket = min(64, s1 - ket0);
-                break;
-            }
-            rem -= nc;
-        }
+        int ket0, nket, g;
+        chunk_kets(a, cut, ch, ket0, nket, g);
         const int ket = ket0 + (lane < nket ? lane : 0);
         const int2 scd = a.pair_sh[ket];
         const int sc = scd.x, sd = scd.y;
         // exact screening of src/fockbuild.cpp:98-115
         const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]),
-                               fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+                               dnl ? fmax(fmax(sDn[sc], sDn[sd]), fmax(sDn[nsh + sc], sDn[nsh + sd]))
+                                   : fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]),
+                                          fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
         const double bound = dn * qab * a.pair_Q[ket];
         const bool active = lane < nket && bound >= lo && bound < hi;
         if (!__any(active)) continue;
@@ -272,7 +322,7 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
         const int nkp = active ? a.pair_np[ket] : 0;
         if (a.counters) {
             cnt_q += __popcll(__ballot(active));
-            cnt_pq += (unsigned long long)(wave_sum(active ? double(a.pair_np[ket]) : 0.0) * nbp + 0.5);
+            cnt_pq += (unsigned long long)(wave_sum(double(nkp)) * nbp + 0.5);
             cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
         const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
@@ -281,10 +331,11 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
                                                               ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq, cut);
         if (active) {
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
-            digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK);
+            digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK, dl,
+                                      sD);
             if constexpr (NDEN > 1)
                 digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab2, jab2, D2, n, a.J2, a.K2, lds,
-                                          sK + (NA + NB) * n);
+                                          sK + (NA + NB) * n, dl, sD + (NA + NB) * n);
         }
     }
     if (a.counters && lane == 0 && cnt_q) {
@@ -384,9 +435,9 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
     const bool same = a.pair_fam != nullptr && a.combo == 0;
     const int bfam = same ? a.pair_fam[bra] : -1, bmem = same ? a.pair_mem[bra] : 0;
     extern __shared__ double sK[];
-    if (lds)
-        for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) sK[i] = 0.0;
-    __syncthreads();
+    const double *sD, *sDn;
+    stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
+    const bool dl = a.d_lds == 2, dnl = a.d_lds != 0;
     const double* D = a.D;
     const double* D2 = a.D2;
     double Dab[NA * NB], jab[NA * NB];
@@ -410,18 +461,8 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
     const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
     unsigned long long cnt_q = 0, cnt_pq = 0, cnt_cpq = 0;
     for (int ch = c0 + wave; ch < c1; ch += W) {
-        int ket0 = 0, nket = 0, rem = ch;
-        for (int g = 0; g < a.ngrp; ++g) {
-            const int mid = cut[2 * g], end = cut[2 * g + 1];
-            const int s0 = a.tail ? mid : a.gbeg[g], s1 = a.tail ? end : mid;
-            const int nc = (s1 - s0 + 63) >> 6;
-            if (rem < nc) {
-                ket0 = s0 + 64 * rem;
-                nket = min(64, s1 - ket0);
-                break;
-            }
-            rem -= nc;
-        }
+        int ket0, nket, g;
+        chunk_kets(a, cut, ch, ket0, nket, g);
         const int ket = ket0 + (lane < nket ? lane : 0);
         const int4 mem4 = a.fam_mem[ket];
         const int mem[FAM_MAXMEM] = {mem4.x, mem4.y, mem4.z, mem4.w};
@@ -436,8 +477,10 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
             if (lane < nket && j >= 0 && !(same && ket == bfam && m > bmem)) {
                 const int2 scd = a.pair_sh[j];
                 const int sc = scd.x, sd = scd.y;
-                const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]), fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]),
-                                                                           fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+                const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]),
+                                       dnl ? fmax(fmax(sDn[sc], sDn[sd]), fmax(sDn[nsh + sc], sDn[nsh + sd]))
+                                           : fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]),
+                                                  fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
                 const double bound = dn * qab * a.pair_Q[j];
                 act[m] = bound >= lo && bound < hi;
                 if (act[m]) dnmax = fmax(dnmax, dn);
@@ -465,10 +508,10 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
                 const int sc = scd.x, sd = scd.y;
                 const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == mem[m] ? 1.0 : 2.0);
                 digest<NA, NB, 1, 1, R>(v + m * NCART, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K,
-                                        lds, sK);
+                                        lds, sK, dl, sD);
                 if constexpr (NDEN > 1)
                     digest<NA, NB, 1, 1, R>(v + m * NCART, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab2, jab2, D2, n,
-                                            a.J2, a.K2, lds, sK + (NA + NB) * n);
+                                            a.J2, a.K2, lds, sK + (NA + NB) * n, dl, sD + (NA + NB) * n);
             }
     }
     if (a.counters && lane == 0 && cnt_q) {
@@ -527,6 +570,21 @@ __global__ __launch_bounds__(256) void k_schwarz(JKArgs a, BoysTables tab, int p
     Qout[pr] = sqrt(mx);
 }
 
+// LDS of one J/K workgroup: the K rows when they fit JK_KROW_LDS_MAX, then
+// the staged density rows as far as the whole stays within JK_DROW_LDS_MAX
+// (norm rows first, D rows too when there is room).  Sets a.k_lds, a.d_lds.
+template <int NA, int NB, int NDEN>
+size_t jk_lds_plan(JKArgs& a) {
+    const size_t rows = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
+    const size_t norms = sizeof(double) * 2 * (size_t)a.nsh;
+    a.k_lds = rows <= JK_KROW_LDS_MAX ? 1 : 0;
+    a.d_lds = 0;
+    if (!a.k_lds) return 0;
+    if (2 * rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 2) a.d_lds = 2;
+    else if (rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 1) a.d_lds = 1;
+    return rows + (a.d_lds == 2 ? rows : 0) + (a.d_lds ? norms : 0);
+}
+
 // mode 0: FP64 kernel over the FP64 heads of the ket groups; mode 2: FP32
 // kernel over the FP32 tails (a.tail set by the caller).  `nblocks`
 // workgroups of the work list in a.blk_bra.
@@ -534,18 +592,17 @@ template <int LA, int LB, int LC, int LD, bool PURE, int NDEN>
 void launch_jk_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     JKArgs a = a0;
-    const size_t kbytes = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
-    a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
     const double inf = __builtin_huge_val();
-    const size_t lds = a.k_lds ? kbytes : 0;
     if (mode == 2) {
         auto k = k_jk<LA, LB, LC, LD, float, PURE, NDEN>;
-        const int th = 64 * JKOccupancy<LA, LB, LC, LD, float>::waves;
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+        constexpr int W = JKOccupancy<LA, LB, LC, LD, float>::waves;
+        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     } else {
         auto k = k_jk<LA, LB, LC, LD, double, PURE, NDEN>;
-        const int th = 64 * JKOccupancy<LA, LB, LC, LD, double>::waves;
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+        constexpr int W = JKOccupancy<LA, LB, LC, LD, double>::waves;
+        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     }
 }
 
@@ -560,18 +617,17 @@ template <int LA, int LB, bool PURE, int NDEN>
 void launch_jk_fam_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     JKArgs a = a0;
-    const size_t kbytes = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
-    a.k_lds = kbytes <= JK_KROW_LDS_MAX ? 1 : 0;
     const double inf = __builtin_huge_val();
-    const size_t lds = a.k_lds ? kbytes : 0;
     if (mode == 2) {
         auto k = k_jk_fam<LA, LB, float, PURE, NDEN>;
-        const int th = 64 * JKWavesFam<LA, LB, float>::value;
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+        constexpr int W = JKWavesFam<LA, LB, float>::value;
+        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     } else {
         auto k = k_jk_fam<LA, LB, double, PURE, NDEN>;
-        const int th = 64 * JKWavesFam<LA, LB, double>::value;
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(th), lds, st, a, tab, a.precision, inf);
+        constexpr int W = JKWavesFam<LA, LB, double>::value;
+        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     }
 }
 

@@ -51,6 +51,7 @@ constexpr long long JK_LIST_MAX_ITEMS = 1LL << 24;   // work items per list and 
 constexpr int GRAD_PROG_KETS = 4, GRAD_PROG_SPLIT = 8;
 constexpr int JK_MAXG = 10;
 constexpr size_t JK_KROW_LDS_MAX = 96 * 1024;
+constexpr size_t JK_DROW_LDS_MAX = 64 * 1024;   // K rows + staged density rows
 
 struct JKArgs {
     const PrimPair* prims;
@@ -73,6 +74,9 @@ struct JKArgs {
     unsigned long long* counters;  // profiling only: per class combo {quartets, primitive quartets, computed ones}
     int combo;               // 6 * X + Y of the launch
     int k_lds;               // set by the launcher: K rows of the bra shells accumulated in LDS
+    int d_lds;               // set by the launcher: the bra shells' rows of the density norms (1) and also
+                             // of D (2) staged in LDS after the K rows
+    int d_lds_max;           // the most staging the launcher may choose (0, 1, 2)
     // device work list of the launch (tasks.hip): workgroup -> bra, and the
     // bra's ket segments per primitive-count group of the ket class
     const int2* cut;         // [bra][JK_MAXG] (mid, end)

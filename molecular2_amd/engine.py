@@ -100,6 +100,7 @@ def lib():
         L.m2_reset_profile.argtypes = [vp]
         L.m2_get_profile_computed.argtypes = [vp, _ll]
         L.m2_set_prim_screening.argtypes = [vp, C.c_double]
+        L.m2_set_prim_screening_fp32.argtypes = [vp, C.c_double]
         L.m2_df_setup.argtypes = [vp]
         L.m2_jk_df.argtypes = [vp, _dp, C.c_int, C.c_int, _dp, _dp]
         L.m2_jk_df_device.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp]
@@ -179,7 +180,7 @@ def lib():
             getattr(L, name).restype = C.c_int
         for name in ("m2_set_profiling", "m2_get_profile", "m2_reset_profile", "m2_ctx_create", "m2_basis_set", "m2_basis_info", "m2_shell_coefs", "m2_schwarz",
                      "m2_shellblock_norm", "m2_one_body", "m2_jk_direct", "m2_jk_direct_device",
-                     "m2_jk_direct_device_noalloc", "m2_get_profile_computed", "m2_set_prim_screening",
+                     "m2_jk_direct_device_noalloc", "m2_get_profile_computed", "m2_set_prim_screening", "m2_set_prim_screening_fp32",
                      "m2_set_mixed_threshold", "m2_last_stats"):
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -482,6 +483,11 @@ class Engine:
         self._chk(self.L.m2_jk_autotune(self.h, C.c_void_p(dD_ptr), precision, M2_MIXED if mixed else M2_FP64,
                                         int(repeats), C.byref(b), C.byref(a)))
         return b.value, a.value
+
+    def set_prim_screening_fp32(self, relative):
+        """primitive screening of the FP32 quartets of a mixed build, relative
+        to precision (default 0.1)"""
+        self._chk(self.L.m2_set_prim_screening_fp32(self.h, float(relative)))
 
     def set_prim_screening(self, relative):
         """primitive quartets below relative * precision (bound q_ab q_cd
