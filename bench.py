@@ -325,6 +325,17 @@ def main():
         sweep.append({"fp32_below": thr, "ms": round(1e3 * tt / a.steps, 3), "abs_dE2_Eh": de,
                       "max_abs_dJ": float(np.max(np.abs(Jt - J64))), "max_abs_dK": float(np.max(np.abs(Kt - K64))),
                       "_t": tt, "_J": Jt, "_K": Kt})
+    # per-class FP32 / FP64 kernel time: one profiled build with every
+    # quartet in FP32 (timing only: the FP32 split of the mixed build keeps
+    # the large quartets in FP64)
+    eng.set_mixed_threshold(1e300)
+    run(a.warmup, True)
+    eng.reset_profile()
+    eng.set_profiling(True)
+    run(a.steps, True)
+    torch.cuda.synchronize()
+    eng.set_profiling(False)
+    prof32 = eng.profile()
     # the reported mixed build uses the SCF's own split, fixed before timing
     fixed = next(x for x in sweep if x["fp32_below"] == 1e-4)
     tmx, Jm, Km, thr_mx = fixed["_t"], fixed["_J"], fixed["_K"], fixed["fp32_below"]
@@ -388,7 +399,11 @@ def main():
                   "max_abs_dJ": float(np.max(np.abs(Jm - J64))), "max_abs_dK": float(np.max(np.abs(Km - K64))),
                   "abs_dE2_Eh": float(abs(emx - e64)), "sweep": sweep,
                   "fastest_split_within_1uEh": best["fp32_below"],
-                  "fastest_speedup_within_1uEh": round(t64 / best["_t"], 3)},
+                  "fastest_speedup_within_1uEh": round(t64 / best["_t"], 3),
+                  "class_fp64_over_fp32_time": {
+                      cf[k]["cls"]: round(cf[k]["ms"] / max(prof32[k][0], 1e-9), 3) for k in cf if k in prof32},
+                  "class_ratio_note": "per class combination, FP64 kernel time over its time with every quartet in "
+                                      "FP32 (a timing-only build)"},
         "classes": {v["cls"]: {"ms_per_build": round(v["ms"] / a.steps, 3),
                                "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3),
                                "quartets": v["quartets"] // a.steps} for v in cf.values()},

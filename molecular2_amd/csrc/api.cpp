@@ -649,6 +649,10 @@ m2_status build_pairs(m2_ctx* ctx) {
             L.bra_fam = fam && X == Y ? ctx->d_pair_fam.p : nullptr;
             L.ngrp = (int)sy.groups.size();
             L.kc = reg ? 64 : JK_HEAVY_KETS;
+            // the split and table-driven classes (l_a + l_b + l_c + l_d >= 6)
+            // stay in FP64 in mixed builds: their few quartets gain less from
+            // FP32 than a second launch costs
+            L.fp32 = PAIR_L[X][0] + PAIR_L[X][1] + PAIR_L[Y][0] + PAIR_L[Y][1] < 6 ? 1 : 0;
             long long maxch = 0;   // chunks of the bra with the most kets
             for (int g = 0; g < L.ngrp; ++g) {
                 L.gbeg[g] = sy.groups[g].begin;

@@ -507,6 +507,133 @@ def gen_class_fam(la, lb):
     return "\n".join(run[:-1] + em + ["};"])
 
 
+# Classes whose J/K quartets run as split register code (k_jk_split): their
+# VRR accumulators do not fit one lane's registers, so the quartet is
+# evaluated in passes, one per Cartesian component c of shell c ("c") or per
+# pair of Cartesian components (c, d) ("cd").  A pass accumulates only the
+# targets [e0|f0] its ket HRR needs -- f = c + d' for the sub-monomials d'
+# of its d -- and returns (a b | c d) for its d (a, b spherical for pure
+# bases); the lower VRR levels are recomputed in every pass.
+SPLIT = {(2, 1, 2, 1): "c", (2, 2, 1, 1): "cd", (2, 2, 2, 0): "c", (2, 2, 2, 1): "cd"}
+
+
+def submonos(d):
+    return [(x, y, z) for x in range(d[0] + 1) for y in range(d[1] + 1) for z in range(d[2] + 1)]
+
+
+def gen_class_split(la, lb, lc, ld):
+    L = la + lb + lc + ld
+    E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
+    Ac, Bc, Cc, Dc = comps(la), comps(lb), comps(lc), comps(ld)
+    mode = SPLIT[(la, lb, lc, ld)]
+    passes = [(ci, 0, len(Dc)) for ci in range(len(Cc))] if mode == "c" else \
+        [(ci, di, 1) for ci in range(len(Cc)) for di in range(len(Dc))]
+
+    def harm(l):
+        if l < 2:
+            return [[(i, 1.0)] for i in range((l + 1) * (l + 2) // 2)]
+        return [[(i, x) for i, x in enumerate(row) if abs(x) > 1e-15] for row in solid_harmonics(l)]
+
+    ha, hb, hc = harm(la), harm(lb), harm(lc)
+    maxnd = max(p[2] for p in passes)
+    em = Emitter()
+    em.out("template <> struct EriSplit<%d, %d, %d, %d> {" % (la, lb, lc, ld))
+    em.out("  static constexpr int L = %d;" % L)
+    em.out("  static constexpr int NPASS = %d;" % len(passes))
+    em.out("  // pass p: Cartesian component PC[p] of shell c, components PD0[p] .. PD0[p] + PND[p] - 1 of d")
+    em.out("  static constexpr int PC[NPASS] = {%s};" % ", ".join(str(p[0]) for p in passes))
+    em.out("  static constexpr int PD0[NPASS] = {%s};" % ", ".join(str(p[1]) for p in passes))
+    em.out("  static constexpr int PND[NPASS] = {%s};" % ", ".join(str(p[2]) for p in passes))
+    em.out("  static constexpr int NOUT = %d;   // outputs of a pass at most, (a, b, d) order" %
+           (len(Ac) * len(Bc) * maxnd))
+    args = ("BPtr bra, int nbra, KPtr ket, int nket, const R ABx, const R ABy, const R ABz, "
+            "const R CDx, const R CDy, const R CDz, const BoysTables& tab, R* __restrict__ out, "
+            "QBPtr bq, const float* __restrict__ kq, float cut")
+    for pi, (ci, d0, nd) in enumerate(passes):
+        c = Cc[ci]
+        ds = Dc[d0:d0 + nd]
+        Fs = sorted({tuple(c[i] + dd[i] for i in range(3)) for d in ds for dd in submonos(d)},
+                    key=lambda f: (sum(f), f))
+        targets = [(e, f) for e in E for f in Fs]
+        em.out("  // pass %d: c = %s, d = %s, %d VRR targets" % (pi, cname(c), " ".join(cname(d) for d in ds),
+                                                               len(targets)))
+        em.out("  template <typename R, bool PURE, bool SCR, typename BPtr, typename KPtr, typename QBPtr>")
+        em.out("  static __device__ __forceinline__ void pass%d(%s) {" % (pi, args))
+        for e, f in targets:
+            em.out("    R a%s_%s = R(0);" % (cname(e), cname(f)))
+        em.out("    const float bq0 = SCR ? float(bq[0]) : 0.f;")
+        em.out("    for (int ik = 0; ik < nket; ++ik) {")
+        em.out("      const float qk = SCR ? kq[ik] : 0.f;")
+        em.out("      if (SCR && qk * bq0 < cut) break;")
+        em.out("      const auto kp = ket[ik];")
+        em.out("      const R q = R(kp.p), Kcd = R(kp.K);")
+        em.out("      const R Qx = R(kp.P[0]), Qy = R(kp.P[1]), Qz = R(kp.P[2]);")
+        em.out("      const R inv2q = R(kp.inv2p);")
+        em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
+        setup, fin = prim_body(la, lb, lc, ld, targets, True, "Kab * Kcd * rs_pq",
+                               lambda e, f: "a%s_%s" % (cname(e), cname(f)))
+        emit_bra_loop(em, L, setup, fin)
+        em.out("    }")
+        node, kdefs = hrr_ket(la, lb, lc, ld)
+        ket_names = {(e, d): node(e, c, d) for e in E for d in ds}
+        for t in kdefs:
+            em.out("    " + t)
+        bseen, bdefs = {}, []
+
+        def bnode(a, b, d):
+            key = (a, b, d)
+            if key in bseen:
+                return bseen[key]
+            if sum(b) == 0:
+                bseen[key] = ket_names[(a, d)]
+                return bseen[key]
+            i = next(k for k in range(3) if b[k] > 0)
+            n1 = bnode(add(a, i), sub(b, i), d)
+            n0 = bnode(a, sub(b, i), d)
+            nm = "h%s_%s_%s" % (cname(a), cname(b), cname(d))
+            bdefs.append("const R %s = %s + AB%s * %s;" % (nm, n1, AX[i], n0))
+            bseen[key] = nm
+            return nm
+
+        cart = {(ia, ib, idd): bnode(a, b, d) for ia, a in enumerate(Ac) for ib, b in enumerate(Bc)
+                for idd, d in enumerate(ds)}
+        for t in bdefs:
+            em.out("    " + t)
+        em.out("    if constexpr (PURE) {")
+        for ia, ra in enumerate(ha):
+            for ib, rb in enumerate(hb):
+                for idd in range(nd):
+                    terms = []
+                    for ca, xa in ra:
+                        for cb, xb in rb:
+                            x = xa * xb
+                            src = cart[(ca, cb, idd)]
+                            terms.append(src if x == 1.0 else "%s * %s" % (lit(x), src))
+                    em.out("      out[%d] = %s;" % ((ia * len(hb) + ib) * nd + idd, " + ".join(terms)))
+        em.out("    } else {")
+        for (ia, ib, idd), nm in sorted(cart.items()):
+            em.out("      out[%d] = %s;" % ((ia * len(Bc) + ib) * nd + idd, nm))
+        em.out("    }")
+        em.out("  }")
+    # spherical components of shell c fed by each Cartesian component
+    cm = [[(m, x) for m, row in enumerate(hc) for i, x in row if i == ci] for ci in range(len(Cc))]
+    w = max(len(t) for t in cm)
+    em.out("  // spherical components m of shell c fed by its Cartesian component i, and the factors")
+    em.out("  static constexpr int CN[%d] = {%s};" % (len(Cc), ", ".join(str(len(t)) for t in cm)))
+    em.out("  static constexpr int CM[%d][%d] = {%s};" % (len(Cc), w, ", ".join(
+        "{%s}" % ", ".join(str(t[k][0]) if k < len(t) else "0" for k in range(w)) for t in cm)))
+    em.out("  static constexpr double CC[%d][%d] = {%s};" % (len(Cc), w, ", ".join(
+        "{%s}" % ", ".join(repr(float(t[k][1])) if k < len(t) else "0.0" for k in range(w)) for t in cm)))
+    em.out("  template <int P, typename R, bool PURE, bool SCR, typename BPtr, typename KPtr, typename QBPtr>")
+    em.out("  static __device__ __forceinline__ void run(%s) {" % args)
+    call = "(bra, nbra, ket, nket, ABx, ABy, ABz, CDx, CDy, CDz, tab, out, bq, kq, cut)"
+    for pi in range(len(passes)):
+        em.out("    %sif constexpr (P == %d) pass%d<R, PURE, SCR>%s;" % ("" if pi == 0 else "else ", pi, pi, call))
+    em.out("  }")
+    em.out("};")
+    return "\n".join(em.lines)
+
+
 def class_list(lmax_bra, lmax_ket):
     out = []
     for la in range(lmax_bra + 1):
@@ -533,6 +660,14 @@ def main(outdir):
         body = gen_class_fam(c[0], c[1]) if (c[2], c[3]) == (0, 0) and (c[0], c[1]) in FAM_BRA else gen_class(*c)
         text = "// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.\n#pragma once\n" + body + "\n"
         path = os.path.join(outdir, "eri_%d%d%d%d.inc" % c)
+        if not os.path.exists(path) or open(path).read() != text:
+            with open(path, "w") as fh:
+                fh.write(text)
+        written.append(path)
+    for c in SPLIT:
+        text = "// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.\n#pragma once\n" + \
+            gen_class_split(*c) + "\n"
+        path = os.path.join(outdir, "eri_split_%d%d%d%d.inc" % c)
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as fh:
                 fh.write(text)
@@ -702,6 +837,16 @@ def write_tus(outdir):
     for X, (la, lb) in enumerate(PAIR_CLASSES):
         for Y in range(X + 1):
             lc, ld = PAIR_CLASSES[Y]
+            if (la, lb, lc, ld) in SPLIT:
+                lines = list(head)
+                lines.append('#include "eri_split_%d%d%d%d.inc"' % (la, lb, lc, ld))
+                lines.append("void launch_jk_%d_%d(int pure, const JKArgs& a, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {" % (X, Y))
+                lines.append("  if (pure) launch_jk_split<%d, %d, %d, %d, true>(a, tab, nblocks, mode, st);" % (la, lb, lc, ld))
+                lines.append("  else launch_jk_split<%d, %d, %d, %d, false>(a, tab, nblocks, mode, st);" % (la, lb, lc, ld))
+                lines.append("}")
+                lines.append("}  // namespace m2")
+                put("jk_%d_%d.hip" % (X, Y), lines)
+                continue
             if (la, lb, lc, ld) in HEAVY:
                 continue
             lines = list(head)
@@ -728,14 +873,18 @@ def write_tus(outdir):
     def heavy(X, Y):
         return PAIR_CLASSES[X] + PAIR_CLASSES[Y] in HEAVY
 
+    def table_jk(X, Y):   # J/K through the table-driven heavy kernel
+        return heavy(X, Y) and PAIR_CLASSES[X] + PAIR_CLASSES[Y] not in SPLIT
+
     for X in range(len(PAIR_CLASSES)):
         for Y in range(X + 1):
-            if not heavy(X, Y):
+            if not table_jk(X, Y):
                 lines.append("void launch_jk_%d_%d(int, const JKArgs&, const BoysTables&, int, int, hipStream_t);" % (X, Y))
     lines.append("typedef void (*jk_combo_launcher)(int, const JKArgs&, const BoysTables&, int, int, hipStream_t);")
     lines.append("const jk_combo_launcher JK_COMBO[%d][%d] = {" % (len(PAIR_CLASSES), len(PAIR_CLASSES)))
     for X in range(len(PAIR_CLASSES)):
-        row = ["launch_jk_%d_%d" % (X, Y) if Y <= X and not heavy(X, Y) else "nullptr" for Y in range(len(PAIR_CLASSES))]
+        row = ["launch_jk_%d_%d" % (X, Y) if Y <= X and not table_jk(X, Y) else "nullptr"
+               for Y in range(len(PAIR_CLASSES))]
         lines.append("  {" + ", ".join(row) + "},")
     lines.append("};")
     lines.append("typedef void (*sw_launcher)(int, const JKArgs&, const BoysTables&, int, int, double*, hipStream_t);")

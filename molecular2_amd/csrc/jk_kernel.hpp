@@ -23,6 +23,7 @@
 #pragma once
 #include "m2_common.hpp"
 #include "boys.hpp"
+#include <type_traits>
 
 namespace m2 {
 
@@ -546,6 +547,285 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
                                                                                double hi) {
     if (a.blk_total && (int)blockIdx.x >= *a.blk_total) return;
     jk_fam_item<LA, LB, R, PURE, NDEN>(a, tab, lo, hi, blockIdx.x);
+}
+
+// Split register classes (csrc/gen_eri.py SPLIT): the quartet runs in one
+// pass per Cartesian component P of shell c, each returning the Cartesian
+// (a b | c_P d) for all a, b, d (a, b then made spherical).  The pass is
+// digested at once: shell c's spherical components m take c_P with the
+// factors T[m][P] (EriSplit::CM / CC), so every contraction of
+// src/fockbuild.cpp:144-149 with a c index runs over the passes:
+//   J(ab) += f sum_d [sum_m T D(c_m, d)] (ab|c_P d)
+//   J(c_m d) += f T sum_ab D(ab) (ab|c_P d)
+//   K(a c_m) += h T sum_bd D(bd) (ab|c_P d),  K(b c_m) likewise with D(ad)
+//   K(bd) += h sum_a [sum_m T D(a c_m)] (ab|c_P d),  K(ad) likewise.
+template <int LA, int LB, int LC, int LD>
+struct EriSplit;
+
+template <int NA, int NB, int NDEN>
+size_t jk_lds_plan(JKArgs& a);
+
+template <int P, int N>
+struct PassLoop {
+    template <typename F>
+    static __device__ __forceinline__ void run(F& f) {
+        f(std::integral_constant<int, P>{});
+        PassLoop<P + 1, N>::run(f);
+    }
+};
+template <int N>
+struct PassLoop<N, N> {
+    template <typename F>
+    static __device__ __forceinline__ void run(F&) {}
+};
+
+// pass P of a split class: its d components [d0, d0 + nd) and the
+// spherical components of shell c its Cartesian component feeds
+template <typename S, bool TC, int P>
+struct PassTerms {
+    static constexpr int c = S::PC[P];
+    static constexpr int d0 = S::PD0[P];
+    static constexpr int nd = S::PND[P];
+    static constexpr int n = TC ? S::CN[c] : 1;
+    static constexpr __device__ int m(int k) { return TC ? S::CM[c][k] : c; }
+    static constexpr __device__ double t(int k) { return TC ? S::CC[c][k] : 1.0; }
+};
+
+template <typename PT, int NA, int NB, typename R>
+__device__ __forceinline__ void split_digest(const R* __restrict__ v, int fa, int fb, int fc, int fd, double f,
+                                             const double* __restrict__ Dab, double* __restrict__ jab,
+                                             const double* __restrict__ D, int n, double* __restrict__ J,
+                                             double* __restrict__ K, bool lds, double* __restrict__ sK, bool dl,
+                                             const double* __restrict__ sD) {
+    const double h = 0.5 * f;
+    constexpr int ND = PT::nd;
+    fd += PT::d0;
+    // J(ab)
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double dcd = 0.0;
+#pragma unroll
+        for (int k = 0; k < PT::n; ++k) dcd += PT::t(k) * D[(fc + PT::m(k)) * n + fd + d];
+        dcd *= f;
+#pragma unroll
+        for (int x = 0; x < NA * NB; ++x) jab[x] += dcd * double(v[x * ND + d]);
+    }
+    // J(c_m d)
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int x = 0; x < NA * NB; ++x) s += Dab[x] * double(v[x * ND + d]);
+#pragma unroll
+        for (int k = 0; k < PT::n; ++k) atomic_add_f64(&J[(fc + PT::m(k)) * n + fd + d], f * PT::t(k) * s);
+    }
+    // K(a c_m) and K(a d)
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, NA + b, fd + d) * double(v[(a * NB + b) * ND + d]);
+#pragma unroll
+        for (int k = 0; k < PT::n; ++k) krow_add(lds, sK, K, n, a, fa + a, fc + PT::m(k), h * PT::t(k) * s);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                double dbc = 0.0;
+#pragma unroll
+                for (int k = 0; k < PT::n; ++k) dbc += PT::t(k) * drow<NA>(dl, sD, D, n, fa, fb, NA + b, fc + PT::m(k));
+                s += dbc * double(v[(a * NB + b) * ND + d]);
+            }
+            krow_add(lds, sK, K, n, a, fa + a, fd + d, h * s);
+        }
+    // K(b c_m) and K(b d)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, a, fd + d) * double(v[(a * NB + b) * ND + d]);
+#pragma unroll
+        for (int k = 0; k < PT::n; ++k) krow_add(lds, sK, K, n, NA + b, fb + b, fc + PT::m(k), h * PT::t(k) * s);
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                double dac = 0.0;
+#pragma unroll
+                for (int k = 0; k < PT::n; ++k) dac += PT::t(k) * drow<NA>(dl, sD, D, n, fa, fb, a, fc + PT::m(k));
+                s += dac * double(v[(a * NB + b) * ND + d]);
+            }
+            krow_add(lds, sK, K, n, NA + b, fb + b, fd + d, h * s);
+        }
+}
+
+// One work item of k_jk_split: as jk_item, with the quartet evaluated and
+// digested pass by pass.  4-wave workgroups, one wavefront per SIMD, so a
+// pass may use the whole register file.
+constexpr int JK_SPLIT_WAVES = 4;
+template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN>
+__device__ __forceinline__ void jk_split_item(const JKArgs& a, const BoysTables& tab, double lo, double hi,
+                                              int witem) {
+    using S = EriSplit<LA, LB, LC, LD>;
+    constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
+    static_assert(LD <= 1, "the d index of a split class is not transformed");
+    constexpr bool TC = PURE && LC >= 2;
+    constexpr int W = JK_SPLIT_WAVES;
+    typedef const __attribute__((address_space(4))) int cint;
+    const int bl = ((cint*)a.blk_bra)[witem];
+    const int lb = witem - ((cint*)a.blk_off)[bl];
+    const int bra = a.bra0 + bl;
+    cint* cut = (cint*)a.cut + 2 * bl * JK_MAXG;
+    int nch = 0;
+    for (int g = 0; g < a.ngrp; ++g) {
+        const int mid = cut[2 * g], end = cut[2 * g + 1];
+        nch += ((a.tail ? end - mid : mid - a.gbeg[g]) + 63) >> 6;
+    }
+    const int c0 = lb * a.block_tasks, c1 = min(c0 + a.block_tasks, nch);
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int2 sab = a.pair_sh[bra];
+    const int sa = __builtin_amdgcn_readfirstlane(sab.x), sb = __builtin_amdgcn_readfirstlane(sab.y);
+    const int n = a.n, nsh = a.nsh;
+    const int fa = a.sh_bf[sa], fb = a.sh_bf[sb];
+    const bool lds = a.k_lds != 0;
+    using PP = typename PrimPairOf<R>::type;
+    const PP* gp = PrimPairOf<R>::get(a);
+    typedef const __attribute__((address_space(4))) PP cPP;
+    const int nbp = a.pair_np[bra];
+    cPP* bprims = (cPP*)(gp + a.pair_off[bra]);
+    typedef const __attribute__((address_space(4))) float cfl;
+    cfl* bq = (cfl*)(a.prim_q + a.pair_off[bra]);
+    extern __shared__ double sK[];
+    const double *sD, *sDn;
+    stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
+    const bool dl = a.d_lds == 2, dnl = a.d_lds != 0;
+    const double* D = a.D;
+    const double* D2 = a.D2;
+    double Dab[NA * NB], jab[NA * NB];
+    double Dab2[NDEN > 1 ? NA * NB : 1], jab2[NDEN > 1 ? NA * NB : 1];
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+            Dab[x * NB + y] = D[(fa + x) * n + fb + y];
+            jab[x * NB + y] = 0.0;
+            if constexpr (NDEN > 1) {
+                Dab2[x * NB + y] = D2[(fa + x) * n + fb + y];
+                jab2[x * NB + y] = 0.0;
+            }
+        }
+    const double* O = a.sh_O;
+    const R ABx = R(O[3 * sa] - O[3 * sb]), ABy = R(O[3 * sa + 1] - O[3 * sb + 1]), ABz = R(O[3 * sa + 2] - O[3 * sb + 2]);
+    const double* Dn = a.Dn;
+    const double dnab = Dn[sa * nsh + sb];
+    const double qab = a.pair_Q[bra];
+    const double fab = (sa == sb ? 1.0 : 2.0) * a.scale;
+    unsigned long long cnt_q = 0, cnt_pq = 0, cnt_cpq = 0;
+    for (int ch = c0 + wave; ch < c1; ch += W) {
+        int ket0, nket, g;
+        chunk_kets(a, cut, ch, ket0, nket, g);
+        const int ket = ket0 + (lane < nket ? lane : 0);
+        const int2 scd = a.pair_sh[ket];
+        const int sc = scd.x, sd = scd.y;
+        // exact screening of src/fockbuild.cpp:98-115
+        const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]),
+                               dnl ? fmax(fmax(sDn[sc], sDn[sd]), fmax(sDn[nsh + sc], sDn[nsh + sd]))
+                                   : fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]),
+                                          fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+        const double bound = dn * qab * a.pair_Q[ket];
+        const bool active = lane < nket && bound >= lo && bound < hi;
+        if (!__any(active)) continue;
+        const float* kq = a.prim_q + a.pair_off[ket];
+        const float cut = float(a.prim_tau / dn);
+        const int nkp = active ? a.pair_np[ket] : 0;
+        if (a.counters) {
+            cnt_q += __popcll(__ballot(active));
+            cnt_pq += (unsigned long long)(wave_sum(double(nkp)) * nbp + 0.5);
+            cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
+        }
+        const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
+        const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+        const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
+        const PP* kprims = gp + a.pair_off[ket];
+        auto pass = [&](auto PC) {
+            constexpr int P = decltype(PC)::value;
+            using PT = PassTerms<S, TC, P>;
+            R v[NA * NB * PT::nd];
+            S::template run<P, R, PURE, true>(bprims, nbp, kprims, nkp, ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq,
+                                              cut);
+            if (active) {
+                split_digest<PT, NA, NB, R>(v, fa, fb, fc, fd, f, Dab, jab, D, n, a.J, a.K, lds, sK, dl, sD);
+                if constexpr (NDEN > 1)
+                    split_digest<PT, NA, NB, R>(v, fa, fb, fc, fd, f, Dab2, jab2, D2, n, a.J2, a.K2, lds,
+                                                sK + (NA + NB) * n, dl, sD + (NA + NB) * n);
+            }
+        };
+        PassLoop<0, S::NPASS>::run(pass);
+    }
+    if (a.counters && lane == 0 && cnt_q) {
+        atomicAdd(&a.counters[3 * a.combo], cnt_q);
+        atomicAdd(&a.counters[3 * a.combo + 1], cnt_pq);
+        atomicAdd(&a.counters[3 * a.combo + 2], cnt_cpq);
+    }
+#pragma unroll
+    for (int x = 0; x < NA * NB; ++x) {
+        const double s = wave_sum(jab[x]);
+        if (lane == 0 && s != 0.0) atomic_add_f64(&a.J[(fa + x / NB) * n + fb + x % NB], s);
+        if constexpr (NDEN > 1) {
+            const double s2 = wave_sum(jab2[x]);
+            if (lane == 0 && s2 != 0.0) atomic_add_f64(&a.J2[(fa + x / NB) * n + fb + x % NB], s2);
+        }
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < NDEN * (NA + NB) * n; i += blockDim.x) {
+        const double s = sK[i];
+        if (s != 0.0) {
+            const int set = i / ((NA + NB) * n), r = i - set * (NA + NB) * n;
+            const int row = r / n, col = r - row * n;
+            atomic_add_f64(&(set ? a.K2 : a.K)[(row < NA ? fa + row : fb + row - NA) * n + col], s);
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN>
+__global__ __launch_bounds__(JK_SPLIT_WAVES * 64, 1) void k_jk_split(JKArgs a, BoysTables tab, double lo, double hi) {
+    if (a.blk_total && (int)blockIdx.x >= *a.blk_total) return;
+    jk_split_item<LA, LB, LC, LD, R, PURE, NDEN>(a, tab, lo, hi, blockIdx.x);
+}
+
+template <int LA, int LB, int LC, int LD, bool PURE, int NDEN>
+void launch_jk_split_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
+    constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
+    JKArgs a = a0;
+    const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+    const double inf = __builtin_huge_val();
+    if (mode == 2)
+        hipLaunchKernelGGL((k_jk_split<LA, LB, LC, LD, float, PURE, NDEN>), dim3(nblocks), dim3(64 * JK_SPLIT_WAVES), lds,
+                           st, a, tab, a.precision, inf);
+    else
+        hipLaunchKernelGGL((k_jk_split<LA, LB, LC, LD, double, PURE, NDEN>), dim3(nblocks), dim3(64 * JK_SPLIT_WAVES),
+                           lds, st, a, tab, a.precision, inf);
+}
+
+template <int LA, int LB, int LC, int LD, bool PURE>
+void launch_jk_split(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode, hipStream_t st) {
+    if (nblocks <= 0) return;
+    if (a0.nden > 1) launch_jk_split_n<LA, LB, LC, LD, PURE, 2>(a0, tab, nblocks, mode, st);
+    else launch_jk_split_n<LA, LB, LC, LD, PURE, 1>(a0, tab, nblocks, mode, st);
 }
 
 // Schwarz: Q(pair) = sqrt(max_ab |(ab|ab)|), one lane per pair

@@ -19,6 +19,7 @@ struct JKList {
     int bound;                // most work items any density gives at this bt (per precision)
     const double* Qk;         // Schwarz bounds the ket ranges are sorted by (pairs or family pairs)
     const int* bra_fam;       // same class with family kets: bra pair -> its family pair (else null)
+    int fp32;                 // mixed builds may run this combination's small quartets in FP32
 };
 
 // one class combination of the gradient's task list (tasks.hip)

@@ -7,7 +7,8 @@
 // (src/fockbuild.cpp:98-115), max|D_blocks| Q_i Q_j >= precision, form a
 // prefix of each group: Q_j >= precision / (Q_i max|D|).  Under mixed
 // precision the prefix splits again at max(precision, fp32_below) / (Q_i
-// max|D|) into an FP64 head and an FP32 tail.  With X == Y only kets j <= i
+// max|D|) into an FP64 head and an FP32 tail (combinations with fp32 unset
+// keep the whole prefix in FP64).  With X == Y only kets j <= i
 // are unique quartets (src/fockbuild.cpp:88-109).  With s-s family kets the
 // groups hold family pairs, sorted by the largest member Q; the triangle is
 // cut at the bra's own family and finished per member by the kernel.
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(256) void k_task_count(const JKList* __restrict__ l
     int ch64 = 0, ch32 = 0;
     const bool any = qi > 0.0 && Dmax > 0.0;
     const double thr = any ? precision / (qi * Dmax) : 0.0;
-    const double thr64 = any ? (mixed ? fmax(precision, fp32_below) / (qi * Dmax) : thr) : 0.0;
+    const double thr64 = any ? (mixed && c.fp32 ? fmax(precision, fp32_below) / (qi * Dmax) : thr) : 0.0;
     for (int g = 0; g < c.ngrp; ++g) {
         const int gb = c.gbeg[g], ge = c.gend[g];
         int end = gb, mid = gb;

@@ -75,3 +75,25 @@ def test_batch_without_minimal_shells_rejects_soad(engine):
     st = L.m2_rhf_batch(0, 1, 1, ip(z), ip(z), dp(d), dp(d), 1, ip(z), ip(z), ip(z), dp(d), dp(d), dp(d), ip(z),
                         C.byref(o), res)
     assert st != 0 and b"minimal shells" in L.m2_rhf_last_error()
+
+
+def test_batch_matches_oracle_energies(engine):
+    """The batch against the oracle's SCF::rhf restatement over its
+    formJKdirect restatement (tests/golden/batch_oracle.json, written by
+    tests/golden/make_batch_oracle.py with the same molecules, guess and
+    convergence options): every molecule both converge agrees to 1e-7 Eh."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "batch_oracle.json")) as fh:
+        ref = json.load(fh)
+    batch = M.random_batch(256, seed=256)
+    mols = [batch[r["index"]] for r in ref["molecules"]]
+    assert [m.natom for m in mols] == [m.natom for m in _small_batch()]
+    print("case: batch of %d molecules vs oracle energies" % len(mols), flush=True)
+    out, _ = engine.rhf_batch(mols, nworkers=4, guess=ref["guess"], **ref["options"])
+    both = 0
+    for r, o in zip(out, ref["molecules"]):
+        if r["converged"] and o["converged"]:
+            both += 1
+            assert abs(r["energy"] - o["energy"]) < 1e-7, (o["index"], r["energy"], o["energy"])
+    assert both >= 4
