@@ -47,9 +47,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // K-row accumulation: LDS row (row < NA: shell a, else shell b) or HBM
-__device__ __forceinline__ void krow_add(bool lds, double* __restrict__ sK, double* __restrict__ K, int n, int row,
-                                         int grow, int col, double v) {
-    if (lds)
+template <int LM>
+__device__ __forceinline__ void krow_add(double* __restrict__ sK, double* __restrict__ K, int n, int row, int grow,
+                                         int col, double v) {
+    if constexpr (LM >= 1)
         __hip_atomic_fetch_add((__attribute__((address_space(3))) double*)(sK + row * n + col), v, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);   // ds_add_f64
     else
@@ -57,20 +58,21 @@ __device__ __forceinline__ void krow_add(bool lds, double* __restrict__ sK, doub
 }
 
 // D(row, col) for a row of the bra shells: rows r < NA are shell a's
-// (global row fa + r), the rest shell b's; from the LDS copy when staged
-template <int NA>
-__device__ __forceinline__ double drow(bool dl, const double* __restrict__ sD, const double* __restrict__ D, int n,
-                                       int fa, int fb, int r, int col) {
-    const double* row = dl ? sD + r * n : D + (r < NA ? fa + r : fb + r - NA) * n;
-    return row[col];
+// (global row fa + r), the rest shell b's; from the LDS copy when staged.
+// LM, the LDS mode of a workgroup: 2 = K rows and D rows in LDS, 1 = K rows
+// only, 0 = neither (K accumulates with global atomics)
+template <int NA, int LM>
+__device__ __forceinline__ double drow(const double* __restrict__ sD, const double* __restrict__ D, int n, int fa,
+                                       int fb, int r, int col) {
+    if constexpr (LM == 2) return sD[r * n + col];
+    else return D[(r < NA ? fa + r : fb + r - NA) * n + col];
 }
 
-template <int NA, int NB, int NC, int ND, typename R>
+template <int NA, int NB, int NC, int ND, int LM, typename R>
 __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, int fc, int fd, double f,
                                        const double* __restrict__ Dab, double* __restrict__ jab,
                                        const double* __restrict__ D, int n, double* __restrict__ J,
-                                       double* __restrict__ K, bool lds, double* __restrict__ sK, bool dl,
-                                       const double* __restrict__ sD) {
+                                       double* __restrict__ K, double* __restrict__ sK, const double* __restrict__ sD) {
     const double h = 0.5 * f;
     // J(ab) += f sum_cd D(cd) v  (lane registers)
 #pragma unroll
@@ -83,7 +85,56 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
 #pragma unroll
                 for (int b = 0; b < NB; ++b) jab[a * NB + b] += dcd * double(v[((a * NB + b) * NC + c) * ND + d]);
         }
-    // J(cd) += f sum_ab D(ab) v
+    // K(ac) += h sum_bd D(bd) v
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) s += drow<NA, LM>(sD, D, n, fa, fb, NA + b, fd + d) * double(v[((a * NB + b) * NC + c) * ND + d]);
+            krow_add<LM>(sK, K, n, a, fa + a, fc + c, h * s);
+        }
+    // K(bd) += h sum_ac D(ac) v
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s += drow<NA, LM>(sD, D, n, fa, fb, a, fc + c) * double(v[((a * NB + b) * NC + c) * ND + d]);
+            krow_add<LM>(sK, K, n, NA + b, fb + b, fd + d, h * s);
+        }
+    // K(ad) += h sum_bc D(bc) v
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s += drow<NA, LM>(sD, D, n, fa, fb, NA + b, fc + c) * double(v[((a * NB + b) * NC + c) * ND + d]);
+            krow_add<LM>(sK, K, n, a, fa + a, fd + d, h * s);
+        }
+    // K(bc) += h sum_ad D(ad) v
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) s += drow<NA, LM>(sD, D, n, fa, fb, a, fd + d) * double(v[((a * NB + b) * NC + c) * ND + d]);
+            krow_add<LM>(sK, K, n, NA + b, fb + b, fc + c, h * s);
+        }
+    // J(cd) += f sum_ab D(ab) v (global atomics last: later loads of this
+    // digestion need not wait for them)
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -95,54 +146,14 @@ __device__ __forceinline__ void digest(const R* __restrict__ v, int fa, int fb, 
                 for (int b = 0; b < NB; ++b) s += Dab[a * NB + b] * double(v[((a * NB + b) * NC + c) * ND + d]);
             atomic_add_f64(&J[(fc + c) * n + fd + d], f * s);
         }
-    // K(ac) += h sum_bd D(bd) v
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            double s = 0.0;
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, NA + b, fd + d) * double(v[((a * NB + b) * NC + c) * ND + d]);
-            krow_add(lds, sK, K, n, a, fa + a, fc + c, h * s);
-        }
-    // K(bd) += h sum_ac D(ac) v
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            double s = 0.0;
-#pragma unroll
-            for (int a = 0; a < NA; ++a)
-#pragma unroll
-                for (int c = 0; c < NC; ++c) s += drow<NA>(dl, sD, D, n, fa, fb, a, fc + c) * double(v[((a * NB + b) * NC + c) * ND + d]);
-            krow_add(lds, sK, K, n, NA + b, fb + b, fd + d, h * s);
-        }
-    // K(ad) += h sum_bc D(bc) v
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            double s = 0.0;
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int c = 0; c < NC; ++c) s += drow<NA>(dl, sD, D, n, fa, fb, NA + b, fc + c) * double(v[((a * NB + b) * NC + c) * ND + d]);
-            krow_add(lds, sK, K, n, a, fa + a, fd + d, h * s);
-        }
-    // K(bc) += h sum_ad D(ad) v
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            double s = 0.0;
-#pragma unroll
-            for (int a = 0; a < NA; ++a)
-#pragma unroll
-                for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, a, fd + d) * double(v[((a * NB + b) * NC + c) * ND + d]);
-            krow_add(lds, sK, K, n, NA + b, fb + b, fc + c, h * s);
-        }
+}
+
+// f(std::integral_constant<int, LM>) for a workgroup's LDS mode (uniform)
+template <typename F>
+__device__ __forceinline__ void with_lds_mode(int lm, F&& f) {
+    if (lm == 2) f(std::integral_constant<int, 2>{});
+    else if (lm == 1) f(std::integral_constant<int, 1>{});
+    else f(std::integral_constant<int, 0>{});
 }
 
 // Primitive quartets whose bound passes the lane's cut (profiling only; the
@@ -281,7 +292,8 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
     extern __shared__ double sK[];
     const double *sD, *sDn;
     stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
-    const bool dl = a.d_lds == 2, dnl = a.d_lds != 0;
+    const bool dnl = a.d_lds != 0;
+    const int lm = lds ? (a.d_lds == 2 ? 2 : 1) : 0;   // LDS mode of the digestion
     const double* D = a.D;
     const double* D2 = a.D2;
     double Dab[NA * NB], jab[NA * NB];
@@ -332,11 +344,14 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
                                                               ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq, cut);
         if (active) {
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
-            digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K, lds, sK, dl,
-                                      sD);
-            if constexpr (NDEN > 1)
-                digest<NA, NB, NC, ND, R>(v, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab2, jab2, D2, n, a.J2, a.K2, lds,
-                                          sK + (NA + NB) * n, dl, sD + (NA + NB) * n);
+            const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+            with_lds_mode(lm, [&](auto M) {
+                constexpr int LM = decltype(M)::value;
+                digest<NA, NB, NC, ND, LM, R>(v, fa, fb, fc, fd, f, Dab, jab, D, n, a.J, a.K, sK, sD);
+                if constexpr (NDEN > 1)
+                    digest<NA, NB, NC, ND, LM, R>(v, fa, fb, fc, fd, f, Dab2, jab2, D2, n, a.J2, a.K2,
+                                                  sK + (NA + NB) * n, sD + (NA + NB) * n);
+            });
         }
     }
     if (a.counters && lane == 0 && cnt_q) {
@@ -438,7 +453,8 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
     extern __shared__ double sK[];
     const double *sD, *sDn;
     stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
-    const bool dl = a.d_lds == 2, dnl = a.d_lds != 0;
+    const bool dnl = a.d_lds != 0;
+    const int lm = lds ? (a.d_lds == 2 ? 2 : 1) : 0;   // LDS mode of the digestion
     const double* D = a.D;
     const double* D2 = a.D2;
     double Dab[NA * NB], jab[NA * NB];
@@ -508,11 +524,14 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
                 const int2 scd = a.pair_sh[mem[m]];
                 const int sc = scd.x, sd = scd.y;
                 const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == mem[m] ? 1.0 : 2.0);
-                digest<NA, NB, 1, 1, R>(v + m * NCART, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab, jab, D, n, a.J, a.K,
-                                        lds, sK, dl, sD);
-                if constexpr (NDEN > 1)
-                    digest<NA, NB, 1, 1, R>(v + m * NCART, fa, fb, a.sh_bf[sc], a.sh_bf[sd], f, Dab2, jab2, D2, n,
-                                            a.J2, a.K2, lds, sK + (NA + NB) * n, dl, sD + (NA + NB) * n);
+                const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+                with_lds_mode(lm, [&](auto M) {
+                    constexpr int LM = decltype(M)::value;
+                    digest<NA, NB, 1, 1, LM, R>(v + m * NCART, fa, fb, fc, fd, f, Dab, jab, D, n, a.J, a.K, sK, sD);
+                    if constexpr (NDEN > 1)
+                        digest<NA, NB, 1, 1, LM, R>(v + m * NCART, fa, fb, fc, fd, f, Dab2, jab2, D2, n, a.J2, a.K2,
+                                                    sK + (NA + NB) * n, sD + (NA + NB) * n);
+                });
             }
     }
     if (a.counters && lane == 0 && cnt_q) {
@@ -591,11 +610,11 @@ struct PassTerms {
     static constexpr __device__ double t(int k) { return TC ? S::CC[c][k] : 1.0; }
 };
 
-template <typename PT, int NA, int NB, typename R>
+template <typename PT, int NA, int NB, int LM, typename R>
 __device__ __forceinline__ void split_digest(const R* __restrict__ v, int fa, int fb, int fc, int fd, double f,
                                              const double* __restrict__ Dab, double* __restrict__ jab,
                                              const double* __restrict__ D, int n, double* __restrict__ J,
-                                             double* __restrict__ K, bool lds, double* __restrict__ sK, bool dl,
+                                             double* __restrict__ K, double* __restrict__ sK,
                                              const double* __restrict__ sD) {
     const double h = 0.5 * f;
     constexpr int ND = PT::nd;
@@ -610,15 +629,6 @@ __device__ __forceinline__ void split_digest(const R* __restrict__ v, int fa, in
 #pragma unroll
         for (int x = 0; x < NA * NB; ++x) jab[x] += dcd * double(v[x * ND + d]);
     }
-    // J(c_m d)
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        double s = 0.0;
-#pragma unroll
-        for (int x = 0; x < NA * NB; ++x) s += Dab[x] * double(v[x * ND + d]);
-#pragma unroll
-        for (int k = 0; k < PT::n; ++k) atomic_add_f64(&J[(fc + PT::m(k)) * n + fd + d], f * PT::t(k) * s);
-    }
     // K(a c_m) and K(a d)
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
@@ -626,9 +636,9 @@ __device__ __forceinline__ void split_digest(const R* __restrict__ v, int fa, in
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, NA + b, fd + d) * double(v[(a * NB + b) * ND + d]);
+            for (int d = 0; d < ND; ++d) s += drow<NA, LM>(sD, D, n, fa, fb, NA + b, fd + d) * double(v[(a * NB + b) * ND + d]);
 #pragma unroll
-        for (int k = 0; k < PT::n; ++k) krow_add(lds, sK, K, n, a, fa + a, fc + PT::m(k), h * PT::t(k) * s);
+        for (int k = 0; k < PT::n; ++k) krow_add<LM>(sK, K, n, a, fa + a, fc + PT::m(k), h * PT::t(k) * s);
     }
 #pragma unroll
     for (int a = 0; a < NA; ++a)
@@ -639,10 +649,10 @@ __device__ __forceinline__ void split_digest(const R* __restrict__ v, int fa, in
             for (int b = 0; b < NB; ++b) {
                 double dbc = 0.0;
 #pragma unroll
-                for (int k = 0; k < PT::n; ++k) dbc += PT::t(k) * drow<NA>(dl, sD, D, n, fa, fb, NA + b, fc + PT::m(k));
+                for (int k = 0; k < PT::n; ++k) dbc += PT::t(k) * drow<NA, LM>(sD, D, n, fa, fb, NA + b, fc + PT::m(k));
                 s += dbc * double(v[(a * NB + b) * ND + d]);
             }
-            krow_add(lds, sK, K, n, a, fa + a, fd + d, h * s);
+            krow_add<LM>(sK, K, n, a, fa + a, fd + d, h * s);
         }
     // K(b c_m) and K(b d)
 #pragma unroll
@@ -651,9 +661,9 @@ __device__ __forceinline__ void split_digest(const R* __restrict__ v, int fa, in
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int d = 0; d < ND; ++d) s += drow<NA>(dl, sD, D, n, fa, fb, a, fd + d) * double(v[(a * NB + b) * ND + d]);
+            for (int d = 0; d < ND; ++d) s += drow<NA, LM>(sD, D, n, fa, fb, a, fd + d) * double(v[(a * NB + b) * ND + d]);
 #pragma unroll
-        for (int k = 0; k < PT::n; ++k) krow_add(lds, sK, K, n, NA + b, fb + b, fc + PT::m(k), h * PT::t(k) * s);
+        for (int k = 0; k < PT::n; ++k) krow_add<LM>(sK, K, n, NA + b, fb + b, fc + PT::m(k), h * PT::t(k) * s);
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -664,11 +674,20 @@ __device__ __forceinline__ void split_digest(const R* __restrict__ v, int fa, in
             for (int a = 0; a < NA; ++a) {
                 double dac = 0.0;
 #pragma unroll
-                for (int k = 0; k < PT::n; ++k) dac += PT::t(k) * drow<NA>(dl, sD, D, n, fa, fb, a, fc + PT::m(k));
+                for (int k = 0; k < PT::n; ++k) dac += PT::t(k) * drow<NA, LM>(sD, D, n, fa, fb, a, fc + PT::m(k));
                 s += dac * double(v[(a * NB + b) * ND + d]);
             }
-            krow_add(lds, sK, K, n, NA + b, fb + b, fd + d, h * s);
+            krow_add<LM>(sK, K, n, NA + b, fb + b, fd + d, h * s);
         }
+    // J(c_m d), global atomics last
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int x = 0; x < NA * NB; ++x) s += Dab[x] * double(v[x * ND + d]);
+#pragma unroll
+        for (int k = 0; k < PT::n; ++k) atomic_add_f64(&J[(fc + PT::m(k)) * n + fd + d], f * PT::t(k) * s);
+    }
 }
 
 // One work item of k_jk_split: as jk_item, with the quartet evaluated and
@@ -711,7 +730,8 @@ __device__ __forceinline__ void jk_split_item(const JKArgs& a, const BoysTables&
     extern __shared__ double sK[];
     const double *sD, *sDn;
     stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
-    const bool dl = a.d_lds == 2, dnl = a.d_lds != 0;
+    const bool dnl = a.d_lds != 0;
+    const int lm = lds ? (a.d_lds == 2 ? 2 : 1) : 0;   // LDS mode of the digestion
     const double* D = a.D;
     const double* D2 = a.D2;
     double Dab[NA * NB], jab[NA * NB];
@@ -766,12 +786,14 @@ __device__ __forceinline__ void jk_split_item(const JKArgs& a, const BoysTables&
             R v[NA * NB * PT::nd];
             S::template run<P, R, PURE, true>(bprims, nbp, kprims, nkp, ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq,
                                               cut);
-            if (active) {
-                split_digest<PT, NA, NB, R>(v, fa, fb, fc, fd, f, Dab, jab, D, n, a.J, a.K, lds, sK, dl, sD);
-                if constexpr (NDEN > 1)
-                    split_digest<PT, NA, NB, R>(v, fa, fb, fc, fd, f, Dab2, jab2, D2, n, a.J2, a.K2, lds,
-                                                sK + (NA + NB) * n, dl, sD + (NA + NB) * n);
-            }
+            if (active)
+                with_lds_mode(lm, [&](auto M) {
+                    constexpr int LM = decltype(M)::value;
+                    split_digest<PT, NA, NB, LM, R>(v, fa, fb, fc, fd, f, Dab, jab, D, n, a.J, a.K, sK, sD);
+                    if constexpr (NDEN > 1)
+                        split_digest<PT, NA, NB, LM, R>(v, fa, fb, fc, fd, f, Dab2, jab2, D2, n, a.J2, a.K2,
+                                                        sK + (NA + NB) * n, sD + (NA + NB) * n);
+                });
         };
         PassLoop<0, S::NPASS>::run(pass);
     }
