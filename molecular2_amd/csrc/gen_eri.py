@@ -1741,17 +1741,25 @@ def gen_deriv_class(la, lb, lc, ld, mask=7):
     em.out("  template <typename R, typename GamFill>")
     em.out("  static __device__ __forceinline__ void run(const PrimPair* __restrict__ bra, int nbra,")
     em.out("      const PrimPair* __restrict__ ket, int nket, const R ABx, const R ABy, const R ABz,")
-    em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, const GamFill& gam_fill, R* __restrict__ g) {")
+    em.out("      const R CDx, const R CDy, const R CDz, const BoysTables& tab, const GamFill& gam_fill, R* __restrict__ g,")
+    em.out("      const float* __restrict__ bq, const float* __restrict__ kq, float cut) {")
     for nm, (T_, _) in sets.items():
         for e, f in T_:
             em.out("    R %s_%s_%s = R(0);" % (nm, cname(e), cname(f)))
+    # primitive screening as in the J/K classes: primitive pair bounds sorted
+    # descending, a lane stops at its first ket primitive below its cut, the
+    # lanes of the bra loop at the first bra primitive below every lane's cut
+    em.out("    const float bq0 = bq[0];")
     em.out("    for (int ik = 0; ik < nket; ++ik) {")
+    em.out("      const float qk = kq[ik];")
+    em.out("      if (qk * bq0 < cut) break;")
     em.out("      const PrimPair kp = ket[ik];")
     em.out("      const R q = R(kp.p), Kcd = R(kp.K), wC = R(2) * R(kp.a);")
     em.out("      const R Qx = R(kp.P[0]), Qy = R(kp.P[1]), Qz = R(kp.P[2]);")
     em.out("      const R inv2q = R(kp.inv2p);")
     em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
     em.out("      for (int ib = 0; ib < nbra; ++ib) {")
+    em.out("        if (!__any(bq[ib] * qk >= cut)) break;")
     em.out("        const PrimPair bp = bra[ib];")
     em.out("        const R p = R(bp.p), Kab = R(bp.K), wA = R(2) * R(bp.a), wB = R(2) * (R(bp.p) - R(bp.a));")
     em.out("        const R inv2p = R(bp.inv2p);")

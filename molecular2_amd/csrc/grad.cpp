@@ -159,6 +159,8 @@ m2_status m2_grad_ex(m2_ctx* ctx, int natom, const double* Z, const double* xyz,
     a.nsh = nsh;
     a.pure = b.pure;
     a.precision = precision;
+    a.prim_q = ctx->d_prim_q.p;
+    a.prim_tau = precision * ctx->prim_screen;
     a.grad = dG.p;
     // D in the Cartesian basis (Dc = T D T^T; T maps each pure d shell's five
     // functions to its six Cartesian components, identity elsewhere), so that

@@ -32,6 +32,12 @@ struct GradArgs {
     const double* Dc;                // D in the Cartesian basis, nc x nc
     const int* sh_cf;                // first Cartesian function of each shell
     int nc;
+    // primitive screening (the J/K kernels' rule): bounds sqrt(max |(ab|ab)|)
+    // of every primitive pair, aligned with prims and sorted descending within
+    // a pair; a primitive quartet is skipped when its bound product times the
+    // quartet's largest density-block norm is below prim_tau (0: none)
+    const float* prim_q;
+    double prim_tau;
 };
 
 __device__ __forceinline__ int ncart_d(int l) { return (l + 1) * (l + 2) / 2; }
@@ -47,13 +53,17 @@ __device__ __forceinline__ void cart_block(const GradArgs& a, int c1, int c2, do
         for (int j = 0; j < N2; ++j) Dt[i * N2 + j] = a.Dc[(size_t)(c1 + i) * a.nc + c2 + j];
 }
 
-__device__ __forceinline__ bool grad_screen(const GradArgs& a, int bra, int ket, int sa, int sb, int sc, int sd) {
-    if (!a.pair_sig[bra] || !a.pair_sig[ket]) return false;
+// max of the six shell-block norms of D over a quartet
+__device__ __forceinline__ double grad_dn(const GradArgs& a, int sa, int sb, int sc, int sd) {
     const int nsh = a.nsh;
     const double* Dn = a.Dn;
-    const double dn = fmax(fmax(Dn[sa * nsh + sb], Dn[sc * nsh + sd]),
-                           fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
-    return dn * a.pair_Q[bra] * a.pair_Q[ket] >= a.precision;
+    return fmax(fmax(Dn[sa * nsh + sb], Dn[sc * nsh + sd]),
+                fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]), fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
+}
+
+__device__ __forceinline__ bool grad_screen(const GradArgs& a, int bra, int ket, int sa, int sb, int sc, int sd) {
+    if (!a.pair_sig[bra] || !a.pair_sig[ket]) return false;
+    return grad_dn(a, sa, sb, sc, sd) * a.pair_Q[bra] * a.pair_Q[ket] >= a.precision;
 }
 
 __device__ __forceinline__ double wave_sum_g(double v) {
@@ -119,11 +129,12 @@ __global__ __launch_bounds__(256, (GradOccupancy<LA, LB, LC, LD, MASK, R>::min_p
                                                0.125 * (Dac[i * NC + k] * Dbd[j * ND + l] + Dad[i * ND + l] * Dbc[j * NC + k])));
                 };
                 const double* O = a.sh_O;
+                const float cut = float(a.prim_tau / grad_dn(a, sa, sb, sc, sd));
                 EriDeriv<LA, LB, LC, LD, MASK>::template run<R>(
                     a.prims + a.pair_off[bra], a.pair_np[bra], a.prims + a.pair_off[ket], a.pair_np[ket],
                     R(O[3 * sa] - O[3 * sb]), R(O[3 * sa + 1] - O[3 * sb + 1]), R(O[3 * sa + 2] - O[3 * sb + 2]),
                     R(O[3 * sc] - O[3 * sd]), R(O[3 * sc + 1] - O[3 * sd + 1]), R(O[3 * sc + 2] - O[3 * sd + 2]), tab,
-                    gam_fill, g);
+                    gam_fill, g, a.prim_q + a.pair_off[bra], a.prim_q + a.pair_off[ket], cut);
             }
             const int at_a = a.sh_atom[sa], at_b = a.sh_atom[sb];
             // A and B are the same for the whole wavefront
