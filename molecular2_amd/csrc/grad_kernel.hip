@@ -153,9 +153,15 @@ __global__ __launch_bounds__(64) void k_grad_prog(GradArgs a, BoysTables tab, co
         const PrimPair* bp = a.prims + a.pair_off[tk.bra];
         const PrimPair* kp = a.prims + a.pair_off[ket];
         const int nbp = a.pair_np[tk.bra], nkp = a.pair_np[ket];
+        // primitive screening (uniform over the workgroup: one quartet)
+        const float* bq = a.prim_q + a.pair_off[tk.bra];
+        const float* kq = a.prim_q + a.pair_off[ket];
+        const float cut = float(a.prim_tau / grad_dn(a, sa, sb, sc, sd));
         for (int ib = 0; ib < nbp; ++ib) {
+            if (bq[ib] * kq[0] < cut) break;
             const PrimPair B = bp[ib];
             for (int ik = 0; ik < nkp; ++ik) {
+                if (bq[ib] * kq[ik] < cut) break;
                 const PrimPair K = kp[ik];
                 const double p = B.p, q = K.p;
                 const double dx = B.P[0] - K.P[0], dy = B.P[1] - K.P[1], dz = B.P[2] - K.P[2];
