@@ -196,19 +196,20 @@ def test_df_fp32_energy_error(engine, h2o20, name):
 
 
 def test_h2o20_gradient_invariances_fd_and_mixed(engine, h2o20):
-    """(H2O)20 RHF gradient: sum_A g_A = 0, sum_A R_A x g_A = 0, two central
-    finite differences of the SCF energy (h = 1e-4 bohr, SOAD start), and the
-    mixed-precision gradient within 1e-6 Eh/bohr of FP64."""
+    """(H2O)20 RHF gradient: sum_A g_A = 0, sum_A R_A x g_A = 0, two
+    four-point finite differences of the SCF energy (h = 2e-3 bohr, SOAD
+    start), and the mixed-precision gradient within 1e-6 Eh/bohr of FP64."""
     mol = h2o20
     print("case: (H2O)20 gradient", flush=True)
     # At the default screening (1e-12) each incremental Delta-D build drops
     # ~1e8 quartets below the threshold, and the incremental Fock matrix
     # drifts by ~1e-9 Eh between resets (the reference's own scheme,
     # src/fockbuild.cpp:176-203), which keeps |dE| from settling at 1e-10.
-    # Screening at 1e-14 brings the drift under ~1e-11.  The energies at +-h
-    # are converged to 1e-11 (at 1e-10 each may sit ~1e-10 from its minimum,
-    # and 1e-10 / 2h is already the 5e-7 tolerance), so the FD error stays
-    # ~1e-11 / 2h.
+    # Screening at 1e-14 and convergence to 1e-11 leave each SCF energy
+    # ~1e-10 Eh of drift of either sign (two-point differences at h = 1e-4
+    # have landed +-5e-7 from the same analytic value), so the derivative is
+    # the four-point stencil at h = 2e-3: noise <= 18e-10 / (12 h) = 7.5e-8,
+    # truncation h^4 E^(5) / 30 ~ 1e-11.
     kw = dict(enconverge=1e-11, densconverge=1e-8, maxiter=100, precision=1e-14)
     r = engine.rhf(mol, gradient=True, **kw)
     assert r["converged"]
@@ -216,12 +217,14 @@ def test_h2o20_gradient_invariances_fd_and_mixed(engine, h2o20):
     assert np.max(np.abs(g.sum(axis=0))) < 1e-6
     torque = np.cross(mol.xyz, g).sum(axis=0)
     assert np.max(np.abs(torque)) < 1e-5, torque
-    h = 1e-4
+    h = 2e-3
     for atom, axis in ((0, 0), (37, 2)):
-        ep = engine.rhf(mol.displaced(atom, axis, h), **kw)
-        em = engine.rhf(mol.displaced(atom, axis, -h), **kw)
-        assert ep["converged"] and em["converged"]
-        fd = (ep["energy"] - em["energy"]) / (2 * h)
+        e = {}
+        for s in (-2, -1, 1, 2):
+            r_s = engine.rhf(mol.displaced(atom, axis, s * h), **kw)
+            assert r_s["converged"]
+            e[s] = r_s["energy"]
+        fd = (8 * (e[1] - e[-1]) - (e[2] - e[-2])) / (12 * h)
         print("  g[%d,%d] analytic %.9f fd %.9f" % (atom, axis, g[atom, axis], fd), flush=True)
         assert abs(g[atom, axis] - fd) < 5e-7, (atom, axis, g[atom, axis], fd)
     rm = engine.rhf(mol, gradient=True, mixed=1, **kw)
