@@ -54,6 +54,20 @@ __device__ __forceinline__ double sqrt_inv<double>(double x) { return rsqrt_pos(
 template <>
 __device__ __forceinline__ float sqrt_inv<float>(float x) { return rsqrtf(x); }
 
+// Byte offset k * RB of row k (k < 2^24): one v_mul_u32_u24 whose result
+// the compiler cannot fold into 64-bit address arithmetic, so that the row
+// loads take the table base from SGPRs and this 32-bit offset from a VGPR.
+template <unsigned RB>
+__device__ __forceinline__ unsigned row_offset(unsigned k) {
+    unsigned off;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(off) : "i"(RB), "v"(k));
+    return off;
+}
+template <typename T>
+__device__ __forceinline__ const T* at_offset(const T* base, unsigned off) {
+    return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
 template <typename R, int M>
 struct BoysEval;
 
@@ -89,12 +103,18 @@ struct BoysEval<double, M> {
         double r[NROW];
         double d, T;
     };
-    // 0 <= T <= TTAB (the row index is clamped to the table)
+    // 0 <= T < TTAB.  The nearest grid index k = rint(T / BOYS_STEP) <= KMAX
+    // comes from the low word of T / BOYS_STEP + 1.5 * 2^52 (whose last
+    // mantissa bit is worth 1, so the addition rounds to an integer), and the
+    // row is addressed by a 32-bit byte offset from the table base.
     static __device__ __forceinline__ void load(double T, Row& w, const BoysTables& tab) {
-        const int k = min(__double2int_rn(T * (1.0 / BOYS_STEP)), KMAX);
-        w.d = k * BOYS_STEP - T;
+        constexpr double RND = 6755399441055744.0;   // 1.5 * 2^52
+        const double x = fma(T, 1.0 / BOYS_STEP, RND);
+        const unsigned k = unsigned(__double2loint(x));
+        w.d = fma(x - RND, BOYS_STEP, -T);
         w.T = T;
-        const double* r = M <= BOYS_DIRECT_MAX ? tab.w + boys_woff(M) + k * (M + 8) : tab.d + k * BOYS_MTAB + M;
+        const double* r = M <= BOYS_DIRECT_MAX ? at_offset(tab.w + boys_woff(M), row_offset<(M + 8) * 8>(k))
+                                               : at_offset(tab.d + M, row_offset<BOYS_MTAB * 8>(k));
 #pragma unroll
         for (int i = 0; i < NROW; ++i) w.r[i] = r[i];
     }
@@ -161,13 +181,18 @@ struct BoysEval<float, M> {
         float r[5];
         float d, T;
     };
-    // 0 <= T <= BOYS_TMAX: F_M by a 5-term Taylor series, the rest by the
+    // 0 <= T < BOYS_TMAX: F_M by a 5-term Taylor series about the grid point
+    // k = rint(T / BOYS_STEP) <= BOYS_NGRID - 1 (taken from the mantissa of
+    // T / BOYS_STEP + 1.5 * 2^23, as in the FP64 case), the rest by the
     // downward recursion
     static __device__ __forceinline__ void load(float T, Row& w, const BoysTables& tab) {
-        const int k = min(__float2int_rn(T * float(1.0 / BOYS_STEP)), BOYS_NGRID - 1);
-        w.d = k * float(BOYS_STEP) - T;
+        constexpr float RND = 12582912.0f;   // 1.5 * 2^23
+        const float x = fmaf(T, float(1.0 / BOYS_STEP), RND);
+        const unsigned k = unsigned(__float_as_int(x) - __float_as_int(RND));
+        w.d = fmaf(x - RND, float(BOYS_STEP), -T);
         w.T = T;
-        const float* r = FW ? tab.fw + (M * BOYS_NGRID + k) * 5 : tab.f + k * BOYS_MTAB + M;
+        const float* r = FW ? at_offset(tab.fw + M * BOYS_NGRID * 5, row_offset<5 * 4>(k))
+                            : at_offset(tab.f + M, row_offset<BOYS_MTAB * 4>(k));
 #pragma unroll
         for (int i = 0; i < 5; ++i) w.r[i] = r[i];
     }

@@ -239,12 +239,53 @@ def hrr_ket(la, lb, lc, ld):
     return node, defs
 
 
-def prim_body(la, lb, lc, ld, targets, interleave, pref, acc):
+def node_deps(kind):
+    """the VRR nodes one node reads"""
+    if kind[0] == "base":
+        return []
+    _, terms, extra = kind
+    deps = [k for _, k in terms]
+    for x in extra:
+        deps += list(x[2:])
+    return deps
+
+
+def emit_fused_acc(em, acc_name, kind, indent, L):
+    """A target no other VRR node reads, added term by term into its
+    accumulator (acc += c v, one FMA each) instead of formed first and added
+    after.  With L = 1 the WP / WQ factors are not formed at all: WP_x v =
+    -d_x (rho_p v), the product rho_p v shared by the three components."""
+    _, terms, extra = kind
+    for c, k in terms:
+        v = vrr_name(*k)
+        if L == 1 and c[:2] == "WP":
+            em.out(indent + "%s -= d%s * (rho_p * %s);" % (acc_name, c[2], v))
+        elif L == 1 and c[:2] == "WQ":
+            em.out(indent + "%s += d%s * (rho_q * %s);" % (acc_name, c[2], v))
+        else:
+            em.out(indent + "%s += %s * %s;" % (acc_name, c, v))
+    for x in extra:
+        if x[0] == "ff":
+            em.out(indent + "%s += %s * inv2q * (%s - rho_q * %s);" % (acc_name, lit(x[1]), vrr_name(*x[2]), vrr_name(*x[3])))
+        elif x[0] == "ee":
+            em.out(indent + "%s += %s * inv2p * (%s - rho_p * %s);" % (acc_name, lit(x[1]), vrr_name(*x[2]), vrr_name(*x[3])))
+        elif x[0] == "ef":
+            em.out(indent + "%s += %s * inv2pq * %s;" % (acc_name, lit(x[1]), vrr_name(*x[2])))
+
+
+def prim_body(la, lb, lc, ld, targets, interleave, pref, acc, fuse=None):
     """Lines of one bra primitive of the bra loop (record `bp`), split into
     (setup up to T, finish from the scaled F_m on); the Boys call sits in
-    between.  `acc(e, f)` names the accumulator of target (e, f)."""
+    between.  `acc(e, f)` names the accumulator of target (e, f).  With
+    `fuse` (default: with interleave) a target that no other node reads is
+    added into its accumulator term by term."""
     L = la + lb + lc + ld
+    if fuse is None:
+        fuse = interleave
     order, nodes = build_vrr(targets, L)
+    read = set()
+    for key in order:
+        read.update(node_deps(nodes[key]))
     ind = "        "
     setup = [ind + "const R p = R(bp.p), Kab = R(bp.K);"]
     if la + lb > 0:
@@ -261,14 +302,23 @@ def prim_body(la, lb, lc, ld, targets, interleave, pref, acc):
     fin.out(ind + "for (int m = 0; m <= %d; ++m) Fm[m] *= pref;" % L)
     if L > 0:
         fin.out(ind + "const R rho_p = q * inv_pq, rho_q = p * inv_pq, inv2pq = R(0.5) * inv_pq;")
-        fin.out(ind + "const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
-        fin.out(ind + "const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
-    if interleave:
+        if L > 1 or not fuse:
+            fin.out(ind + "const R WPx = -rho_p * dx, WPy = -rho_p * dy, WPz = -rho_p * dz;")
+            fin.out(ind + "const R WQx = rho_q * dx, WQy = rho_q * dy, WQz = rho_q * dz;")
+    tset = set(targets)
+
+    def fused(key):
+        e, f, m = key
+        return fuse and m == 0 and (e, f) in tset and key not in read and nodes[key][0] != "base"
+
+    if interleave or fuse:
         # each target is accumulated right after its VRR node (shorter live ranges)
-        tset = set(targets)
         for key in order:
-            emit_vrr(fin, [key], nodes, ind)
             e, f, m = key
+            if fused(key):
+                emit_fused_acc(fin, acc(e, f), nodes[key], ind, L)
+                continue
+            emit_vrr(fin, [key], nodes, ind)
             if m == 0 and (e, f) in tset:
                 fin.out(ind + "%s += %s;" % (acc(e, f), vrr_name(e, f, 0)))
     else:
@@ -294,7 +344,7 @@ def emit_bra_loop(em, L, setup, fin):
     em.out("      }")
 
 
-def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
+def gen_class(la, lb, lc, ld, deriv=False, interleave=True, fuse=None):
     L = la + lb + lc + ld
     em = Emitter()
     E = [c for Le in range(la, la + lb + 1) for c in comps(Le)]
@@ -333,7 +383,7 @@ def gen_class(la, lb, lc, ld, deriv=False, interleave=True):
         em.out("      const R inv2q = R(kp.inv2p);")
         em.out("      const R QCx = R(kp.PA[0]), QCy = R(kp.PA[1]), QCz = R(kp.PA[2]);")
     setup, fin = prim_body(la, lb, lc, ld, targets, interleave, "Kab * Kcd * rs_pq",
-                           lambda e, f: "a%s_%s" % (cname(e), cname(f)))
+                           lambda e, f: "a%s_%s" % (cname(e), cname(f)), fuse=fuse)
     emit_bra_loop(em, L, setup, fin)
     em.out("    }")
     # HRR: ket first for every e in E, then bra
@@ -468,7 +518,7 @@ def gen_class_fam(la, lb):
                 for t in lines[i_kp:i_bra]]
     targets = [(e, (0, 0, 0)) for e in E]
     setup, fin = prim_body(la, lb, 0, 0, targets, False, "Kab * rs_pq",
-                           lambda e, f: "t%s_%s" % (cname(e), cname(f)))
+                           lambda e, f: "t%s_%s" % (cname(e), cname(f)), fuse=True)
     bra_loop = Emitter()
     emit_bra_loop(bra_loop, la + lb, setup, fin)
     em = []
@@ -502,7 +552,7 @@ def gen_class_fam(la, lb):
             em.append("  " + t.replace("out[", "outm["))
         em.append("    }")
     em.append("  }")
-    run = gen_class(la, lb, 0, 0, interleave=False).splitlines()
+    run = gen_class(la, lb, 0, 0, interleave=False, fuse=True).splitlines()
     assert run[-1] == "};"
     return "\n".join(run[:-1] + em + ["};"])
 
