@@ -1115,11 +1115,13 @@ m2_status m2_ctx_create(m2_ctx** out, int device) {
     std::vector<float> tabf(tab.size());
     for (int k = 0; k < BOYS_NGRID; ++k) boys_host(BOYS_MTAB - 1, k * BOYS_STEP, &tab[(size_t)k * BOYS_MTAB]);
     for (size_t i = 0; i < tab.size(); ++i) tabf[i] = (float)tab[i];
-    // compact windows F_0 .. F_(M+7) of the low-m classes (same grid)
+    // compact rows {F_(M+7), e^-T} of the low-m classes (same grid)
     std::vector<double> tabw(BOYS_WSIZE);
     for (int M = 0; M <= BOYS_DIRECT_MAX; ++M)
-        for (int k = 0; k <= BOYS_WK[M]; ++k)
-            for (int m = 0; m < M + 8; ++m) tabw[boys_woff(M) + (size_t)k * (M + 8) + m] = tab[(size_t)k * BOYS_MTAB + m];
+        for (int k = 0; k <= BOYS_WK[M]; ++k) {
+            tabw[boys_woff(M) + (size_t)k * 2] = tab[(size_t)k * BOYS_MTAB + M + 7];
+            tabw[boys_woff(M) + (size_t)k * 2 + 1] = std::exp(-(k * BOYS_STEP));
+        }
     std::vector<float> tabfw(BOYS_FWSIZE);
     for (int M = 0; M <= BOYS_FW_MAX; ++M)
         for (int k = 0; k < BOYS_NGRID; ++k)

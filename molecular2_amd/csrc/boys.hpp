@@ -2,9 +2,12 @@
 //
 // FP64: 8-term Taylor expansion about the nearest grid point (|dT| <= 0.05,
 // truncation < 1e-15 relative), using F_m^(k) = (-1)^k F_{m+k}.  For
-// M <= BOYS_DIRECT_MAX every F_m comes from its own expansion over the same
-// table row (no exponential); above it F_M is expanded and the rest follow by
-// the stable downward recursion F_m = (2T F_{m+1} + e^-T) / (2m+1).  For
+// M <= BOYS_DIRECT_MAX every F_m comes from its own expansion; the values
+// F_0 .. F_(M+7) at the grid point are generated from a 16-byte table row,
+// F_(M+7)(T0) and e^-T0, by the downward recursion F_m = (2 T0 F_(m+1) +
+// e^-T0) / (2m+1) (stable: every step adds positive terms), so a lookup is
+// one 16-byte load.  Above BOYS_DIRECT_MAX F_M is expanded from a full table
+// row and the rest follow by the same recursion at T with e^-T.  For
 // T >= BOYS_TMAX (60) the asymptotic F_0 = sqrt(pi/T)/2 (erfc(sqrt(60)) <
 // 1e-27) and F_{m+1} = (2m+1) F_m / (2T) (the e^-T term is below 1e-26).
 // FP32: 5-term Taylor from a float table (truncation < 3e-9 relative) with
@@ -22,14 +25,14 @@ struct BoysTables {
 };
 
 // Compact tables of the low-m classes: for M = 0..3, rows k = 0..BOYS_WK[M]
-// of F_0 .. F_(M+7) at T = k BOYS_STEP (M + 8 doubles a row), back to back
-// from BoysTables::w.  Beyond BOYS_WK[M] BOYS_STEP the asymptotic form
-// F_m = (2m-1)!! sqrt(pi) / (2^(m+1) T^(m+1/2)) is already within 1e-15
+// of {F_(M+7)(T_k), e^-T_k} at T_k = k BOYS_STEP (two doubles a row), back
+// to back from BoysTables::w.  Beyond BOYS_WK[M] BOYS_STEP the asymptotic
+// form F_m = (2m-1)!! sqrt(pi) / (2^(m+1) T^(m+1/2)) is already within 1e-15
 // relative (the neglected part is ~e^-T T^(m-1/2) / Gamma(m+1/2) of it), so
-// the tables stop there: 21-39 KB each, against 125 KB for the full table,
-// and they stay in the vector L1 cache of the kernels that read them.
+// the tables stop there: 5-7 KB each, which stay in the vector L1 cache of
+// the kernels that read them.
 constexpr int BOYS_WK[4] = {330, 380, 400, 440};
-constexpr int boys_woff(int M) { return M == 0 ? 0 : boys_woff(M - 1) + (BOYS_WK[M - 1] + 1) * (M - 1 + 8); }
+constexpr int boys_woff(int M) { return M == 0 ? 0 : boys_woff(M - 1) + (BOYS_WK[M - 1] + 1) * 2; }
 constexpr int BOYS_WSIZE = boys_woff(4);
 // FP32 windows F_M .. F_(M+4) (5 floats a row, all BOYS_NGRID rows) for
 // M = 0..BOYS_FW_MAX, back to back from BoysTables::fw: 12 KB each
@@ -98,10 +101,11 @@ struct BoysEval<double, M> {
 
     // The table branch in two phases: load() gathers the row (and the offset
     // d from the grid point), eval() sums the Taylor series.
-    static constexpr int NROW = M <= BOYS_DIRECT_MAX ? M + 8 : 8;
+    static constexpr bool DIRECT = M <= BOYS_DIRECT_MAX;
+    static constexpr int NROW = DIRECT ? 2 : 8;
     struct Row {
-        double r[NROW];
-        double d, T;
+        double r[NROW];   // {F_(M+7)(T0), e^-T0} (DIRECT) or F_M .. F_(M+7) at T0
+        double d, T, t2;  // T0 - T, T, 2 T0
     };
     // 0 <= T < TTAB.  The nearest grid index k = rint(T / BOYS_STEP) <= KMAX
     // comes from the low word of T / BOYS_STEP + 1.5 * 2^52 (whose last
@@ -111,17 +115,28 @@ struct BoysEval<double, M> {
         constexpr double RND = 6755399441055744.0;   // 1.5 * 2^52
         const double x = fma(T, 1.0 / BOYS_STEP, RND);
         const unsigned k = unsigned(__double2loint(x));
-        w.d = fma(x - RND, BOYS_STEP, -T);
+        const double kd = x - RND;
+        w.d = fma(kd, BOYS_STEP, -T);
         w.T = T;
-        const double* r = M <= BOYS_DIRECT_MAX ? at_offset(tab.w + boys_woff(M), row_offset<(M + 8) * 8>(k))
-                                               : at_offset(tab.d + M, row_offset<BOYS_MTAB * 8>(k));
+        w.t2 = kd * (2.0 * BOYS_STEP);
+        if constexpr (DIRECT) {
+            const double2 v = *at_offset(reinterpret_cast<const double2*>(tab.w + boys_woff(M)), row_offset<16>(k));
+            w.r[0] = v.x;
+            w.r[1] = v.y;
+        } else {
+            const double* r = at_offset(tab.d + M, row_offset<BOYS_MTAB * 8>(k));
 #pragma unroll
-        for (int i = 0; i < NROW; ++i) w.r[i] = r[i];
+            for (int i = 0; i < NROW; ++i) w.r[i] = r[i];
+        }
     }
     static __device__ __forceinline__ void eval(const Row& w, double* F) {
         const double d = w.d;
-        const double* r = w.r;
-        if constexpr (M <= BOYS_DIRECT_MAX) {
+        if constexpr (DIRECT) {
+            // F_m(T0), m = M+7 .. 0, by the downward recursion
+            double r[M + 8];
+            r[M + 7] = w.r[0];
+#pragma unroll
+            for (int m = M + 6; m >= 0; --m) r[m] = fma(w.t2, r[m + 1], w.r[1]) * (1.0 / (2 * m + 1));
             const double d2 = d * 0.5, d3 = d * (1.0 / 3.0), d4 = d * 0.25, d5 = d * 0.2, d6 = d * (1.0 / 6.0),
                          d7 = d * (1.0 / 7.0);
 #pragma unroll
@@ -135,6 +150,7 @@ struct BoysEval<double, M> {
                 F[m] = r[m] + d * s;
             }
         } else {
+            const double* r = w.r;
             double s = r[7];
             s = r[6] + d * (1.0 / 7.0) * s;
             s = r[5] + d * (1.0 / 6.0) * s;
