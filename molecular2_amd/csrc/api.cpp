@@ -1124,8 +1124,10 @@ m2_status m2_ctx_create(m2_ctx** out, int device) {
         }
     std::vector<float> tabfw(BOYS_FWSIZE);
     for (int M = 0; M <= BOYS_FW_MAX; ++M)
-        for (int k = 0; k < BOYS_NGRID; ++k)
-            for (int j = 0; j < 5; ++j) tabfw[((size_t)M * BOYS_NGRID + k) * 5 + j] = tabf[(size_t)k * BOYS_MTAB + M + j];
+        for (int k = 0; k < BOYS_NGRID; ++k) {
+            tabfw[((size_t)M * BOYS_NGRID + k) * 2] = tabf[(size_t)k * BOYS_MTAB + M + 4];
+            tabfw[((size_t)M * BOYS_NGRID + k) * 2 + 1] = (float)std::exp(-(k * BOYS_STEP));
+        }
     if (ctx->d_boys.reserve(tab.size()) != hipSuccess || ctx->d_boys_f.reserve(tab.size()) != hipSuccess ||
         ctx->d_boys_w.reserve(tabw.size()) != hipSuccess || ctx->d_boys_fw.reserve(tabfw.size()) != hipSuccess ||
         hipMemcpy(ctx->d_boys_fw.p, tabfw.data(), sizeof(float) * tabfw.size(), hipMemcpyHostToDevice) != hipSuccess ||

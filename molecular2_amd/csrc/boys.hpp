@@ -10,8 +10,9 @@
 // row and the rest follow by the same recursion at T with e^-T.  For
 // T >= BOYS_TMAX (60) the asymptotic F_0 = sqrt(pi/T)/2 (erfc(sqrt(60)) <
 // 1e-27) and F_{m+1} = (2m+1) F_m / (2T) (the e^-T term is below 1e-26).
-// FP32: 5-term Taylor from a float table (truncation < 3e-9 relative) with
-// the same downward recursion; the large-T branch is evaluated in FP64.
+// FP32: 5-term Taylor expansion of F_M (truncation < 3e-9 relative) from an
+// 8-byte row {F_(M+4)(T0), e^-T0} and the same recursion at T0, the lower F_m
+// by the downward recursion at T; the large-T branch is evaluated in FP64.
 #pragma once
 #include "m2_common.hpp"
 
@@ -34,10 +35,10 @@ struct BoysTables {
 constexpr int BOYS_WK[4] = {330, 380, 400, 440};
 constexpr int boys_woff(int M) { return M == 0 ? 0 : boys_woff(M - 1) + (BOYS_WK[M - 1] + 1) * 2; }
 constexpr int BOYS_WSIZE = boys_woff(4);
-// FP32 windows F_M .. F_(M+4) (5 floats a row, all BOYS_NGRID rows) for
-// M = 0..BOYS_FW_MAX, back to back from BoysTables::fw: 12 KB each
+// FP32 windows {F_(M+4)(T_k), e^-T_k} (two floats a row, all BOYS_NGRID rows)
+// for M = 0..BOYS_FW_MAX, back to back from BoysTables::fw: 4.8 KB each
 constexpr int BOYS_FW_MAX = 8;
-constexpr int BOYS_FWSIZE = (BOYS_FW_MAX + 1) * BOYS_NGRID * 5;
+constexpr int BOYS_FWSIZE = (BOYS_FW_MAX + 1) * BOYS_NGRID * 2;
 
 // 1/sqrt(x) for finite x > 0: the hardware estimate (v_rsq_f64, ~2^-23
 // relative) refined by one third-order step of the iteration for x^(-1/2),
@@ -193,9 +194,10 @@ struct BoysEval<float, M> {
     }
 
     static constexpr bool FW = M <= BOYS_FW_MAX;
+    static constexpr int NROW = FW ? 2 : 5;
     struct Row {
-        float r[5];
-        float d, T;
+        float r[NROW];   // {F_(M+4)(T0), e^-T0} (FW) or F_M .. F_(M+4) at T0
+        float d, T, t2;  // T0 - T, T, 2 T0
     };
     // 0 <= T < BOYS_TMAX: F_M by a 5-term Taylor series about the grid point
     // k = rint(T / BOYS_STEP) <= BOYS_NGRID - 1 (taken from the mantissa of
@@ -205,20 +207,37 @@ struct BoysEval<float, M> {
         constexpr float RND = 12582912.0f;   // 1.5 * 2^23
         const float x = fmaf(T, float(1.0 / BOYS_STEP), RND);
         const unsigned k = unsigned(__float_as_int(x) - __float_as_int(RND));
-        w.d = fmaf(x - RND, float(BOYS_STEP), -T);
+        const float kd = x - RND;
+        w.d = fmaf(kd, float(BOYS_STEP), -T);
         w.T = T;
-        const float* r = FW ? at_offset(tab.fw + M * BOYS_NGRID * 5, row_offset<5 * 4>(k))
-                            : at_offset(tab.f + M, row_offset<BOYS_MTAB * 4>(k));
+        w.t2 = kd * float(2.0 * BOYS_STEP);
+        if constexpr (FW) {
+            const float2 v = *at_offset(reinterpret_cast<const float2*>(tab.fw + M * BOYS_NGRID * 2), row_offset<8>(k));
+            w.r[0] = v.x;
+            w.r[1] = v.y;
+        } else {
+            const float* r = at_offset(tab.f + M, row_offset<BOYS_MTAB * 4>(k));
 #pragma unroll
-        for (int i = 0; i < 5; ++i) w.r[i] = r[i];
+            for (int i = 0; i < 5; ++i) w.r[i] = r[i];
+        }
     }
     static __device__ __forceinline__ void eval(const Row& w, float* F) {
         const float d = w.d;
-        float s = w.r[4];
-        s = w.r[3] + d * (1.0f / 4.0f) * s;
-        s = w.r[2] + d * (1.0f / 3.0f) * s;
-        s = w.r[1] + d * (1.0f / 2.0f) * s;
-        s = w.r[0] + d * s;
+        float r[5];
+        if constexpr (FW) {
+            // F_(M+3) .. F_M at T0 by the downward recursion from F_(M+4)(T0)
+            r[4] = w.r[0];
+#pragma unroll
+            for (int j = 3; j >= 0; --j) r[j] = fmaf(w.t2, r[j + 1], w.r[1]) * (1.0f / (2 * (M + j) + 1));
+        } else {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) r[j] = w.r[j];
+        }
+        float s = r[4];
+        s = r[3] + d * (1.0f / 4.0f) * s;
+        s = r[2] + d * (1.0f / 3.0f) * s;
+        s = r[1] + d * (1.0f / 2.0f) * s;
+        s = r[0] + d * s;
         F[M] = s;
         if constexpr (M > 0) {
             const float e = __expf(-w.T);
