@@ -120,6 +120,7 @@ JKArgs jk_args(m2_ctx* ctx) {
     JKArgs a{};
     a.prims = ctx->d_prims.p;
     a.prims_f = ctx->d_prims_f.p;
+    a.kets_s = ctx->d_kets_s.p;
     a.pair_off = ctx->d_pair_off.p;
     a.pair_np = ctx->d_pair_np.p;
     a.pair_sh = ctx->d_pair_sh.p;
@@ -494,8 +495,19 @@ m2_status build_pairs(m2_ctx* ctx) {
         HIPCHK(ctx->d_prim_q.reserve(std::max<size_t>(prims.size(), 1)));
         std::vector<float> q;
         for (int i = 0; i < npair; ++i) q.insert(q.end(), pairs[i].q.begin(), pairs[i].q.end());
-        if (q.size() == prims.size() && !q.empty())
+        if (q.size() == prims.size() && !q.empty()) {
             HIPCHK(hipMemcpy(ctx->d_prim_q.p, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice));
+            std::vector<KetS> ks(prims.size());
+            for (size_t i = 0; i < prims.size(); ++i) {
+                ks[i].p = prims[i].p;
+                ks[i].K = prims[i].K;
+                for (int k = 0; k < 3; ++k) ks[i].P[k] = prims[i].P[k];
+                ks[i].q = q[i];
+                ks[i].pad = 0.0f;
+            }
+            HIPCHK(ctx->d_kets_s.reserve(ks.size()));
+            HIPCHK(hipMemcpy(ctx->d_kets_s.p, ks.data(), sizeof(KetS) * ks.size(), hipMemcpyHostToDevice));
+        }
         return M2_OK;
     };
     m2_status st = upload();
@@ -1155,6 +1167,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     }
     ctx->d_prims.release();
     ctx->d_prims_f.release();
+    ctx->d_kets_s.release();
     ctx->d_pair_off.release();
     ctx->d_pair_np.release();
     ctx->d_pair_sh.release();

@@ -374,7 +374,7 @@ def gen_class(la, lb, lc, ld, deriv=False, interleave=True, fuse=None):
     # Quartets between those limits are computed, which only adds accuracy.
     em.out("    const float bq0 = SCR ? float(bq[0]) : 0.f;")
     em.out("    for (int ik = 0; ik < nket; ++ik) {")
-    em.out("      const float qk = SCR ? kq[ik] : 0.f;")
+    em.out("      const float qk = SCR ? ket_bound(ket, kq, ik) : 0.f;")
     em.out("      if (SCR && qk * bq0 < cut) break;")
     em.out("      const auto kp = ket[ik];")
     em.out("      const R q = R(kp.p), Kcd = R(kp.K);")

@@ -340,7 +340,12 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
         }
         const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
         R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
-        EriClass<LA, LB, LC, LD>::template run<R, PURE, true>(bprims, nbp, gp + a.pair_off[ket], nkp,
+        // s-s kets of the FP64 kernels through their compact records
+        const auto kprims = [&]() {
+            if constexpr (LC + LD == 0 && std::is_same<R, double>::value) return a.kets_s + a.pair_off[ket];
+            else return gp + a.pair_off[ket];
+        }();
+        EriClass<LA, LB, LC, LD>::template run<R, PURE, true>(bprims, nbp, kprims, nkp,
                                                               ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq, cut);
         if (active) {
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);

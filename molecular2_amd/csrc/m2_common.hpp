@@ -29,6 +29,27 @@ struct alignas(16) BraPair {
 };
 static_assert(sizeof(BraPair<double>) == 80 && sizeof(BraPair<float>) == 48, "BraPair layout");
 
+// The FP64 record a primitive pair is read through when it is the ket of an
+// s-s pair class: the fields such kets use and the pair's primitive bound,
+// 48 bytes instead of 80 (the lanes' ket records are the kernels' main
+// L2 reads).
+struct alignas(16) KetS {
+    double p, K;
+    double P[3];
+    float q;   // bound of the primitive pair (its prim_q entry)
+    float pad;
+};
+static_assert(sizeof(KetS) == 48, "KetS layout");
+
+// bound of ket primitive ik: from the bound array, or from a KetS record
+template <typename KP>
+__device__ __forceinline__ float ket_bound(const KP* __restrict__, const float* __restrict__ kq, int ik) {
+    return kq[ik];
+}
+__device__ __forceinline__ float ket_bound(const KetS* __restrict__ ket, const float* __restrict__, int ik) {
+    return ket[ik].q;
+}
+
 // Boys table: F_m(T_k) at T_k = k * BOYS_STEP, k = 0..BOYS_NGRID-1,
 // m = 0..BOYS_MTAB-1.  Beyond BOYS_TMAX the asymptotic branch is used.
 constexpr int BOYS_MTAB = 26;
@@ -56,6 +77,7 @@ constexpr size_t JK_DROW_LDS_MAX = 64 * 1024;   // K rows + staged density rows
 struct JKArgs {
     const PrimPair* prims;
     const BraPair<float>* prims_f;   // the same records in FP32
+    const KetS* kets_s;              // the same records as s-s kets (FP64)
     const int* pair_off;     // first PrimPair of each shell pair
     const int* pair_np;      // number of PrimPairs of each shell pair
     const int2* pair_sh;     // (a, b) shell indices, la >= lb
