@@ -132,6 +132,7 @@ JKArgs jk_args(m2_ctx* ctx) {
     a.fam_q = ctx->d_fam_q.p;
     a.prim_tau = 0.0;
     a.d_lds_max = 2;
+    a.boys_lds_max = 1;
     a.n = ctx->obs.nbf;
     a.nsh = ctx->obs.nsh;
     if (ctx->fam_on) {

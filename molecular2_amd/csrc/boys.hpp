@@ -253,4 +253,32 @@ struct BoysEval<float, M> {
     }
 };
 
+// Bytes of the compact rows of BoysEval<R, M> (0: it reads the full tables)
+template <typename R, int M>
+constexpr int boys_rows_bytes() {
+    return sizeof(R) == 8 ? (M <= BOYS_DIRECT_MAX ? (BOYS_WK[M <= BOYS_DIRECT_MAX ? M : 0] + 1) * 16 : 0)
+                          : (M <= BOYS_FW_MAX ? BOYS_NGRID * 8 : 0);
+}
+
+// The workgroup copies the compact rows of BoysEval<R, M> to LDS at dst
+// (16-byte aligned, boys_rows_bytes<R, M>() bytes) and gets tables whose
+// compact rows are read from there; the caller synchronises before use.
+template <typename R, int M>
+__device__ __forceinline__ BoysTables stage_boys_rows(const BoysTables& tab, double* dst) {
+    static_assert(boys_rows_bytes<R, M>() > 0, "no compact rows");
+    BoysTables t = tab;
+    if constexpr (sizeof(R) == 8) {
+        const double2* src = reinterpret_cast<const double2*>(tab.w + boys_woff(M));
+        double2* d = reinterpret_cast<double2*>(dst);
+        for (int i = threadIdx.x; i <= BOYS_WK[M]; i += blockDim.x) d[i] = src[i];
+        t.w = dst - boys_woff(M);
+    } else {
+        const float2* src = reinterpret_cast<const float2*>(tab.fw + M * BOYS_NGRID * 2);
+        float2* d = reinterpret_cast<float2*>(dst);
+        for (int i = threadIdx.x; i < BOYS_NGRID; i += blockDim.x) d[i] = src[i];
+        t.fw = reinterpret_cast<const float*>(dst) - M * BOYS_NGRID * 2;
+    }
+    return t;
+}
+
 }  // namespace m2

@@ -255,8 +255,8 @@ __device__ __forceinline__ void stage_bra_rows(const JKArgs& a, int sa, int sb, 
 // NDEN = 2 (unrestricted build): every quartet is digested with D and with D2
 // into (J, K) and (J2, K2); the LDS then holds the K rows of both.
 // One work item of k_jk: bra pair and up to a.block_tasks of its ket chunks.
-template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN>
-__device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, double lo, double hi, int witem) {
+template <int LA, int LB, int LC, int LD, typename R, bool PURE, int NDEN, bool BL>
+__device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab0, double lo, double hi, int witem) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NC = NFun<LC, PURE>::value, ND = NFun<LD, PURE>::value;
     constexpr int W = JKOccupancy<LA, LB, LC, LD, R>::waves;
@@ -290,6 +290,10 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab, 
     typedef const __attribute__((address_space(4))) float cfl;
     cfl* bq = (cfl*)(a.prim_q + a.pair_off[bra]);
     extern __shared__ double sK[];
+    // BL: the class's compact Boys rows read from LDS (copied here; the
+    // bra-row staging below ends with the workgroup barrier)
+    BoysTables tab = tab0;
+    if constexpr (BL) tab = stage_boys_rows<R, LA + LB + LC + LD>(tab0, sK + a.boys_off / sizeof(double));
     const double *sD, *sDn;
     stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
     const bool dnl = a.d_lds != 0;
@@ -391,7 +395,12 @@ __global__ __launch_bounds__((JKOccupancy<LA, LB, LC, LD, R>::waves * 64),
                              (JKOccupancy<LA, LB, LC, LD, R>::min_per_eu)) void k_jk(JKArgs a, BoysTables tab, double lo,
                                                                                  double hi) {
     if (a.blk_total && (int)blockIdx.x >= *a.blk_total) return;   // grid sized from a bound: past the total
-    jk_item<LA, LB, LC, LD, R, PURE, NDEN>(a, tab, lo, hi, blockIdx.x);
+    if constexpr (boys_rows_bytes<R, LA + LB + LC + LD>() > 0) {
+        if (a.boys_lds) jk_item<LA, LB, LC, LD, R, PURE, NDEN, true>(a, tab, lo, hi, blockIdx.x);
+        else jk_item<LA, LB, LC, LD, R, PURE, NDEN, false>(a, tab, lo, hi, blockIdx.x);
+    } else {
+        jk_item<LA, LB, LC, LD, R, PURE, NDEN, false>(a, tab, lo, hi, blockIdx.x);
+    }
 }
 
 // (ab|ss) with s-s family kets: one lane per ket family pair.  Its primitive
@@ -420,8 +429,8 @@ struct JKWavesFam {
     static constexpr int min_per_eu = ss ? 5 : 1;
 };
 
-template <int LA, int LB, typename R, bool PURE, int NDEN>
-__device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& tab, double lo, double hi,
+template <int LA, int LB, typename R, bool PURE, int NDEN, bool BL>
+__device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& tab0, double lo, double hi,
                                             int witem) {
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     constexpr int NCART = EriClass<LA, LB, 0, 0>::NCART;
@@ -456,6 +465,8 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
     const bool same = a.pair_fam != nullptr && a.combo == 0;
     const int bfam = same ? a.pair_fam[bra] : -1, bmem = same ? a.pair_mem[bra] : 0;
     extern __shared__ double sK[];
+    BoysTables tab = tab0;   // BL: compact Boys rows from LDS, as in jk_item
+    if constexpr (BL) tab = stage_boys_rows<R, LA + LB>(tab0, sK + a.boys_off / sizeof(double));
     const double *sD, *sDn;
     stage_bra_rows<NA, NB, NDEN>(a, sa, sb, fa, fb, sK, sD, sDn);
     const bool dnl = a.d_lds != 0;
@@ -570,7 +581,12 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
                              (JKWavesFam<LA, LB, R>::min_per_eu)) void k_jk_fam(JKArgs a, BoysTables tab, double lo,
                                                                                double hi) {
     if (a.blk_total && (int)blockIdx.x >= *a.blk_total) return;
-    jk_fam_item<LA, LB, R, PURE, NDEN>(a, tab, lo, hi, blockIdx.x);
+    if constexpr (boys_rows_bytes<R, LA + LB>() > 0) {
+        if (a.boys_lds) jk_fam_item<LA, LB, R, PURE, NDEN, true>(a, tab, lo, hi, blockIdx.x);
+        else jk_fam_item<LA, LB, R, PURE, NDEN, false>(a, tab, lo, hi, blockIdx.x);
+    } else {
+        jk_fam_item<LA, LB, R, PURE, NDEN, false>(a, tab, lo, hi, blockIdx.x);
+    }
 }
 
 // Split register classes (csrc/gen_eri.py SPLIT): the quartet runs in one
@@ -586,8 +602,8 @@ __global__ __launch_bounds__((JKWavesFam<LA, LB, R>::value * 64),
 template <int LA, int LB, int LC, int LD>
 struct EriSplit;
 
-template <int NA, int NB, int NDEN>
-size_t jk_lds_plan(JKArgs& a);
+template <int NA, int NB, int NDEN, int BB = 0>
+size_t jk_lds_plan(JKArgs& a, int waves = 4);
 
 template <int P, int N>
 struct PassLoop {
@@ -880,16 +896,35 @@ __global__ __launch_bounds__(256) void k_schwarz(JKArgs a, BoysTables tab, int p
 // LDS of one J/K workgroup: the K rows when they fit JK_KROW_LDS_MAX, then
 // the staged density rows as far as the whole stays within JK_DROW_LDS_MAX
 // (norm rows first, D rows too when there is room).  Sets a.k_lds, a.d_lds.
-template <int NA, int NB, int NDEN>
-size_t jk_lds_plan(JKArgs& a) {
+// BB > 0: the kernel can also read its BB bytes of compact Boys rows from
+// LDS; they are added (a.boys_lds, a.boys_off) when that keeps the workgroups
+// per CU the LDS allows (160 KB per CU, 512-byte granules), or leaves room
+// for at least 5 waves per SIMD.
+template <int NA, int NB, int NDEN, int BB>
+size_t jk_lds_plan(JKArgs& a, int waves) {
     const size_t rows = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
     const size_t norms = sizeof(double) * 2 * (size_t)a.nsh;
     a.k_lds = rows <= JK_KROW_LDS_MAX ? 1 : 0;
     a.d_lds = 0;
-    if (!a.k_lds) return 0;
-    if (2 * rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 2) a.d_lds = 2;
-    else if (rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 1) a.d_lds = 1;
-    return rows + (a.d_lds == 2 ? rows : 0) + (a.d_lds ? norms : 0);
+    a.boys_lds = 0;
+    a.boys_off = 0;
+    size_t base = 0;
+    if (a.k_lds) {
+        if (2 * rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 2) a.d_lds = 2;
+        else if (rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 1) a.d_lds = 1;
+        base = rows + (a.d_lds == 2 ? rows : 0) + (a.d_lds ? norms : 0);
+    }
+    if (BB > 0 && a.boys_lds_max) {
+        constexpr size_t CU = 160 * 1024, G = 512;
+        const size_t off = (base + 15) & ~size_t(15), with = off + BB;
+        auto per_cu = [&](size_t b) { return b ? CU / ((b + G - 1) / G * G) : CU; };
+        if (with <= CU && (per_cu(with) >= per_cu(base) || per_cu(with) * (size_t)waves >= 20)) {
+            a.boys_lds = 1;
+            a.boys_off = (int)off;
+            return with;
+        }
+    }
+    return base;
 }
 
 // mode 0: FP64 kernel over the FP64 heads of the ket groups; mode 2: FP32
@@ -900,15 +935,16 @@ void launch_jk_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int mode,
     constexpr int NA = NFun<LA, PURE>::value, NB = NFun<LB, PURE>::value;
     JKArgs a = a0;
     const double inf = __builtin_huge_val();
+    constexpr int M = LA + LB + LC + LD;
     if (mode == 2) {
         auto k = k_jk<LA, LB, LC, LD, float, PURE, NDEN>;
         constexpr int W = JKOccupancy<LA, LB, LC, LD, float>::waves;
-        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        const size_t lds = jk_lds_plan<NA, NB, NDEN, boys_rows_bytes<float, M>()>(a, W);
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     } else {
         auto k = k_jk<LA, LB, LC, LD, double, PURE, NDEN>;
         constexpr int W = JKOccupancy<LA, LB, LC, LD, double>::waves;
-        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        const size_t lds = jk_lds_plan<NA, NB, NDEN, boys_rows_bytes<double, M>()>(a, W);
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     }
 }
@@ -928,12 +964,12 @@ void launch_jk_fam_n(const JKArgs& a0, const BoysTables& tab, int nblocks, int m
     if (mode == 2) {
         auto k = k_jk_fam<LA, LB, float, PURE, NDEN>;
         constexpr int W = JKWavesFam<LA, LB, float>::value;
-        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        const size_t lds = jk_lds_plan<NA, NB, NDEN, boys_rows_bytes<float, LA + LB>()>(a, W);
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     } else {
         auto k = k_jk_fam<LA, LB, double, PURE, NDEN>;
         constexpr int W = JKWavesFam<LA, LB, double>::value;
-        const size_t lds = jk_lds_plan<NA, NB, NDEN>(a);
+        const size_t lds = jk_lds_plan<NA, NB, NDEN, boys_rows_bytes<double, LA + LB>()>(a, W);
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(64 * W), lds, st, a, tab, a.precision, inf);
     }
 }

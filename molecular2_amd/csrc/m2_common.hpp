@@ -99,6 +99,9 @@ struct JKArgs {
     int d_lds;               // set by the launcher: the bra shells' rows of the density norms (1) and also
                              // of D (2) staged in LDS after the K rows
     int d_lds_max;           // the most staging the launcher may choose (0, 1, 2)
+    int boys_lds;            // set by the launcher: the class's compact Boys rows staged in LDS at byte boys_off
+    int boys_off;
+    int boys_lds_max;        // 0: never stage the Boys rows
     // device work list of the launch (tasks.hip): workgroup -> bra, and the
     // bra's ket segments per primitive-count group of the ket class
     const int2* cut;         // [bra][JK_MAXG] (mid, end)
