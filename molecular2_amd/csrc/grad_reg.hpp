@@ -72,6 +72,10 @@ __device__ __forceinline__ double wave_sum_g(double v) {
     return v;
 }
 
+// doubles of the per-block atom bins, rounded up to keep what follows them
+// 16-byte aligned
+__host__ __device__ constexpr int grad_bins_len(int natom) { return (natom * 3 + 1) & ~1; }
+
 // register classes: one quartet per lane, per-block LDS bins for the atoms
 // R: arithmetic of the integrals and their contraction (float for the
 // quartets a mixed-precision gradient leaves below its FP32 threshold); the
@@ -89,7 +93,10 @@ __global__ __launch_bounds__(256, (GradOccupancy<LA, LB, LC, LD, MASK, R>::min_p
     GradArgs a, BoysTables tab, const Task* __restrict__ tasks, int ntasks, int natom) {
     constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
     constexpr int NC = (LC + 1) * (LC + 2) / 2, ND = (LD + 1) * (LD + 2) / 2;
-    extern __shared__ double bins[];   // natom * 3
+    extern __shared__ double bins[];   // natom * 3, then the compact Boys rows of the class (if any)
+    constexpr int LP = LA + LB + LC + LD + 1;
+    BoysTables tabx = tab;
+    if constexpr (boys_rows_bytes<R, LP>() > 0) tabx = stage_boys_rows<R, LP>(tab, bins + grad_bins_len(natom));
     for (int i = threadIdx.x; i < natom * 3; i += blockDim.x) bins[i] = 0.0;
     __syncthreads();
     const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -133,7 +140,7 @@ __global__ __launch_bounds__(256, (GradOccupancy<LA, LB, LC, LD, MASK, R>::min_p
                 EriDeriv<LA, LB, LC, LD, MASK>::template run<R>(
                     a.prims + a.pair_off[bra], a.pair_np[bra], a.prims + a.pair_off[ket], a.pair_np[ket],
                     R(O[3 * sa] - O[3 * sb]), R(O[3 * sa + 1] - O[3 * sb + 1]), R(O[3 * sa + 2] - O[3 * sb + 2]),
-                    R(O[3 * sc] - O[3 * sd]), R(O[3 * sc + 1] - O[3 * sd + 1]), R(O[3 * sc + 2] - O[3 * sd + 2]), tab,
+                    R(O[3 * sc] - O[3 * sd]), R(O[3 * sc + 1] - O[3 * sd + 1]), R(O[3 * sc + 2] - O[3 * sd + 2]), tabx,
                     gam_fill, g, a.prim_q + a.pair_off[bra], a.prim_q + a.pair_off[ket], cut);
             }
             const int at_a = a.sh_atom[sa], at_b = a.sh_atom[sb];
@@ -164,8 +171,9 @@ __global__ __launch_bounds__(256, (GradOccupancy<LA, LB, LC, LD, MASK, R>::min_p
 
 template <int LA, int LB, int LC, int LD, int MASK, typename R>
 void launch_reg(const GradArgs& a, const BoysTables& tab, const Task* t, int nt, int natom, hipStream_t st) {
-    hipLaunchKernelGGL((k_grad_reg<LA, LB, LC, LD, MASK, R>), dim3((nt + 3) / 4), dim3(256),
-                       sizeof(double) * natom * 3, st, a, tab, t, nt, natom);
+    const size_t lds = sizeof(double) * grad_bins_len(natom) + boys_rows_bytes<R, LA + LB + LC + LD + 1>();
+    hipLaunchKernelGGL((k_grad_reg<LA, LB, LC, LD, MASK, R>), dim3((nt + 3) / 4), dim3(256), lds, st, a, tab, t, nt,
+                       natom);
 }
 
 }  // namespace m2

@@ -897,9 +897,11 @@ __global__ __launch_bounds__(256) void k_schwarz(JKArgs a, BoysTables tab, int p
 // the staged density rows as far as the whole stays within JK_DROW_LDS_MAX
 // (norm rows first, D rows too when there is room).  Sets a.k_lds, a.d_lds.
 // BB > 0: the kernel can also read its BB bytes of compact Boys rows from
-// LDS; they are added (a.boys_lds, a.boys_off) when that keeps the workgroups
-// per CU the LDS allows (160 KB per CU, 512-byte granules), or leaves room
-// for at least 5 waves per SIMD.
+// LDS.  They are added (a.boys_lds, a.boys_off) when that keeps the
+// workgroups per CU the LDS allowed (160 KB per CU, 512-byte granules) or
+// leaves room for at least 5 waves per SIMD, if need be in place of the
+// staged D rows (the Boys rows are read once per primitive quartet, the D
+// rows once per quartet).
 template <int NA, int NB, int NDEN, int BB>
 size_t jk_lds_plan(JKArgs& a, int waves) {
     const size_t rows = sizeof(double) * (size_t)NDEN * (NA + NB) * a.n;
@@ -908,23 +910,26 @@ size_t jk_lds_plan(JKArgs& a, int waves) {
     a.d_lds = 0;
     a.boys_lds = 0;
     a.boys_off = 0;
-    size_t base = 0;
+    auto base = [&](int dl) { return a.k_lds ? rows + (dl == 2 ? rows : 0) + (dl ? norms : 0) : 0; };
     if (a.k_lds) {
         if (2 * rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 2) a.d_lds = 2;
         else if (rows + norms <= JK_DROW_LDS_MAX && a.d_lds_max >= 1) a.d_lds = 1;
-        base = rows + (a.d_lds == 2 ? rows : 0) + (a.d_lds ? norms : 0);
     }
     if (BB > 0 && a.boys_lds_max) {
         constexpr size_t CU = 160 * 1024, G = 512;
-        const size_t off = (base + 15) & ~size_t(15), with = off + BB;
         auto per_cu = [&](size_t b) { return b ? CU / ((b + G - 1) / G * G) : CU; };
-        if (with <= CU && (per_cu(with) >= per_cu(base) || per_cu(with) * (size_t)waves >= 20)) {
-            a.boys_lds = 1;
-            a.boys_off = (int)off;
-            return with;
+        const size_t occ = per_cu(base(a.d_lds));
+        for (int dl = a.d_lds; dl >= (a.d_lds ? 1 : 0); --dl) {
+            const size_t off = (base(dl) + 15) & ~size_t(15), with = off + BB;
+            if (with <= CU && (per_cu(with) >= occ || per_cu(with) * (size_t)waves >= 20)) {
+                a.d_lds = dl;
+                a.boys_lds = 1;
+                a.boys_off = (int)off;
+                return with;
+            }
         }
     }
-    return base;
+    return base(a.d_lds);
 }
 
 // mode 0: FP64 kernel over the FP64 heads of the ket groups; mode 2: FP32
