@@ -137,6 +137,7 @@ JKArgs jk_args(m2_ctx* ctx) {
     a.nsh = ctx->obs.nsh;
     if (ctx->fam_on) {
         a.fam_prims = ctx->d_fam_prims.p;
+        a.fam_s = ctx->d_fam_s.p;
         a.fam_prims_f = ctx->d_fam_prims_f.p;
         a.fam_km = ctx->d_fam_km.p;
         a.fam_km_f = ctx->d_fam_km_f.p;
@@ -343,7 +344,17 @@ static m2_status build_families(m2_ctx* ctx) {
     HIPCHK(ctx->d_fam_mem.reserve(nfp));
     HIPCHK(ctx->d_pair_fam.reserve(npair));
     HIPCHK(ctx->d_pair_mem.reserve(npair));
+    std::vector<FamS> fs(prims.size());
+    for (size_t i = 0; i < prims.size(); ++i) {
+        fs[i] = FamS{};
+        fs[i].p = prims[i].p;
+        for (int k = 0; k < 3; ++k) fs[i].P[k] = prims[i].P[k];
+        for (int m = 0; m < FAM_MAXMEM; ++m) fs[i].km[m] = km[FAM_MAXMEM * i + m];
+        fs[i].q = fq[i];
+    }
+    HIPCHK(ctx->d_fam_s.reserve(std::max<size_t>(fs.size(), 1)));
     if (!prims.empty()) {
+        HIPCHK(hipMemcpy(ctx->d_fam_s.p, fs.data(), sizeof(FamS) * fs.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_fam_prims.p, prims.data(), sizeof(PrimPair) * prims.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_fam_prims_f.p, prims_f.data(), sizeof(BraPair<float>) * prims.size(),
                          hipMemcpyHostToDevice));

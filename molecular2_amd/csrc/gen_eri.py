@@ -538,7 +538,7 @@ def gen_class_fam(la, lb):
         em.append("      R t%s = R(0);" % a[1:])
     em.extend(bra_loop.lines)
     for m in range(FAM_MAXMEM):
-        em.append("      const R c%d = R(km[%d * ik + %d]);" % (m, FAM_MAXMEM, m))
+        em.append("      const R c%d = R(fam_factor(km, ket, ik, %d));" % (m, m))
     for m in range(FAM_MAXMEM):
         for a in accs:
             em.append("      %s_m%d += c%d * t%s;" % (a, m, m, a[1:]))

@@ -532,7 +532,12 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
             cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
         R v[FAM_MAXMEM * NCART];
-        EriClass<LA, LB, 0, 0>::template run_fam<R, PURE, true>(bprims, nbp, fprims + off, fkm + FAM_MAXMEM * off,
+        // FP64: the family primitive pairs through their FamS records
+        const auto kprims = [&]() {
+            if constexpr (std::is_same<R, double>::value) return a.fam_s + off;
+            else return fprims + off;
+        }();
+        EriClass<LA, LB, 0, 0>::template run_fam<R, PURE, true>(bprims, nbp, kprims, fkm + FAM_MAXMEM * off,
                                                                 nkp, ABx, ABy, ABz, tab, v, bq, kq, cut);
 #pragma unroll
         for (int m = 0; m < FAM_MAXMEM; ++m)

@@ -122,6 +122,7 @@ struct JKArgs {
     // primitive pairs among up to FAM_MAXMEM contracted member pairs
     int fam_kets;            // set by the host: ket indices of this launch are family pairs
     const PrimPair* fam_prims;        // primitive pairs, K without contraction coefficients
+    const struct FamS* fam_s;         // the same with their factors and bounds (FP64 kernels)
     const BraPair<float>* fam_prims_f;
     const double* fam_km;    // [prim][FAM_MAXMEM] member factors K c_c c_d (0 for absent members)
     const float* fam_km_f;
@@ -140,6 +141,31 @@ struct JKArgs {
 };
 
 constexpr int FAM_MAXMEM = 4;
+
+// The FP64 record a family primitive pair is read through by the family
+// kernels: its centre, exponent sum, member factors and bound in one
+// 80-byte record (instead of an 80-byte pair record of which 32 bytes are
+// read, the factors and the bound in two more arrays)
+struct alignas(16) FamS {
+    double p;
+    double P[3];
+    double km[FAM_MAXMEM];   // member factors K c_c c_d (0 for absent members)
+    float q;                 // bound of the family primitive pair
+    float pad[3];
+};
+static_assert(sizeof(FamS) == 80, "FamS layout");
+__device__ __forceinline__ float ket_bound(const FamS* __restrict__ ket, const float* __restrict__, int ik) {
+    return ket[ik].q;
+}
+// member factor m of ket primitive ik: from the factor array, or from a FamS record
+template <typename KP, typename KM>
+__device__ __forceinline__ KM fam_factor(const KM* __restrict__ km, const KP* __restrict__, int ik, int m) {
+    return km[FAM_MAXMEM * ik + m];
+}
+template <typename KM>
+__device__ __forceinline__ double fam_factor(const KM* __restrict__, const FamS* __restrict__ ket, int ik, int m) {
+    return ket[ik].km[m];
+}
 // class combinations (bra X | ket Y) run with s-s family kets: Y = (ss) and
 // X = (ss) or (pp) whenever the basis has families; X = (ps) or (ds) only when
 // the family members are mostly significant together (`wide`), since those
