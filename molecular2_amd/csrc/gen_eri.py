@@ -1647,13 +1647,17 @@ def write_deriv_programs(outdir):
 
 DERIV_REGISTER = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0), (1, 1, 1, 0), (2, 0, 1, 0),
                   (2, 1, 0, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0), (2, 2, 0, 0),
-                  (2, 2, 1, 0), (2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0)]
+                  (2, 2, 1, 0), (2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0), (2, 2, 1, 1), (2, 2, 2, 1)]
 # classes whose three derivative centres run as three lighter passes
 DERIV_SPLIT = {(1, 1, 1, 0), (2, 0, 1, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 1, 2, 0),
-               (2, 2, 0, 0), (2, 2, 1, 0), (2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0)}
+               (2, 2, 0, 0), (2, 2, 1, 0), (2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0), (2, 2, 1, 1), (2, 2, 2, 1)}
 # split classes whose every centre pass is split again over the c components
-# of Gamma (fewer accumulators per pass, each pass with its own VRR)
-DERIV_CSPLIT = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0)}
+# of Gamma (fewer accumulators per pass, each pass with its own VRR).  The
+# FP64 passes of (dd|pp) and (dd|dp) still spill (1.3-4.3 K VGPRs each, to
+# scratch) and run faster than the LDS interpreter all the same: (H2O)20
+# cc-pVDZ gradient 0.176 -> 0.165 s.  (dd|dd) stays in the interpreter: its
+# register passes take over half an hour to compile.
+DERIV_CSPLIT = {(2, 1, 1, 1), (2, 1, 2, 1), (2, 2, 2, 0), (2, 2, 1, 1), (2, 2, 2, 1)}
 
 
 def gen_deriv_class(la, lb, lc, ld, mask=7):
