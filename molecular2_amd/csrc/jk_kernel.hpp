@@ -191,13 +191,14 @@ template <int LA, int LB, int LC, int LD, typename R>
 struct JKWaves {
     static constexpr int value = EriClass<LA, LB, LC, LD>::NACC * (int)sizeof(R) <= 280 ? 8 : 4;
 };
-// (ps|ss) in FP64 runs 4-wave workgroups held to 5 waves per SIMD (its 102
-// VGPRs fit 5; 8-wave workgroups reach only 4); elsewhere no minimum
+// (ps|ss) in FP64 runs 4-wave workgroups held to 4 waves per SIMD (<= 128
+// VGPRs: room for its next ket primitive in flight; its LDS, with the staged
+// D and Boys rows, allows 4 workgroups per CU at n = 480); elsewhere no minimum
 template <int LA, int LB, int LC, int LD, typename R>
 struct JKOccupancy {
     static constexpr bool ps = LA == 1 && LB == 0 && LC == 0 && LD == 0 && sizeof(R) == 8;
     static constexpr int waves = ps ? 4 : JKWaves<LA, LB, LC, LD, R>::value;
-    static constexpr int min_per_eu = ps ? 5 : 1;
+    static constexpr int min_per_eu = ps ? 4 : 1;
 };
 
 // Work item -> ket chunk `ch` of the bra's list: first ket and count (<= 64)

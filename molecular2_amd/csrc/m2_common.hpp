@@ -41,6 +41,20 @@ struct alignas(16) KetS {
 };
 static_assert(sizeof(KetS) == 48, "KetS layout");
 
+// ket primitive ik when ik < nket (a zero record otherwise), for loading the
+// next ket primitive while the current one is computed
+template <typename KP>
+__device__ __forceinline__ KP load_ket(const KP* __restrict__ ket, int ik, int nket) {
+    return ik < nket ? ket[ik] : KP{};
+}
+// bound of a loaded ket primitive: from the bound array, or from its KetS record
+template <typename KP>
+__device__ __forceinline__ float ket_q(const KP&, const float* __restrict__ kq, int ik) {
+    return kq[ik];
+}
+struct KetS;
+__device__ __forceinline__ float ket_q(const KetS& k, const float* __restrict__, int ik);
+
 // bound of ket primitive ik: from the bound array, or from a KetS record
 template <typename KP>
 __device__ __forceinline__ float ket_bound(const KP* __restrict__, const float* __restrict__ kq, int ik) {
@@ -49,6 +63,7 @@ __device__ __forceinline__ float ket_bound(const KP* __restrict__, const float* 
 __device__ __forceinline__ float ket_bound(const KetS* __restrict__ ket, const float* __restrict__, int ik) {
     return ket[ik].q;
 }
+__device__ __forceinline__ float ket_q(const KetS& k, const float* __restrict__, int) { return k.q; }
 
 // Boys table: F_m(T_k) at T_k = k * BOYS_STEP, k = 0..BOYS_NGRID-1,
 // m = 0..BOYS_MTAB-1.  Beyond BOYS_TMAX the asymptotic branch is used.
