@@ -121,6 +121,7 @@ JKArgs jk_args(m2_ctx* ctx) {
     a.prims = ctx->d_prims.p;
     a.prims_f = ctx->d_prims_f.p;
     a.kets_s = ctx->d_kets_s.p;
+    a.kets_sf = ctx->d_kets_sf.p;
     a.pair_off = ctx->d_pair_off.p;
     a.pair_np = ctx->d_pair_np.p;
     a.pair_sh = ctx->d_pair_sh.p;
@@ -138,6 +139,7 @@ JKArgs jk_args(m2_ctx* ctx) {
     if (ctx->fam_on) {
         a.fam_prims = ctx->d_fam_prims.p;
         a.fam_s = ctx->d_fam_s.p;
+        a.fam_sf = ctx->d_fam_sf.p;
         a.fam_prims_f = ctx->d_fam_prims_f.p;
         a.fam_km = ctx->d_fam_km.p;
         a.fam_km_f = ctx->d_fam_km_f.p;
@@ -352,9 +354,19 @@ static m2_status build_families(m2_ctx* ctx) {
         for (int m = 0; m < FAM_MAXMEM; ++m) fs[i].km[m] = km[FAM_MAXMEM * i + m];
         fs[i].q = fq[i];
     }
+    std::vector<FamSf> fsf(prims.size());
+    for (size_t i = 0; i < prims.size(); ++i) {
+        fsf[i] = FamSf{};
+        fsf[i].p = (float)prims[i].p;
+        for (int k = 0; k < 3; ++k) fsf[i].P[k] = (float)prims[i].P[k];
+        for (int m = 0; m < FAM_MAXMEM; ++m) fsf[i].km[m] = (float)km[FAM_MAXMEM * i + m];
+        fsf[i].q = fq[i];
+    }
     HIPCHK(ctx->d_fam_s.reserve(std::max<size_t>(fs.size(), 1)));
+    HIPCHK(ctx->d_fam_sf.reserve(std::max<size_t>(fsf.size(), 1)));
     if (!prims.empty()) {
         HIPCHK(hipMemcpy(ctx->d_fam_s.p, fs.data(), sizeof(FamS) * fs.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_fam_sf.p, fsf.data(), sizeof(FamSf) * fsf.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_fam_prims.p, prims.data(), sizeof(PrimPair) * prims.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_fam_prims_f.p, prims_f.data(), sizeof(BraPair<float>) * prims.size(),
                          hipMemcpyHostToDevice));
@@ -519,6 +531,15 @@ m2_status build_pairs(m2_ctx* ctx) {
             }
             HIPCHK(ctx->d_kets_s.reserve(ks.size()));
             HIPCHK(hipMemcpy(ctx->d_kets_s.p, ks.data(), sizeof(KetS) * ks.size(), hipMemcpyHostToDevice));
+            std::vector<KetSf> ksf(prims.size());
+            for (size_t i = 0; i < prims.size(); ++i) {
+                ksf[i].p = (float)prims[i].p;
+                ksf[i].K = (float)prims[i].K;
+                for (int k = 0; k < 3; ++k) ksf[i].P[k] = (float)prims[i].P[k];
+                ksf[i].q = q[i];
+            }
+            HIPCHK(ctx->d_kets_sf.reserve(ksf.size()));
+            HIPCHK(hipMemcpy(ctx->d_kets_sf.p, ksf.data(), sizeof(KetSf) * ksf.size(), hipMemcpyHostToDevice));
         }
         return M2_OK;
     };
@@ -1180,6 +1201,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_prims.release();
     ctx->d_prims_f.release();
     ctx->d_kets_s.release();
+    ctx->d_kets_sf.release();
     ctx->d_pair_off.release();
     ctx->d_pair_np.release();
     ctx->d_pair_sh.release();

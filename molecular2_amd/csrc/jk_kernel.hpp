@@ -345,9 +345,10 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab0,
         }
         const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
         R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
-        // s-s kets of the FP64 kernels through their compact records
+        // s-s kets through their compact records
         const auto kprims = [&]() {
             if constexpr (LC + LD == 0 && std::is_same<R, double>::value) return a.kets_s + a.pair_off[ket];
+            else if constexpr (LC + LD == 0) return a.kets_sf + a.pair_off[ket];
             else return gp + a.pair_off[ket];
         }();
         EriClass<LA, LB, LC, LD>::template run<R, PURE, true>(bprims, nbp, kprims, nkp,
@@ -533,10 +534,10 @@ __device__ __forceinline__ void jk_fam_item(const JKArgs& a, const BoysTables& t
             cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
         R v[FAM_MAXMEM * NCART];
-        // FP64: the family primitive pairs through their FamS records
+        // the family primitive pairs through their FamS / FamSf records
         const auto kprims = [&]() {
             if constexpr (std::is_same<R, double>::value) return a.fam_s + off;
-            else return fprims + off;
+            else return a.fam_sf + off;
         }();
         EriClass<LA, LB, 0, 0>::template run_fam<R, PURE, true>(bprims, nbp, kprims, fkm + FAM_MAXMEM * off,
                                                                 nkp, ABx, ABy, ABz, tab, v, bq, kq, cut);

@@ -65,6 +65,18 @@ __device__ __forceinline__ float ket_bound(const KetS* __restrict__ ket, const f
 }
 __device__ __forceinline__ float ket_q(const KetS& k, const float* __restrict__, int) { return k.q; }
 
+// The same for the FP32 kernels (24 bytes)
+struct alignas(8) KetSf {
+    float p, K;
+    float P[3];
+    float q;
+};
+static_assert(sizeof(KetSf) == 24, "KetSf layout");
+__device__ __forceinline__ float ket_bound(const KetSf* __restrict__ ket, const float* __restrict__, int ik) {
+    return ket[ik].q;
+}
+__device__ __forceinline__ float ket_q(const KetSf& k, const float* __restrict__, int) { return k.q; }
+
 // Boys table: F_m(T_k) at T_k = k * BOYS_STEP, k = 0..BOYS_NGRID-1,
 // m = 0..BOYS_MTAB-1.  Beyond BOYS_TMAX the asymptotic branch is used.
 constexpr int BOYS_MTAB = 26;
@@ -93,6 +105,7 @@ struct JKArgs {
     const PrimPair* prims;
     const BraPair<float>* prims_f;   // the same records in FP32
     const KetS* kets_s;              // the same records as s-s kets (FP64)
+    const struct KetSf* kets_sf;     // and as s-s kets of the FP32 kernels
     const int* pair_off;     // first PrimPair of each shell pair
     const int* pair_np;      // number of PrimPairs of each shell pair
     const int2* pair_sh;     // (a, b) shell indices, la >= lb
@@ -138,6 +151,7 @@ struct JKArgs {
     int fam_kets;            // set by the host: ket indices of this launch are family pairs
     const PrimPair* fam_prims;        // primitive pairs, K without contraction coefficients
     const struct FamS* fam_s;         // the same with their factors and bounds (FP64 kernels)
+    const struct FamSf* fam_sf;       // (FP32 kernels)
     const BraPair<float>* fam_prims_f;
     const double* fam_km;    // [prim][FAM_MAXMEM] member factors K c_c c_d (0 for absent members)
     const float* fam_km_f;
@@ -179,6 +193,22 @@ __device__ __forceinline__ KM fam_factor(const KM* __restrict__ km, const KP* __
 }
 template <typename KM>
 __device__ __forceinline__ double fam_factor(const KM* __restrict__, const FamS* __restrict__ ket, int ik, int m) {
+    return ket[ik].km[m];
+}
+// The same for the FP32 family kernels (48 bytes)
+struct alignas(16) FamSf {
+    float p;
+    float P[3];
+    float km[FAM_MAXMEM];
+    float q;
+    float pad[3];
+};
+static_assert(sizeof(FamSf) == 48, "FamSf layout");
+__device__ __forceinline__ float ket_bound(const FamSf* __restrict__ ket, const float* __restrict__, int ik) {
+    return ket[ik].q;
+}
+template <typename KM>
+__device__ __forceinline__ float fam_factor(const KM* __restrict__, const FamSf* __restrict__ ket, int ik, int m) {
     return ket[ik].km[m];
 }
 // class combinations (bra X | ket Y) run with s-s family kets: Y = (ss) and
