@@ -121,6 +121,7 @@ JKArgs jk_args(m2_ctx* ctx) {
     a.prims = ctx->d_prims.p;
     a.prims_f = ctx->d_prims_f.p;
     a.kets_s = ctx->d_kets_s.p;
+    a.pair_rec = ctx->d_pair_rec.p;
     a.kets_sf = ctx->d_kets_sf.p;
     a.pair_off = ctx->d_pair_off.p;
     a.pair_np = ctx->d_pair_np.p;
@@ -658,6 +659,23 @@ m2_status build_pairs(m2_ctx* ctx) {
     if (st) return st;
     ctx->pair_Q = Qs;
     HIPCHK(hipMemcpy(ctx->d_pair_Q.p, Qs.data(), sizeof(double) * npair, hipMemcpyHostToDevice));
+    {
+        std::vector<PairRec> rec(npair);
+        for (int i = 0; i < npair; ++i) {
+            PairRec& r = rec[i];
+            r = PairRec{};
+            r.sc = ctx->pair_sh[i].x;
+            r.sd = ctx->pair_sh[i].y;
+            r.off = ctx->pair_off[i];
+            r.np = ctx->pair_np[i];
+            r.Q = Qs[i];
+            for (int k = 0; k < 3; ++k) r.CD[k] = b.sh[r.sc].O[k] - b.sh[r.sd].O[k];
+            r.fc = b.sh[r.sc].bf;
+            r.fd = b.sh[r.sd].bf;
+        }
+        HIPCHK(ctx->d_pair_rec.reserve(npair));
+        HIPCHK(hipMemcpy(ctx->d_pair_rec.p, rec.data(), sizeof(PairRec) * npair, hipMemcpyHostToDevice));
+    }
     for (int c = 0; c < NPAIRCLASS; ++c) ctx->cls[c] = PairClassSeg{};
     int i0 = 0;
     while (i0 < npair) {
@@ -1201,6 +1219,7 @@ void m2_ctx_destroy(m2_ctx* ctx) {
     ctx->d_prims.release();
     ctx->d_prims_f.release();
     ctx->d_kets_s.release();
+    ctx->d_pair_rec.release();
     ctx->d_kets_sf.release();
     ctx->d_pair_off.release();
     ctx->d_pair_np.release();

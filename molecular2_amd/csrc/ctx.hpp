@@ -109,6 +109,7 @@ struct m2_ctx {
     m2::DevBuf<m2::PrimPair> d_prims;
     m2::DevBuf<m2::BraPair<float>> d_prims_f;
     m2::DevBuf<m2::KetS> d_kets_s;   // the pair records as s-s kets (with their bounds)
+    m2::DevBuf<m2::PairRec> d_pair_rec;   // shell pairs as kets: shells, primitives, Q, C - D, first bfs
     m2::DevBuf<m2::KetSf> d_kets_sf;
     m2::DevBuf<m2::FamS> d_fam_s;    // the family primitive pairs with their factors and bounds
     m2::DevBuf<m2::FamSf> d_fam_sf;

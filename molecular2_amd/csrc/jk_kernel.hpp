@@ -193,12 +193,15 @@ struct JKWaves {
 };
 // (ps|ss) in FP64 runs 4-wave workgroups held to 4 waves per SIMD (<= 128
 // VGPRs: room for its next ket primitive in flight; its LDS, with the staged
-// D and Boys rows, allows 4 workgroups per CU at n = 480); elsewhere no minimum
+// D and Boys rows, allows 4 workgroups per CU at n = 480); (ps|ps) in FP64 is
+// held to 4 waves per SIMD too (it would take 129 VGPRs, 3 waves); elsewhere
+// no minimum
 template <int LA, int LB, int LC, int LD, typename R>
 struct JKOccupancy {
     static constexpr bool ps = LA == 1 && LB == 0 && LC == 0 && LD == 0 && sizeof(R) == 8;
+    static constexpr bool psps = LA == 1 && LB == 0 && LC == 1 && LD == 0 && sizeof(R) == 8;
     static constexpr int waves = ps ? 4 : JKWaves<LA, LB, LC, LD, R>::value;
-    static constexpr int min_per_eu = ps ? 4 : 1;
+    static constexpr int min_per_eu = ps || psps ? 4 : 1;
 };
 
 // Work item -> ket chunk `ch` of the bra's list: first ket and count (<= 64)
@@ -325,37 +328,37 @@ __device__ __forceinline__ void jk_item(const JKArgs& a, const BoysTables& tab0,
         int ket0, nket, g;
         chunk_kets(a, cut, ch, ket0, nket, g);
         const int ket = ket0 + (lane < nket ? lane : 0);
-        const int2 scd = a.pair_sh[ket];
-        const int sc = scd.x, sd = scd.y;
+        const PairRec pr = a.pair_rec[ket];   // one 64-byte record per ket
+        const int sc = pr.sc, sd = pr.sd;
         // exact screening of src/fockbuild.cpp:98-115
         const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]),
                                dnl ? fmax(fmax(sDn[sc], sDn[sd]), fmax(sDn[nsh + sc], sDn[nsh + sd]))
                                    : fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]),
                                           fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
-        const double bound = dn * qab * a.pair_Q[ket];
+        const double bound = dn * qab * pr.Q;
         const bool active = lane < nket && bound >= lo && bound < hi;
         if (!__any(active)) continue;
-        const float* kq = a.prim_q + a.pair_off[ket];
+        const float* kq = a.prim_q + pr.off;
         const float cut = float(a.prim_tau / dn);
-        const int nkp = active ? a.pair_np[ket] : 0;
+        const int nkp = active ? pr.np : 0;
         if (a.counters) {
             cnt_q += __popcll(__ballot(active));
             cnt_pq += (unsigned long long)(wave_sum(double(nkp)) * nbp + 0.5);
             cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
-        const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
+        const R CDx = R(pr.CD[0]), CDy = R(pr.CD[1]), CDz = R(pr.CD[2]);
         R v[NA * NB * NC * ND > EriClass<LA, LB, LC, LD>::NCART ? NA * NB * NC * ND : EriClass<LA, LB, LC, LD>::NCART];
         // s-s kets through their compact records
         const auto kprims = [&]() {
-            if constexpr (LC + LD == 0 && std::is_same<R, double>::value) return a.kets_s + a.pair_off[ket];
-            else if constexpr (LC + LD == 0) return a.kets_sf + a.pair_off[ket];
-            else return gp + a.pair_off[ket];
+            if constexpr (LC + LD == 0 && std::is_same<R, double>::value) return a.kets_s + pr.off;
+            else if constexpr (LC + LD == 0) return a.kets_sf + pr.off;
+            else return gp + pr.off;
         }();
         EriClass<LA, LB, LC, LD>::template run<R, PURE, true>(bprims, nbp, kprims, nkp,
                                                               ABx, ABy, ABz, CDx, CDy, CDz, tab, v, bq, kq, cut);
         if (active) {
             const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
-            const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+            const int fc = a.pair_rec[ket].fc, fd = a.pair_rec[ket].fd;   // read late: not live in the loop
             with_lds_mode(lm, [&](auto M) {
                 constexpr int LM = decltype(M)::value;
                 digest<NA, NB, NC, ND, LM, R>(v, fa, fb, fc, fd, f, Dab, jab, D, n, a.J, a.K, sK, sD);
@@ -786,28 +789,28 @@ __device__ __forceinline__ void jk_split_item(const JKArgs& a, const BoysTables&
         int ket0, nket, g;
         chunk_kets(a, cut, ch, ket0, nket, g);
         const int ket = ket0 + (lane < nket ? lane : 0);
-        const int2 scd = a.pair_sh[ket];
-        const int sc = scd.x, sd = scd.y;
+        const PairRec pr = a.pair_rec[ket];   // one 64-byte record per ket
+        const int sc = pr.sc, sd = pr.sd;
         // exact screening of src/fockbuild.cpp:98-115
         const double dn = fmax(fmax(dnab, Dn[sc * nsh + sd]),
                                dnl ? fmax(fmax(sDn[sc], sDn[sd]), fmax(sDn[nsh + sc], sDn[nsh + sd]))
                                    : fmax(fmax(Dn[sa * nsh + sc], Dn[sa * nsh + sd]),
                                           fmax(Dn[sb * nsh + sc], Dn[sb * nsh + sd])));
-        const double bound = dn * qab * a.pair_Q[ket];
+        const double bound = dn * qab * pr.Q;
         const bool active = lane < nket && bound >= lo && bound < hi;
         if (!__any(active)) continue;
-        const float* kq = a.prim_q + a.pair_off[ket];
+        const float* kq = a.prim_q + pr.off;
         const float cut = float(a.prim_tau / dn);
-        const int nkp = active ? a.pair_np[ket] : 0;
+        const int nkp = active ? pr.np : 0;
         if (a.counters) {
             cnt_q += __popcll(__ballot(active));
             cnt_pq += (unsigned long long)(wave_sum(double(nkp)) * nbp + 0.5);
             cnt_cpq += (unsigned long long)(wave_sum(double(screened_prims(bq, nbp, kq, nkp, cut))) + 0.5);
         }
-        const R CDx = R(O[3 * sc] - O[3 * sd]), CDy = R(O[3 * sc + 1] - O[3 * sd + 1]), CDz = R(O[3 * sc + 2] - O[3 * sd + 2]);
-        const int fc = a.sh_bf[sc], fd = a.sh_bf[sd];
+        const R CDx = R(pr.CD[0]), CDy = R(pr.CD[1]), CDz = R(pr.CD[2]);
+        const int fc = pr.fc, fd = pr.fd;
         const double f = fab * (sc == sd ? 1.0 : 2.0) * (bra == ket ? 1.0 : 2.0);
-        const PP* kprims = gp + a.pair_off[ket];
+        const PP* kprims = gp + pr.off;
         auto pass = [&](auto PC) {
             constexpr int P = decltype(PC)::value;
             using PT = PassTerms<S, TC, P>;

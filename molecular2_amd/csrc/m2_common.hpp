@@ -29,6 +29,18 @@ struct alignas(16) BraPair {
 };
 static_assert(sizeof(BraPair<double>) == 80 && sizeof(BraPair<float>) == 48, "BraPair layout");
 
+// The per-chunk data of a shell pair read as a ket by the J/K kernels, in
+// one 64-byte record (instead of ~10 scattered loads): its shells, first
+// primitive pair and count, Schwarz bound, C - D and first basis functions
+struct alignas(16) PairRec {
+    int sc, sd, off, np;
+    double Q;
+    double CD[3];
+    int fc, fd;
+    int pad[2];
+};
+static_assert(sizeof(PairRec) == 64, "PairRec layout");
+
 // The FP64 record a primitive pair is read through when it is the ket of an
 // s-s pair class: the fields such kets use and the pair's primitive bound,
 // 48 bytes instead of 80 (the lanes' ket records are the kernels' main
@@ -105,6 +117,7 @@ struct JKArgs {
     const PrimPair* prims;
     const BraPair<float>* prims_f;   // the same records in FP32
     const KetS* kets_s;              // the same records as s-s kets (FP64)
+    const PairRec* pair_rec;         // shell pairs as kets (per-chunk data)
     const struct KetSf* kets_sf;     // and as s-s kets of the FP32 kernels
     const int* pair_off;     // first PrimPair of each shell pair
     const int* pair_np;      // number of PrimPairs of each shell pair
