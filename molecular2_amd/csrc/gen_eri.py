@@ -768,7 +768,10 @@ def flop_model(la, lb, lc, ld):
       f_cont: per contracted quartet (HRR, solid-harmonic transform),
       n_eri:  spherical integrals per quartet."""
     L = la + lb + lc + ld
-    text = gen_class(la, lb, lc, ld)
+    # the textbook DAG: every VRR node formed, then added to its accumulator
+    # (the emitted kernels add terminal nodes into their accumulators term
+    # by term, which saves instructions, not operations of the scheme)
+    text = gen_class(la, lb, lc, ld, fuse=False)
     lines = text.splitlines()
     i0 = next(i for i, t in enumerate(lines) if "for (int ib = 0" in t)
     i1 = next(i for i in range(i0, len(lines)) if lines[i].strip() == "}")
