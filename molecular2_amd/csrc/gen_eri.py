@@ -540,14 +540,27 @@ def gen_class_fam(la, lb):
         for a in accs:
             em.append("    R %s_m%d = R(0);" % (a, m))
     em.append("    const float bq0 = SCR ? float(bq[0]) : 0.f;")
-    em.append("    for (int ik = 0; ik < nket; ++ik) {")
-    em.extend(screen)
-    em.extend(ket_head)
+    if (la, lb) == (0, 0):
+        # (ss|ss) loads its next family primitive while the current one is computed
+        em.append("    auto kp_next = load_ket(ket, 0, nket);")
+        em.append("    for (int ik = 0; ik < nket; ++ik) {")
+        em.append("      const auto kp = kp_next;")
+        em.append("      kp_next = load_ket(ket, ik + 1, nket);")
+        em.append("      const float qk = SCR ? ket_q(kp, kq, ik) : 0.f;")
+        em.append("      if (SCR && qk * bq0 < cut) break;")
+        em.extend(t for t in ket_head if t.strip() != "const auto kp = ket[ik];")
+    else:
+        em.append("    for (int ik = 0; ik < nket; ++ik) {")
+        em.extend(screen)
+        em.extend(ket_head)
     for a in accs:
         em.append("      R t%s = R(0);" % a[1:])
     em.extend(bra_loop.lines)
     for m in range(FAM_MAXMEM):
-        em.append("      const R c%d = R(fam_factor(km, ket, ik, %d));" % (m, m))
+        if (la, lb) == (0, 0):
+            em.append("      const R c%d = R(fam_factor_of(kp, km, ik, %d));" % (m, m))
+        else:
+            em.append("      const R c%d = R(fam_factor(km, ket, ik, %d));" % (m, m))
     for m in range(FAM_MAXMEM):
         for a in accs:
             em.append("      %s_m%d += c%d * t%s;" % (a, m, m, a[1:]))

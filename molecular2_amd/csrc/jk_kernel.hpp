@@ -428,10 +428,11 @@ struct FamPrimOf<float> {
 
 template <int LA, int LB, typename R>
 struct JKWavesFam {
-    // (ss|ss) in FP64 (91 VGPRs) runs 4-wave workgroups held to 5 waves per SIMD
+    // (ss|ss) in FP64 runs 4-wave workgroups held to 4 waves per SIMD (<= 128
+    // VGPRs: room for its next family primitive in flight)
     static constexpr bool ss = LA == 0 && LB == 0 && sizeof(R) == 8;
     static constexpr int value = ss ? 4 : EriClass<LA, LB, 0, 0>::NACC_FAM * (int)sizeof(R) <= 280 ? 8 : 4;
-    static constexpr int min_per_eu = ss ? 5 : 1;
+    static constexpr int min_per_eu = ss ? 4 : 1;
 };
 
 template <int LA, int LB, typename R, bool PURE, int NDEN, bool BL>

@@ -208,6 +208,18 @@ template <typename KM>
 __device__ __forceinline__ double fam_factor(const KM* __restrict__, const FamS* __restrict__ ket, int ik, int m) {
     return ket[ik].km[m];
 }
+// member factor m of a loaded family primitive: from the factor array, or
+// from its FamS / FamSf record
+template <typename KP, typename KM>
+__device__ __forceinline__ KM fam_factor_of(const KP&, const KM* __restrict__ km, int ik, int m) {
+    return km[FAM_MAXMEM * ik + m];
+}
+template <typename KM>
+__device__ __forceinline__ double fam_factor_of(const FamS& k, const KM* __restrict__, int, int m) {
+    return k.km[m];
+}
+__device__ __forceinline__ float ket_q(const FamS& k, const float* __restrict__, int) { return k.q; }
+
 // The same for the FP32 family kernels (48 bytes)
 struct alignas(16) FamSf {
     float p;
@@ -224,6 +236,11 @@ template <typename KM>
 __device__ __forceinline__ float fam_factor(const KM* __restrict__, const FamSf* __restrict__ ket, int ik, int m) {
     return ket[ik].km[m];
 }
+template <typename KM>
+__device__ __forceinline__ float fam_factor_of(const FamSf& k, const KM* __restrict__, int, int m) {
+    return k.km[m];
+}
+__device__ __forceinline__ float ket_q(const FamSf& k, const float* __restrict__, int) { return k.q; }
 // class combinations (bra X | ket Y) run with s-s family kets: Y = (ss) and
 // X = (ss) or (pp) whenever the basis has families; X = (ps) or (ds) only when
 // the family members are mostly significant together (`wide`), since those
