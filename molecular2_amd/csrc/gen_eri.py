@@ -503,6 +503,12 @@ FAM_MAXMEM = 4
 # waves per SIMD with it: 4.8 -> 4.6 ms on (H2O)20 cc-pVDZ) and (pp|ss)
 # (226 VGPRs either way, 2 waves per SIMD)
 FAM_PREFETCH = {(0, 0), (1, 1)}
+# register classes that load their next ket primitive ahead: those whose
+# kernels stay within the same waves per SIMD with the extra record (2 waves
+# at <= 256 VGPRs or 1 wave; (ps|ss), a family-bra class, is set in
+# gen_class_fam)
+PREFETCH = {(1, 1, 1, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 0, 0), (2, 1, 1, 0), (2, 1, 2, 0),
+            (2, 2, 1, 0)}
 
 
 def gen_class_fam(la, lb):
@@ -735,7 +741,8 @@ def main(outdir):
             classes.add((lp, 0, lq, 0))
     written = []
     for c in sorted(classes):
-        body = gen_class_fam(c[0], c[1]) if (c[2], c[3]) == (0, 0) and (c[0], c[1]) in FAM_BRA else gen_class(*c)
+        body = gen_class_fam(c[0], c[1]) if (c[2], c[3]) == (0, 0) and (c[0], c[1]) in FAM_BRA else \
+            gen_class(*c, prefetch=c in PREFETCH)
         text = "// Generated by molecular2_amd/csrc/gen_eri.py -- do not edit.\n#pragma once\n" + body + "\n"
         path = os.path.join(outdir, "eri_%d%d%d%d.inc" % c)
         if not os.path.exists(path) or open(path).read() != text:
